@@ -1,0 +1,385 @@
+/*
+ * rpx.h -- C ABI of the MI355X (gfx950) ray-tracing and gausslet field-summation
+ * library (librpx.so).
+ *
+ * Every entry point takes plain pointers and sizes.  Ray state lives in device
+ * memory (HBM) as structure-of-arrays; the caller owns every buffer (it may come
+ * from any allocator, torch included).  The library owns only scene handles.
+ * Functions never throw across the ABI: they return an rpx_status_t and
+ * rpx_last_error() gives a message for the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the raypier repository):
+ *   raypier/core/ctracer.pyx   trace_segment_c / trace_one_face_segment_c /
+ *                              trace_gausslet_c / trace_one_face_gausslet_c,
+ *                              FaceList.intersect_c, compute_orientation_c,
+ *                              RayCollection.copy_as_array / from_array
+ *   raypier/core/cfaces.pyx    Face.intersect_c / compute_normal_c (per face type)
+ *   raypier/core/cshapes.pyx   Shape.point_inside_c
+ *   raypier/core/cdistortions.pyx  ZernikeDistortion.z_offset(_and_gradient)_c
+ *   raypier/core/cmaterials.pyx    InterfaceMaterial.eval_child_ray_c /
+ *                                  eval_parabasal_ray_c, BaseDispersionCurve
+ *   raypier/core/cfields.pyx   sum_gaussian_modes, evaluate_modes
+ *   raypier/core/fields.py     evaluate_neighbours_gc
+ */
+#ifndef RPX_H
+#define RPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RPX_ABI_VERSION 1
+
+/* ------------------------------------------------------------------ status */
+typedef int32_t rpx_status_t;
+#define RPX_OK              0
+#define RPX_E_INVALID       1   /* bad argument / malformed scene table        */
+#define RPX_E_UNSUPPORTED   2   /* face / material / shape type not on device  */
+#define RPX_E_CAPACITY      3   /* output buffer too small; *n_out = required  */
+#define RPX_E_HIP           4   /* HIP runtime error                           */
+#define RPX_E_NO_DEVICE     5   /* no usable gfx950 device                     */
+
+/* ------------------------------------------------------------ ray flags   */
+/* ctracer.pyx:20-22 */
+#define RPX_REFL_RAY   (1u << 0)
+#define RPX_GAUSSLET   (1u << 1)
+#define RPX_PARABASAL  (1u << 2)
+#define RPX_NO_FACE    0xFFFFFFFFu   /* end_face_idx / parent_idx of "-1" */
+
+/* --------------------------------------------------------- ray SoA layout */
+/* A generation of rays in HBM.  Field k of ray i lives at f64[k*capacity+i]
+ * (resp. u32[k*capacity+i]).  The 21 doubles and 5 uint32 are exactly the
+ * fields of ctracer.pxd:40-48 ray_t (188 bytes per ray). */
+enum rpx_ray_f64_field {
+    RPX_OX = 0, RPX_OY, RPX_OZ,          /* origin            */
+    RPX_DX, RPX_DY, RPX_DZ,              /* direction         */
+    RPX_NX, RPX_NY, RPX_NZ,              /* normal            */
+    RPX_EX, RPX_EY, RPX_EZ,              /* E_vector          */
+    RPX_N_RE, RPX_N_IM,                  /* refractive_index  */
+    RPX_E1_RE, RPX_E1_IM,                /* E1_amp            */
+    RPX_E2_RE, RPX_E2_IM,                /* E2_amp            */
+    RPX_LENGTH, RPX_PHASE, RPX_ACCUM,    /* length, phase, accumulated_path */
+    RPX_N_F64_FIELDS                     /* = 21 */
+};
+enum rpx_ray_u32_field {
+    RPX_WL_IDX = 0, RPX_PARENT_IDX, RPX_END_FACE_IDX, RPX_RAY_IDENT, RPX_RAY_TYPE_ID,
+    RPX_N_U32_FIELDS                     /* = 5 */
+};
+/* Parabasal rays of a gausslet (ctracer.pxd:54-64): 6 rays x 10 doubles,
+ * field (j*10 + f) of gausslet i at para[(j*10+f)*capacity + i],
+ * f = ox,oy,oz,dx,dy,dz,nx,ny,nz,length. */
+#define RPX_N_PARA        6
+#define RPX_PARA_FIELDS   10
+
+typedef struct rpx_rays {
+    double   *f64;       /* RPX_N_F64_FIELDS * capacity doubles            */
+    uint32_t *u32;       /* RPX_N_U32_FIELDS * capacity uint32             */
+    double   *para;      /* gausslets only: 60 * capacity doubles, else 0  */
+    uint64_t  capacity;  /* field stride, in elements                      */
+} rpx_rays_t;
+
+/* ------------------------------------------------------------ scene table */
+/* Transform of a FaceList: p' = M p + t  (ctracer.pxd:67-69 transform_t). */
+typedef struct rpx_transform {
+    double m[9];
+    double t[3];
+} rpx_transform_t;
+
+typedef struct rpx_face_list {
+    rpx_transform_t trans;       /* FaceList.transform          */
+    rpx_transform_t inv_trans;   /* FaceList.inverse_transform  */
+    int32_t first_face;          /* index into rpx_scene_desc.faces */
+    int32_t n_faces;
+    int32_t pad[2];
+} rpx_face_list_t;
+
+/* Face types (cfaces.pyx). */
+enum rpx_face_type {
+    RPX_FACE_CIRCULAR = 1,         /* CircularFace          cfaces.pyx:140  */
+    RPX_FACE_SHAPED_PLANAR = 2,    /* ShapedPlanarFace      cfaces.pyx:194  */
+    RPX_FACE_RECTANGULAR = 3,      /* RectangularFace       cfaces.pyx:354  */
+    RPX_FACE_POLYGON = 4,          /* PolygonFace           cfaces.pyx:1077 */
+    RPX_FACE_SPHERICAL = 5,        /* SphericalFace         cfaces.pyx:410  */
+    RPX_FACE_SHAPED_SPHERICAL = 6, /* ShapedSphericalFace   cfaces.pyx:501  */
+    RPX_FACE_EXTRUDED_PLANAR = 7,  /* ExtrudedPlanarFace    cfaces.pyx:611  */
+    RPX_FACE_CONIC = 8,            /* ConicRevolutionFace   cfaces.pyx:1751 */
+    RPX_FACE_ASPHERIC = 9,         /* AsphericFace          cfaces.pyx:1885 */
+    RPX_FACE_DISTORTION = 10       /* DistortionFace        cfaces.pyx:2323 */
+};
+
+#define RPX_FACE_NPARAM 16
+/* Parameter slots p[] per face type:
+ *   CIRCULAR          diameter, offset, z_plane
+ *   SHAPED_PLANAR     z_height
+ *   RECTANGULAR       length, width, offset, z_plane
+ *   POLYGON           z_plane                    (aux0 = vertex offset, aux1 = count)
+ *   SPHERICAL         diameter, curvature, z_height
+ *   SHAPED_SPHERICAL  curvature, z_height
+ *   EXTRUDED_PLANAR   x1, y1, x2, y2, z1, z2, normal.x, normal.y, normal.z
+ *   CONIC             curvature, z_height, conic_const
+ *   ASPHERIC          curvature, z_height, conic_const, A4, A6, A8, A10, A12, A14, A16, atol
+ *   DISTORTION        accuracy                   (aux0 = base face index, aux1 = zernike index)
+ * A DISTORTION face's base face is an ordinary record of the face table that
+ * belongs to no face list (n_faces of every list excludes it). */
+typedef struct rpx_face {
+    int32_t type;            /* rpx_face_type                               */
+    int32_t global_idx;      /* Face.idx: position in chain(face_lists)     */
+    int32_t face_list;       /* owning face list, -1 for a distortion base  */
+    int32_t material;        /* index into materials, -1 = no children      */
+    int32_t invert_normal;   /* Face.invert_normal (orientation flip)       */
+    int32_t invert_normals;  /* CircularFace / ShapedFace.invert_normals     */
+    int32_t shape;           /* root op index into shape_ops, -1 = none      */
+    int32_t aux0, aux1;
+    int32_t pad;
+    double  tolerance;       /* Face.tolerance                              */
+    double  p[RPX_FACE_NPARAM];
+} rpx_face_t;
+
+/* Aperture shapes (cshapes.pyx) as postfix programs.  A face's `shape` field
+ * is the index of the program's last (root) op; the root op's `begin` field
+ * is the index of its first op.  Ops begin..root run on a boolean stack. */
+enum rpx_shape_opcode {
+    RPX_SHAPE_TRUE = 1,     /* Shape base class: always inside   ctracer.pyx:1659 */
+    RPX_SHAPE_CIRCLE = 2,   /* CircleShape     cshapes.pyx:102 p = cx, cy, radius */
+    RPX_SHAPE_RECT = 3,     /* RectangleShape  cshapes.pyx:119 p = cx, cy, width, height */
+    RPX_SHAPE_POLYGON = 4,  /* PolygonShape    cshapes.pyx:139 a = vertex offset, b = count */
+    RPX_SHAPE_NOT = 5,      /* InvertShape     cshapes.pyx:34  */
+    RPX_SHAPE_AND = 6,      /* BooleanAND      cshapes.pyx:55  */
+    RPX_SHAPE_OR = 7,       /* BooleanOR       cshapes.pyx:60  */
+    RPX_SHAPE_XOR = 8       /* BooleanXOR      cshapes.pyx:65  */
+};
+typedef struct rpx_shape_op {
+    int32_t op;
+    int32_t a, b;
+    int32_t begin;           /* root op: index of the program's first op */
+    double  p[4];
+} rpx_shape_op_t;
+
+/* Dispersion curve (cmaterials.pyx:104-239): n(wl) by formula 0/1/2/3/5 over
+ * coefs[0..n_coefs), imaginary part 1e-4*absorption/(4 pi)*wl. */
+#define RPX_MAX_DISP_COEFS 18
+typedef struct rpx_dispersion {
+    int32_t formula_id;
+    int32_t n_coefs;
+    double  absorption;
+    double  coefs[RPX_MAX_DISP_COEFS];
+} rpx_dispersion_t;
+
+enum rpx_material_type {
+    RPX_MAT_NONE = 0,               /* InterfaceMaterial base: no children     */
+    RPX_MAT_OPAQUE = 1,             /* OpaqueMaterial            cmaterials.pyx:242  */
+    RPX_MAT_TRANSPARENT = 2,        /* TransparentMaterial       cmaterials.pyx:254  */
+    RPX_MAT_PEC = 3,                /* PECMaterial               cmaterials.pyx:281  */
+    RPX_MAT_PARTIAL = 4,            /* PartiallyReflectiveMaterial cmaterials.pyx:322 */
+    RPX_MAT_DIELECTRIC = 5,         /* DielectricMaterial        cmaterials.pyx:554  */
+    RPX_MAT_FULL_DIELECTRIC = 6,    /* FullDielectricMaterial    cmaterials.pyx:727  */
+    RPX_MAT_FULL_DISPERSIVE = 7,    /* FullDielectricDispersiveMaterial cmaterials.pyx:875 */
+    RPX_MAT_SINGLE_LAYER = 8,       /* SingleLayerCoatedMaterial cmaterials.pyx:1017 */
+    RPX_MAT_COATED_DISPERSIVE = 9,  /* CoatedDispersiveMaterial  cmaterials.pyx:1187 */
+    RPX_MAT_GRATING = 10            /* DiffractionGratingMaterial cmaterials.pyx:1437 */
+};
+
+typedef struct rpx_material {
+    int32_t type;                   /* rpx_material_type                         */
+    int32_t order;                  /* grating order                             */
+    int32_t pad[2];
+    double  reflection_threshold;
+    double  transmission_threshold;
+    double  n_inside[2];            /* complex (re, im), non-dispersive types    */
+    double  n_outside[2];
+    double  n_coating[2];
+    double  thickness;              /* coating thickness in microns              */
+    double  reflectivity;           /* PartiallyReflectiveMaterial               */
+    double  lines_per_mm;           /* grating                                   */
+    double  efficiency;             /* grating                                   */
+    double  origin[3];              /* grating origin                            */
+    double  pad2;
+    rpx_dispersion_t disp_inside;   /* dispersive types                          */
+    rpx_dispersion_t disp_outside;
+    rpx_dispersion_t disp_coating;
+} rpx_material_t;
+
+/* Zernike distortion (cdistortions.pyx:323-514) compiled to straight-line form.
+ * The reference evaluates radial polynomials by a memoised recursion over a
+ * 3 x k_max workspace.  At scene upload the recursion is executed symbolically,
+ * preserving the workspace slot each value is cached in (including slots shared
+ * by different (n,m) pairs), giving a list of instructions
+ *     slot[dst] = combine(op, value(a), value(b), value(c), value(d), value(e))
+ * where value(x) = slot[x] for x >= 0, 0.0 for x == -1, 1.0 for x == -2, and
+ *   op 0 (R):        r*(a + b) - c
+ *   op 1 (R'):       ((a + b) + r*(c + d)) - e
+ *   op 2 (R/r):      (a + b) - c
+ * Then, per coefficient i (sorted by ANSI j), Z terms use value(r_ref[i]),
+ * value(rp_ref[i]), value(ror_ref[i]).  Two programs exist per distortion:
+ * `sag` (z_offset_c: radial values only) and `grad` (z_offset_and_gradient_c). */
+typedef struct rpx_zernike_insn {
+    int32_t op;
+    int32_t dst;
+    int32_t src[5];
+    int32_t pad;
+} rpx_zernike_insn_t;
+
+typedef struct rpx_zernike_coef {
+    int32_t n, m;             /* m signed (ANSI)                          */
+    int32_t r_ref, rp_ref, ror_ref;   /* value refs in the grad program   */
+    int32_t sag_ref;                  /* value ref in the sag program     */
+    double  value;
+} rpx_zernike_coef_t;
+
+typedef struct rpx_zernike {
+    double  unit_radius;
+    int32_t n_coefs;
+    int32_t coef_begin;       /* index into zernike_coefs                  */
+    int32_t sag_begin, sag_len;     /* range in zernike_insns              */
+    int32_t grad_begin, grad_len;
+    int32_t n_slots;          /* 3 * k_max                                 */
+    int32_t pad;
+} rpx_zernike_t;
+
+#define RPX_MAX_ZERNIKE_SLOTS 96
+
+typedef struct rpx_scene_desc {
+    const rpx_face_list_t   *face_lists;   int32_t n_face_lists;
+    const rpx_face_t        *faces;        int32_t n_faces;
+    const rpx_material_t    *materials;    int32_t n_materials;
+    const rpx_shape_op_t    *shape_ops;    int32_t n_shape_ops;
+    const double            *vertices;     int32_t n_vertices;   /* (x, y) pairs */
+    const rpx_zernike_t     *zernikes;     int32_t n_zernikes;
+    const rpx_zernike_coef_t *zernike_coefs; int32_t n_zernike_coefs;
+    const rpx_zernike_insn_t *zernike_insns; int32_t n_zernike_insns;
+} rpx_scene_desc_t;
+
+typedef struct rpx_scene rpx_scene_t;   /* opaque, device-resident copy */
+
+/* ----------------------------------------------------------------- library */
+int32_t     rpx_abi_version(void);
+const char *rpx_last_error(void);
+/* Number of bytes of each host-side struct, for binding checks. */
+int32_t     rpx_struct_size(const char *name);
+
+/* Upload a scene (face lists, faces, materials, shapes, distortions) to the
+ * current device.  Replaces the per-generation walk over Python FaceList /
+ * Face / InterfaceMaterial objects in ctracer.pyx:2062-2118 and the
+ * FaceList.sync_transforms calls of ctracer.pyx:2173-2211. */
+rpx_status_t rpx_scene_create(const rpx_scene_desc_t *desc, rpx_scene_t **out);
+rpx_status_t rpx_scene_destroy(rpx_scene_t *scene);
+/* Largest number of children one ray can produce in this scene. */
+int32_t      rpx_scene_max_children(const rpx_scene_t *scene);
+
+/* Workspace bytes needed to trace n_in rays (counts, block offsets). */
+uint64_t rpx_trace_workspace_bytes(uint64_t n_in);
+
+/* One non-sequential generation: ctracer.pyx:2062-2118 trace_segment_c.
+ *   in        parent generation; length and end_face_idx are updated in place
+ *   wavelengths  device array of vacuum wavelengths (microns) indexed by
+ *             wavelength_idx (the materials' `wavelengths`)
+ *   out       children in parent-major order, reflected before transmitted
+ *   face_hits device uint32[n_global_faces], incremented per hit (Face.count)
+ *   n_out     host pointer; receives the number of children.  If it exceeds
+ *             out->capacity, RPX_E_CAPACITY is returned and nothing is written
+ *             to `out` (the in-place parent update still happened).
+ * The call is stream-ordered and returns after the stream has completed. */
+rpx_status_t rpx_trace_segment(const rpx_scene_t *scene,
+                               const rpx_rays_t *in, uint64_t n_in,
+                               float max_length,
+                               const double *wavelengths, uint64_t n_wavelengths,
+                               rpx_rays_t *out, uint64_t *n_out,
+                               uint32_t *face_hits,
+                               void *workspace, uint64_t workspace_bytes,
+                               void *stream);
+
+/* Sequential variant, one prescribed face: ctracer.pyx:2121-2170
+ * trace_one_face_segment_c.  face_list / face_idx select FaceList and the
+ * index of the face inside it. */
+rpx_status_t rpx_trace_one_face_segment(const rpx_scene_t *scene,
+                               int32_t face_list, int32_t face_idx,
+                               const rpx_rays_t *in, uint64_t n_in,
+                               float max_length,
+                               const double *wavelengths, uint64_t n_wavelengths,
+                               rpx_rays_t *out, uint64_t *n_out,
+                               uint32_t *face_hits,
+                               void *workspace, uint64_t workspace_bytes,
+                               void *stream);
+
+/* Gausslet generation: ctracer.pyx:2214-2281 trace_gausslet_c with
+ * trace_parabasal_rays (ctracer.pyx:2350-2385).  `in`/`out` carry para[].
+ * face_list < 0 selects the non-sequential search; otherwise the single face
+ * (face_list, face_idx) as in trace_one_face_gausslet_c (ctracer.pyx:2284). */
+rpx_status_t rpx_trace_gausslet(const rpx_scene_t *scene,
+                               int32_t face_list, int32_t face_idx,
+                               const rpx_rays_t *in, uint64_t n_in,
+                               double max_length,
+                               const double *wavelengths, uint64_t n_wavelengths,
+                               rpx_rays_t *out, uint64_t *n_out,
+                               uint32_t *face_hits,
+                               void *workspace, uint64_t workspace_bytes,
+                               void *stream);
+
+/* Per-object queries, batched over n device-resident inputs.
+ *   rpx_query_face: what = 0 -> out[i] = Face.intersect_c(p1[i], p2[i],
+ *   is_base_ray).dist (cfaces.pyx, -1 for no intersection); what = 1 ->
+ *   out[3i..3i+2] = Face.compute_normal_c(p1[i]) (local frame).
+ *   face_rec indexes the scene's face table. */
+rpx_status_t rpx_query_face(const rpx_scene_t *scene, int32_t face_rec, int32_t what,
+                            const double *p1, const double *p2, uint64_t n,
+                            int32_t is_base_ray, double *out, void *stream);
+/* Shape.point_inside_c (cshapes.pyx) of the program rooted at `root`;
+ * xy = n (x, y) pairs, out = n int32 0/1. */
+rpx_status_t rpx_query_shape(const rpx_scene_t *scene, int32_t root, const double *xy,
+                             uint64_t n, int32_t *out, void *stream);
+/* ZernikeDistortion.z_offset_c -> sag[i] and z_offset_and_gradient_c ->
+ * grad[3i..3i+2] = (dz/dx, dz/dy, z) (cdistortions.pyx:416-514).
+ * Either output may be null. */
+rpx_status_t rpx_query_zernike(const rpx_scene_t *scene, int32_t zernike, const double *xy,
+                               uint64_t n, double *sag, double *grad, void *stream);
+/* InterfaceMaterial.eval_child_ray_c for n independent (ray, point,
+ * orientation) inputs (ctracer.pyx:1620-1632).  Ray i's children go to
+ * out[3i + k], k < counts[i]; idx_base + i is written as their parent_idx. */
+rpx_status_t rpx_eval_child_rays(const rpx_scene_t *scene, int32_t material,
+                                 const rpx_rays_t *in, uint64_t n, uint32_t idx_base,
+                                 const double *points, const double *normals,
+                                 const double *tangents,
+                                 const double *wavelengths, uint64_t n_wavelengths,
+                                 rpx_rays_t *out, uint32_t *counts, void *stream);
+
+/* AoS <-> SoA: RayCollection.copy_as_array / from_array (ctracer.pyx:1048,
+ * 1142) for the packed 188-byte ray_t and 668-byte gausslet_t layouts. */
+rpx_status_t rpx_pack_rays(const rpx_rays_t *soa, uint64_t n, void *aos,
+                           int32_t gausslet, void *stream);
+rpx_status_t rpx_unpack_rays(const void *aos, uint64_t n, rpx_rays_t *soa,
+                             int32_t gausslet, void *stream);
+
+/* evaluate_neighbours_gc (core/fields.py:114-136) fused with evaluate_modes
+ * (cfields.pyx:156-228): per gausslet, the 3 complex mode coefficients.
+ *   modes  device complex128[n][3] (interleaved re, im) */
+rpx_status_t rpx_gausslet_modes(const rpx_rays_t *gausslets, uint64_t n,
+                                double blending, double *modes, void *stream);
+
+/* evaluate_modes (cfields.pyx:218-228) on explicit x, y, dx, dy arrays of
+ * shape [n][row] (device, float64). */
+rpx_status_t rpx_evaluate_modes(const double *x, const double *y,
+                                const double *dx, const double *dy,
+                                uint64_t n, int32_t row, double blending,
+                                double *modes, void *stream);
+
+/* sum_gaussian_modes (cfields.pyx:52-115).
+ *   rays      base rays (SoA, only the ray part is read)
+ *   modes     device complex128[n_rays][3]
+ *   wavelengths device float64 table indexed by wavelength_idx
+ *   points    device float64[n_pts][3]
+ *   out       device complex128[n_pts][3], overwritten
+ *   precision 0 = FP64 (the reference arithmetic),
+ *             1 = FP32 amplitudes with double-single phase */
+rpx_status_t rpx_sum_gaussian_modes(const rpx_rays_t *rays, uint64_t n_rays,
+                                    const double *modes,
+                                    const double *wavelengths, uint64_t n_wavelengths,
+                                    const double *points, uint64_t n_pts,
+                                    double time_ps, double *out,
+                                    int32_t precision, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RPX_H */

@@ -1,0 +1,205 @@
+"""CPU oracle for the raypier tracing loop and gausslet field sum.
+
+TEST INFRASTRUCTURE ONLY.  This package (rpx_oracle.c restating the
+reference Cython statement by statement, built by oracle/Makefile) is imported
+by tests/, by __graft_entry__.smoke() as the checker and by bench.py's
+cpu_baseline leg.  The product library never links or calls it.
+
+Functions take the scene tables compiled by raypier_optics_amd._scene and
+numpy arrays of the packed ray_dtype / gausslet_dtype records.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        lib = ctypes.CDLL(LIB)
+        vp, u64, i32, f64 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double
+        lib.oracle_trace_segment.restype = u64
+        lib.oracle_trace_segment.argtypes = [vp, i32, i32, vp, u64, ctypes.c_float, vp, u64, vp, u64, vp]
+        lib.oracle_trace_gausslet.restype = u64
+        lib.oracle_trace_gausslet.argtypes = [vp, i32, i32, vp, u64, f64, vp, u64, vp, u64, vp]
+        lib.oracle_face_intersect.restype = f64
+        lib.oracle_face_intersect.argtypes = [vp, i32, vp, vp, i32]
+        lib.oracle_face_normal.argtypes = [vp, i32, vp, vp]
+        lib.oracle_shape_inside.restype = i32
+        lib.oracle_shape_inside.argtypes = [vp, i32, f64, f64]
+        lib.oracle_zernike.argtypes = [vp, i32, vp, vp, u64, vp, vp]
+        lib.oracle_eval_child_ray.restype = u64
+        lib.oracle_eval_child_ray.argtypes = [vp, i32, vp, ctypes.c_uint32, vp, vp, vp, vp, u64, vp, u64]
+        lib.oracle_dispersion.argtypes = [vp, vp, u64, vp]
+        lib.oracle_evaluate_modes.argtypes = [vp, vp, vp, vp, u64, i32, f64, vp]
+        lib.oracle_neighbours_gc.argtypes = [vp, u64, vp, vp, vp, vp]
+        lib.oracle_sum_gaussian_modes.argtypes = [vp, u64, vp, vp, u64, vp, u64, f64, vp]
+        lib.oracle_sizes.restype = i32
+        lib.oracle_sizes.argtypes = [i32]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+class _Desc(object):
+    """Keeps a scene's host tables alive while the oracle reads them."""
+
+    def __init__(self, tables):
+        self.tables = tables
+        self.desc = tables.desc()
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.desc)
+
+
+def trace_segment(tables, rays, max_length, wavelengths, one_list=-1, one_face=-1, max_children=3):
+    """One generation; `rays` (ray_dtype) is updated in place.  Returns
+    (children, face_hits)."""
+    lib = load()
+    d = _Desc(tables)
+    wl = np.ascontiguousarray(wavelengths, dtype=np.float64)
+    cap = max_children * len(rays) + 1
+    out = np.zeros(cap, rays.dtype)
+    hits = np.zeros(max(tables.n_global_faces, 1), np.uint32)
+    n = lib.oracle_trace_segment(d.ref, one_list, one_face, _p(rays), len(rays), max_length, _p(wl), len(wl),
+                                 _p(out), cap, _p(hits))
+    assert n <= cap
+    return out[:n].copy(), hits
+
+
+def trace_gausslet(tables, gs, max_length, wavelengths, one_list=-1, one_face=-1, max_children=3):
+    lib = load()
+    d = _Desc(tables)
+    wl = np.ascontiguousarray(wavelengths, dtype=np.float64)
+    cap = max_children * len(gs) + 1
+    out = np.zeros(cap, gs.dtype)
+    hits = np.zeros(max(tables.n_global_faces, 1), np.uint32)
+    n = lib.oracle_trace_gausslet(d.ref, one_list, one_face, _p(gs), len(gs), max_length, _p(wl), len(wl),
+                                  _p(out), cap, _p(hits))
+    assert n <= cap
+    return out[:n].copy(), hits
+
+
+def trace_rays(tables, rays, max_length, wavelengths, recursion_limit, gausslet=False):
+    """Generation loop of core/tracer.py:9-47 over an already compiled scene."""
+    rays = rays.copy()
+    if gausslet:
+        rays["base_ray"]["length"] = max_length
+        rays["para_rays"]["length"] = max_length
+    else:
+        rays["length"] = max_length
+    traced = []
+    count = 0
+    hits = np.zeros(max(tables.n_global_faces, 1), np.uint64)
+    while len(rays) > 0 and count < recursion_limit:
+        traced.append(rays)
+        if gausslet:
+            rays, h = trace_gausslet(tables, rays, max_length, wavelengths)
+        else:
+            rays, h = trace_segment(tables, rays, max_length, wavelengths)
+        hits += h
+        count += 1
+    return traced, hits
+
+
+def face_intersect(tables, rec, p1, p2, is_base_ray):
+    lib = load()
+    d = _Desc(tables)
+    a = np.ascontiguousarray(p1, np.float64)
+    b = np.ascontiguousarray(p2, np.float64)
+    return lib.oracle_face_intersect(d.ref, rec, _p(a), _p(b), int(is_base_ray))
+
+
+def face_normal(tables, rec, p):
+    lib = load()
+    d = _Desc(tables)
+    a = np.ascontiguousarray(p, np.float64)
+    out = np.zeros(3)
+    lib.oracle_face_normal(d.ref, rec, _p(a), _p(out))
+    return out
+
+
+def shape_inside(tables, root, x, y):
+    lib = load()
+    d = _Desc(tables)
+    return bool(lib.oracle_shape_inside(d.ref, root, float(x), float(y)))
+
+
+def zernike(tables, zi, x, y):
+    lib = load()
+    d = _Desc(tables)
+    x = np.ascontiguousarray(x, np.float64).ravel()
+    y = np.ascontiguousarray(y, np.float64).ravel()
+    sag = np.zeros(len(x))
+    grad = np.zeros((len(x), 3))
+    lib.oracle_zernike(d.ref, zi, _p(x), _p(y), len(x), _p(sag), _p(grad))
+    return sag, grad
+
+
+def eval_child_ray(tables, mat, ray, idx, point, normal, tangent, wavelengths=()):
+    """Children of one ray (ray_dtype record array of length 1, mutated as the
+    reference mutates nothing here)."""
+    lib = load()
+    d = _Desc(tables)
+    r = np.ascontiguousarray(ray).copy()
+    wl = np.ascontiguousarray(wavelengths, np.float64)
+    out = np.zeros(4, r.dtype)
+    pt, nm, tg = (np.ascontiguousarray(v, np.float64) for v in (point, normal, tangent))
+    n = lib.oracle_eval_child_ray(d.ref, mat, _p(r), idx, _p(pt), _p(nm), _p(tg), _p(wl) if len(wl) else None,
+                                  len(wl), _p(out), 4)
+    return out[:n].copy()
+
+
+def dispersion(disp_rec, wl):
+    lib = load()
+    rec = np.ascontiguousarray(disp_rec)
+    w = np.ascontiguousarray(wl, np.float64)
+    out = np.zeros(2 * len(w))
+    lib.oracle_dispersion(_p(rec), _p(w), len(w), _p(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def evaluate_modes(x, y, dx, dy, blending=1.0):
+    lib = load()
+    x, y, dx, dy = (np.ascontiguousarray(v, np.float64) for v in (x, y, dx, dy))
+    n, size = x.shape
+    out = np.zeros((n, 3), np.complex128)
+    lib.oracle_evaluate_modes(_p(x), _p(y), _p(dx), _p(dy), n, size, blending, _p(out))
+    return out
+
+
+def neighbours_gc(gs):
+    lib = load()
+    g = np.ascontiguousarray(gs)
+    n = len(g)
+    X, Y, DX, DY = (np.zeros((n, 6)) for _ in range(4))
+    lib.oracle_neighbours_gc(_p(g), n, _p(X), _p(Y), _p(DX), _p(DY))
+    return X, Y, DX, DY
+
+
+def sum_gaussian_modes(rays, modes, wavelengths, points, time_ps=0.0):
+    lib = load()
+    r = np.ascontiguousarray(rays)
+    m = np.ascontiguousarray(modes, np.complex128)
+    wl = np.ascontiguousarray(wavelengths, np.float64)
+    pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    out = np.zeros((len(pts), 3), np.complex128)
+    lib.oracle_sum_gaussian_modes(_p(r), len(r), _p(m), _p(wl), len(wl), _p(pts), len(pts), time_ps, _p(out))
+    return out

@@ -1,0 +1,539 @@
+// Face intersection, normals, aperture shapes and Zernike distortions (FP64).
+//
+// Each routine follows the arithmetic of the reference face it stands in for;
+// the reference line is cited beside it.  `is_base` is the reference's
+// is_base_ray flag: parabasal rays skip most aperture tests.
+#pragma once
+#include "rpx_internal.h"
+
+namespace rpx {
+
+// --------------------------------------------------------------- shapes
+// PolygonShape.point_inside_c (cshapes.pyx:150-168)
+RPX_DEV bool polygon_shape_inside(const double *pts, int size, double X, double Y) {
+    bool ct = false;
+    double y1 = pts[2 * (size - 1) + 1], x1 = pts[2 * (size - 1)];
+    for (int i = 0; i < size; ++i) {
+        double y2 = pts[2 * i + 1], x2 = pts[2 * i];
+        if ((y1 <= Y && Y < y2) || (y2 <= Y && Y < y1)) {
+            if ((x1 + ((Y - y1) * (x2 - x1)) / (y2 - y1)) > X) ct = !ct;
+        }
+        y1 = y2;
+        x1 = x2;
+    }
+    return ct;
+}
+
+// point_in_polygon_c used by PolygonFace (cfaces.pyx:1050-1068)
+RPX_DEV bool polygon_face_inside(const double *pts, int size, double X, double Y) {
+    bool ct = false;
+    double y1 = pts[2 * (size - 1) + 1], x1 = pts[2 * (size - 1)];
+    for (int i = 0; i < size; ++i) {
+        double y2 = pts[2 * i + 1], x2 = pts[2 * i];
+        double h = (Y - y1) / (y2 - y1);
+        if (0.0 < h && h <= 1.0) {
+            double x = x1 + h * (x2 - x1);
+            if (x > X) ct = !ct;
+        }
+        y1 = y2;
+        x1 = x2;
+    }
+    return ct;
+}
+
+// Shape tree (cshapes.pyx:34-168) evaluated as a postfix program.
+RPX_DEV bool shape_inside(const SceneView &S, int root, double x, double y) {
+    if (root < 0) return true;
+    uint32_t stack = 0;
+    int sp = 0;
+    int begin = S.shape_ops[root].begin;
+    for (int i = begin; i <= root; ++i) {
+        const rpx_shape_op_t &op = S.shape_ops[i];
+        uint32_t v = 0;
+        switch (op.op) {
+        case RPX_SHAPE_TRUE:
+            v = 1;
+            break;
+        case RPX_SHAPE_CIRCLE: {
+            double dx = x - op.p[0], dy = y - op.p[1];
+            v = ((dx * dx) + (dy * dy) < (op.p[2] * op.p[2])) ? 1u : 0u;
+        } break;
+        case RPX_SHAPE_RECT: {
+            double dx = x - op.p[0], dy = y - op.p[1];
+            v = ((2 * fabs(dx) < op.p[2]) && (2 * fabs(dy)) < op.p[3]) ? 1u : 0u;
+        } break;
+        case RPX_SHAPE_POLYGON:
+            v = polygon_shape_inside(S.vertices + 2 * op.a, op.b, x, y) ? 1u : 0u;
+            break;
+        case RPX_SHAPE_NOT:
+            --sp;
+            v = ((stack >> sp) & 1u) ^ 1u;
+            stack &= ~(1u << sp);
+            break;
+        default: {  // AND / OR / XOR
+            --sp;
+            uint32_t b = (stack >> sp) & 1u;
+            stack &= ~(1u << sp);
+            --sp;
+            uint32_t a = (stack >> sp) & 1u;
+            stack &= ~(1u << sp);
+            v = op.op == RPX_SHAPE_AND ? (a & b) : op.op == RPX_SHAPE_OR ? (a | b) : (a ^ b);
+        } break;
+        }
+        stack |= (v << sp);
+        ++sp;
+    }
+    return ((stack >> (sp - 1)) & 1u) != 0;
+}
+
+// ------------------------------------------------------------- Zernike
+// Straight-line form of the memoised radial recursion of
+// cdistortions.pyx:149-316 (see rpx_zernike_insn_t).
+RPX_DEV double zref(const double *slot, int ref) {
+    return ref >= 0 ? slot[ref] : (ref == -1 ? 0.0 : 1.0);
+}
+
+RPX_DEV void zernike_run(const SceneView &S, int begin, int len, double r, double *slot) {
+    for (int i = begin; i < begin + len; ++i) {
+        const rpx_zernike_insn_t &in = S.zinsns[i];
+        double a = zref(slot, in.src[0]), b = zref(slot, in.src[1]), c = zref(slot, in.src[2]);
+        double v;
+        if (in.op == 0) {
+            v = r * (a + b);
+            v -= c;
+        } else if (in.op == 1) {
+            double d = zref(slot, in.src[3]), e = zref(slot, in.src[4]);
+            v = a + b;
+            v += r * (c + d);
+            v -= e;
+        } else {
+            v = a + b;
+            v -= c;
+        }
+        slot[in.dst] = v;
+    }
+}
+
+// ZernikeDistortion.z_offset_c (cdistortions.pyx:416-450)
+RPX_DEV double zernike_sag(const SceneView &S, int zi, double x, double y) {
+    const rpx_zernike_t &Z = S.zernikes[zi];
+    double slot[RPX_MAX_ZERNIKE_SLOTS];
+    x /= Z.unit_radius;
+    y /= Z.unit_radius;
+    double r = sqrt(x * x + y * y);
+    double theta = atan2(y, x);
+    zernike_run(S, Z.sag_begin, Z.sag_len, r, slot);
+    double z = 0.0;
+    for (int i = 0; i < Z.n_coefs; ++i) {
+        const rpx_zernike_coef_t &c = S.zcoefs[Z.coef_begin + i];
+        double N = (c.m == 0) ? sqrt((double)(c.n + 1)) : sqrt((double)(2 * (c.n + 1)));
+        N *= c.value;
+        double PH = (c.m >= 0) ? cos(c.m * theta) : -sin(c.m * theta);
+        double R = zref(slot, c.sag_ref);
+        z += N * R * PH;
+    }
+    return z;
+}
+
+// ZernikeDistortion.z_offset_and_gradient_c (cdistortions.pyx:456-514):
+// returns (dz/dx, dz/dy, z).
+RPX_DEV V3 zernike_grad(const SceneView &S, int zi, double x, double y) {
+    const rpx_zernike_t &Z = S.zernikes[zi];
+    double slot[RPX_MAX_ZERNIKE_SLOTS];
+    x /= Z.unit_radius;
+    y /= Z.unit_radius;
+    double r = sqrt(x * x + y * y);
+    double theta = atan2(y, x);
+    zernike_run(S, Z.grad_begin, Z.grad_len, r, slot);
+    V3 out = v3(0.0, 0.0, 0.0);
+    double st = sin(theta), ct = cos(theta);
+    for (int i = 0; i < Z.n_coefs; ++i) {
+        const rpx_zernike_coef_t &c = S.zcoefs[Z.coef_begin + i];
+        double PH, PHp;
+        if (c.m >= 0) {
+            PH = cos(c.m * theta);
+            PHp = (double)(-c.m) * sin(c.m * theta);
+        } else {
+            PH = -sin(c.m * theta);
+            PHp = (double)(-c.m) * cos(c.m * theta);
+        }
+        double R = zref(slot, c.r_ref), Rp = zref(slot, c.rp_ref), Ror = zref(slot, c.ror_ref);
+        double N = (c.m == 0) ? sqrt((double)(c.n + 1)) : sqrt((double)(2 * (c.n + 1)));
+        N *= c.value;
+        out.z += N * R * PH;
+        out.x += N * (Rp * ct * PH + Ror * (-st) * PHp);
+        out.y += N * (Rp * st * PH + Ror * (ct)*PHp);
+    }
+    out.x /= Z.unit_radius;
+    out.y /= Z.unit_radius;
+    return out;
+}
+
+// ----------------------------------------------------------- conic / asphere
+// intersect_conic (cfaces.pyx:1695-1747)
+RPX_DEV double intersect_conic(V3 a, V3 d, double curvature, double conic_const) {
+    double beta = 1 + conic_const;
+    double R = -curvature;
+    double b2 = beta * beta;
+    double A = b2 * (d.z * d.z) + beta * (d.x * d.x) + beta * (d.y * d.y);
+    double B = -2 * R * beta * d.z + 2 * a.x * beta * d.x + 2 * a.y * beta * d.y + 2 * a.z * b2 * d.z;
+    double C = -2 * R * a.z * beta + (a.x * a.x) * beta + (a.y * a.y) * beta + (a.z * a.z) * b2;
+    double D = B * B - 4 * A * C;
+    if (D < 0) return -1;
+    D = sqrt(D);
+    if (R * beta * d.z <= 0) return (-B + D) / (2 * A);
+    return (-B - D) / (2 * A);
+}
+
+RPX_DEV double ipow(double x, int n) { return pow(x, (double)n); }
+
+struct Asph {
+    double R, beta, A4, A6, A8, A10, A12, A14, A16;
+    V3 a, d;
+};
+
+// eval_aspheric_impf (cfaces.pyx:1854-1863)
+RPX_DEV double asph_impf(const Asph &A, double alpha) {
+    double tx = A.a.x + alpha * A.d.x, ty = A.a.y + alpha * A.d.y;
+    double r2 = (tx * tx + ty * ty);
+    double out = r2;
+    out /= A.R * (1 + sqrt(1 - A.beta * r2 / (A.R * A.R)));
+    out -= A.a.z + alpha * A.d.z;
+    out += A.A4 * (r2 * r2) + A.A6 * ipow(r2, 3) + A.A8 * ipow(r2, 4) + A.A10 * ipow(r2, 5) +
+           A.A12 * ipow(r2, 6) + A.A14 * ipow(r2, 7) + A.A16 * ipow(r2, 8);
+    return out;
+}
+
+// eval_aspheric_grad (cfaces.pyx:1866-1882)
+RPX_DEV double asph_grad(const Asph &A, double alpha) {
+    double tx = A.a.x + alpha * A.d.x, ty = A.a.y + alpha * A.d.y;
+    double r2 = (tx * tx + ty * ty);
+    double dx = A.d.x * tx;
+    double dy = A.d.y * ty;
+    double out = A.A10 * (10 * dx + 10 * dy) * ipow(r2, 4);
+    out += A.A12 * (12 * dx + 12 * dy) * ipow(r2, 5);
+    out += A.A14 * (14 * dx + 14 * dy) * ipow(r2, 6);
+    out += A.A16 * (16 * dx + 16 * dy) * ipow(r2, 7);
+    out += A.A4 * (4 * dx + 4 * dy) * (r2);
+    out += A.A6 * (6 * dx + 6 * dy) * (r2 * r2);
+    out += A.A8 * (8 * dx + 8 * dy) * ipow(r2, 3) - A.d.z;
+    out += (2 * dx + 2 * dy) / (A.R * (sqrt(1 - A.beta * (r2) / (A.R * A.R)) + 1));
+    double s = sqrt(1 - A.beta * r2 / (A.R * A.R));
+    double s1 = s + 1;
+    out += A.beta * (2 * dx + 2 * dy) * (r2) / (2 * ipow(A.R, 3) * s * (s1 * s1));
+    return out;
+}
+
+// ------------------------------------------------------------- intersect
+RPX_DEV double face_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V3 p2, int is_base);
+
+// Base-face intersection used by DistortionFace (no recursion into a second
+// distortion level: the host refuses nested distortions).
+RPX_DEV double plain_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V3 p2, int is_base) {
+    const double tol = f.tolerance;
+    switch (f.type) {
+    case RPX_FACE_CIRCULAR: {  // cfaces.pyx:151-178
+        double max_length = sep(p1, p2);
+        double h = (f.p[2] - p1.z) / (p2.z - p1.z);
+        double d = f.p[0];
+        if ((h < tol) || (h > 1.0)) return -1;
+        double X = p1.x + h * (p2.x - p1.x) - f.p[1];
+        double Y = p1.y + h * (p2.y - p1.y);
+        if (is_base && (X * X + Y * Y) > (d * d / 4)) return -1;
+        return h * max_length;
+    }
+    case RPX_FACE_SHAPED_PLANAR: {  // cfaces.pyx:201-226
+        double max_length = sep(p1, p2);
+        double h = (f.p[0] - p1.z) / (p2.z - p1.z);
+        if ((h < tol) || (h > 1.0)) return -1;
+        double X = p1.x + h * (p2.x - p1.x);
+        double Y = p1.y + h * (p2.y - p1.y);
+        if (is_base && !shape_inside(S, f.shape, X, Y)) return -1;
+        return h * max_length;
+    }
+    case RPX_FACE_RECTANGULAR: {  // cfaces.pyx:365-397
+        double max_length = sep(p1, p2);
+        double h = (f.p[3] - p1.z) / (p2.z - p1.z);
+        double lngth = f.p[0], wdth = f.p[1];
+        if ((h < tol) || (h > 1.0)) return -1;
+        if (is_base) {
+            double X = p1.x + h * (p2.x - p1.x) - f.p[2];
+            double Y = p1.y + h * (p2.y - p1.y);
+            if (X * X > lngth * lngth / 4) return -1;
+            if (Y * Y > wdth * wdth / 4) return -1;
+        }
+        return h * max_length;
+    }
+    case RPX_FACE_POLYGON: {  // cfaces.pyx:1093-1108
+        double max_length = sep(p1, p2);
+        double h = (f.p[0] - p1.z) / (p2.z - p1.z);
+        if ((h < tol) || (h > 1.0)) return -1;
+        double X = p1.x + h * (p2.x - p1.x);
+        double Y = p1.y + h * (p2.y - p1.y);
+        if (is_base && polygon_face_inside(S.vertices + 2 * f.aux0, f.aux1, X, Y)) return h * max_length;
+        return -1;
+    }
+    case RPX_FACE_SPHERICAL:
+    case RPX_FACE_SHAPED_SPHERICAL: {  // cfaces.pyx:421-486 / 513-576
+        const bool shaped = f.type == RPX_FACE_SHAPED_SPHERICAL;
+        const double curvature = shaped ? f.p[0] : f.p[1];
+        const double z_height = shaped ? f.p[1] : f.p[2];
+        V3 s = sub(p2, p1);
+        double cz = z_height - curvature;
+        V3 d = p1;
+        d.z -= cz;
+        double A = mag_sq(s);
+        double B = 2 * dot(s, d);
+        double C = mag_sq(d) - curvature * curvature;
+        double D = B * B - 4 * A * C;
+        if (D < 0) return -1;
+        D = sqrt(D);
+        double a1 = (-B + D) / (2 * A);
+        V3 pt1 = add(p1, mul(s, a1));
+        double a2 = (-B - D) / (2 * A);
+        V3 pt2 = add(p1, mul(s, a2));
+        if (curvature >= 0) {
+            if (pt1.z < cz) a1 = RPX_INF;
+            if (pt2.z < cz) a2 = RPX_INF;
+        } else {
+            if (pt1.z > cz) a1 = RPX_INF;
+            if (pt2.z > cz) a2 = RPX_INF;
+        }
+        if (is_base) {
+            if (shaped) {
+                if (!shape_inside(S, f.shape, pt1.x, pt1.y)) a1 = RPX_INF;
+                if (!shape_inside(S, f.shape, pt2.x, pt2.y)) a2 = RPX_INF;
+            } else {
+                double DD = f.p[0] * f.p[0] / 4.;
+                if ((pt1.x * pt1.x + pt1.y * pt1.y) > DD) a1 = RPX_INF;
+                if ((pt2.x * pt2.x + pt2.y * pt2.y) > DD) a2 = RPX_INF;
+            }
+        }
+        if (a2 < a1) a1 = a2;
+        if (a1 > 1.0 || a1 < tol) return -1;
+        return a1 * sep(p1, p2);
+    }
+    case RPX_FACE_EXTRUDED_PLANAR: {  // cfaces.pyx:665-699
+        double ux = f.p[0], uy = f.p[1];
+        double vx = f.p[2] - ux, vy = f.p[3] - uy;
+        V3 s = sub(p2, p1);
+        double a;
+        if (is_base) {
+            a = (s.y * (ux - p1.x) - s.x * (uy - p1.y)) / (s.x * vy - s.y * vx);
+            if (a < 0) return -1;
+            if (a > 1) return -1;
+        }
+        a = (vx * (p1.y - uy) - vy * (p1.x - ux)) / (s.x * vy - s.y * vx);
+        if (is_base) {
+            double dz = a * (p2.z - p1.z);
+            double zz = p1.z + dz;
+            if (f.p[4] < zz && zz < f.p[5]) return a * mag(s);
+            return -1;
+        }
+        return a * mag(s);
+    }
+    case RPX_FACE_CONIC: {  // cfaces.pyx:1767-1798
+        V3 d = sub(p2, p1);
+        V3 a = p1;
+        a.z -= f.p[1];
+        double a1 = intersect_conic(a, d, f.p[0], f.p[2]);
+        V3 pt1 = add(a, mul(d, a1));
+        if (is_base && !shape_inside(S, f.shape, pt1.x, pt1.y)) return -1;
+        if (a1 > 1.0 || a1 < tol) return -1;
+        return a1 * sep(p1, p2);
+    }
+    case RPX_FACE_ASPHERIC: {  // cfaces.pyx:1909-1976
+        double atol = f.p[10];
+        double tl = atol * atol;
+        V3 d = sub(p2, p1);
+        V3 a = p1;
+        a.z -= f.p[1];
+        double a1 = intersect_conic(a, d, f.p[0], f.p[2]);
+        Asph A;
+        A.R = -f.p[0];
+        A.beta = 1 + f.p[2];
+        A.A4 = f.p[3]; A.A6 = f.p[4]; A.A8 = f.p[5]; A.A10 = f.p[6];
+        A.A12 = f.p[7]; A.A14 = f.p[8]; A.A16 = f.p[9];
+        A.a = a;
+        A.d = d;
+        double fv = asph_impf(A, a1), f_last = fv;
+        double dz = -fv / asph_grad(A, a1);
+        bool converged = false;
+        for (int i = 0; i < 100; ++i) {
+            a1 += dz;
+            if (dz * dz < tl) { converged = true; break; }
+            fv = asph_impf(A, a1);
+            if (fabs(fv) > fabs(f_last)) return -1;
+            f_last = fv;
+            dz = -fv / asph_grad(A, a1);
+        }
+        if (!converged) return -1;
+        V3 pt1 = add(a, mul(d, a1));
+        if (is_base && !shape_inside(S, f.shape, pt1.x, pt1.y)) return -1;
+        if (a1 > 1.0 || a1 < tol) return -1;
+        return a1 * sep(p1, p2);
+    }
+    default:
+        return -1;
+    }
+}
+
+// Local-frame normal of a face (compute_normal_c of each class).
+RPX_DEV V3 plain_normal(const SceneView &S, const rpx_face_t &f, V3 p) {
+    switch (f.type) {
+    case RPX_FACE_CIRCULAR:  // cfaces.pyx:180-191
+        return v3(0, 0, f.invert_normals ? 1 : -1);
+    case RPX_FACE_SHAPED_PLANAR:  // cfaces.pyx:228-236
+        return v3(0, 0, 1);
+    case RPX_FACE_RECTANGULAR:
+    case RPX_FACE_POLYGON:  // cfaces.pyx:399-407, 1110-1118
+        return v3(0, 0, -1);
+    case RPX_FACE_SPHERICAL:
+    case RPX_FACE_SHAPED_SPHERICAL: {  // cfaces.pyx:488-498
+        const bool shaped = f.type == RPX_FACE_SHAPED_SPHERICAL;
+        const double curvature = shaped ? f.p[0] : f.p[1];
+        const double z_height = shaped ? f.p[1] : f.p[2];
+        p.z -= (z_height - curvature);
+        if (curvature < 0) {
+            p.z = -p.z;
+            p.y = -p.y;
+            p.x = -p.x;
+        }
+        return norm(p);
+    }
+    case RPX_FACE_EXTRUDED_PLANAR:  // cfaces.pyx:701-702
+        return v3(f.p[6], f.p[7], f.p[8]);
+    case RPX_FACE_CONIC: {  // cfaces.pyx:1800-1823
+        double R = -f.p[0];
+        double beta = 1 + f.p[2];
+        int sign = f.invert_normals ? -1 : 1;
+        p.z -= f.p[1];
+        V3 g;
+        g.z = 2 * beta * (R - beta * p.z);
+        g.x = -p.x * 2 * beta;
+        g.y = -p.y * 2 * beta;
+        if ((R * beta) < 0) sign *= -1;
+        g.z *= sign;
+        g.y *= sign;
+        g.x *= sign;
+        return norm(g);
+    }
+    case RPX_FACE_ASPHERIC: {  // cfaces.pyx:1978-2007
+        double R = -f.p[0];
+        double beta = 1 + f.p[2];
+        int sign = f.invert_normals ? -1 : 1;
+        p.z -= f.p[1];
+        double r2 = p.x * p.x + p.y * p.y;
+        double root = sqrt(1 - (beta * (r2) / (R * R)));
+        double A4 = f.p[3], A6 = f.p[4], A8 = f.p[5], A10 = f.p[6], A12 = f.p[7], A14 = f.p[8], A16 = f.p[9];
+        double df = 10 * A10 * ipow(r2, 4) + 8 * A8 * ipow(r2, 3) + 6 * A6 * (r2 * r2) + 4 * A4 * r2;
+        df += 16 * A16 * ipow(r2, 7) + 14 * A14 * ipow(r2, 6) + 12 * A12 * ipow(r2, 5);
+        df += 2 / (R * (1 + root));
+        double r1 = 1 + root;
+        df += beta * (r2) / (ipow(R, 3) * root * (r1 * r1));
+        V3 g;
+        g.z = 1.0;
+        g.x = -df * p.x;
+        g.y = -df * p.y;
+        g.z *= sign;
+        g.y *= sign;
+        g.x *= sign;
+        return norm(g);
+    }
+    default:
+        return p;
+    }
+}
+
+// DistortionFace.intersect_c (cfaces.pyx:2339-2416)
+RPX_DEV double distortion_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V3 p2) {
+    const rpx_face_t &face = S.faces[f.aux0];
+    const int zi = f.aux1;
+    double a1, a2, h = sep(p2, p1);
+    double tolerance = f.p[0];
+    a2 = plain_intersect(S, face, p1, p2, 0);
+    if (a2 > h || a2 < f.tolerance) return -1;
+    V3 d = sub(p2, p1);
+    V3 pt1 = add(p1, mul(d, a2 / h));
+    V3 dxdyz = zernike_grad(S, zi, pt1.x, pt1.y);
+    V3 n = plain_normal(S, face, pt1);
+    pt1.z += dxdyz.z;
+    n.x /= n.z;
+    n.y /= n.z;
+    n.x -= dxdyz.x;
+    n.y -= dxdyz.y;
+    V3 o = sub(p1, pt1);
+    a1 = -h * dot(o, n) / dot(d, n);
+    if (a1 < fabs(dxdyz.z)) return -1;
+    for (int i = 0; i < 20; ++i) {
+        pt1 = add(p1, mul(d, a1 / h));
+        if (fabs(a1 - a2) < tolerance) break;
+        double z_shift = zernike_sag(S, zi, pt1.x, pt1.y);
+        V3 q1 = p1, q2 = p2;
+        q1.z -= z_shift;
+        q2.z -= z_shift;
+        a2 = plain_intersect(S, face, q1, q2, 0);
+        pt1 = add(q1, mul(d, a2 / h));
+        n = plain_normal(S, face, pt1);
+        dxdyz = zernike_grad(S, zi, pt1.x, pt1.y);
+        pt1.z += dxdyz.z;
+        n.x /= n.z;
+        n.y /= n.z;
+        n.x -= dxdyz.x;
+        n.y -= dxdyz.y;
+        o = sub(p1, pt1);
+        a2 = a1;
+        a1 = -h * dot(o, n) / dot(d, n);
+    }
+    if (!shape_inside(S, f.shape, pt1.x, pt1.y)) return -1;
+    return a1;
+}
+
+RPX_DEV double face_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V3 p2, int is_base) {
+    if (f.type == RPX_FACE_DISTORTION) return distortion_intersect(S, f, p1, p2);
+    return plain_intersect(S, f, p1, p2, is_base);
+}
+
+// DistortionFace.compute_normal_c (cfaces.pyx:2418-2431)
+RPX_DEV V3 face_normal(const SceneView &S, const rpx_face_t &f, V3 p) {
+    if (f.type != RPX_FACE_DISTORTION) return plain_normal(S, f, p);
+    V3 dxdyz = zernike_grad(S, f.aux1, p.x, p.y);
+    V3 p1 = p;
+    p1.z -= dxdyz.z;
+    V3 n = plain_normal(S, S.faces[f.aux0], p1);
+    n.x /= n.z;
+    n.y /= n.z;
+    n.z = 1.0;
+    n.x -= dxdyz.x;
+    n.y -= dxdyz.y;
+    return norm(n);
+}
+
+// compute_tangent_c: (1,0,0) by default, (0,0,1) for ExtrudedPlanarFace
+// (ctracer.pyx:1786-1791, cfaces.pyx:704-709).
+RPX_DEV V3 face_tangent(const rpx_face_t &f) {
+    if (f.type == RPX_FACE_EXTRUDED_PLANAR) return v3(0.0, 0.0, 1.0);
+    return v3(1.0, 0.0, 0.0);
+}
+
+struct Orientation {
+    V3 normal, tangent;
+};
+
+// FaceList.compute_orientation_c (ctracer.pyx:1939-1953)
+RPX_DEV Orientation compute_orientation(const SceneView &S, const rpx_face_list_t &fl,
+                                        const rpx_face_t &f, V3 point) {
+    point = transform(fl.inv_trans, point);
+    Orientation o;
+    o.normal = face_normal(S, f, point);
+    o.tangent = face_tangent(f);
+    if (f.invert_normal) {
+        o.normal = neg(o.normal);
+        o.tangent = neg(o.tangent);
+    }
+    o.normal = rotate(fl.trans, o.normal);
+    o.tangent = rotate(fl.trans, o.tangent);
+    return o;
+}
+
+}  // namespace rpx

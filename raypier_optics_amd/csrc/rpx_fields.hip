@@ -1,0 +1,241 @@
+// Gausslet E-field summation on gfx950.
+//
+//   k_modes_gc    one gausslet per lane: parabasal offsets/slopes
+//                 (evaluate_neighbours_gc, core/fields.py:114-136) fused with
+//                 the closed-form 3x3 least-squares mode fit
+//                 (evaluate_one_mode, cfields.pyx:156-215).
+//   k_field_prep  per-gausslet invariants of sum_gaussian_modes
+//                 (cfields.pyx:76-99) into a structure-of-arrays table.
+//   k_field_sum   one evaluation point per lane; gausslet invariants stream
+//                 through LDS in tiles and every lane sums them in gausslet
+//                 order, so each point's sum has the reference's order.
+#include <hip/hip_runtime.h>
+#include "rpx_materials.h"
+#include "rpx_host.h"
+
+using namespace rpx;
+
+// --------------------------------------------------------------- mode fit
+RPX_DEV void fit_one_mode(const double *x, const double *y, const double *dx, const double *dy, int size,
+                          double blending, double *o) {
+    double a00 = 0, a01 = 0, a02 = 0, a11 = 0, a12 = 0, a22 = 0, b0 = 0, b1 = 0, b2 = 0;
+    for (int i = 0; i < size; ++i) {
+        double xi2 = x[i] * x[i];
+        double yi2 = y[i] * y[i];
+        a00 += xi2 * xi2;
+        a01 += 2 * xi2 * x[i] * y[i];
+        a02 += xi2 * yi2;
+        a11 += 4 * xi2 * yi2;
+        a12 += 2 * x[i] * y[i] * yi2;
+        a22 += yi2 * yi2;
+        b0 += xi2;
+        b1 += 2 * x[i] * y[i];
+        b2 += yi2;
+    }
+    double den = (-a00 * a11 * a22 + a00 * (a12 * a12) + (a01 * a01) * a22 - 2 * a01 * a02 * a12 + (a02 * a02) * a11);
+    o[1] = blending * (-b0 * (a11 * a22 - (a12 * a12)) + b1 * (a01 * a22 - a02 * a12) - b2 * (a01 * a12 - a02 * a11)) / den;
+    o[3] = -blending * (b0 * (a01 * a22 - a02 * a12) - b1 * (a00 * a22 - (a02 * a02)) + b2 * (a00 * a12 - a01 * a02)) / den;
+    o[5] = blending * (-b0 * (a01 * a12 - a02 * a11) + b1 * (a00 * a12 - a01 * a02) - b2 * (a00 * a11 - (a01 * a01))) / den;
+    a00 = a01 = a02 = a11 = a12 = a22 = b0 = b1 = b2 = 0;
+    for (int i = 0; i < size; ++i) {
+        double xi2 = x[i] * x[i];
+        double yi2 = y[i] * y[i];
+        a00 += xi2;
+        a01 += x[i] * y[i];
+        a11 += xi2 + yi2;
+        a22 += yi2;
+        b0 += dx[i] * x[i];
+        b1 += dx[i] * y[i] + dy[i] * x[i];
+        b2 += dy[i] * y[i];
+    }
+    a12 = a01 * a01;
+    den = (a00 * a12 - a00 * a11 * a22 + a12 * a22);
+    o[0] = (-a12 * b2 + a01 * a22 * b1 + b0 * (a12 - a11 * a22)) / den;
+    o[2] = -(a00 * a01 * b2 - a00 * a22 * b1 + a01 * a22 * b0) / den;
+    o[4] = (a00 * a01 * b1 - a12 * b0 - b2 * (a00 * a11 - a12)) / den;
+}
+
+__global__ void k_modes_gc(rpx_rays_t g, uint64_t n, double blending, double *modes) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t c = g.capacity;
+    const double *f = g.f64;
+    V3 o = v3(f[RPX_OX * c + i], f[RPX_OY * c + i], f[RPX_OZ * c + i]);
+    V3 d = v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]);
+    V3 E = v3(f[RPX_EX * c + i], f[RPX_EY * c + i], f[RPX_EZ * c + i]);
+    V3 H = cross(E, d);  // numpy.cross(E, direction)
+    double x[6], y[6], dx[6], dy[6];
+    for (int j = 0; j < RPX_N_PARA; ++j) {
+        const uint64_t b = (uint64_t)j * RPX_PARA_FIELDS;
+        V3 po = v3(g.para[(b + 0) * c + i], g.para[(b + 1) * c + i], g.para[(b + 2) * c + i]);
+        V3 nd = v3(g.para[(b + 3) * c + i], g.para[(b + 4) * c + i], g.para[(b + 5) * c + i]);
+        V3 off = sub(po, o);
+        double dz = (nd.x * d.x + nd.y * d.y) + nd.z * d.z;
+        x[j] = (off.x * E.x + off.y * E.y) + off.z * E.z;
+        y[j] = (off.x * H.x + off.y * H.y) + off.z * H.z;
+        dx[j] = ((nd.x * E.x + nd.y * E.y) + nd.z * E.z) / dz;
+        dy[j] = ((nd.x * H.x + nd.y * H.y) + nd.z * H.z) / dz;
+    }
+    fit_one_mode(x, y, dx, dy, RPX_N_PARA, blending, modes + 6 * i);
+}
+
+#define RPX_MAX_ROW 32
+
+__global__ void k_modes_xy(const double *X, const double *Y, const double *DX, const double *DY, uint64_t n,
+                           int row, double blending, double *modes) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fit_one_mode(X + i * row, Y + i * row, DX + i * row, DY + i * row, row, blending, modes + 6 * i);
+}
+
+extern "C" rpx_status_t rpx_gausslet_modes(const rpx_rays_t *g, uint64_t n, double blending, double *modes,
+                                           void *stream) {
+    if (n == 0) return RPX_OK;
+    if (!g || !g->f64 || !g->para || !modes || g->capacity < n)
+        return rpx_set_error(RPX_E_INVALID, "gausslet buffers");
+    hipLaunchKernelGGL(k_modes_gc, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *g, n,
+                       blending, modes);
+    RPX_HIP_TRY(hipGetLastError());
+    return RPX_OK;
+}
+
+extern "C" rpx_status_t rpx_evaluate_modes(const double *x, const double *y, const double *dx, const double *dy,
+                                           uint64_t n, int32_t row, double blending, double *modes, void *stream) {
+    if (n == 0) return RPX_OK;
+    if (!x || !y || !dx || !dy || !modes || row < 1 || row > RPX_MAX_ROW)
+        return rpx_set_error(RPX_E_INVALID, "mode-fit buffers or row size");
+    hipLaunchKernelGGL(k_modes_xy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, x, y, dx,
+                       dy, n, row, blending, modes);
+    RPX_HIP_TRY(hipGetLastError());
+    return RPX_OK;
+}
+
+// ------------------------------------------------------------ field sum FP64
+// Per-gausslet invariants, SoA with stride G: origin(3) E(3) H(3) dir(3)
+// kz(2) phase A(2) B(2) C(2) detG0(2) inv_root_area E1(2) E2(2) = 30.
+#define RPX_GP 30
+
+__global__ void k_field_prep(rpx_rays_t r, uint64_t n, const double *modes, WlTable wl, double ct, double *gp) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Ray ray = load_ray(r, i);
+    V3 E = norm(ray.E_vector);
+    V3 H = norm(cross(ray.direction, E));
+    double k = 2000.0 * M_PI / wl[ray.wl_idx];
+    double phase = ray.phase + (ray.accum * k) - (ct * k / ray.n.re);
+    Cx kz = cmul(ray.n, cx(k, 0.0));
+    double invk = 2. / kz.re;
+    Cx A = cx(modes[6 * i], modes[6 * i + 1]);
+    Cx B = cx(modes[6 * i + 2], modes[6 * i + 3]);
+    Cx C = cx(modes[6 * i + 4], modes[6 * i + 5]);
+    double ira = sqrt(sqrt(A.im * C.im - (B.im * B.im)) * (2.0 / M_PI));
+    A.im *= invk;
+    B.im *= invk;
+    C.im *= invk;
+    Cx detG0 = csub(cmul(A, C), cmul(B, B));
+    double v[RPX_GP] = {ray.origin.x, ray.origin.y, ray.origin.z, E.x, E.y, E.z, H.x, H.y, H.z,
+                        ray.direction.x, ray.direction.y, ray.direction.z, kz.re, kz.im, phase,
+                        A.re, A.im, B.re, B.im, C.re, C.im, detG0.re, detG0.im, ira,
+                        ray.E1.re, ray.E1.im, ray.E2.re, ray.E2.im, 0.0, 0.0};
+    for (int k2 = 0; k2 < RPX_GP; ++k2) gp[(uint64_t)k2 * n + i] = v[k2];
+}
+
+#define FS_BLOCK 256
+#define FS_TILE 64
+
+__global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64_t G, const double *points,
+                                                        uint64_t M, double *out) {
+    __shared__ double tile[RPX_GP][FS_TILE];
+    const uint64_t ip = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = ip < M;
+    V3 pt0 = active ? v3(points[3 * ip], points[3 * ip + 1], points[3 * ip + 2]) : v3(0, 0, 0);
+    double o0r = 0, o0i = 0, o1r = 0, o1i = 0, o2r = 0, o2i = 0;
+    const Cx rootI = cx(sqrt(0.5), sqrt(0.5));
+    const Cx I = cx(0.0, 1.0);
+    for (uint64_t base = 0; base < G; base += FS_TILE) {
+        const int nt = (int)((G - base) < FS_TILE ? (G - base) : FS_TILE);
+        __syncthreads();
+        for (int e = threadIdx.x; e < RPX_GP * FS_TILE; e += FS_BLOCK) {
+            const int q = e / FS_TILE, g = e - q * FS_TILE;
+            tile[q][g] = (g < nt) ? gp[(uint64_t)q * G + base + g] : 0.0;
+        }
+        __syncthreads();
+        if (!active) continue;
+        for (int g = 0; g < nt; ++g) {
+            V3 O = v3(tile[0][g], tile[1][g], tile[2][g]);
+            V3 E = v3(tile[3][g], tile[4][g], tile[5][g]);
+            V3 H = v3(tile[6][g], tile[7][g], tile[8][g]);
+            V3 D = v3(tile[9][g], tile[10][g], tile[11][g]);
+            Cx kz = cx(tile[12][g], tile[13][g]);
+            double phase = tile[14][g];
+            Cx A = cx(tile[15][g], tile[16][g]), B = cx(tile[17][g], tile[18][g]);
+            Cx C = cx(tile[19][g], tile[20][g]), detG0 = cx(tile[21][g], tile[22][g]);
+            double ira = tile[23][g];
+            Cx E1a = cx(tile[24][g], tile[25][g]), E2a = cx(tile[26][g], tile[27][g]);
+            // calc_mode_U (cfields.pyx:118-151)
+            V3 pt = sub(pt0, O);
+            double x = dot(pt, E), y = dot(pt, H), z = dot(pt, D);
+            Cx denom = cmul(cadd(cadd(cx(1, 0), cmul(cx(z, 0), cadd(A, C))), cmul(cx(z * z, 0), detG0)), cx(2, 0));
+            Cx zd = cmul(cx(z, 0), detG0);
+            Cx AA = cdiv(cadd(A, zd), denom);
+            Cx CC = cdiv(cadd(C, zd), denom);
+            Cx inner = cadd(cx(z, 0), cmul(AA, cx(x * x, 0)));
+            inner = cadd(inner, cdiv(cmul(B, cx(2 * x * y, 0)), denom));
+            inner = cadd(inner, cmul(CC, cx(y * y, 0)));
+            Cx arg = cadd(cx(phase, 0), cmul(kz, inner));
+            Cx U = cexp_(cmul(I, arg));
+            Cx q1 = cadd(cx(1, 0), cmul(cx(z, 0), A));
+            Cx q2 = cadd(cx(1, 0), cmul(cx(z, 0), C));
+            Cx zb = cmul(cx(z, 0), B);
+            Cx s = csqrt_(cmul(csub(cmul(q1, q2), cmul(zb, zb)), I));
+            U = cdiv(U, s);
+            U = cmul(U, cmul(cx(ira, 0), rootI));
+            Cx E1 = cmul(E1a, U), E2 = cmul(E2a, U);
+            Cx t0 = cadd(cmul(E1, cx(E.x, 0)), cmul(E2, cx(H.x, 0)));
+            Cx t1 = cadd(cmul(E1, cx(E.y, 0)), cmul(E2, cx(H.y, 0)));
+            Cx t2 = cadd(cmul(E1, cx(E.z, 0)), cmul(E2, cx(H.z, 0)));
+            o0r += t0.re; o0i += t0.im;
+            o1r += t1.re; o1i += t1.im;
+            o2r += t2.re; o2i += t2.im;
+        }
+    }
+    if (active) {
+        out[6 * ip + 0] = o0r; out[6 * ip + 1] = o0i;
+        out[6 * ip + 2] = o1r; out[6 * ip + 3] = o1i;
+        out[6 * ip + 4] = o2r; out[6 * ip + 5] = o2i;
+    }
+}
+
+extern "C" rpx_status_t rpx_sum_gaussian_modes(const rpx_rays_t *rays, uint64_t n_rays, const double *modes,
+                                               const double *wavelengths, uint64_t n_wavelengths,
+                                               const double *points, uint64_t n_pts, double time_ps, double *out,
+                                               int32_t precision, void *stream_) {
+    hipStream_t stream = (hipStream_t)stream_;
+    if (precision != 0) return rpx_set_error(RPX_E_UNSUPPORTED, "only the FP64 field sum is built");
+    if (n_pts == 0) return RPX_OK;
+    if (!points || !out) return rpx_set_error(RPX_E_INVALID, "null point/output buffer");
+    if (n_rays == 0) {
+        RPX_HIP_TRY(hipMemsetAsync(out, 0, sizeof(double) * 6 * n_pts, stream));
+        return RPX_OK;
+    }
+    if (!rays || !rays->f64 || !rays->u32 || rays->capacity < n_rays || !modes)
+        return rpx_set_error(RPX_E_INVALID, "ray/mode buffers");
+    WlTable wl;
+    wl.wl = wavelengths;
+    wl.n = wavelengths ? n_wavelengths : 0;
+    double *gp = nullptr;
+    RPX_HIP_TRY(hipMallocAsync((void **)&gp, sizeof(double) * RPX_GP * n_rays, stream));
+    const double ct = 0.299792458 * time_ps;
+    hipLaunchKernelGGL(k_field_prep, dim3((unsigned)((n_rays + 255) / 256)), dim3(256), 0, stream, *rays, n_rays,
+                       modes, wl, ct, gp);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_field_sum, dim3((unsigned)((n_pts + FS_BLOCK - 1) / FS_BLOCK)), dim3(FS_BLOCK), 0,
+                           stream, gp, n_rays, points, n_pts, out);
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipFreeAsync(gp, stream);
+    if (e != hipSuccess) return rpx_set_error(RPX_E_HIP, std::string("field sum launch: ") + hipGetErrorString(e));
+    if (e2 != hipSuccess) return rpx_set_error(RPX_E_HIP, std::string("field sum free: ") + hipGetErrorString(e2));
+    return RPX_OK;
+}

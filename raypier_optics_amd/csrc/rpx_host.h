@@ -1,0 +1,21 @@
+// Host-side internals shared by the library's translation units.
+#pragma once
+#include <string>
+#include "rpx_internal.h"
+
+struct rpx_scene {
+    rpx::SceneView view;
+    void *device_block;
+    int32_t n_global_faces;
+    int32_t list_sizes[16];
+};
+
+// Records the message for rpx_last_error() and returns `code`.
+rpx_status_t rpx_set_error(rpx_status_t code, const std::string &msg);
+
+#define RPX_HIP_TRY(expr)                                                                     \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            return rpx_set_error(RPX_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)

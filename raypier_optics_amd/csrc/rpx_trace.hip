@@ -1,0 +1,716 @@
+// Generation tracer for gfx950: one generation of rays (or gausslets) against
+// a device-resident scene, with stable parent-major compaction of children.
+//
+// Pipeline per generation (three launches, no inter-block waiting):
+//   k_intersect  one ray per lane: nearest-face search over every face list
+//                (faces staged in LDS), in-place parent update (length,
+//                end_face_idx), Face.count histogram, child count via the
+//                material; per-block child totals by wavefront ballots.
+//   k_scan       one workgroup: exclusive scan of the block totals.
+//   k_emit       per-block exclusive scan of child counts (ballot bit-planes +
+//                LDS), then each lane re-evaluates its material and writes its
+//                children at their parent-major position.
+#include <hip/hip_runtime.h>
+#include <string.h>
+#include <stdio.h>
+#include <string>
+#include <vector>
+#include <mutex>
+#include "rpx_materials.h"
+#include "rpx_host.h"
+
+using namespace rpx;
+
+// ---------------------------------------------------------------- errors
+static thread_local std::string g_last_error;
+
+rpx_status_t rpx_set_error(rpx_status_t code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+static rpx_status_t set_error(rpx_status_t code, const std::string &msg) { return rpx_set_error(code, msg); }
+#define HIP_TRY(expr) RPX_HIP_TRY(expr)
+
+extern "C" const char *rpx_last_error(void) { return g_last_error.c_str(); }
+extern "C" int32_t rpx_abi_version(void) { return RPX_ABI_VERSION; }
+
+extern "C" int32_t rpx_struct_size(const char *name) {
+    if (!name) return -1;
+    std::string n(name);
+    if (n == "rpx_transform_t") return sizeof(rpx_transform_t);
+    if (n == "rpx_face_list_t") return sizeof(rpx_face_list_t);
+    if (n == "rpx_face_t") return sizeof(rpx_face_t);
+    if (n == "rpx_shape_op_t") return sizeof(rpx_shape_op_t);
+    if (n == "rpx_dispersion_t") return sizeof(rpx_dispersion_t);
+    if (n == "rpx_material_t") return sizeof(rpx_material_t);
+    if (n == "rpx_zernike_insn_t") return sizeof(rpx_zernike_insn_t);
+    if (n == "rpx_zernike_coef_t") return sizeof(rpx_zernike_coef_t);
+    if (n == "rpx_zernike_t") return sizeof(rpx_zernike_t);
+    if (n == "rpx_rays_t") return sizeof(rpx_rays_t);
+    if (n == "rpx_scene_desc_t") return sizeof(rpx_scene_desc_t);
+    return -1;
+}
+
+// ----------------------------------------------------------------- scene
+
+static int material_max_children(int type) {
+    switch (type) {
+    case RPX_MAT_TRANSPARENT:
+    case RPX_MAT_PEC:
+    case RPX_MAT_DIELECTRIC:
+    case RPX_MAT_GRATING:
+        return 1;
+    case RPX_MAT_PARTIAL:
+    case RPX_MAT_FULL_DIELECTRIC:
+    case RPX_MAT_FULL_DISPERSIVE:
+    case RPX_MAT_SINGLE_LAYER:
+    case RPX_MAT_COATED_DISPERSIVE:
+        return 2;
+    default:
+        return 0;
+    }
+}
+
+#define RPX_LDS_FACES 64
+#define RPX_LDS_LISTS 16
+#define RPX_BLOCK 256
+
+static rpx_status_t validate_scene(const rpx_scene_desc_t *d) {
+    if (!d) return set_error(RPX_E_INVALID, "scene desc is null");
+    if (d->n_face_lists < 0 || d->n_faces < 0 || d->n_materials < 0)
+        return set_error(RPX_E_INVALID, "negative table size");
+    if (d->n_faces > RPX_LDS_FACES)
+        return set_error(RPX_E_UNSUPPORTED, "more than 64 faces in one scene");
+    if (d->n_face_lists > RPX_LDS_LISTS)
+        return set_error(RPX_E_UNSUPPORTED, "more than 16 face lists in one scene");
+    int expect = 0;
+    for (int l = 0; l < d->n_face_lists; ++l) {
+        const rpx_face_list_t &fl = d->face_lists[l];
+        if (fl.first_face != expect || fl.n_faces < 0)
+            return set_error(RPX_E_INVALID, "face lists must tile the face table in order");
+        expect += fl.n_faces;
+    }
+    if (expect > d->n_faces) return set_error(RPX_E_INVALID, "face lists exceed face table");
+    for (int i = 0; i < d->n_faces; ++i) {
+        const rpx_face_t &f = d->faces[i];
+        if (f.type < RPX_FACE_CIRCULAR || f.type > RPX_FACE_DISTORTION)
+            return set_error(RPX_E_UNSUPPORTED, "unknown face type " + std::to_string(f.type));
+        if (f.material >= d->n_materials) return set_error(RPX_E_INVALID, "face material index");
+        if (f.shape >= d->n_shape_ops) return set_error(RPX_E_INVALID, "face shape index");
+        if (f.global_idx < 0) return set_error(RPX_E_INVALID, "negative global face index");
+        if (f.type == RPX_FACE_POLYGON &&
+            (f.aux1 < 1 || f.aux0 < 0 || f.aux0 + f.aux1 > d->n_vertices))
+            return set_error(RPX_E_INVALID, "polygon vertex range");
+        if (f.type == RPX_FACE_DISTORTION) {
+            if (f.aux0 < 0 || f.aux0 >= d->n_faces || d->faces[f.aux0].type == RPX_FACE_DISTORTION)
+                return set_error(RPX_E_INVALID, "distortion base face");
+            if (f.aux1 < 0 || f.aux1 >= d->n_zernikes) return set_error(RPX_E_INVALID, "distortion index");
+        }
+    }
+    for (int i = 0; i < d->n_shape_ops; ++i) {
+        const rpx_shape_op_t &o = d->shape_ops[i];
+        if (o.op < RPX_SHAPE_TRUE || o.op > RPX_SHAPE_XOR) return set_error(RPX_E_INVALID, "shape opcode");
+        if (o.op == RPX_SHAPE_POLYGON && (o.b < 1 || o.a < 0 || o.a + o.b > d->n_vertices))
+            return set_error(RPX_E_INVALID, "shape polygon range");
+    }
+    for (int i = 0; i < d->n_faces; ++i) {  // postfix programs must be well formed
+        int root = d->faces[i].shape;
+        if (root < 0) continue;
+        int begin = d->shape_ops[root].begin, depth = 0;
+        if (begin < 0 || begin > root) return set_error(RPX_E_INVALID, "shape program range");
+        for (int k = begin; k <= root; ++k) {
+            int op = d->shape_ops[k].op;
+            if (op <= RPX_SHAPE_POLYGON) ++depth;
+            else if (op == RPX_SHAPE_NOT) { if (depth < 1) return set_error(RPX_E_INVALID, "shape stack"); }
+            else { if (depth < 2) return set_error(RPX_E_INVALID, "shape stack"); --depth; }
+            if (depth > 31) return set_error(RPX_E_UNSUPPORTED, "shape program deeper than 31");
+        }
+        if (depth != 1) return set_error(RPX_E_INVALID, "shape program does not reduce to one value");
+    }
+    for (int i = 0; i < d->n_materials; ++i) {
+        const rpx_material_t &m = d->materials[i];
+        if (m.type < RPX_MAT_NONE || m.type > RPX_MAT_GRATING)
+            return set_error(RPX_E_UNSUPPORTED, "unknown material type " + std::to_string(m.type));
+        const rpx_dispersion_t *ds[3] = {&m.disp_inside, &m.disp_outside, &m.disp_coating};
+        for (auto *dd : ds)
+            if (dd->n_coefs < 0 || dd->n_coefs > RPX_MAX_DISP_COEFS)
+                return set_error(RPX_E_INVALID, "dispersion coefficient count");
+    }
+    for (int i = 0; i < d->n_zernikes; ++i) {
+        const rpx_zernike_t &z = d->zernikes[i];
+        if (z.n_slots < 0 || z.n_slots > RPX_MAX_ZERNIKE_SLOTS)
+            return set_error(RPX_E_UNSUPPORTED, "Zernike workspace too large");
+        if (z.coef_begin < 0 || z.coef_begin + z.n_coefs > d->n_zernike_coefs)
+            return set_error(RPX_E_INVALID, "Zernike coefficient range");
+        if (z.sag_begin < 0 || z.sag_begin + z.sag_len > d->n_zernike_insns ||
+            z.grad_begin < 0 || z.grad_begin + z.grad_len > d->n_zernike_insns)
+            return set_error(RPX_E_INVALID, "Zernike program range");
+        for (int k = 0; k < z.n_coefs; ++k) {
+            const rpx_zernike_coef_t &c = d->zernike_coefs[z.coef_begin + k];
+            int refs[4] = {c.r_ref, c.rp_ref, c.ror_ref, c.sag_ref};
+            for (int r : refs)
+                if (r < -2 || r >= z.n_slots) return set_error(RPX_E_INVALID, "Zernike value ref");
+        }
+        int ranges[2][2] = {{z.sag_begin, z.sag_len}, {z.grad_begin, z.grad_len}};
+        for (auto &rg : ranges)
+            for (int k = rg[0]; k < rg[0] + rg[1]; ++k) {
+                const rpx_zernike_insn_t &in = d->zernike_insns[k];
+                if (in.dst < 0 || in.dst >= z.n_slots || in.op < 0 || in.op > 2)
+                    return set_error(RPX_E_INVALID, "Zernike instruction");
+                for (int s = 0; s < 5; ++s)
+                    if (in.src[s] < -2 || in.src[s] >= z.n_slots)
+                        return set_error(RPX_E_INVALID, "Zernike instruction operand");
+            }
+    }
+    return RPX_OK;
+}
+
+template <class T>
+static size_t push_table(std::vector<char> &blob, const T *src, int n) {
+    size_t off = (blob.size() + 255) & ~size_t(255);
+    blob.resize(off + sizeof(T) * (size_t)(n > 0 ? n : 0));
+    if (n > 0) memcpy(blob.data() + off, src, sizeof(T) * (size_t)n);
+    return off;
+}
+
+extern "C" rpx_status_t rpx_scene_create(const rpx_scene_desc_t *d, rpx_scene_t **out) {
+    if (!out) return set_error(RPX_E_INVALID, "out is null");
+    *out = nullptr;
+    rpx_status_t st = validate_scene(d);
+    if (st != RPX_OK) return st;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_error(RPX_E_NO_DEVICE, "no HIP device");
+    std::vector<char> blob;
+    size_t o_fl = push_table(blob, d->face_lists, d->n_face_lists);
+    size_t o_f = push_table(blob, d->faces, d->n_faces);
+    size_t o_m = push_table(blob, d->materials, d->n_materials);
+    size_t o_s = push_table(blob, d->shape_ops, d->n_shape_ops);
+    size_t o_v = push_table(blob, d->vertices, 2 * d->n_vertices);
+    size_t o_z = push_table(blob, d->zernikes, d->n_zernikes);
+    size_t o_zc = push_table(blob, d->zernike_coefs, d->n_zernike_coefs);
+    size_t o_zi = push_table(blob, d->zernike_insns, d->n_zernike_insns);
+    blob.resize(((blob.size() + 255) & ~size_t(255)) + 256);
+    char *dev = nullptr;
+    HIP_TRY(hipMalloc((void **)&dev, blob.size()));
+    hipError_t e = hipMemcpy(dev, blob.data(), blob.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(dev);
+        return set_error(RPX_E_HIP, std::string("scene upload: ") + hipGetErrorString(e));
+    }
+    rpx_scene *s = new rpx_scene();
+    s->device_block = dev;
+    SceneView &v = s->view;
+    v.face_lists = (const rpx_face_list_t *)(dev + o_fl);
+    v.faces = (const rpx_face_t *)(dev + o_f);
+    v.materials = (const rpx_material_t *)(dev + o_m);
+    v.shape_ops = (const rpx_shape_op_t *)(dev + o_s);
+    v.vertices = (const double *)(dev + o_v);
+    v.zernikes = (const rpx_zernike_t *)(dev + o_z);
+    v.zcoefs = (const rpx_zernike_coef_t *)(dev + o_zc);
+    v.zinsns = (const rpx_zernike_insn_t *)(dev + o_zi);
+    v.n_face_lists = d->n_face_lists;
+    v.n_faces = d->n_faces;
+    v.n_materials = d->n_materials;
+    v.n_shape_ops = d->n_shape_ops;
+    v.n_vertices = d->n_vertices;
+    v.n_zernikes = d->n_zernikes;
+    v.n_zcoefs = d->n_zernike_coefs;
+    v.n_zinsns = d->n_zernike_insns;
+    int nlist = 0, maxc = 0, nglob = 0;
+    for (int l = 0; l < d->n_face_lists; ++l) nlist += d->face_lists[l].n_faces;
+    v.n_list_faces = nlist;
+    for (int i = 0; i < nlist; ++i) {
+        const rpx_face_t &f = d->faces[i];
+        if (f.global_idx + 1 > nglob) nglob = f.global_idx + 1;
+        int mc = f.material >= 0 ? material_max_children(d->materials[f.material].type) : 0;
+        if (mc > maxc) maxc = mc;
+    }
+    v.max_children = maxc;
+    s->n_global_faces = nglob;
+    for (int l = 0; l < d->n_face_lists; ++l) s->list_sizes[l] = d->face_lists[l].n_faces;
+    *out = s;
+    return RPX_OK;
+}
+
+extern "C" rpx_status_t rpx_scene_destroy(rpx_scene_t *s) {
+    if (!s) return RPX_OK;
+    hipError_t e = hipFree(s->device_block);
+    delete s;
+    if (e != hipSuccess) return set_error(RPX_E_HIP, hipGetErrorString(e));
+    return RPX_OK;
+}
+
+extern "C" int32_t rpx_scene_max_children(const rpx_scene_t *s) { return s ? s->view.max_children : -1; }
+
+// ------------------------------------------------------------ workspace
+struct Workspace {
+    uint32_t *tags;          // per ray: child count (bits 0..7) | (face record + 1) << 8
+    uint32_t *block_totals;  // per block
+    uint64_t *block_offsets; // per block, exclusive
+    uint64_t *total;         // 1
+};
+
+static uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+
+extern "C" uint64_t rpx_trace_workspace_bytes(uint64_t n_in) {
+    uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
+    return align256(4 * n_in) + align256(4 * nb) + align256(8 * nb) + 256;
+}
+
+static Workspace carve(void *ws, uint64_t n_in) {
+    uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
+    char *p = (char *)ws;
+    Workspace w;
+    w.tags = (uint32_t *)p;
+    p += align256(4 * n_in);
+    w.block_totals = (uint32_t *)p;
+    p += align256(4 * nb);
+    w.block_offsets = (uint64_t *)p;
+    p += align256(8 * nb);
+    w.total = (uint64_t *)p;
+    return w;
+}
+
+// --------------------------------------------------------------- kernels
+struct TraceArgs {
+    SceneView S;
+    rpx_rays_t in;
+    rpx_rays_t out;
+    uint64_t n_in;
+    double max_length;      // already rounded to float for ray generations
+    WlTable wavelengths;
+    uint32_t *face_hits;
+    Workspace ws;
+    int32_t one_list, one_face;  // one-face mode: list and face index inside it (-1 = off)
+};
+
+// Stage face lists and faces in LDS; hand back a view that points there.
+__device__ __forceinline__ SceneView stage_scene(const SceneView &S, rpx_face_list_t *lds_lists,
+                                                  rpx_face_t *lds_faces) {
+    const int t = threadIdx.x;
+    {
+        const int nw = S.n_faces * (int)(sizeof(rpx_face_t) / 8);
+        const double *src = (const double *)S.faces;
+        double *dst = (double *)lds_faces;
+        for (int k = t; k < nw; k += blockDim.x) dst[k] = src[k];
+    }
+    {
+        const int nw = S.n_face_lists * (int)(sizeof(rpx_face_list_t) / 8);
+        const double *src = (const double *)S.face_lists;
+        double *dst = (double *)lds_lists;
+        for (int k = t; k < nw; k += blockDim.x) dst[k] = src[k];
+    }
+    __syncthreads();
+    SceneView v = S;
+    v.faces = lds_faces;
+    v.face_lists = lds_lists;
+    return v;
+}
+
+struct CountEmit {
+    int n = 0;
+    RPX_DEV void operator()(const Ray &) { ++n; }
+};
+
+// Block-wide exclusive scan of small per-lane counts (0..7) with wavefront
+// ballots: three bit-planes, popcount below the lane, wave totals in LDS.
+__device__ __forceinline__ uint32_t block_scan_small(uint32_t c, uint32_t *lds_wave, uint32_t *block_total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        uint64_t m = __ballot((c >> b) & 1u);
+        pre += (uint32_t)__popcll(m & lt) << b;
+        tot += (uint32_t)__popcll(m) << b;
+    }
+    if (lane == 0) lds_wave[wave] = tot;
+    __syncthreads();
+    uint32_t base = 0, all = 0;
+    for (int w = 0; w < nwave; ++w) {
+        uint32_t v = lds_wave[w];
+        if (w < wave) base += v;
+        all += v;
+    }
+    *block_total = all;
+    return base + pre;
+}
+
+template <bool GAUSS>
+__global__ __launch_bounds__(RPX_BLOCK) void k_intersect(TraceArgs A) {
+    __shared__ rpx_face_t lds_faces[RPX_LDS_FACES];
+    __shared__ rpx_face_list_t lds_lists[RPX_LDS_LISTS];
+    __shared__ uint32_t lds_hist[RPX_LDS_FACES];
+    __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
+    const SceneView S = stage_scene(A.S, lds_lists, lds_faces);
+    for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
+    __syncthreads();
+
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t count = 0, tag = 0;
+    if (i < A.n_in) {
+        const double *f = A.in.f64;
+        const uint64_t c = A.in.capacity;
+        V3 origin = v3(f[RPX_OX * c + i], f[RPX_OY * c + i], f[RPX_OZ * c + i]);
+        V3 dir = v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]);
+        double length = GAUSS ? f[RPX_LENGTH * c + i] : A.max_length;
+        uint32_t end_face = RPX_NO_FACE;
+        int hit_rec = -1, hit_list = -1;
+        V3 point = add(origin, mul(dir, A.max_length));
+        if (A.one_list < 0) {
+            // trace_segment_c / FaceList.intersect_c (ctracer.pyx:2093-2104, 1882-1904)
+            for (int l = 0; l < S.n_face_lists; ++l) {
+                const rpx_face_list_t &fl = S.face_lists[l];
+                V3 p1 = transform(fl.inv_trans, origin);
+                V3 p2 = transform(fl.inv_trans, point);
+                for (int k = fl.first_face; k < fl.first_face + fl.n_faces; ++k) {
+                    const rpx_face_t &fc = S.faces[k];
+                    double dist = face_intersect(S, fc, p1, p2, 1);
+                    if (fc.tolerance < dist && dist < length) {
+                        length = dist;
+                        end_face = (uint32_t)fc.global_idx;
+                        hit_rec = k;
+                        hit_list = l;
+                    }
+                }
+            }
+        } else {
+            // FaceList.intersect_one_face_c (ctracer.pyx:1861-1879)
+            const rpx_face_list_t &fl = S.face_lists[A.one_list];
+            const int k = fl.first_face + A.one_face;
+            const rpx_face_t &fc = S.faces[k];
+            V3 p1 = transform(fl.inv_trans, origin);
+            V3 p2 = transform(fl.inv_trans, point);
+            double dist = face_intersect(S, fc, p1, p2, 1);
+            if (fc.tolerance < dist && dist < length) {
+                length = dist;
+                end_face = (uint32_t)fc.global_idx;
+                hit_rec = k;
+                hit_list = A.one_list;
+            }
+        }
+        A.in.f64[RPX_LENGTH * c + i] = length;
+        A.in.u32[RPX_END_FACE_IDX * c + i] = end_face;
+        if (hit_rec >= 0) {
+            atomicAdd(&lds_hist[hit_rec], 1u);
+            const rpx_face_t &fc = S.faces[hit_rec];
+            const rpx_face_list_t &fl = S.face_lists[hit_list];
+            bool para_ok = true;
+            if (GAUSS) {
+                // trace_parabasal_rays (ctracer.pyx:2363-2373): every parabasal
+                // ray must hit the same face; lengths update in place.
+                const double *P = A.in.para;
+                for (int j = 0; j < RPX_N_PARA && para_ok; ++j) {
+                    const uint64_t b = (uint64_t)j * RPX_PARA_FIELDS;
+                    V3 po = v3(P[(b + 0) * c + i], P[(b + 1) * c + i], P[(b + 2) * c + i]);
+                    V3 pd = v3(P[(b + 3) * c + i], P[(b + 4) * c + i], P[(b + 5) * c + i]);
+                    double plen = P[(b + 9) * c + i];
+                    V3 pend = add(po, mul(pd, A.max_length));
+                    V3 q1 = transform(fl.inv_trans, po);
+                    V3 q2 = transform(fl.inv_trans, pend);
+                    double dist = face_intersect(S, fc, q1, q2, 0);
+                    if (fc.tolerance < dist && dist < plen) {
+                        A.in.para[(b + 9) * c + i] = dist;
+                    } else {
+                        para_ok = false;
+                    }
+                }
+            }
+            if (para_ok && fc.material >= 0) {
+                Ray r = load_ray(A.in, i);
+                r.length = length;
+                r.end_face_idx = end_face;
+                V3 hp = add(r.origin, mul(r.direction, length));
+                Orientation o = compute_orientation(S, fl, fc, hp);
+                CountEmit ce;
+                eval_children(S.materials[fc.material], r, (uint32_t)i, hp, o, A.wavelengths, ce);
+                count = (uint32_t)ce.n;
+            }
+            tag = count | ((uint32_t)(hit_rec + 1) << 8);
+        }
+        A.ws.tags[i] = tag;
+    }
+    uint32_t block_total;
+    block_scan_small(count, lds_wave, &block_total);
+    if (threadIdx.x == 0) A.ws.block_totals[blockIdx.x] = block_total;
+    __syncthreads();
+    if (A.face_hits) {
+        for (int k = threadIdx.x; k < S.n_faces; k += blockDim.x) {
+            uint32_t h = lds_hist[k];
+            if (h) atomicAdd(&A.face_hits[S.faces[k].global_idx], h);
+        }
+    }
+}
+
+// Exclusive scan of block totals in a single workgroup of 1024 lanes.
+__global__ __launch_bounds__(1024) void k_scan(const uint32_t *totals, uint64_t nb, uint64_t *offsets,
+                                               uint64_t *grand_total) {
+    __shared__ uint64_t lds[1024];
+    uint64_t carry = 0;
+    for (uint64_t base = 0; base < nb; base += 1024) {
+        uint64_t k = base + threadIdx.x;
+        uint64_t v = (k < nb) ? totals[k] : 0;
+        lds[threadIdx.x] = v;
+        __syncthreads();
+        for (int s = 1; s < 1024; s <<= 1) {  // Hillis-Steele inclusive scan
+            uint64_t add = (threadIdx.x >= (unsigned)s) ? lds[threadIdx.x - s] : 0;
+            __syncthreads();
+            lds[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (k < nb) offsets[k] = carry + lds[threadIdx.x] - v;
+        uint64_t chunk = lds[1023];
+        __syncthreads();
+        carry += chunk;
+    }
+    if (threadIdx.x == 0) *grand_total = carry;
+}
+
+struct StoreEmit {
+    const rpx_rays_t *out;
+    uint64_t pos;
+    RPX_DEV void operator()(const Ray &r) { store_ray(*out, pos++, r); }
+};
+
+// Collects up to 3 children of one gausslet base ray.
+struct CollectEmit {
+    Ray r[3];
+    int n = 0;
+    RPX_DEV void operator()(const Ray &x) {
+        if (n < 3) r[n] = x;
+        ++n;
+    }
+};
+
+template <bool GAUSS>
+__global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
+    __shared__ rpx_face_t lds_faces[RPX_LDS_FACES];
+    __shared__ rpx_face_list_t lds_lists[RPX_LDS_LISTS];
+    __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
+    const SceneView S = stage_scene(A.S, lds_lists, lds_faces);
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t tag = (i < A.n_in) ? A.ws.tags[i] : 0u;
+    const uint32_t count = tag & 0xffu;
+    uint32_t block_total;
+    const uint32_t local = block_scan_small(count, lds_wave, &block_total);
+    if (count == 0) return;
+    const uint64_t pos = A.ws.block_offsets[blockIdx.x] + local;
+    const int rec = (int)(tag >> 8) - 1;
+    const rpx_face_t &fc = S.faces[rec];
+    const rpx_face_list_t &fl = S.face_lists[fc.face_list];
+    const rpx_material_t &m = S.materials[fc.material];
+    Ray r = load_ray(A.in, i);
+    V3 hp = add(r.origin, mul(r.direction, r.length));
+    Orientation o = compute_orientation(S, fl, fc, hp);
+    if (!GAUSS) {
+        StoreEmit se{&A.out, pos};
+        eval_children(m, r, (uint32_t)i, hp, o, A.wavelengths, se);
+        return;
+    }
+    // Gausslets: children of the base ray, then 6 parabasal rays each
+    // (ctracer.pyx:2375-2385), lengths reset to max_length (:2280).
+    CollectEmit ce;
+    eval_children(m, r, (uint32_t)i, hp, o, A.wavelengths, ce);
+    const uint64_t c = A.in.capacity, co = A.out.capacity;
+    for (int k = 0; k < ce.n; ++k) {
+        Ray child = ce.r[k];
+        child.length = A.max_length;
+        store_ray(A.out, pos + k, child);
+    }
+    for (int j = 0; j < RPX_N_PARA; ++j) {
+        const uint64_t b = (uint64_t)j * RPX_PARA_FIELDS;
+        const double *P = A.in.para;
+        V3 po = v3(P[(b + 0) * c + i], P[(b + 1) * c + i], P[(b + 2) * c + i]);
+        V3 pd = v3(P[(b + 3) * c + i], P[(b + 4) * c + i], P[(b + 5) * c + i]);
+        double plen = P[(b + 9) * c + i];
+        V3 pp = add(po, mul(pd, plen));
+        Orientation po_ = compute_orientation(S, fl, fc, pp);
+        for (int k = 0; k < ce.n; ++k) {
+            Para q = eval_parabasal(m, ce.r[k], pd, pp, po_, ce.r[k].ray_type_id, A.wavelengths);
+            double *Q = A.out.para;
+            const uint64_t oi = pos + k;
+            Q[(b + 0) * co + oi] = q.origin.x;
+            Q[(b + 1) * co + oi] = q.origin.y;
+            Q[(b + 2) * co + oi] = q.origin.z;
+            Q[(b + 3) * co + oi] = q.direction.x;
+            Q[(b + 4) * co + oi] = q.direction.y;
+            Q[(b + 5) * co + oi] = q.direction.z;
+            Q[(b + 6) * co + oi] = q.normal.x;
+            Q[(b + 7) * co + oi] = q.normal.y;
+            Q[(b + 8) * co + oi] = q.normal.z;
+            Q[(b + 9) * co + oi] = A.max_length;
+        }
+    }
+}
+
+// ------------------------------------------------------------- host side
+static rpx_status_t check_rays(const rpx_rays_t *r, uint64_t n, bool gauss, const char *what) {
+    if (!r) return set_error(RPX_E_INVALID, std::string(what) + " is null");
+    if (r->capacity < n) return set_error(RPX_E_INVALID, std::string(what) + " capacity below count");
+    if (n > 0 && (!r->f64 || !r->u32)) return set_error(RPX_E_INVALID, std::string(what) + " has null fields");
+    if (gauss && n > 0 && !r->para) return set_error(RPX_E_INVALID, std::string(what) + " lacks parabasal rays");
+    return RPX_OK;
+}
+
+static rpx_status_t run_generation(const rpx_scene_t *scene, int one_list, int one_face, bool gauss,
+                                   const rpx_rays_t *in, uint64_t n_in, double max_length,
+                                   const double *wavelengths, uint64_t n_wavelengths, rpx_rays_t *out,
+                                   uint64_t *n_out, uint32_t *face_hits, void *workspace,
+                                   uint64_t workspace_bytes, void *stream_) {
+    if (!scene) return set_error(RPX_E_INVALID, "scene is null");
+    if (!n_out) return set_error(RPX_E_INVALID, "n_out is null");
+    *n_out = 0;
+    rpx_status_t st = check_rays(in, n_in, gauss, "input rays");
+    if (st) return st;
+    if (!out) return set_error(RPX_E_INVALID, "output rays are null");
+    if (n_in == 0) return RPX_OK;
+    if (n_in > 0xFFFFFFFFull) return set_error(RPX_E_UNSUPPORTED, "more than 2^32-1 rays in one generation");
+    if (!wavelengths && n_wavelengths) return set_error(RPX_E_INVALID, "wavelength table is null");
+    if (workspace_bytes < rpx_trace_workspace_bytes(n_in) || !workspace)
+        return set_error(RPX_E_INVALID, "workspace too small");
+    if (one_list >= 0) {
+        if (one_list >= scene->view.n_face_lists || one_face < 0 || one_face >= scene->list_sizes[one_list])
+            return set_error(RPX_E_INVALID, "face list / face index out of range");
+    }
+    hipStream_t stream = (hipStream_t)stream_;
+    TraceArgs A;
+    A.S = scene->view;
+    A.in = *in;
+    A.out = *out;
+    A.n_in = n_in;
+    A.max_length = max_length;
+    A.wavelengths.wl = wavelengths;
+    A.wavelengths.n = wavelengths ? n_wavelengths : 0;
+    A.face_hits = face_hits;
+    A.ws = carve(workspace, n_in);
+    A.one_list = one_list;
+    A.one_face = one_face;
+    const uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
+    if (gauss)
+        hipLaunchKernelGGL(k_intersect<true>, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
+    else
+        hipLaunchKernelGGL(k_intersect<false>, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, A.ws.block_totals, nb, A.ws.block_offsets,
+                       A.ws.total);
+    HIP_TRY(hipGetLastError());
+    uint64_t total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, A.ws.total, sizeof(total), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    *n_out = total;
+    if (total == 0) return RPX_OK;
+    if (total > out->capacity)
+        return set_error(RPX_E_CAPACITY, "output capacity " + std::to_string(out->capacity) + " below " +
+                                             std::to_string(total) + " children");
+    st = check_rays(out, total, gauss, "output rays");
+    if (st) return st;
+    if (gauss)
+        hipLaunchKernelGGL(k_emit<true>, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
+    else
+        hipLaunchKernelGGL(k_emit<false>, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream));
+    return RPX_OK;
+}
+
+extern "C" rpx_status_t rpx_trace_segment(const rpx_scene_t *scene, const rpx_rays_t *in, uint64_t n_in,
+                                          float max_length, const double *wavelengths,
+                                          uint64_t n_wavelengths, rpx_rays_t *out, uint64_t *n_out,
+                                          uint32_t *face_hits, void *workspace, uint64_t workspace_bytes,
+                                          void *stream) {
+    return run_generation(scene, -1, -1, false, in, n_in, (double)max_length, wavelengths, n_wavelengths, out,
+                          n_out, face_hits, workspace, workspace_bytes, stream);
+}
+
+extern "C" rpx_status_t rpx_trace_one_face_segment(const rpx_scene_t *scene, int32_t face_list,
+                                                   int32_t face_idx, const rpx_rays_t *in, uint64_t n_in,
+                                                   float max_length, const double *wavelengths,
+                                                   uint64_t n_wavelengths, rpx_rays_t *out, uint64_t *n_out,
+                                                   uint32_t *face_hits, void *workspace,
+                                                   uint64_t workspace_bytes, void *stream) {
+    if (!scene || face_list < 0 || face_list >= scene->view.n_face_lists || face_idx < 0)
+        return set_error(RPX_E_INVALID, "face list / face index out of range");
+    return run_generation(scene, face_list, face_idx, false, in, n_in, (double)max_length, wavelengths,
+                          n_wavelengths, out, n_out, face_hits, workspace, workspace_bytes, stream);
+}
+
+extern "C" rpx_status_t rpx_trace_gausslet(const rpx_scene_t *scene, int32_t face_list, int32_t face_idx,
+                                           const rpx_rays_t *in, uint64_t n_in, double max_length,
+                                           const double *wavelengths, uint64_t n_wavelengths, rpx_rays_t *out,
+                                           uint64_t *n_out, uint32_t *face_hits, void *workspace,
+                                           uint64_t workspace_bytes, void *stream) {
+    if (!scene) return set_error(RPX_E_INVALID, "scene is null");
+    if (face_list >= scene->view.n_face_lists || (face_list >= 0 && face_idx < 0))
+        return set_error(RPX_E_INVALID, "face list / face index out of range");
+    return run_generation(scene, face_list, face_list >= 0 ? face_idx : -1, true, in, n_in, max_length,
+                          wavelengths, n_wavelengths, out, n_out, face_hits, workspace, workspace_bytes, stream);
+}
+
+// ------------------------------------------------------ AoS <-> SoA
+// Packed ray_t: 21 doubles (168 B) then 5 uint32 (20 B) = 188 B = 47 words.
+// gausslet_t appends 6 x 10 doubles = 480 B -> 167 words.
+__global__ void k_pack(rpx_rays_t soa, uint64_t n, uint32_t *aos, int words_per) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * (uint64_t)words_per) return;
+    const uint64_t i = g / words_per;
+    const int w = (int)(g - i * words_per);
+    const uint64_t c = soa.capacity;
+    uint32_t v;
+    if (w < 42) {
+        v = ((const uint32_t *)(soa.f64 + (uint64_t)(w >> 1) * c + i))[w & 1];
+    } else if (w < 47) {
+        v = soa.u32[(uint64_t)(w - 42) * c + i];
+    } else {
+        const int pw = w - 47;
+        v = ((const uint32_t *)(soa.para + (uint64_t)(pw >> 1) * c + i))[pw & 1];
+    }
+    aos[g] = v;
+}
+
+__global__ void k_unpack(const uint32_t *aos, uint64_t n, rpx_rays_t soa, int words_per) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * (uint64_t)words_per) return;
+    const uint64_t i = g / words_per;
+    const int w = (int)(g - i * words_per);
+    const uint64_t c = soa.capacity;
+    const uint32_t v = aos[g];
+    if (w < 42) {
+        ((uint32_t *)(soa.f64 + (uint64_t)(w >> 1) * c + i))[w & 1] = v;
+    } else if (w < 47) {
+        soa.u32[(uint64_t)(w - 42) * c + i] = v;
+    } else {
+        const int pw = w - 47;
+        ((uint32_t *)(soa.para + (uint64_t)(pw >> 1) * c + i))[pw & 1] = v;
+    }
+}
+
+extern "C" rpx_status_t rpx_pack_rays(const rpx_rays_t *soa, uint64_t n, void *aos, int32_t gausslet,
+                                      void *stream) {
+    rpx_status_t st = check_rays(soa, n, gausslet != 0, "rays");
+    if (st) return st;
+    if (n == 0) return RPX_OK;
+    if (!aos) return set_error(RPX_E_INVALID, "AoS buffer is null");
+    const int words = gausslet ? 167 : 47;
+    const uint64_t total = n * words;
+    const uint64_t nb = (total + 255) / 256;
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, *soa, n, (uint32_t *)aos,
+                       words);
+    HIP_TRY(hipGetLastError());
+    return RPX_OK;
+}
+
+extern "C" rpx_status_t rpx_unpack_rays(const void *aos, uint64_t n, rpx_rays_t *soa, int32_t gausslet,
+                                        void *stream) {
+    rpx_status_t st = check_rays(soa, n, gausslet != 0, "rays");
+    if (st) return st;
+    if (n == 0) return RPX_OK;
+    if (!aos) return set_error(RPX_E_INVALID, "AoS buffer is null");
+    const int words = gausslet ? 167 : 47;
+    const uint64_t total = n * words;
+    const uint64_t nb = (total + 255) / 256;
+    hipLaunchKernelGGL(k_unpack, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, (const uint32_t *)aos,
+                       n, *soa, words);
+    HIP_TRY(hipGetLastError());
+    return RPX_OK;
+}

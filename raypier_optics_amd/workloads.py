@@ -1,0 +1,210 @@
+"""Optical systems and seeded ray fields of the benchmark configurations,
+built with the drop-in API.
+
+  singlet_c1      N-BK7 biconvex singlet (R=+/-50 mm, CT 5 mm, D 25 mm),
+                  uncoated Fresnel interfaces with zero thresholds (reflected
+                  children kept), opaque detector at the paraxial focus;
+                  collimated hexagonal field, 10,000 rays over 20 mm.
+  asphere_doublet_c2
+                  even asphere (k=-0.6, A4..A10) in N-BK7 followed by a
+                  cemented N-BK7/N-SF5 doublet R=(+61.5, -44.2, -128.2) mm,
+                  single-layer MgF2 (n=1.38) quarter-wave AR coating on the
+                  outer faces, planar detector; 4,194,304 rays uniform on a
+                  6 mm-radius disk, angles U(+/-2 deg), per-ray wavelength
+                  U(0.45, 0.65) um.
+
+Sign conventions follow raypier: SphericalFace.curvature = z_vertex -
+z_centre, and a material's "outside" is the side its face normal points to.
+"""
+import math
+
+import numpy as np
+
+from ._abi import ray_dtype, NO_FACE
+from .ctracer import RayCollection, FaceList
+from .cfaces import SphericalFace, CircularFace, AsphericFace, ShapedPlanarFace
+from .cshapes import CircleShape
+from .cmaterials import (BaseDispersionCurve, NondispersiveCurve, FullDielectricDispersiveMaterial,
+                         CoatedDispersiveMaterial, OpaqueMaterial)
+
+# Sellmeier formula-2 coefficients [0, B1, C1, B2, C2, B3, C3] (C in um^2)
+N_BK7 = (0.0, 1.03961212, 0.00600069867, 0.231792344, 0.0200179144, 1.01046945, 103.560653)
+N_SF5 = (0.0, 1.52481889, 0.011254756, 0.187085527, 0.0588995392, 1.42729015, 129.141675)
+N_SF11 = (0.0, 1.73759695, 0.013188707, 0.313747346, 0.0623068142, 1.89878101, 155.23629)
+
+
+def glass(coefs, wl_min=0.3, wl_max=2.5):
+    return BaseDispersionCurve(2, np.array(coefs), 0.0, wl_min, wl_max)
+
+
+def sellmeier_n(coefs, wl):
+    wl2 = wl * wl
+    n2 = 1.0 + coefs[0]
+    for i in range(3):
+        n2 += coefs[2 * i + 1] * wl2 / (wl2 - coefs[2 * i + 2])
+    return math.sqrt(n2)
+
+
+# ------------------------------------------------------------- ray fields
+def base_rays(n):
+    r = np.zeros(n, ray_dtype)
+    r["direction"] = (0.0, 0.0, 1.0)
+    r["E_vector"] = (1.0, 0.0, 0.0)
+    r["refractive_index"] = 1.0
+    r["E1_amp"] = 1.0
+    r["length"] = np.inf
+    r["parent_idx"] = NO_FACE
+    r["end_face_idx"] = NO_FACE
+    r["ray_ident"] = np.arange(n, dtype=np.uint32)
+    return r
+
+
+def hex_field(n_rays, radius, z0):
+    """Collimated hexagonal grid inside a circle, trimmed to the n_rays points
+    nearest the axis (ties broken by index)."""
+    area = math.pi * radius * radius
+    spacing = math.sqrt(2.0 * area / (math.sqrt(3.0) * n_rays))
+    m = int(radius / spacing) + 3
+    i, j = np.meshgrid(np.arange(-m, m + 1), np.arange(-m, m + 1), indexing="ij")
+    x = (i + 0.5 * j) * spacing
+    y = j * (math.sqrt(3.0) / 2.0) * spacing
+    x, y = x.ravel(), y.ravel()
+    r2 = x * x + y * y
+    order = np.lexsort((np.arange(len(r2)), r2))[:n_rays]
+    r = base_rays(n_rays)
+    r["origin"][:, 0] = x[order]
+    r["origin"][:, 1] = y[order]
+    r["origin"][:, 2] = z0
+    return r
+
+
+def disk_field(n_rays, radius, z0, max_angle_deg, wl_range, seed):
+    """Uniform positions on a disk, direction angles U(+/-max_angle), one
+    wavelength per ray (wavelength_idx = ray index)."""
+    rng = np.random.default_rng(seed)
+    rr = radius * np.sqrt(rng.random(n_rays))
+    th = 2 * np.pi * rng.random(n_rays)
+    ax = np.radians(rng.uniform(-max_angle_deg, max_angle_deg, n_rays))
+    ay = np.radians(rng.uniform(-max_angle_deg, max_angle_deg, n_rays))
+    d = np.stack([np.tan(ax), np.tan(ay), np.ones(n_rays)], axis=1)
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    r = base_rays(n_rays)
+    r["origin"][:, 0] = rr * np.cos(th)
+    r["origin"][:, 1] = rr * np.sin(th)
+    r["origin"][:, 2] = z0
+    r["direction"] = d
+    e = np.cross(d, np.array([0.0, 1.0, 0.0]))   # linear polarisation along ~x
+    r["E_vector"] = e / np.linalg.norm(e, axis=1)[:, None]
+    r["wavelength_idx"] = np.arange(n_rays, dtype=np.uint32)
+    wl = rng.uniform(wl_range[0], wl_range[1], n_rays)
+    return r, wl
+
+
+# ---------------------------------------------------------------- systems
+class Workload(object):
+    def __init__(self, name, rays, wavelengths, face_lists, recursion_limit, max_length, info):
+        self.name = name
+        self.rays = rays                  # host ray_dtype array
+        self.wavelengths = np.asarray(wavelengths, np.float64)
+        self.face_lists = face_lists
+        self.recursion_limit = recursion_limit
+        self.max_length = max_length
+        self.info = info
+
+    def collection(self):
+        rc = RayCollection.from_array(self.rays)
+        rc.wavelengths = self.wavelengths
+        return rc
+
+    def subset(self, idx):
+        """The same system with a subset of the input rays (wavelength table
+        reindexed when rays carry their own wavelength)."""
+        rays = self.rays[idx].copy()
+        wl = self.wavelengths
+        if len(wl) == len(self.rays) and len(wl) > 1:
+            wl = wl[rays["wavelength_idx"]]
+            rays["wavelength_idx"] = np.arange(len(rays), dtype=np.uint32)
+        return Workload(self.name, rays, wl, self.face_lists, self.recursion_limit, self.max_length, self.info)
+
+
+def prepare_faces(face_lists, wavelengths, max_length):
+    """Global face indices, counts and material wavelengths exactly as
+    trace_rays sets them (core/tracer.py:28-34); returns all_faces."""
+    all_faces = [f for fl in face_lists for f in fl.faces]
+    for i, f in enumerate(all_faces):
+        f.idx = i
+        f.count = 0
+        f.material.wavelengths = np.asarray(wavelengths)
+        f.max_length = max_length
+    return all_faces
+
+
+def singlet_c1(n_rays=10000, depth=4):
+    wl = 0.55
+    air = NondispersiveCurve(1.0)
+    bk7 = glass(N_BK7)
+    front = FullDielectricDispersiveMaterial(dispersion_inside=air, dispersion_outside=bk7,
+                                             reflection_threshold=0.0, transmission_threshold=0.0)
+    back = FullDielectricDispersiveMaterial(dispersion_inside=bk7, dispersion_outside=air,
+                                            reflection_threshold=0.0, transmission_threshold=0.0)
+    lens = FaceList()
+    lens.faces = [SphericalFace(diameter=25.0, z_height=-2.5, curvature=-50.0, material=front),
+                  SphericalFace(diameter=25.0, z_height=2.5, curvature=50.0, material=back)]
+    n = sellmeier_n(N_BK7, wl)
+    # thick-lens paraxial back focal length
+    R1, R2, t = 50.0, -50.0, 5.0
+    P = (n - 1) * (1 / R1 - 1 / R2 + (n - 1) * t / (n * R1 * R2))
+    efl = 1.0 / P
+    bfl = efl * (1 - (n - 1) * t / (n * R1))
+    det = FaceList()
+    det.faces = [CircularFace(diameter=60.0, z_plane=2.5 + bfl, material=OpaqueMaterial())]
+    rays = hex_field(n_rays, 10.0, -10.0)
+    info = {"n_glass": n, "efl_mm": efl, "bfl_mm": bfl, "detector_z_mm": 2.5 + bfl}
+    return Workload("singlet_c1", rays, [wl], [lens, det], depth, 100.0, info)
+
+
+def asphere_doublet_c2(n_rays=4194304, seed=12345):
+    air = NondispersiveCurve(1.0)
+    bk7, sf5 = glass(N_BK7), glass(N_SF5)
+    mgf2 = NondispersiveCurve(1.38)
+    t_coat = 0.55 / (4 * 1.38)
+    shape = CircleShape(radius=10.0)
+    # asphere: optical R = +30 mm, k=-0.6, A4..A10 of the prescription;
+    # AsphericFace takes negated A terms with inverted normals (as lens
+    # classes build it), planar back face at CT = 6 mm
+    asph_mat = CoatedDispersiveMaterial(dispersion_inside=bk7, dispersion_outside=air, dispersion_coating=mgf2,
+                                        coating_thickness=t_coat)
+    A = {"A4": -2e-5, "A6": 1e-7, "A8": -3e-10, "A10": 5e-13}
+    asphere = FaceList()
+    asphere.faces = [AsphericFace(shape=shape, material=asph_mat, conic_const=-0.6, z_height=0.0, curvature=30.0,
+                                  A4=-A["A4"], A6=-A["A6"], A8=-A["A8"], A10=-A["A10"], invert_normals=True,
+                                  atol=1e-16),
+                     ShapedPlanarFace(shape=shape, material=asph_mat, z_height=6.0)]
+    zA, ct1, ct2 = 16.0, 6.0, 3.0
+    m1 = CoatedDispersiveMaterial(dispersion_inside=air, dispersion_outside=bk7, dispersion_coating=mgf2,
+                                  coating_thickness=t_coat)
+    m2 = CoatedDispersiveMaterial(dispersion_inside=bk7, dispersion_outside=sf5, dispersion_coating=mgf2,
+                                  coating_thickness=0.0)
+    m3 = CoatedDispersiveMaterial(dispersion_inside=sf5, dispersion_outside=air, dispersion_coating=mgf2,
+                                  coating_thickness=t_coat)
+    doublet = FaceList()
+    doublet.faces = [SphericalFace(diameter=25.0, z_height=zA, curvature=-61.5, material=m1),
+                     SphericalFace(diameter=25.0, z_height=zA + ct1, curvature=44.2, material=m2),
+                     SphericalFace(diameter=25.0, z_height=zA + ct1 + ct2, curvature=128.2, material=m3)]
+    # paraxial focus at 0.55 um by a y-nu trace of the five refracting surfaces
+    wl0 = 0.55
+    nb, nf = sellmeier_n(N_BK7, wl0), sellmeier_n(N_SF5, wl0)
+    surfs = [(0.0, 30.0, 1.0, nb), (6.0, 1e300, nb, 1.0), (zA, 61.5, 1.0, nb), (zA + ct1, -44.2, nb, nf),
+             (zA + ct1 + ct2, -128.2, nf, 1.0)]
+    y, u, z_prev = 1.0, 0.0, 0.0
+    for z, R, n1, n2 in surfs:
+        y = y + u * (z - z_prev)
+        u = (n1 * u - y * (n2 - n1) / R) / n2
+        z_prev = z
+    focus = z_prev - y / u
+    det = FaceList()
+    det.faces = [CircularFace(diameter=80.0, z_plane=focus, material=OpaqueMaterial())]
+    rays, wl = disk_field(n_rays, 6.0, -5.0, 2.0, (0.45, 0.65), seed)
+    info = {"asphere_R_mm": 30.0, "asphere_CT_mm": 6.0, "doublet_z_mm": zA, "CT1_mm": ct1, "CT2_mm": ct2,
+            "coating_um": t_coat, "focus_z_mm": focus, "seed": seed}
+    return Workload("asphere_doublet_c2", rays, wl, [asphere, doublet, det], 10, 200.0, info)
