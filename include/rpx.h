@@ -119,7 +119,9 @@ enum rpx_face_type {
  *   SHAPED_SPHERICAL  curvature, z_height
  *   EXTRUDED_PLANAR   x1, y1, x2, y2, z1, z2, normal.x, normal.y, normal.z
  *   CONIC             curvature, z_height, conic_const
- *   ASPHERIC          curvature, z_height, conic_const, A4, A6, A8, A10, A12, A14, A16, atol
+ *   ASPHERIC          curvature, z_height, conic_const, A4, A6, A8, A10, A12, A14, A16, atol,
+ *                     R^3 (R = -curvature), sag-band half-width over the aperture
+ *                     (-1 = no band), bound on |dsag/dr| over the aperture
  *   DISTORTION        accuracy                   (aux0 = base face index, aux1 = zernike index)
  * A DISTORTION face's base face is an ordinary record of the face table that
  * belongs to no face list (n_faces of every list excludes it). */
@@ -316,6 +318,16 @@ rpx_status_t rpx_trace_gausslet(const rpx_scene_t *scene,
                                uint32_t *face_hits,
                                void *workspace, uint64_t workspace_bytes,
                                void *stream);
+
+/* Kernel timing with HIP events recorded on the stream each generation runs
+ * on (no counterpart in the reference: measurement support).  While enabled,
+ * every trace call adds the duration of its three kernels to the running
+ * totals: ms[0..2] = intersect/count, block scan, emit; launches[0..2] the
+ * number of launches.  rpx_timing_read also reports bytes moved by each
+ * kernel's algorithm (read + written, per the ray record sizes) in bytes[0..2]
+ * and resets the totals. */
+rpx_status_t rpx_timing_enable(int32_t on);
+rpx_status_t rpx_timing_read(double *ms, uint64_t *launches, double *bytes);
 
 /* Per-object queries, batched over n device-resident inputs.
  *   rpx_query_face: what = 0 -> out[i] = Face.intersect_c(p1[i], p2[i],

@@ -35,6 +35,8 @@ _SIGNATURES = {
                                            P(_abi.Rays), P(c_u64), c_void_p, c_void_p, c_u64, c_void_p]),
     "rpx_trace_gausslet": (c_i32, [c_void_p, c_i32, c_i32, P(_abi.Rays), c_u64, c_f64, c_void_p, c_u64,
                                    P(_abi.Rays), P(c_u64), c_void_p, c_void_p, c_u64, c_void_p]),
+    "rpx_timing_enable": (c_i32, [c_i32]),
+    "rpx_timing_read": (c_i32, [c_void_p, c_void_p, c_void_p]),
     "rpx_query_face": (c_i32, [c_void_p, c_i32, c_i32, c_void_p, c_void_p, c_u64, c_i32, c_void_p, c_void_p]),
     "rpx_query_shape": (c_i32, [c_void_p, c_i32, c_void_p, c_u64, c_void_p, c_void_p]),
     "rpx_query_zernike": (c_i32, [c_void_p, c_i32, c_void_p, c_u64, c_void_p, c_void_p, c_void_p]),

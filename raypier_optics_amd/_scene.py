@@ -5,7 +5,10 @@ Objects describe themselves through `_rpx_fill_face`, `_rpx_fill_material`,
 `_rpx_shape_ops` and `_rpx_compile` methods; an object without one is not
 supported on the device and is refused here (RPX_E_UNSUPPORTED semantics).
 """
+import collections
 import ctypes
+import hashlib
+
 import numpy as np
 
 from . import _abi
@@ -58,27 +61,63 @@ class SceneTables(object):
                 best = max(best, mc.get(int(self.materials[f["material"]]["type"]), 0))
         return best
 
+    def key(self):
+        """Content key: two scenes with equal tables share one upload."""
+        parts = [a.tobytes() for a in (self.face_lists, self.faces, self.materials, self.shape_ops,
+                                       self.vertices, self.zernikes, self.zcoefs, self.zinsns)]
+        return hashlib.sha1(b"|".join(parts)).hexdigest()
+
     def handle(self):
-        """Upload to the current GPU once; returns the rpx_scene_t pointer."""
+        """The uploaded rpx_scene_t for these tables on the current GPU."""
         if self._handle is None:
-            from ._lib import load, check, require_gpu
-            require_gpu()
-            lib = load()
-            h = ctypes.c_void_p()
-            d = self.desc()
-            check(lib.rpx_scene_create(ctypes.byref(d), ctypes.byref(h)))
-            self._handle = h
-        return self._handle
+            self._handle = _DEVICE_SCENES.get(self)
+        return self._handle.ptr
+
+
+class _DeviceScene(object):
+    def __init__(self, tables):
+        from ._lib import load, check, require_gpu
+        torch = require_gpu()
+        self.device = torch.cuda.current_device()
+        lib = load()
+        h = ctypes.c_void_p()
+        d = tables.desc()
+        check(lib.rpx_scene_create(ctypes.byref(d), ctypes.byref(h)))
+        self.ptr = h
 
     def __del__(self):
-        h = getattr(self, "_handle", None)
-        if h is not None and h.value:
+        if self.ptr is not None and self.ptr.value:
             try:
                 from ._lib import load
-                load().rpx_scene_destroy(h)
+                load().rpx_scene_destroy(self.ptr)
             except Exception:
                 pass
-            self._handle = None
+            self.ptr = None
+
+
+class _SceneCache(object):
+    """Recently uploaded scenes keyed by table contents and device, so that
+    tracing the same optics again reuses the device copy."""
+
+    def __init__(self, size=16):
+        self.size = size
+        self.entries = collections.OrderedDict()
+
+    def get(self, tables):
+        import torch
+        key = (tables.key(), torch.cuda.current_device())
+        ent = self.entries.get(key)
+        if ent is None:
+            ent = _DeviceScene(tables)
+            self.entries[key] = ent
+            while len(self.entries) > self.size:
+                self.entries.popitem(last=False)
+        else:
+            self.entries.move_to_end(key)
+        return ent
+
+
+_DEVICE_SCENES = _SceneCache()
 
 
 class SceneBuilder(object):

@@ -4,6 +4,8 @@ Each class keeps the reference's constructor keywords and attributes and
 compiles to one rpx_face_t record; intersection and normals run on the GPU
 (rpx_faces.h).  Parameter slots follow include/rpx.h.
 """
+import math
+
 import numpy as np
 
 from . import _abi
@@ -195,10 +197,32 @@ class AsphericFace(ShapedFace):
             setattr(self, k, kwds.get(k, 0.0))
         self.atol = kwds.get("atol", 1.0e-8)
 
+    def _sag_band(self):
+        """Half-width of the z band holding the surface over the aperture and
+        a bound on |dsag/dr| there, or (-1, 0) when either is unbounded."""
+        from .cshapes import bound_radius
+        rb = bound_radius(self.shape)
+        R = -float(self.curvature)
+        beta = 1.0 + float(self.conic_const)
+        A = [self.A4, self.A6, self.A8, self.A10, self.A12, self.A14, self.A16]
+        if not math.isfinite(rb) or R == 0.0:
+            return -1.0, 0.0
+        arg = 1.0 - max(beta, 0.0) * rb * rb / (R * R)
+        if arg <= 1e-6:
+            return -1.0, 0.0
+        band = rb * rb / abs(R) + sum(abs(a) * rb ** (2 * k + 4) for k, a in enumerate(A))
+        slope = rb / (abs(R) * math.sqrt(arg)) + sum((2 * k + 4) * abs(a) * rb ** (2 * k + 3) for k, a in enumerate(A))
+        if not (math.isfinite(band) and math.isfinite(slope)):
+            return -1.0, 0.0
+        return band, slope
+
     def _rpx_fill_face(self, rec, b):
         rec["type"] = _abi.FACE_ASPHERIC
-        rec["p"][:11] = (self.curvature, self.z_height, self.conic_const, self.A4, self.A6, self.A8,
-                         self.A10, self.A12, self.A14, self.A16, self.atol)
+        R = -float(self.curvature)
+        band, slope = self._sag_band()
+        # R**3 through the C library's pow, as the reference evaluates it
+        rec["p"][:14] = (self.curvature, self.z_height, self.conic_const, self.A4, self.A6, self.A8,
+                         self.A10, self.A12, self.A14, self.A16, self.atol, R ** 3, band, slope)
         self._fill_shape(rec, b)
 
 

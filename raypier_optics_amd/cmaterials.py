@@ -13,6 +13,20 @@ from . import _abi
 from .ctracer import InterfaceMaterial
 
 
+_extent_cache = {"key": None, "extent": None}
+
+
+def wavelength_extent(wl):
+    """(min, max) of a wavelength array, cached for the most recent array
+    object so that repeated traces of one collection scan it once."""
+    wl = np.asarray(wl, dtype=np.double)
+    key = (id(wl), wl.ctypes.data if wl.size else 0, wl.size, wl.strides)
+    if _extent_cache["key"] != key:
+        ext = (float(wl.min()), float(wl.max())) if wl.size else (np.inf, -np.inf)
+        _extent_cache["key"], _extent_cache["extent"] = key, ext
+    return _extent_cache["extent"]
+
+
 class BaseDispersionCurve(object):
     """n(wavelength) by formula 0/1/2/3/5 (cmaterials.pyx:152-239)."""
 
@@ -29,9 +43,12 @@ class BaseDispersionCurve(object):
         self.wavelength_max = wavelength_max
 
     def check_range(self, wavelen):
-        wl = np.asarray(wavelen, dtype=np.double)
-        bad = (wl < self.wavelength_min) | (wl > self.wavelength_max)
-        if bad.any():
+        """Raise ValueError if any wavelength is outside the curve's range
+        (c_evaluate_n, cmaterials.pyx:218-224)."""
+        lo, hi = wavelength_extent(wavelen)
+        if lo < self.wavelength_min or hi > self.wavelength_max:
+            wl = np.asarray(wavelen, dtype=np.double)
+            bad = (wl < self.wavelength_min) | (wl > self.wavelength_max)
             wvl = float(wl[bad][0])
             raise ValueError("Wavelength (%f) outside range of dispersion curve (%f -> %f)" %
                              (wvl, self.wavelength_min, self.wavelength_max))

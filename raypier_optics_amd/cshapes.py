@@ -3,10 +3,19 @@
 A shape tree compiles to a postfix program of the scene table; the predicate
 itself runs on the GPU (cshapes.pyx:34-168 semantics).
 """
+import math
+
 import numpy as np
 
 from . import _abi
 from .ctracer import Shape
+
+
+def bound_radius(shape):
+    """Radius of a disk about the origin containing every inside point of
+    the shape (infinite when unbounded)."""
+    f = getattr(shape, "_rpx_bound_radius", None)
+    return f() if f is not None else float("inf")
 
 
 class LogicalOpShape(Shape):
@@ -28,6 +37,9 @@ class InvertShape(LogicalOpShape):
 
     def __init__(self, shape):
         self.shape = shape
+
+    def _rpx_bound_radius(self):
+        return float("inf")
 
     def _rpx_children(self):
         return [self.shape]
@@ -53,13 +65,22 @@ class BooleanShape(LogicalOpShape):
 class BooleanAND(BooleanShape):
     _op = _abi.SHAPE_AND
 
+    def _rpx_bound_radius(self):
+        return min(bound_radius(self.shape1), bound_radius(self.shape2))
+
 
 class BooleanOR(BooleanShape):
     _op = _abi.SHAPE_OR
 
+    def _rpx_bound_radius(self):
+        return max(bound_radius(self.shape1), bound_radius(self.shape2))
+
 
 class BooleanXOR(BooleanShape):
     _op = _abi.SHAPE_XOR
+
+    def _rpx_bound_radius(self):
+        return max(bound_radius(self.shape1), bound_radius(self.shape2))
 
 
 class BasicShape(LogicalOpShape):
@@ -87,6 +108,9 @@ class CircleShape(BasicShape):
         super().__init__(**kwds)
         self.radius = kwds.get("radius", 1.0)
 
+    def _rpx_bound_radius(self):
+        return math.hypot(self.centre_x, self.centre_y) + abs(self.radius)
+
     def _rpx_shape_ops(self, rec, builder):
         rec["op"] = _abi.SHAPE_CIRCLE
         rec["p"][:3] = (self.centre_x, self.centre_y, self.radius)
@@ -99,6 +123,9 @@ class RectangleShape(BasicShape):
         super().__init__(**kwds)
         self.width = kwds.get("width", 5.0)
         self.height = kwds.get("height", 7.0)
+
+    def _rpx_bound_radius(self):
+        return math.hypot(self.centre_x, self.centre_y) + 0.5 * math.hypot(self.width, self.height)
 
     def _rpx_shape_ops(self, rec, builder):
         rec["op"] = _abi.SHAPE_RECT
@@ -121,6 +148,11 @@ class PolygonShape(BasicShape):
     @coordinates.setter
     def coordinates(self, val):
         self._coordinates = np.ascontiguousarray(val, dtype=np.float64).reshape(-1, 2)
+
+    def _rpx_bound_radius(self):
+        if len(self._coordinates) == 0:
+            return float("inf")
+        return float(np.hypot(self._coordinates[:, 0], self._coordinates[:, 1]).max())
 
     def _rpx_shape_ops(self, rec, builder):
         off, n = builder.add_vertices(self._coordinates)

@@ -185,10 +185,27 @@ RPX_DEV double intersect_conic(V3 a, V3 d, double curvature, double conic_const)
     return (-B - D) / (2 * A);
 }
 
-RPX_DEV double ipow(double x, int n) { return pow(x, (double)n); }
+// x^2 .. x^nmax into pw[2..nmax].  x^2 is the rounded product (the
+// reference's x**2); higher powers come from a double-double product chain
+// rounded once, i.e. correctly rounded like the C library's pow for these
+// integer exponents, at a fraction of pow's cost.
+RPX_DEV void pow_chain(double x, double *pw, int nmax) {
+    double hi = x * x;
+    double lo = fma(x, x, -hi);
+    pw[2] = hi;
+    for (int k = 3; k <= nmax; ++k) {
+        double p = hi * x;
+        double e = fma(hi, x, -p);
+        lo = fma(lo, x, e);
+        double s = p + lo;
+        lo = lo - (s - p);
+        hi = s;
+        pw[k] = hi + lo;
+    }
+}
 
 struct Asph {
-    double R, beta, A4, A6, A8, A10, A12, A14, A16;
+    double R, beta, A4, A6, A8, A10, A12, A14, A16, R3;
     V3 a, d;
 };
 
@@ -196,11 +213,13 @@ struct Asph {
 RPX_DEV double asph_impf(const Asph &A, double alpha) {
     double tx = A.a.x + alpha * A.d.x, ty = A.a.y + alpha * A.d.y;
     double r2 = (tx * tx + ty * ty);
+    double pw[9];
+    pow_chain(r2, pw, 8);
     double out = r2;
     out /= A.R * (1 + sqrt(1 - A.beta * r2 / (A.R * A.R)));
     out -= A.a.z + alpha * A.d.z;
-    out += A.A4 * (r2 * r2) + A.A6 * ipow(r2, 3) + A.A8 * ipow(r2, 4) + A.A10 * ipow(r2, 5) +
-           A.A12 * ipow(r2, 6) + A.A14 * ipow(r2, 7) + A.A16 * ipow(r2, 8);
+    out += A.A4 * pw[2] + A.A6 * pw[3] + A.A8 * pw[4] + A.A10 * pw[5] + A.A12 * pw[6] + A.A14 * pw[7] +
+           A.A16 * pw[8];
     return out;
 }
 
@@ -210,17 +229,19 @@ RPX_DEV double asph_grad(const Asph &A, double alpha) {
     double r2 = (tx * tx + ty * ty);
     double dx = A.d.x * tx;
     double dy = A.d.y * ty;
-    double out = A.A10 * (10 * dx + 10 * dy) * ipow(r2, 4);
-    out += A.A12 * (12 * dx + 12 * dy) * ipow(r2, 5);
-    out += A.A14 * (14 * dx + 14 * dy) * ipow(r2, 6);
-    out += A.A16 * (16 * dx + 16 * dy) * ipow(r2, 7);
+    double pw[8];
+    pow_chain(r2, pw, 7);
+    double out = A.A10 * (10 * dx + 10 * dy) * pw[4];
+    out += A.A12 * (12 * dx + 12 * dy) * pw[5];
+    out += A.A14 * (14 * dx + 14 * dy) * pw[6];
+    out += A.A16 * (16 * dx + 16 * dy) * pw[7];
     out += A.A4 * (4 * dx + 4 * dy) * (r2);
-    out += A.A6 * (6 * dx + 6 * dy) * (r2 * r2);
-    out += A.A8 * (8 * dx + 8 * dy) * ipow(r2, 3) - A.d.z;
+    out += A.A6 * (6 * dx + 6 * dy) * pw[2];
+    out += A.A8 * (8 * dx + 8 * dy) * pw[3] - A.d.z;
     out += (2 * dx + 2 * dy) / (A.R * (sqrt(1 - A.beta * (r2) / (A.R * A.R)) + 1));
     double s = sqrt(1 - A.beta * r2 / (A.R * A.R));
     double s1 = s + 1;
-    out += A.beta * (2 * dx + 2 * dy) * (r2) / (2 * ipow(A.R, 3) * s * (s1 * s1));
+    out += A.beta * (2 * dx + 2 * dy) * (r2) / (2 * A.R3 * s * (s1 * s1));
     return out;
 }
 
@@ -345,6 +366,16 @@ RPX_DEV double plain_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V
     case RPX_FACE_ASPHERIC: {  // cfaces.pyx:1909-1976
         double atol = f.p[10];
         double tl = atol * atol;
+        if (is_base && f.p[12] >= 0.0) {
+            // A base-ray hit is accepted only inside the aperture, where the
+            // surface lies within z_height +/- p[12]; the converged root
+            // satisfies the sag equation to within the Newton tolerance.  A
+            // segment whose z-range misses that band cannot be accepted.
+            double len = sep(p1, p2);
+            double m = f.p[12] + 1e-9 * (1.0 + fabs(f.p[1]) + f.p[12]) + 10.0 * atol * len * (1.0 + f.p[13]);
+            double zlo = fmin(p1.z, p2.z), zhi = fmax(p1.z, p2.z);
+            if (zhi < f.p[1] - m || zlo > f.p[1] + m) return -1;
+        }
         V3 d = sub(p2, p1);
         V3 a = p1;
         a.z -= f.p[1];
@@ -354,6 +385,7 @@ RPX_DEV double plain_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V
         A.beta = 1 + f.p[2];
         A.A4 = f.p[3]; A.A6 = f.p[4]; A.A8 = f.p[5]; A.A10 = f.p[6];
         A.A12 = f.p[7]; A.A14 = f.p[8]; A.A16 = f.p[9];
+        A.R3 = f.p[11];
         A.a = a;
         A.d = d;
         double fv = asph_impf(A, a1), f_last = fv;
@@ -426,11 +458,13 @@ RPX_DEV V3 plain_normal(const SceneView &S, const rpx_face_t &f, V3 p) {
         double r2 = p.x * p.x + p.y * p.y;
         double root = sqrt(1 - (beta * (r2) / (R * R)));
         double A4 = f.p[3], A6 = f.p[4], A8 = f.p[5], A10 = f.p[6], A12 = f.p[7], A14 = f.p[8], A16 = f.p[9];
-        double df = 10 * A10 * ipow(r2, 4) + 8 * A8 * ipow(r2, 3) + 6 * A6 * (r2 * r2) + 4 * A4 * r2;
-        df += 16 * A16 * ipow(r2, 7) + 14 * A14 * ipow(r2, 6) + 12 * A12 * ipow(r2, 5);
+        double pw[8];
+        pow_chain(r2, pw, 7);
+        double df = 10 * A10 * pw[4] + 8 * A8 * pw[3] + 6 * A6 * pw[2] + 4 * A4 * r2;
+        df += 16 * A16 * pw[7] + 14 * A14 * pw[6] + 12 * A12 * pw[5];
         df += 2 / (R * (1 + root));
         double r1 = 1 + root;
-        df += beta * (r2) / (ipow(R, 3) * root * (r1 * r1));
+        df += beta * (r2) / (f.p[11] * root * (r1 * r1));
         V3 g;
         g.z = 1.0;
         g.x = -df * p.x;

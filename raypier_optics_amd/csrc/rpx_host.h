@@ -1,6 +1,7 @@
 // Host-side internals shared by the library's translation units.
 #pragma once
 #include <string>
+#include <vector>
 #include "rpx_internal.h"
 
 struct rpx_scene {
@@ -8,7 +9,11 @@ struct rpx_scene {
     void *device_block;
     int32_t n_global_faces;
     int32_t list_sizes[16];
+    std::vector<rpx_shape_op_t> host_shape_ops;   // for validating query roots
 };
+
+// Checks that ops[begin..root] is a well-formed postfix program.
+rpx_status_t rpx_check_shape_program(const rpx_shape_op_t *ops, int n_ops, int root);
 
 // Records the message for rpx_last_error() and returns `code`.
 rpx_status_t rpx_set_error(rpx_status_t code, const std::string &msg);

@@ -84,7 +84,12 @@ extern "C" rpx_status_t rpx_query_face(const rpx_scene_t *scene, int32_t face_re
 
 extern "C" rpx_status_t rpx_query_shape(const rpx_scene_t *scene, int32_t root, const double *xy, uint64_t n,
                                         int32_t *out, void *stream) {
-    if (!scene || root >= scene->view.n_shape_ops) return rpx_set_error(RPX_E_INVALID, "shape root out of range");
+    if (!scene) return rpx_set_error(RPX_E_INVALID, "scene is null");
+    {
+        rpx_status_t st = rpx_check_shape_program(scene->host_shape_ops.data(), (int)scene->host_shape_ops.size(),
+                                                  root);
+        if (st != RPX_OK) return st;
+    }
     if (n == 0) return RPX_OK;
     if (!xy || !out) return rpx_set_error(RPX_E_INVALID, "null query buffer");
     hipLaunchKernelGGL(k_query_shape, dim3(nblocks(n)), dim3(256), 0, (hipStream_t)stream, scene->view, root, xy, n,

@@ -75,6 +75,23 @@ static int material_max_children(int type) {
 #define RPX_LDS_LISTS 16
 #define RPX_BLOCK 256
 
+rpx_status_t rpx_check_shape_program(const rpx_shape_op_t *ops, int n_ops, int root) {
+    if (root < 0) return RPX_OK;
+    if (root >= n_ops) return set_error(RPX_E_INVALID, "shape root out of range");
+    int begin = ops[root].begin, depth = 0;
+    if (begin < 0 || begin > root) return set_error(RPX_E_INVALID, "shape program range");
+    for (int k = begin; k <= root; ++k) {
+        int op = ops[k].op;
+        if (op < RPX_SHAPE_TRUE || op > RPX_SHAPE_XOR) return set_error(RPX_E_INVALID, "shape opcode");
+        if (op <= RPX_SHAPE_POLYGON) ++depth;
+        else if (op == RPX_SHAPE_NOT) { if (depth < 1) return set_error(RPX_E_INVALID, "shape stack"); }
+        else { if (depth < 2) return set_error(RPX_E_INVALID, "shape stack"); --depth; }
+        if (depth > 31) return set_error(RPX_E_UNSUPPORTED, "shape program deeper than 31");
+    }
+    if (depth != 1) return set_error(RPX_E_INVALID, "shape program does not reduce to one value");
+    return RPX_OK;
+}
+
 static rpx_status_t validate_scene(const rpx_scene_desc_t *d) {
     if (!d) return set_error(RPX_E_INVALID, "scene desc is null");
     if (d->n_face_lists < 0 || d->n_faces < 0 || d->n_materials < 0)
@@ -91,6 +108,9 @@ static rpx_status_t validate_scene(const rpx_scene_desc_t *d) {
         expect += fl.n_faces;
     }
     if (expect > d->n_faces) return set_error(RPX_E_INVALID, "face lists exceed face table");
+    for (int l = 0; l < d->n_face_lists; ++l)
+        for (int k = d->face_lists[l].first_face; k < d->face_lists[l].first_face + d->face_lists[l].n_faces; ++k)
+            if (d->faces[k].face_list != l) return set_error(RPX_E_INVALID, "face_list back-reference");
     for (int i = 0; i < d->n_faces; ++i) {
         const rpx_face_t &f = d->faces[i];
         if (f.type < RPX_FACE_CIRCULAR || f.type > RPX_FACE_DISTORTION)
@@ -114,18 +134,8 @@ static rpx_status_t validate_scene(const rpx_scene_desc_t *d) {
             return set_error(RPX_E_INVALID, "shape polygon range");
     }
     for (int i = 0; i < d->n_faces; ++i) {  // postfix programs must be well formed
-        int root = d->faces[i].shape;
-        if (root < 0) continue;
-        int begin = d->shape_ops[root].begin, depth = 0;
-        if (begin < 0 || begin > root) return set_error(RPX_E_INVALID, "shape program range");
-        for (int k = begin; k <= root; ++k) {
-            int op = d->shape_ops[k].op;
-            if (op <= RPX_SHAPE_POLYGON) ++depth;
-            else if (op == RPX_SHAPE_NOT) { if (depth < 1) return set_error(RPX_E_INVALID, "shape stack"); }
-            else { if (depth < 2) return set_error(RPX_E_INVALID, "shape stack"); --depth; }
-            if (depth > 31) return set_error(RPX_E_UNSUPPORTED, "shape program deeper than 31");
-        }
-        if (depth != 1) return set_error(RPX_E_INVALID, "shape program does not reduce to one value");
+        rpx_status_t st = rpx_check_shape_program(d->shape_ops, d->n_shape_ops, d->faces[i].shape);
+        if (st != RPX_OK) return st;
     }
     for (int i = 0; i < d->n_materials; ++i) {
         const rpx_material_t &m = d->materials[i];
@@ -228,6 +238,7 @@ extern "C" rpx_status_t rpx_scene_create(const rpx_scene_desc_t *d, rpx_scene_t 
     }
     v.max_children = maxc;
     s->n_global_faces = nglob;
+    s->host_shape_ops.assign(d->shape_ops, d->shape_ops + d->n_shape_ops);
     for (int l = 0; l < d->n_face_lists; ++l) s->list_sizes[l] = d->face_lists[l].n_faces;
     *out = s;
     return RPX_OK;
@@ -246,16 +257,17 @@ extern "C" int32_t rpx_scene_max_children(const rpx_scene_t *s) { return s ? s->
 // ------------------------------------------------------------ workspace
 struct Workspace {
     uint32_t *tags;          // per ray: child count (bits 0..7) | (face record + 1) << 8
-    uint32_t *block_totals;  // per block
+    uint32_t *block_totals;  // per block: children
+    uint32_t *block_parents; // per block: rays with at least one child
     uint64_t *block_offsets; // per block, exclusive
-    uint64_t *total;         // 1
+    uint64_t *total;         // [0] children, [1] parents with children
 };
 
 static uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 
 extern "C" uint64_t rpx_trace_workspace_bytes(uint64_t n_in) {
     uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
-    return align256(4 * n_in) + align256(4 * nb) + align256(8 * nb) + 256;
+    return align256(4 * n_in) + 2 * align256(4 * nb) + align256(8 * nb) + 256;
 }
 
 static Workspace carve(void *ws, uint64_t n_in) {
@@ -265,6 +277,8 @@ static Workspace carve(void *ws, uint64_t n_in) {
     w.tags = (uint32_t *)p;
     p += align256(4 * n_in);
     w.block_totals = (uint32_t *)p;
+    p += align256(4 * nb);
+    w.block_parents = (uint32_t *)p;
     p += align256(4 * nb);
     w.block_offsets = (uint64_t *)p;
     p += align256(8 * nb);
@@ -433,10 +447,16 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_intersect(TraceArgs A) {
     }
     uint32_t block_total;
     block_scan_small(count, lds_wave, &block_total);
-    if (threadIdx.x == 0) A.ws.block_totals[blockIdx.x] = block_total;
+    __syncthreads();
+    uint32_t block_par;
+    block_scan_small(count ? 1u : 0u, lds_wave, &block_par);
+    if (threadIdx.x == 0) {
+        A.ws.block_totals[blockIdx.x] = block_total;
+        A.ws.block_parents[blockIdx.x] = block_par;
+    }
     __syncthreads();
     if (A.face_hits) {
-        for (int k = threadIdx.x; k < S.n_faces; k += blockDim.x) {
+        for (int k = threadIdx.x; k < S.n_list_faces; k += blockDim.x) {
             uint32_t h = lds_hist[k];
             if (h) atomicAdd(&A.face_hits[S.faces[k].global_idx], h);
         }
@@ -444,10 +464,12 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_intersect(TraceArgs A) {
 }
 
 // Exclusive scan of block totals in a single workgroup of 1024 lanes.
-__global__ __launch_bounds__(1024) void k_scan(const uint32_t *totals, uint64_t nb, uint64_t *offsets,
-                                               uint64_t *grand_total) {
+__global__ __launch_bounds__(1024) void k_scan(const uint32_t *totals, const uint32_t *parents, uint64_t nb,
+                                               uint64_t *offsets, uint64_t *grand_total) {
     __shared__ uint64_t lds[1024];
     uint64_t carry = 0;
+    uint64_t mine = 0;
+    for (uint64_t k = threadIdx.x; k < nb; k += 1024) mine += parents[k];
     for (uint64_t base = 0; base < nb; base += 1024) {
         uint64_t k = base + threadIdx.x;
         uint64_t v = (k < nb) ? totals[k] : 0;
@@ -464,13 +486,30 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t *totals, uint64_t 
         __syncthreads();
         carry += chunk;
     }
-    if (threadIdx.x == 0) *grand_total = carry;
+    __syncthreads();
+    lds[threadIdx.x] = mine;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (threadIdx.x < (unsigned)s) lds[threadIdx.x] += lds[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        grand_total[0] = carry;
+        grand_total[1] = lds[0];
+    }
 }
 
+// Writes at most `room` children (the count reserved by the counting pass),
+// so a lane can never write outside its slot range.
 struct StoreEmit {
     const rpx_rays_t *out;
     uint64_t pos;
-    RPX_DEV void operator()(const Ray &r) { store_ray(*out, pos++, r); }
+    uint32_t room;
+    RPX_DEV void operator()(const Ray &r) {
+        if (room == 0) return;
+        store_ray(*out, pos++, r);
+        --room;
+    }
 };
 
 // Collects up to 3 children of one gausslet base ray.
@@ -504,7 +543,7 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
     V3 hp = add(r.origin, mul(r.direction, r.length));
     Orientation o = compute_orientation(S, fl, fc, hp);
     if (!GAUSS) {
-        StoreEmit se{&A.out, pos};
+        StoreEmit se{&A.out, pos, count};
         eval_children(m, r, (uint32_t)i, hp, o, A.wavelengths, se);
         return;
     }
@@ -513,7 +552,8 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
     CollectEmit ce;
     eval_children(m, r, (uint32_t)i, hp, o, A.wavelengths, ce);
     const uint64_t c = A.in.capacity, co = A.out.capacity;
-    for (int k = 0; k < ce.n; ++k) {
+    const int nk = min(min(ce.n, 3), (int)count);
+    for (int k = 0; k < nk; ++k) {
         Ray child = ce.r[k];
         child.length = A.max_length;
         store_ray(A.out, pos + k, child);
@@ -526,7 +566,7 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
         double plen = P[(b + 9) * c + i];
         V3 pp = add(po, mul(pd, plen));
         Orientation po_ = compute_orientation(S, fl, fc, pp);
-        for (int k = 0; k < ce.n; ++k) {
+        for (int k = 0; k < nk; ++k) {
             Para q = eval_parabasal(m, ce.r[k], pd, pp, po_, ce.r[k].ray_type_id, A.wavelengths);
             double *Q = A.out.para;
             const uint64_t oi = pos + k;
@@ -543,6 +583,45 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
         }
     }
 }
+
+// ------------------------------------------------------------- timing
+static std::mutex g_timing_mutex;
+static bool g_timing_on = false;
+static double g_timing_ms[3] = {0, 0, 0};
+static uint64_t g_timing_launches[3] = {0, 0, 0};
+static double g_timing_bytes[3] = {0, 0, 0};
+
+extern "C" rpx_status_t rpx_timing_enable(int32_t on) {
+    std::lock_guard<std::mutex> g(g_timing_mutex);
+    g_timing_on = on != 0;
+    return RPX_OK;
+}
+
+extern "C" rpx_status_t rpx_timing_read(double *ms, uint64_t *launches, double *bytes) {
+    std::lock_guard<std::mutex> g(g_timing_mutex);
+    for (int k = 0; k < 3; ++k) {
+        if (ms) ms[k] = g_timing_ms[k];
+        if (launches) launches[k] = g_timing_launches[k];
+        if (bytes) bytes[k] = g_timing_bytes[k];
+        g_timing_ms[k] = 0;
+        g_timing_launches[k] = 0;
+        g_timing_bytes[k] = 0;
+    }
+    return RPX_OK;
+}
+
+struct EventSet {
+    hipEvent_t e[5];
+    bool ok = false;
+    EventSet() {
+        ok = true;
+        for (auto &x : e)
+            if (hipEventCreate(&x) != hipSuccess) ok = false;
+    }
+    ~EventSet() {
+        for (auto &x : e) (void)hipEventDestroy(x);
+    }
+};
 
 // ------------------------------------------------------------- host side
 static rpx_status_t check_rays(const rpx_rays_t *r, uint64_t n, bool gauss, const char *what) {
@@ -587,30 +666,70 @@ static rpx_status_t run_generation(const rpx_scene_t *scene, int one_list, int o
     A.one_list = one_list;
     A.one_face = one_face;
     const uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
+    bool timing;
+    {
+        std::lock_guard<std::mutex> g(g_timing_mutex);
+        timing = g_timing_on;
+    }
+    static thread_local EventSet ev;
+    timing = timing && ev.ok;
+    if (timing) HIP_TRY(hipEventRecord(ev.e[0], stream));
     if (gauss)
         hipLaunchKernelGGL(k_intersect<true>, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
     else
         hipLaunchKernelGGL(k_intersect<false>, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, A.ws.block_totals, nb, A.ws.block_offsets,
-                       A.ws.total);
+    if (timing) HIP_TRY(hipEventRecord(ev.e[1], stream));
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, A.ws.block_totals, A.ws.block_parents, nb,
+                       A.ws.block_offsets, A.ws.total);
     HIP_TRY(hipGetLastError());
-    uint64_t total = 0;
-    HIP_TRY(hipMemcpyAsync(&total, A.ws.total, sizeof(total), hipMemcpyDeviceToHost, stream));
+    if (timing) HIP_TRY(hipEventRecord(ev.e[2], stream));
+    uint64_t totals[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(totals, A.ws.total, sizeof(totals), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
+    const uint64_t total = totals[0], parents_with_children = totals[1];
     *n_out = total;
+    float t_int = 0, t_scan = 0;
+    if (timing) {
+        HIP_TRY(hipEventElapsedTime(&t_int, ev.e[0], ev.e[1]));
+        HIP_TRY(hipEventElapsedTime(&t_scan, ev.e[1], ev.e[2]));
+        // algorithmic bytes: parent geometry read, length/end_face written
+        const double geom = gauss ? (48.0 + 6 * 48.0) : 48.0;
+        std::lock_guard<std::mutex> g(g_timing_mutex);
+        g_timing_ms[0] += t_int;
+        g_timing_ms[1] += t_scan;
+        g_timing_launches[0] += 1;
+        g_timing_launches[1] += 1;
+        g_timing_bytes[0] += (double)n_in * (geom + 12.0);
+        g_timing_bytes[1] += (double)nb * 16.0;
+    }
     if (total == 0) return RPX_OK;
     if (total > out->capacity)
         return set_error(RPX_E_CAPACITY, "output capacity " + std::to_string(out->capacity) + " below " +
                                              std::to_string(total) + " children");
     st = check_rays(out, total, gauss, "output rays");
     if (st) return st;
+    if (timing) HIP_TRY(hipEventRecord(ev.e[3], stream));
     if (gauss)
         hipLaunchKernelGGL(k_emit<true>, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
     else
         hipLaunchKernelGGL(k_emit<false>, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
     HIP_TRY(hipGetLastError());
+    if (timing) HIP_TRY(hipEventRecord(ev.e[4], stream));
     HIP_TRY(hipStreamSynchronize(stream));
+    if (timing) {
+        float t_emit = 0;
+        HIP_TRY(hipEventElapsedTime(&t_emit, ev.e[3], ev.e[4]));
+        // algorithmic bytes: the 100 non-geometry bytes of each parent with
+        // children (its geometry was counted in the intersect pass) plus every
+        // child record written
+        const double rest = gauss ? 100.0 + 6 * 80.0 : 100.0;
+        const double rec = gauss ? 188.0 + 6 * 80.0 : 188.0;
+        std::lock_guard<std::mutex> g(g_timing_mutex);
+        g_timing_ms[2] += t_emit;
+        g_timing_launches[2] += 1;
+        g_timing_bytes[2] += (double)total * rec + (double)parents_with_children * rest;
+    }
     return RPX_OK;
 }
 

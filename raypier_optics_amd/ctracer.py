@@ -822,7 +822,11 @@ class Face(object):
         self.count = 0
 
     def update(self):
-        """Pull the `params` attributes from the owner."""
+        """Pull the `params` attributes from the owner (ctracer.pyx:1761-1767).
+        A face without an owner keeps its own attributes (the reference
+        raises AttributeError there)."""
+        if self.owner is None:
+            return
         for name in self.params:
             setattr(self, name, getattr(self.owner, name))
 
