@@ -246,14 +246,14 @@ RPX_DEV double asph_grad(const Asph &A, double alpha) {
 }
 
 // ------------------------------------------------------------- intersect
-RPX_DEV double face_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V3 p2, int is_base);
-
 // Base-face intersection used by DistortionFace (no recursion into a second
 // distortion level: the host refuses nested distortions).
+template <uint32_t F>
 RPX_DEV double plain_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V3 p2, int is_base) {
     const double tol = f.tolerance;
     switch (f.type) {
-    case RPX_FACE_CIRCULAR: {  // cfaces.pyx:151-178
+    case RPX_FACE_CIRCULAR: {
+        if constexpr ((F & F_PLANE) == 0) return -1;  // cfaces.pyx:151-178
         double max_length = sep(p1, p2);
         double h = (f.p[2] - p1.z) / (p2.z - p1.z);
         double d = f.p[0];
@@ -263,7 +263,8 @@ RPX_DEV double plain_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V
         if (is_base && (X * X + Y * Y) > (d * d / 4)) return -1;
         return h * max_length;
     }
-    case RPX_FACE_SHAPED_PLANAR: {  // cfaces.pyx:201-226
+    case RPX_FACE_SHAPED_PLANAR: {
+        if constexpr ((F & F_PLANE) == 0) return -1;  // cfaces.pyx:201-226
         double max_length = sep(p1, p2);
         double h = (f.p[0] - p1.z) / (p2.z - p1.z);
         if ((h < tol) || (h > 1.0)) return -1;
@@ -272,7 +273,8 @@ RPX_DEV double plain_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V
         if (is_base && !shape_inside(S, f.shape, X, Y)) return -1;
         return h * max_length;
     }
-    case RPX_FACE_RECTANGULAR: {  // cfaces.pyx:365-397
+    case RPX_FACE_RECTANGULAR: {
+        if constexpr ((F & F_PLANE) == 0) return -1;  // cfaces.pyx:365-397
         double max_length = sep(p1, p2);
         double h = (f.p[3] - p1.z) / (p2.z - p1.z);
         double lngth = f.p[0], wdth = f.p[1];
@@ -285,7 +287,8 @@ RPX_DEV double plain_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V
         }
         return h * max_length;
     }
-    case RPX_FACE_POLYGON: {  // cfaces.pyx:1093-1108
+    case RPX_FACE_POLYGON: {
+        if constexpr ((F & F_PLANE) == 0) return -1;  // cfaces.pyx:1093-1108
         double max_length = sep(p1, p2);
         double h = (f.p[0] - p1.z) / (p2.z - p1.z);
         if ((h < tol) || (h > 1.0)) return -1;
@@ -295,7 +298,8 @@ RPX_DEV double plain_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V
         return -1;
     }
     case RPX_FACE_SPHERICAL:
-    case RPX_FACE_SHAPED_SPHERICAL: {  // cfaces.pyx:421-486 / 513-576
+    case RPX_FACE_SHAPED_SPHERICAL: {
+        if constexpr ((F & F_SPHERE) == 0) return -1;  // cfaces.pyx:421-486 / 513-576
         const bool shaped = f.type == RPX_FACE_SHAPED_SPHERICAL;
         const double curvature = shaped ? f.p[0] : f.p[1];
         const double z_height = shaped ? f.p[1] : f.p[2];
@@ -334,7 +338,8 @@ RPX_DEV double plain_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V
         if (a1 > 1.0 || a1 < tol) return -1;
         return a1 * sep(p1, p2);
     }
-    case RPX_FACE_EXTRUDED_PLANAR: {  // cfaces.pyx:665-699
+    case RPX_FACE_EXTRUDED_PLANAR: {
+        if constexpr ((F & F_PLANE) == 0) return -1;  // cfaces.pyx:665-699
         double ux = f.p[0], uy = f.p[1];
         double vx = f.p[2] - ux, vy = f.p[3] - uy;
         V3 s = sub(p2, p1);
@@ -353,7 +358,8 @@ RPX_DEV double plain_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V
         }
         return a * mag(s);
     }
-    case RPX_FACE_CONIC: {  // cfaces.pyx:1767-1798
+    case RPX_FACE_CONIC: {
+        if constexpr ((F & F_CONIC) == 0) return -1;  // cfaces.pyx:1767-1798
         V3 d = sub(p2, p1);
         V3 a = p1;
         a.z -= f.p[1];
@@ -363,7 +369,8 @@ RPX_DEV double plain_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V
         if (a1 > 1.0 || a1 < tol) return -1;
         return a1 * sep(p1, p2);
     }
-    case RPX_FACE_ASPHERIC: {  // cfaces.pyx:1909-1976
+    case RPX_FACE_ASPHERIC: {
+        if constexpr ((F & F_ASPHERE) == 0) return -1;  // cfaces.pyx:1909-1976
         double atol = f.p[10];
         double tl = atol * atol;
         if (is_base && f.p[12] >= 0.0) {
@@ -411,6 +418,7 @@ RPX_DEV double plain_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V
 }
 
 // Local-frame normal of a face (compute_normal_c of each class).
+template <uint32_t F>
 RPX_DEV V3 plain_normal(const SceneView &S, const rpx_face_t &f, V3 p) {
     switch (f.type) {
     case RPX_FACE_CIRCULAR:  // cfaces.pyx:180-191
@@ -421,7 +429,8 @@ RPX_DEV V3 plain_normal(const SceneView &S, const rpx_face_t &f, V3 p) {
     case RPX_FACE_POLYGON:  // cfaces.pyx:399-407, 1110-1118
         return v3(0, 0, -1);
     case RPX_FACE_SPHERICAL:
-    case RPX_FACE_SHAPED_SPHERICAL: {  // cfaces.pyx:488-498
+    case RPX_FACE_SHAPED_SPHERICAL: {
+        if constexpr ((F & F_SPHERE) == 0) return p;  // cfaces.pyx:488-498
         const bool shaped = f.type == RPX_FACE_SHAPED_SPHERICAL;
         const double curvature = shaped ? f.p[0] : f.p[1];
         const double z_height = shaped ? f.p[1] : f.p[2];
@@ -435,7 +444,8 @@ RPX_DEV V3 plain_normal(const SceneView &S, const rpx_face_t &f, V3 p) {
     }
     case RPX_FACE_EXTRUDED_PLANAR:  // cfaces.pyx:701-702
         return v3(f.p[6], f.p[7], f.p[8]);
-    case RPX_FACE_CONIC: {  // cfaces.pyx:1800-1823
+    case RPX_FACE_CONIC: {
+        if constexpr ((F & F_CONIC) == 0) return p;  // cfaces.pyx:1800-1823
         double R = -f.p[0];
         double beta = 1 + f.p[2];
         int sign = f.invert_normals ? -1 : 1;
@@ -450,7 +460,8 @@ RPX_DEV V3 plain_normal(const SceneView &S, const rpx_face_t &f, V3 p) {
         g.x *= sign;
         return norm(g);
     }
-    case RPX_FACE_ASPHERIC: {  // cfaces.pyx:1978-2007
+    case RPX_FACE_ASPHERIC: {
+        if constexpr ((F & F_ASPHERE) == 0) return p;  // cfaces.pyx:1978-2007
         double R = -f.p[0];
         double beta = 1 + f.p[2];
         int sign = f.invert_normals ? -1 : 1;
@@ -480,17 +491,18 @@ RPX_DEV V3 plain_normal(const SceneView &S, const rpx_face_t &f, V3 p) {
 }
 
 // DistortionFace.intersect_c (cfaces.pyx:2339-2416)
+template <uint32_t F>
 RPX_DEV double distortion_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V3 p2) {
     const rpx_face_t &face = S.faces[f.aux0];
     const int zi = f.aux1;
     double a1, a2, h = sep(p2, p1);
     double tolerance = f.p[0];
-    a2 = plain_intersect(S, face, p1, p2, 0);
+    a2 = plain_intersect<F>(S, face, p1, p2, 0);
     if (a2 > h || a2 < f.tolerance) return -1;
     V3 d = sub(p2, p1);
     V3 pt1 = add(p1, mul(d, a2 / h));
     V3 dxdyz = zernike_grad(S, zi, pt1.x, pt1.y);
-    V3 n = plain_normal(S, face, pt1);
+    V3 n = plain_normal<F>(S, face, pt1);
     pt1.z += dxdyz.z;
     n.x /= n.z;
     n.y /= n.z;
@@ -506,9 +518,9 @@ RPX_DEV double distortion_intersect(const SceneView &S, const rpx_face_t &f, V3 
         V3 q1 = p1, q2 = p2;
         q1.z -= z_shift;
         q2.z -= z_shift;
-        a2 = plain_intersect(S, face, q1, q2, 0);
+        a2 = plain_intersect<F>(S, face, q1, q2, 0);
         pt1 = add(q1, mul(d, a2 / h));
-        n = plain_normal(S, face, pt1);
+        n = plain_normal<F>(S, face, pt1);
         dxdyz = zernike_grad(S, zi, pt1.x, pt1.y);
         pt1.z += dxdyz.z;
         n.x /= n.z;
@@ -523,18 +535,25 @@ RPX_DEV double distortion_intersect(const SceneView &S, const rpx_face_t &f, V3 
     return a1;
 }
 
+template <uint32_t F>
 RPX_DEV double face_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V3 p2, int is_base) {
-    if (f.type == RPX_FACE_DISTORTION) return distortion_intersect(S, f, p1, p2);
-    return plain_intersect(S, f, p1, p2, is_base);
+    if constexpr ((F & F_DISTORT) != 0) {
+        if (f.type == RPX_FACE_DISTORTION) return distortion_intersect<F>(S, f, p1, p2);
+    }
+    return plain_intersect<F>(S, f, p1, p2, is_base);
 }
 
 // DistortionFace.compute_normal_c (cfaces.pyx:2418-2431)
+template <uint32_t F>
 RPX_DEV V3 face_normal(const SceneView &S, const rpx_face_t &f, V3 p) {
-    if (f.type != RPX_FACE_DISTORTION) return plain_normal(S, f, p);
+    if constexpr ((F & F_DISTORT) == 0) {
+        return plain_normal<F>(S, f, p);
+    }
+    if (f.type != RPX_FACE_DISTORTION) return plain_normal<F>(S, f, p);
     V3 dxdyz = zernike_grad(S, f.aux1, p.x, p.y);
     V3 p1 = p;
     p1.z -= dxdyz.z;
-    V3 n = plain_normal(S, S.faces[f.aux0], p1);
+    V3 n = plain_normal<F>(S, S.faces[f.aux0], p1);
     n.x /= n.z;
     n.y /= n.z;
     n.z = 1.0;
@@ -555,11 +574,12 @@ struct Orientation {
 };
 
 // FaceList.compute_orientation_c (ctracer.pyx:1939-1953)
+template <uint32_t F>
 RPX_DEV Orientation compute_orientation(const SceneView &S, const rpx_face_list_t &fl,
                                         const rpx_face_t &f, V3 point) {
     point = transform(fl.inv_trans, point);
     Orientation o;
-    o.normal = face_normal(S, f, point);
+    o.normal = face_normal<F>(S, f, point);
     o.tangent = face_tangent(f);
     if (f.invert_normal) {
         o.normal = neg(o.normal);

@@ -9,6 +9,7 @@ struct rpx_scene {
     void *device_block;
     int32_t n_global_faces;
     int32_t list_sizes[16];
+    uint32_t features;        // face / material families present (F_*, M_*)
     std::vector<rpx_shape_op_t> host_shape_ops;   // for validating query roots
 };
 

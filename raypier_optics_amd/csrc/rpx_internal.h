@@ -107,6 +107,24 @@ RPX_DEV Cx cexp_(Cx z) {
     return cx(e * c, e * s);
 }
 
+// ---------------------------------------------------------- feature masks
+// Face and material families; generation kernels are instantiated for a few
+// masks and a scene runs the smallest one that covers what it uses.
+enum : uint32_t {
+    F_PLANE = 1u << 0,      // circular, shaped planar, rectangular, polygon, extruded
+    F_SPHERE = 1u << 1,     // spherical, shaped spherical
+    F_CONIC = 1u << 2,
+    F_ASPHERE = 1u << 3,
+    F_DISTORT = 1u << 4,
+    M_SIMPLE = 1u << 5,     // none, opaque, transparent, PEC, partial, dielectric
+    M_FRESNEL = 1u << 6,    // full dielectric, full dispersive
+    M_COATED = 1u << 7,     // single layer, coated dispersive
+    M_GRATING = 1u << 8,
+    V_ALL = 0x1ffu,
+    V_LENS = F_PLANE | F_SPHERE | F_CONIC | F_ASPHERE | M_SIMPLE | M_FRESNEL | M_COATED,
+    V_SPHERES = F_PLANE | F_SPHERE | M_SIMPLE | M_FRESNEL | M_COATED
+};
+
 // ------------------------------------------------------------------- scene
 // Device view of an uploaded scene (pointers into one device allocation).
 struct SceneView {

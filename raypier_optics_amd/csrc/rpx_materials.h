@@ -162,11 +162,12 @@ RPX_DEV double power_frac(double nre, Cx a, Cx b, double P_in) {
 
 // ------------------------------------------------------------- children
 // InterfaceMaterial.eval_child_ray_c of every supported material.
-template <class Emit>
+template <uint32_t F, class Emit>
 RPX_DEV void eval_children(const rpx_material_t &m, const Ray &in, uint32_t idx, V3 point,
                            const Orientation &orient, const WlTable &wavelengths, Emit &emit) {
     switch (m.type) {
-    case RPX_MAT_TRANSPARENT: {  // cmaterials.pyx:260-278
+    case RPX_MAT_TRANSPARENT: {
+        if constexpr ((F & M_SIMPLE) == 0) return;  // cmaterials.pyx:260-278
         V3 normal = norm(orient.normal);
         Ray sp = convert_to_sp(in, normal);
         sp.accum += sp.length * sp.n.re;
@@ -177,7 +178,8 @@ RPX_DEV void eval_children(const rpx_material_t &m, const Ray &in, uint32_t idx,
         sp.ray_type_id &= ~RPX_REFL_RAY;
         emit(sp);
     } break;
-    case RPX_MAT_PEC: {  // cmaterials.pyx:285-319
+    case RPX_MAT_PEC: {
+        if constexpr ((F & M_SIMPLE) == 0) return;  // cmaterials.pyx:285-319
         V3 normal = norm(orient.normal);
         Ray sp = convert_to_sp(in, normal);
         double cosTheta = dot(normal, in.direction);
@@ -194,7 +196,8 @@ RPX_DEV void eval_children(const rpx_material_t &m, const Ray &in, uint32_t idx,
         sp.ray_type_id |= RPX_REFL_RAY;
         emit(sp);
     } break;
-    case RPX_MAT_PARTIAL: {  // cmaterials.pyx:345-397
+    case RPX_MAT_PARTIAL: {
+        if constexpr ((F & M_SIMPLE) == 0) return;  // cmaterials.pyx:345-397
         V3 normal = norm(orient.normal);
         V3 in_dir = norm(in.direction);
         Ray sp = convert_to_sp(in, normal);
@@ -224,7 +227,8 @@ RPX_DEV void eval_children(const rpx_material_t &m, const Ray &in, uint32_t idx,
         sp2.ray_type_id &= ~RPX_REFL_RAY;
         emit(sp2);
     } break;
-    case RPX_MAT_DIELECTRIC: {  // cmaterials.pyx:587-680
+    case RPX_MAT_DIELECTRIC: {
+        if constexpr ((F & M_SIMPLE) == 0) return;  // cmaterials.pyx:587-680
         V3 normal = norm(orient.normal);
         V3 in_dir = norm(in.direction);
         Ray sp = convert_to_sp(in, normal);
@@ -281,7 +285,8 @@ RPX_DEV void eval_children(const rpx_material_t &m, const Ray &in, uint32_t idx,
         emit(sp);
     } break;
     case RPX_MAT_FULL_DIELECTRIC:
-    case RPX_MAT_FULL_DISPERSIVE: {  // cmaterials.pyx:755-872, 896-1013
+    case RPX_MAT_FULL_DISPERSIVE: {
+        if constexpr ((F & M_FRESNEL) == 0) return;  // cmaterials.pyx:755-872, 896-1013
         Sides sd = material_indices(m, m.type == RPX_MAT_FULL_DISPERSIVE ? wavelengths[in.wl_idx] : 0.0);
         V3 normal = norm(orient.normal);
         V3 in_dir = norm(in.direction);
@@ -345,7 +350,8 @@ RPX_DEV void eval_children(const rpx_material_t &m, const Ray &in, uint32_t idx,
         }
     } break;
     case RPX_MAT_SINGLE_LAYER:
-    case RPX_MAT_COATED_DISPERSIVE: {  // cmaterials.pyx:1026-1182, 1228-1391
+    case RPX_MAT_COATED_DISPERSIVE: {
+        if constexpr ((F & M_COATED) == 0) return;  // cmaterials.pyx:1026-1182, 1228-1391
         const double wavelength = wavelengths[in.wl_idx];  // both read _wavelengths
         Sides sd = material_indices(m, wavelength);
         V3 normal = norm(orient.normal);
@@ -433,7 +439,8 @@ RPX_DEV void eval_children(const rpx_material_t &m, const Ray &in, uint32_t idx,
             emit(sp);
         }
     } break;
-    case RPX_MAT_GRATING: {  // cmaterials.pyx:1472-1542
+    case RPX_MAT_GRATING: {
+        if constexpr ((F & M_GRATING) == 0) return;  // cmaterials.pyx:1472-1542
         V3 normal = norm(orient.normal);
         V3 tangent = norm(orient.tangent);
         V3 tangent2 = cross(normal, tangent);
@@ -475,6 +482,7 @@ RPX_DEV void eval_children(const rpx_material_t &m, const Ray &in, uint32_t idx,
 // eval_parabasal_ray_c: InterfaceMaterial default (ctracer.pyx:1588-1610),
 // DielectricMaterial (cmaterials.pyx:683-724, inherited by FullDielectric and
 // SingleLayerCoated), CoatedDispersiveMaterial (:1393-1434), grating (:1544-1599).
+template <uint32_t F>
 RPX_DEV Para eval_parabasal(const rpx_material_t &m, const Ray &base, V3 direction, V3 point,
                             const Orientation &orient, uint32_t ray_type_id, const WlTable &wavelengths) {
     Para out;
@@ -484,6 +492,7 @@ RPX_DEV Para eval_parabasal(const rpx_material_t &m, const Ray &base, V3 directi
     case RPX_MAT_FULL_DIELECTRIC:
     case RPX_MAT_SINGLE_LAYER:
     case RPX_MAT_COATED_DISPERSIVE: {
+        if constexpr ((F & (M_SIMPLE | M_FRESNEL | M_COATED)) == 0) break;
         direction = norm(direction);
         double cosTheta = dot(normal, direction);
         V3 ctn = mul(normal, cosTheta);
@@ -511,6 +520,7 @@ RPX_DEV Para eval_parabasal(const rpx_material_t &m, const Ray &base, V3 directi
         }
     } break;
     case RPX_MAT_GRATING: {
+        if constexpr ((F & M_GRATING) == 0) break;
         V3 tangent = norm(orient.tangent);
         V3 tangent2 = cross(normal, tangent);
         double wavelen = wavelengths[base.wl_idx];

@@ -15,9 +15,9 @@ __global__ void k_query_face(SceneView S, int rec, int what, const double *p1, c
     V3 a = v3(p1[3 * i], p1[3 * i + 1], p1[3 * i + 2]);
     if (what == 0) {
         V3 b = v3(p2[3 * i], p2[3 * i + 1], p2[3 * i + 2]);
-        out[i] = face_intersect(S, f, a, b, is_base);
+        out[i] = face_intersect<V_ALL>(S, f, a, b, is_base);
     } else {
-        V3 nrm = face_normal(S, f, a);
+        V3 nrm = face_normal<V_ALL>(S, f, a);
         out[3 * i] = nrm.x;
         out[3 * i + 1] = nrm.y;
         out[3 * i + 2] = nrm.z;
@@ -63,7 +63,7 @@ __global__ void k_eval_child(SceneView S, int mat, rpx_rays_t in, uint64_t n, ui
     o.tangent = v3(tan_[3 * i], tan_[3 * i + 1], tan_[3 * i + 2]);
     V3 p = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
     SlotEmit se{&out, 3 * i};
-    eval_children(S.materials[mat], r, idx_base + (uint32_t)i, p, o, wl, se);
+    eval_children<V_ALL>(S.materials[mat], r, idx_base + (uint32_t)i, p, o, wl, se);
     counts[i] = (uint32_t)se.n;
 }
 

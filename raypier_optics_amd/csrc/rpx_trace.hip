@@ -237,6 +237,25 @@ extern "C" rpx_status_t rpx_scene_create(const rpx_scene_desc_t *d, rpx_scene_t 
         if (mc > maxc) maxc = mc;
     }
     v.max_children = maxc;
+    uint32_t feat = 0;
+    for (int i = 0; i < d->n_faces; ++i) {
+        switch (d->faces[i].type) {
+        case RPX_FACE_SPHERICAL: case RPX_FACE_SHAPED_SPHERICAL: feat |= F_SPHERE; break;
+        case RPX_FACE_CONIC: feat |= F_CONIC; break;
+        case RPX_FACE_ASPHERIC: feat |= F_ASPHERE; break;
+        case RPX_FACE_DISTORTION: feat |= F_DISTORT; break;
+        default: feat |= F_PLANE; break;
+        }
+    }
+    for (int i = 0; i < d->n_materials; ++i) {
+        switch (d->materials[i].type) {
+        case RPX_MAT_FULL_DIELECTRIC: case RPX_MAT_FULL_DISPERSIVE: feat |= M_FRESNEL; break;
+        case RPX_MAT_SINGLE_LAYER: case RPX_MAT_COATED_DISPERSIVE: feat |= M_COATED; break;
+        case RPX_MAT_GRATING: feat |= M_GRATING; break;
+        default: feat |= M_SIMPLE; break;
+        }
+    }
+    s->features = feat;
     s->n_global_faces = nglob;
     s->host_shape_ops.assign(d->shape_ops, d->shape_ops + d->n_shape_ops);
     for (int l = 0; l < d->n_face_lists; ++l) s->list_sizes[l] = d->face_lists[l].n_faces;
@@ -351,7 +370,7 @@ __device__ __forceinline__ uint32_t block_scan_small(uint32_t c, uint32_t *lds_w
     return base + pre;
 }
 
-template <bool GAUSS>
+template <bool GAUSS, uint32_t F>
 __global__ __launch_bounds__(RPX_BLOCK) void k_intersect(TraceArgs A) {
     __shared__ rpx_face_t lds_faces[RPX_LDS_FACES];
     __shared__ rpx_face_list_t lds_lists[RPX_LDS_LISTS];
@@ -380,7 +399,7 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_intersect(TraceArgs A) {
                 V3 p2 = transform(fl.inv_trans, point);
                 for (int k = fl.first_face; k < fl.first_face + fl.n_faces; ++k) {
                     const rpx_face_t &fc = S.faces[k];
-                    double dist = face_intersect(S, fc, p1, p2, 1);
+                    double dist = face_intersect<F>(S, fc, p1, p2, 1);
                     if (fc.tolerance < dist && dist < length) {
                         length = dist;
                         end_face = (uint32_t)fc.global_idx;
@@ -396,7 +415,7 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_intersect(TraceArgs A) {
             const rpx_face_t &fc = S.faces[k];
             V3 p1 = transform(fl.inv_trans, origin);
             V3 p2 = transform(fl.inv_trans, point);
-            double dist = face_intersect(S, fc, p1, p2, 1);
+            double dist = face_intersect<F>(S, fc, p1, p2, 1);
             if (fc.tolerance < dist && dist < length) {
                 length = dist;
                 end_face = (uint32_t)fc.global_idx;
@@ -423,7 +442,7 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_intersect(TraceArgs A) {
                     V3 pend = add(po, mul(pd, A.max_length));
                     V3 q1 = transform(fl.inv_trans, po);
                     V3 q2 = transform(fl.inv_trans, pend);
-                    double dist = face_intersect(S, fc, q1, q2, 0);
+                    double dist = face_intersect<F>(S, fc, q1, q2, 0);
                     if (fc.tolerance < dist && dist < plen) {
                         A.in.para[(b + 9) * c + i] = dist;
                     } else {
@@ -436,9 +455,9 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_intersect(TraceArgs A) {
                 r.length = length;
                 r.end_face_idx = end_face;
                 V3 hp = add(r.origin, mul(r.direction, length));
-                Orientation o = compute_orientation(S, fl, fc, hp);
+                Orientation o = compute_orientation<F>(S, fl, fc, hp);
                 CountEmit ce;
-                eval_children(S.materials[fc.material], r, (uint32_t)i, hp, o, A.wavelengths, ce);
+                eval_children<F>(S.materials[fc.material], r, (uint32_t)i, hp, o, A.wavelengths, ce);
                 count = (uint32_t)ce.n;
             }
             tag = count | ((uint32_t)(hit_rec + 1) << 8);
@@ -522,7 +541,7 @@ struct CollectEmit {
     }
 };
 
-template <bool GAUSS>
+template <bool GAUSS, uint32_t F>
 __global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
     __shared__ rpx_face_t lds_faces[RPX_LDS_FACES];
     __shared__ rpx_face_list_t lds_lists[RPX_LDS_LISTS];
@@ -541,16 +560,16 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
     const rpx_material_t &m = S.materials[fc.material];
     Ray r = load_ray(A.in, i);
     V3 hp = add(r.origin, mul(r.direction, r.length));
-    Orientation o = compute_orientation(S, fl, fc, hp);
+    Orientation o = compute_orientation<F>(S, fl, fc, hp);
     if (!GAUSS) {
         StoreEmit se{&A.out, pos, count};
-        eval_children(m, r, (uint32_t)i, hp, o, A.wavelengths, se);
+        eval_children<F>(m, r, (uint32_t)i, hp, o, A.wavelengths, se);
         return;
     }
     // Gausslets: children of the base ray, then 6 parabasal rays each
     // (ctracer.pyx:2375-2385), lengths reset to max_length (:2280).
     CollectEmit ce;
-    eval_children(m, r, (uint32_t)i, hp, o, A.wavelengths, ce);
+    eval_children<F>(m, r, (uint32_t)i, hp, o, A.wavelengths, ce);
     const uint64_t c = A.in.capacity, co = A.out.capacity;
     const int nk = min(min(ce.n, 3), (int)count);
     for (int k = 0; k < nk; ++k) {
@@ -565,9 +584,9 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
         V3 pd = v3(P[(b + 3) * c + i], P[(b + 4) * c + i], P[(b + 5) * c + i]);
         double plen = P[(b + 9) * c + i];
         V3 pp = add(po, mul(pd, plen));
-        Orientation po_ = compute_orientation(S, fl, fc, pp);
+        Orientation po_ = compute_orientation<F>(S, fl, fc, pp);
         for (int k = 0; k < nk; ++k) {
-            Para q = eval_parabasal(m, ce.r[k], pd, pp, po_, ce.r[k].ray_type_id, A.wavelengths);
+            Para q = eval_parabasal<F>(m, ce.r[k], pd, pp, po_, ce.r[k].ray_type_id, A.wavelengths);
             double *Q = A.out.para;
             const uint64_t oi = pos + k;
             Q[(b + 0) * co + oi] = q.origin.x;
@@ -582,6 +601,41 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
             Q[(b + 9) * co + oi] = A.max_length;
         }
     }
+}
+
+// ------------------------------------------------------------- variants
+static uint32_t scene_variant(const rpx_scene_t *s) {
+    if ((s->features & ~(uint32_t)V_SPHERES) == 0) return V_SPHERES;
+    if ((s->features & ~(uint32_t)V_LENS) == 0) return V_LENS;
+    return V_ALL;
+}
+
+template <uint32_t F>
+static void launch_intersect_f(bool gauss, dim3 grid, hipStream_t stream, const TraceArgs &A) {
+    if (gauss)
+        hipLaunchKernelGGL((k_intersect<true, F>), grid, dim3(RPX_BLOCK), 0, stream, A);
+    else
+        hipLaunchKernelGGL((k_intersect<false, F>), grid, dim3(RPX_BLOCK), 0, stream, A);
+}
+
+template <uint32_t F>
+static void launch_emit_f(bool gauss, dim3 grid, hipStream_t stream, const TraceArgs &A) {
+    if (gauss)
+        hipLaunchKernelGGL((k_emit<true, F>), grid, dim3(RPX_BLOCK), 0, stream, A);
+    else
+        hipLaunchKernelGGL((k_emit<false, F>), grid, dim3(RPX_BLOCK), 0, stream, A);
+}
+
+static void launch_intersect(bool gauss, uint32_t variant, dim3 grid, hipStream_t stream, const TraceArgs &A) {
+    if (variant == V_SPHERES) launch_intersect_f<V_SPHERES>(gauss, grid, stream, A);
+    else if (variant == V_LENS) launch_intersect_f<V_LENS>(gauss, grid, stream, A);
+    else launch_intersect_f<V_ALL>(gauss, grid, stream, A);
+}
+
+static void launch_emit(bool gauss, uint32_t variant, dim3 grid, hipStream_t stream, const TraceArgs &A) {
+    if (variant == V_SPHERES) launch_emit_f<V_SPHERES>(gauss, grid, stream, A);
+    else if (variant == V_LENS) launch_emit_f<V_LENS>(gauss, grid, stream, A);
+    else launch_emit_f<V_ALL>(gauss, grid, stream, A);
 }
 
 // ------------------------------------------------------------- timing
@@ -674,10 +728,8 @@ static rpx_status_t run_generation(const rpx_scene_t *scene, int one_list, int o
     static thread_local EventSet ev;
     timing = timing && ev.ok;
     if (timing) HIP_TRY(hipEventRecord(ev.e[0], stream));
-    if (gauss)
-        hipLaunchKernelGGL(k_intersect<true>, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
-    else
-        hipLaunchKernelGGL(k_intersect<false>, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
+    const uint32_t variant = scene_variant(scene);
+    launch_intersect(gauss, variant, dim3((unsigned)nb), stream, A);
     HIP_TRY(hipGetLastError());
     if (timing) HIP_TRY(hipEventRecord(ev.e[1], stream));
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, A.ws.block_totals, A.ws.block_parents, nb,
@@ -710,10 +762,7 @@ static rpx_status_t run_generation(const rpx_scene_t *scene, int one_list, int o
     st = check_rays(out, total, gauss, "output rays");
     if (st) return st;
     if (timing) HIP_TRY(hipEventRecord(ev.e[3], stream));
-    if (gauss)
-        hipLaunchKernelGGL(k_emit<true>, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
-    else
-        hipLaunchKernelGGL(k_emit<false>, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
+    launch_emit(gauss, variant, dim3((unsigned)nb), stream, A);
     HIP_TRY(hipGetLastError());
     if (timing) HIP_TRY(hipEventRecord(ev.e[4], stream));
     HIP_TRY(hipStreamSynchronize(stream));
