@@ -29,6 +29,21 @@ HBM_PEAK_GBS = 8000.0
 KERNELS = ("k_intersect", "k_scan", "k_emit")
 
 
+def measured_traffic(kernel, rays):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE profile of this workload (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_asphere_doublet_c2.json")
+    try:
+        with open(path) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = prof.get("kernels", {}).get(kernel)
+    if prof.get("rays") != rays or k is None:
+        return None
+    return k["traffic_bytes_per_launch"]
+
+
 def interactions(traced):
     return int(sum(int(np.count_nonzero(g.end_face_idx != 0xFFFFFFFF)) for g in traced))
 
@@ -50,6 +65,17 @@ def cpu_baseline(w, sample):
                       (sample, n_int, dt)}
 
 
+def copy_calibration(torch, mib):
+    """Three plain copies of `mib` MiB: a kernel of known traffic (reads and
+    writes mib MiB each) for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +84,8 @@ def main():
     ap.add_argument("--rays", type=int, default=4194304)
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run device-to-device copies of this size first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding)",
@@ -69,6 +97,8 @@ def main():
     from raypier_optics_amd.tracer import trace_rays
     lib = _lib.load()
     torch.cuda.set_device(0)
+    if args.copy_calibration_mib:
+        copy_calibration(torch, args.copy_calibration_mib)
 
     w = workloads.asphere_doublet_c2(n_rays=args.rays)
     rays = w.collection()
@@ -128,7 +158,8 @@ def main():
                    "rays": args.rays, "interactions_per_step": n_int, "generations": w.recursion_limit,
                    "per_ray_wavelength_um": [0.45, 0.65]},
         "roofline": {"bound": "hbm", "kernel": KERNELS[k], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(KERNELS[k], args.rays),
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
                      "kernel_time_share": kernel_share},
         "fp32split": None,
