@@ -26,7 +26,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ray-surface interactions/s on 1 GPU (FP64 vs FP32-split); p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ("k_intersect", "k_scan", "k_emit")
 
 
 def measured_traffic(kernel, rays):
@@ -93,7 +92,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from raypier_optics_amd import workloads, _lib
+    from raypier_optics_amd import workloads, _lib, _abi
     from raypier_optics_amd.tracer import trace_rays
     lib = _lib.load()
     torch.cuda.set_device(0)
@@ -123,9 +122,11 @@ def main():
         del traced
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    ms = (ctypes.c_double * 3)()
-    launches = (ctypes.c_uint64 * 3)()
-    nbytes = (ctypes.c_double * 3)()
+    KERNELS = _abi.TIMING_KERNELS
+    nk = len(KERNELS)
+    ms = (ctypes.c_double * nk)()
+    launches = (ctypes.c_uint64 * nk)()
+    nbytes = (ctypes.c_double * nk)()
     lib.rpx_timing_read(ms, launches, nbytes)
     lib.rpx_timing_enable(0)
     if n_int is None:
@@ -134,11 +135,11 @@ def main():
 
     ms_per_step = 1000.0 * dt / args.steps
     value = n_int * args.steps / dt
-    k = int(np.argmax([ms[i] for i in range(3)]))
+    k = int(np.argmax([ms[i] for i in range(nk)]))
     avg_ms = ms[k] / max(launches[k], 1)
     bytes_per_launch = nbytes[k] / max(launches[k], 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    kernel_share = {KERNELS[i]: round(ms[i] / max(sum(ms), 1e-9), 4) for i in range(3)}
+    kernel_share = {KERNELS[i]: round(ms[i] / max(sum(ms), 1e-9), 4) for i in range(nk) if launches[i]}
 
     out = {
         "metric": METRIC,

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RPX_ABI_VERSION 1
+#define RPX_ABI_VERSION 2
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t rpx_status_t;
@@ -321,11 +321,20 @@ rpx_status_t rpx_trace_gausslet(const rpx_scene_t *scene,
 
 /* Kernel timing with HIP events recorded on the stream each generation runs
  * on (no counterpart in the reference: measurement support).  While enabled,
- * every trace call adds the duration of its three kernels to the running
- * totals: ms[0..2] = intersect/count, block scan, emit; launches[0..2] the
- * number of launches.  rpx_timing_read also reports bytes moved by each
- * kernel's algorithm (read + written, per the ray record sizes) in bytes[0..2]
- * and resets the totals. */
+ * every trace call adds the duration of its kernels to running totals, one
+ * slot per kernel (below).  rpx_timing_read copies RPX_TIMING_SLOTS entries
+ * of: ms = total duration, launches = number of launches, bytes = bytes the
+ * kernel's algorithm reads + writes (per the ray record sizes); then resets
+ * the totals.  Ray generations run k_clear + k_generation; gausslet
+ * generations run k_intersect + k_scan + k_emit. */
+enum {
+    RPX_TIMING_INTERSECT = 0,
+    RPX_TIMING_SCAN = 1,
+    RPX_TIMING_EMIT = 2,
+    RPX_TIMING_GENERATION = 3,
+    RPX_TIMING_CLEAR = 4,
+    RPX_TIMING_SLOTS = 5
+};
 rpx_status_t rpx_timing_enable(int32_t on);
 rpx_status_t rpx_timing_read(double *ms, uint64_t *launches, double *bytes);
 

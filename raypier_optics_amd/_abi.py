@@ -33,6 +33,11 @@ gausslet_dtype = np.dtype([('base_ray', ray_dtype),
 
 assert ray_dtype.itemsize == 188 and para_dtype.itemsize == 80 and gausslet_dtype.itemsize == 668
 
+ABI_VERSION = 2          # RPX_ABI_VERSION of include/rpx.h
+
+# timing slots (RPX_TIMING_*)
+TIMING_KERNELS = ("k_intersect", "k_scan", "k_emit", "k_generation", "k_clear")
+
 REFL_RAY = 1 << 0
 GAUSSLET = 1 << 1
 PARABASAL = 1 << 2

@@ -72,7 +72,7 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.rpx_abi_version() != 1:
+    if lib.rpx_abi_version() != _abi.ABI_VERSION:
         raise OSError("librpx ABI version mismatch")
     for name, size in _abi.STRUCT_SIZES.items():
         got = lib.rpx_struct_size(name.encode())

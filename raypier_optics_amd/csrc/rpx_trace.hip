@@ -274,19 +274,31 @@ extern "C" rpx_status_t rpx_scene_destroy(rpx_scene_t *s) {
 extern "C" int32_t rpx_scene_max_children(const rpx_scene_t *s) { return s ? s->view.max_children : -1; }
 
 // ------------------------------------------------------------ workspace
+// Generation control words (device), cleared by k_clear before each
+// single-pass generation and read back by the host afterwards.
+struct GenControl {
+    uint64_t children;       // children written (grand total)
+    uint64_t parents;        // rays with at least one child
+    uint64_t hits;           // rays whose nearest-face search found a face
+    uint64_t error;          // nonzero: a look-back wait ran out of polls
+    uint32_t tile_counter;   // next tile id handed to a starting workgroup
+    uint32_t pad[7];
+};
+
 struct Workspace {
     uint32_t *tags;          // per ray: child count (bits 0..7) | (face record + 1) << 8
     uint32_t *block_totals;  // per block: children
     uint32_t *block_parents; // per block: rays with at least one child
     uint64_t *block_offsets; // per block, exclusive
-    uint64_t *total;         // [0] children, [1] parents with children
+    uint64_t *tile_status;   // per tile: look-back flag (bits 62..63) | value
+    GenControl *ctl;
 };
 
 static uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 
 extern "C" uint64_t rpx_trace_workspace_bytes(uint64_t n_in) {
     uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
-    return align256(4 * n_in) + 2 * align256(4 * nb) + align256(8 * nb) + 256;
+    return align256(4 * n_in) + 2 * align256(4 * nb) + 2 * align256(8 * nb) + align256(sizeof(GenControl));
 }
 
 static Workspace carve(void *ws, uint64_t n_in) {
@@ -301,7 +313,9 @@ static Workspace carve(void *ws, uint64_t n_in) {
     p += align256(4 * nb);
     w.block_offsets = (uint64_t *)p;
     p += align256(8 * nb);
-    w.total = (uint64_t *)p;
+    w.tile_status = (uint64_t *)p;
+    p += align256(8 * nb);
+    w.ctl = (GenControl *)p;
     return w;
 }
 
@@ -484,7 +498,7 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_intersect(TraceArgs A) {
 
 // Exclusive scan of block totals in a single workgroup of 1024 lanes.
 __global__ __launch_bounds__(1024) void k_scan(const uint32_t *totals, const uint32_t *parents, uint64_t nb,
-                                               uint64_t *offsets, uint64_t *grand_total) {
+                                               uint64_t *offsets, GenControl *ctl) {
     __shared__ uint64_t lds[1024];
     uint64_t carry = 0;
     uint64_t mine = 0;
@@ -513,8 +527,8 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t *totals, const uin
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        grand_total[0] = carry;
-        grand_total[1] = lds[0];
+        ctl->children = carry;
+        ctl->parents = lds[0];
     }
 }
 
@@ -603,6 +617,200 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
     }
 }
 
+// --------------------------------------------------- single-pass generation
+// One launch per ray generation: intersect, evaluate the material once, and
+// place the children with a decoupled look-back scan over tiles of RPX_BLOCK
+// rays.  Tiles are handed out by an atomic counter in launch order, so every
+// tile a workgroup waits for belongs to a workgroup that is already running;
+// the wait is also bounded by a poll limit (error flag instead of a hang).
+
+#define RPX_LOOKBACK_POLLS (1u << 22)
+static constexpr uint64_t ST_AGG = 1ull << 62;     // tile aggregate published
+static constexpr uint64_t ST_PREFIX = 2ull << 62;  // inclusive prefix published
+static constexpr uint64_t ST_VALUE = (1ull << 62) - 1;
+
+__global__ __launch_bounds__(256) void k_clear(uint64_t *tile_status, uint64_t nb, GenControl *ctl) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nb) tile_status[k] = 0;
+    if (k == 0) {
+        ctl->children = 0;
+        ctl->parents = 0;
+        ctl->hits = 0;
+        ctl->error = 0;
+        ctl->tile_counter = 0;
+    }
+}
+
+__device__ __forceinline__ uint64_t status_load(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Run by one whole wavefront.  Publishes the tile's aggregate, sums the
+// predecessors' aggregates 64 tiles at a time until an inclusive prefix is
+// found, publishes this tile's inclusive prefix and returns the exclusive one.
+__device__ uint64_t lookback(uint64_t *status, uint32_t tile, uint64_t agg, GenControl *ctl) {
+    const int lane = threadIdx.x & 63;
+    if (tile == 0) {
+        if (lane == 0) status_store(&status[0], ST_PREFIX | agg);
+        return 0;
+    }
+    if (lane == 0) status_store(&status[tile], ST_AGG | agg);
+    uint64_t excl = 0;
+    int64_t end = (int64_t)tile - 1;
+    uint32_t polls = 0;
+    for (;;) {
+        const int64_t idx = end - lane;
+        const uint64_t st = (idx >= 0) ? status_load(&status[idx]) : ST_PREFIX;
+        const uint64_t flag = st >> 62;
+        const uint64_t pm = __ballot(flag == 2);
+        const int first = pm ? (__ffsll((unsigned long long)pm) - 1) : 64;
+        const uint64_t window = (first >= 63) ? ~0ull : ((2ull << first) - 1);
+        if (__ballot(flag == 0) & window) {
+            if (++polls > RPX_LOOKBACK_POLLS) {
+                if (lane == 0) atomicAdd((unsigned long long *)&ctl->error, 1ull);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        uint64_t v = (lane <= first) ? (st & ST_VALUE) : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        excl += v;
+        if (first < 64) break;
+        end -= 64;
+    }
+    if (lane == 0) status_store(&status[tile], ST_PREFIX | (excl + agg));
+    return excl;
+}
+
+// Collects the (at most two) children of one ray as a two-slot shift
+// register: r0 is the latest child, r1 the one before.  Conditional or
+// indexed slot writes make the compiler keep the pair in scratch memory;
+// the unconditional shift keeps both in registers.
+struct PairEmit {
+    Ray r0, r1;
+    int n = 0;
+    RPX_DEV void operator()(const Ray &x) {
+        r1 = r0;
+        r0 = x;
+        ++n;
+    }
+    RPX_DEV const Ray &first() const { return n > 1 ? r1 : r0; }
+};
+
+template <uint32_t F>
+__global__ __launch_bounds__(RPX_BLOCK) void k_generation(TraceArgs A) {
+    __shared__ rpx_face_t lds_faces[RPX_LDS_FACES];
+    __shared__ rpx_face_list_t lds_lists[RPX_LDS_LISTS];
+    __shared__ uint32_t lds_hist[RPX_LDS_FACES];
+    __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
+    __shared__ uint32_t lds_tile, lds_parents, lds_hits;
+    __shared__ uint64_t lds_excl;
+    const SceneView S = stage_scene(A.S, lds_lists, lds_faces);
+    for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
+    if (threadIdx.x == 0) {
+        lds_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
+        lds_parents = 0;
+        lds_hits = 0;
+    }
+    __syncthreads();
+    const uint32_t tile = lds_tile;
+    const uint64_t i = (uint64_t)tile * RPX_BLOCK + threadIdx.x;
+    PairEmit ce;
+    bool hit = false;
+    if (i < A.n_in) {
+        const double *f = A.in.f64;
+        const uint64_t c = A.in.capacity;
+        V3 origin = v3(f[RPX_OX * c + i], f[RPX_OY * c + i], f[RPX_OZ * c + i]);
+        V3 dir = v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]);
+        double length = A.max_length;
+        uint32_t end_face = RPX_NO_FACE;
+        int hit_rec = -1, hit_list = -1;
+        V3 point = add(origin, mul(dir, A.max_length));
+        if (A.one_list < 0) {
+            // trace_segment_c / FaceList.intersect_c (ctracer.pyx:2093-2104, 1882-1904)
+            for (int l = 0; l < S.n_face_lists; ++l) {
+                const rpx_face_list_t &fl = S.face_lists[l];
+                V3 p1 = transform(fl.inv_trans, origin);
+                V3 p2 = transform(fl.inv_trans, point);
+                for (int k = fl.first_face; k < fl.first_face + fl.n_faces; ++k) {
+                    const rpx_face_t &fc = S.faces[k];
+                    double dist = face_intersect<F>(S, fc, p1, p2, 1);
+                    if (fc.tolerance < dist && dist < length) {
+                        length = dist;
+                        end_face = (uint32_t)fc.global_idx;
+                        hit_rec = k;
+                        hit_list = l;
+                    }
+                }
+            }
+        } else {
+            // FaceList.intersect_one_face_c (ctracer.pyx:1861-1879)
+            const rpx_face_list_t &fl = S.face_lists[A.one_list];
+            const int k = fl.first_face + A.one_face;
+            const rpx_face_t &fc = S.faces[k];
+            V3 p1 = transform(fl.inv_trans, origin);
+            V3 p2 = transform(fl.inv_trans, point);
+            double dist = face_intersect<F>(S, fc, p1, p2, 1);
+            if (fc.tolerance < dist && dist < length) {
+                length = dist;
+                end_face = (uint32_t)fc.global_idx;
+                hit_rec = k;
+                hit_list = A.one_list;
+            }
+        }
+        A.in.f64[RPX_LENGTH * c + i] = length;
+        A.in.u32[RPX_END_FACE_IDX * c + i] = end_face;
+        if (hit_rec >= 0) {
+            hit = true;
+            atomicAdd(&lds_hist[hit_rec], 1u);
+            const rpx_face_t &fc = S.faces[hit_rec];
+            if (fc.material >= 0) {
+                const rpx_face_list_t &fl = S.face_lists[hit_list];
+                Ray r = load_ray(A.in, i);
+                r.length = length;
+                r.end_face_idx = end_face;
+                V3 hp = add(r.origin, mul(r.direction, length));
+                Orientation o = compute_orientation<F>(S, fl, fc, hp);
+                eval_children<F>(S.materials[fc.material], r, (uint32_t)i, hp, o, A.wavelengths, ce);
+            }
+        }
+    }
+    const uint32_t count = (uint32_t)min(ce.n, 2);
+    uint32_t block_total;
+    const uint32_t local = block_scan_small(count, lds_wave, &block_total);
+    {
+        const uint64_t mp = __ballot(count != 0), mh = __ballot(hit);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&lds_parents, (uint32_t)__popcll(mp));
+            atomicAdd(&lds_hits, (uint32_t)__popcll(mh));
+        }
+    }
+    if (threadIdx.x < 64) {
+        const uint64_t excl = lookback(A.ws.tile_status, tile, block_total, A.ws.ctl);
+        if (threadIdx.x == 0) lds_excl = excl;
+    }
+    __syncthreads();
+    const uint64_t pos = lds_excl + local;
+    if (count > 0 && pos < A.out.capacity) store_ray(A.out, pos, ce.first());
+    if (count > 1 && pos + 1 < A.out.capacity) store_ray(A.out, pos + 1, ce.r0);
+    if (threadIdx.x == 0) {
+        if (lds_parents) atomicAdd((unsigned long long *)&A.ws.ctl->parents, (unsigned long long)lds_parents);
+        if (lds_hits) atomicAdd((unsigned long long *)&A.ws.ctl->hits, (unsigned long long)lds_hits);
+        if ((uint64_t)tile == (A.n_in - 1) / RPX_BLOCK) A.ws.ctl->children = lds_excl + block_total;
+    }
+    if (A.face_hits) {
+        for (int k = threadIdx.x; k < S.n_list_faces; k += blockDim.x) {
+            uint32_t h = lds_hist[k];
+            if (h) atomicAdd(&A.face_hits[S.faces[k].global_idx], h);
+        }
+    }
+}
+
 // ------------------------------------------------------------- variants
 static uint32_t scene_variant(const rpx_scene_t *s) {
     if ((s->features & ~(uint32_t)V_SPHERES) == 0) return V_SPHERES;
@@ -632,6 +840,12 @@ static void launch_intersect(bool gauss, uint32_t variant, dim3 grid, hipStream_
     else launch_intersect_f<V_ALL>(gauss, grid, stream, A);
 }
 
+static void launch_generation(uint32_t variant, dim3 grid, hipStream_t stream, const TraceArgs &A) {
+    if (variant == V_SPHERES) hipLaunchKernelGGL((k_generation<V_SPHERES>), grid, dim3(RPX_BLOCK), 0, stream, A);
+    else if (variant == V_LENS) hipLaunchKernelGGL((k_generation<V_LENS>), grid, dim3(RPX_BLOCK), 0, stream, A);
+    else hipLaunchKernelGGL((k_generation<V_ALL>), grid, dim3(RPX_BLOCK), 0, stream, A);
+}
+
 static void launch_emit(bool gauss, uint32_t variant, dim3 grid, hipStream_t stream, const TraceArgs &A) {
     if (variant == V_SPHERES) launch_emit_f<V_SPHERES>(gauss, grid, stream, A);
     else if (variant == V_LENS) launch_emit_f<V_LENS>(gauss, grid, stream, A);
@@ -641,9 +855,9 @@ static void launch_emit(bool gauss, uint32_t variant, dim3 grid, hipStream_t str
 // ------------------------------------------------------------- timing
 static std::mutex g_timing_mutex;
 static bool g_timing_on = false;
-static double g_timing_ms[3] = {0, 0, 0};
-static uint64_t g_timing_launches[3] = {0, 0, 0};
-static double g_timing_bytes[3] = {0, 0, 0};
+static double g_timing_ms[RPX_TIMING_SLOTS] = {};
+static uint64_t g_timing_launches[RPX_TIMING_SLOTS] = {};
+static double g_timing_bytes[RPX_TIMING_SLOTS] = {};
 
 extern "C" rpx_status_t rpx_timing_enable(int32_t on) {
     std::lock_guard<std::mutex> g(g_timing_mutex);
@@ -653,7 +867,7 @@ extern "C" rpx_status_t rpx_timing_enable(int32_t on) {
 
 extern "C" rpx_status_t rpx_timing_read(double *ms, uint64_t *launches, double *bytes) {
     std::lock_guard<std::mutex> g(g_timing_mutex);
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < RPX_TIMING_SLOTS; ++k) {
         if (ms) ms[k] = g_timing_ms[k];
         if (launches) launches[k] = g_timing_launches[k];
         if (bytes) bytes[k] = g_timing_bytes[k];
@@ -727,17 +941,54 @@ static rpx_status_t run_generation(const rpx_scene_t *scene, int one_list, int o
     }
     static thread_local EventSet ev;
     timing = timing && ev.ok;
-    if (timing) HIP_TRY(hipEventRecord(ev.e[0], stream));
     const uint32_t variant = scene_variant(scene);
+    if (!gauss) {
+        if (out->capacity > 0 && (!out->f64 || !out->u32))
+            return set_error(RPX_E_INVALID, "output rays have null fields");
+        if (timing) HIP_TRY(hipEventRecord(ev.e[0], stream));
+        hipLaunchKernelGGL(k_clear, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, stream, A.ws.tile_status, nb,
+                           A.ws.ctl);
+        HIP_TRY(hipGetLastError());
+        if (timing) HIP_TRY(hipEventRecord(ev.e[1], stream));
+        launch_generation(variant, dim3((unsigned)nb), stream, A);
+        HIP_TRY(hipGetLastError());
+        if (timing) HIP_TRY(hipEventRecord(ev.e[2], stream));
+        GenControl ctl;
+        HIP_TRY(hipMemcpyAsync(&ctl, A.ws.ctl, sizeof(ctl), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (ctl.error) return set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
+        *n_out = ctl.children;
+        if (timing) {
+            float t_clear = 0, t_gen = 0;
+            HIP_TRY(hipEventElapsedTime(&t_clear, ev.e[0], ev.e[1]));
+            HIP_TRY(hipEventElapsedTime(&t_gen, ev.e[1], ev.e[2]));
+            std::lock_guard<std::mutex> g(g_timing_mutex);
+            g_timing_ms[RPX_TIMING_CLEAR] += t_clear;
+            g_timing_launches[RPX_TIMING_CLEAR] += 1;
+            g_timing_bytes[RPX_TIMING_CLEAR] += (double)nb * 8.0;
+            // algorithmic bytes: geometry of every input ray, its length and
+            // end face written back, the other 100 bytes of each ray that hit,
+            // and every child record
+            g_timing_ms[RPX_TIMING_GENERATION] += t_gen;
+            g_timing_launches[RPX_TIMING_GENERATION] += 1;
+            g_timing_bytes[RPX_TIMING_GENERATION] +=
+                (double)n_in * (48.0 + 12.0) + (double)ctl.hits * 100.0 + (double)ctl.children * 188.0;
+        }
+        if (ctl.children > out->capacity)
+            return set_error(RPX_E_CAPACITY, "output capacity " + std::to_string(out->capacity) + " below " +
+                                                 std::to_string(ctl.children) + " children");
+        return RPX_OK;
+    }
+    if (timing) HIP_TRY(hipEventRecord(ev.e[0], stream));
     launch_intersect(gauss, variant, dim3((unsigned)nb), stream, A);
     HIP_TRY(hipGetLastError());
     if (timing) HIP_TRY(hipEventRecord(ev.e[1], stream));
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, A.ws.block_totals, A.ws.block_parents, nb,
-                       A.ws.block_offsets, A.ws.total);
+                       A.ws.block_offsets, A.ws.ctl);
     HIP_TRY(hipGetLastError());
     if (timing) HIP_TRY(hipEventRecord(ev.e[2], stream));
     uint64_t totals[2] = {0, 0};
-    HIP_TRY(hipMemcpyAsync(totals, A.ws.total, sizeof(totals), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(totals, A.ws.ctl, sizeof(totals), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     const uint64_t total = totals[0], parents_with_children = totals[1];
     *n_out = total;
@@ -748,12 +999,12 @@ static rpx_status_t run_generation(const rpx_scene_t *scene, int one_list, int o
         // algorithmic bytes: parent geometry read, length/end_face written
         const double geom = gauss ? (48.0 + 6 * 48.0) : 48.0;
         std::lock_guard<std::mutex> g(g_timing_mutex);
-        g_timing_ms[0] += t_int;
-        g_timing_ms[1] += t_scan;
-        g_timing_launches[0] += 1;
-        g_timing_launches[1] += 1;
-        g_timing_bytes[0] += (double)n_in * (geom + 12.0);
-        g_timing_bytes[1] += (double)nb * 16.0;
+        g_timing_ms[RPX_TIMING_INTERSECT] += t_int;
+        g_timing_ms[RPX_TIMING_SCAN] += t_scan;
+        g_timing_launches[RPX_TIMING_INTERSECT] += 1;
+        g_timing_launches[RPX_TIMING_SCAN] += 1;
+        g_timing_bytes[RPX_TIMING_INTERSECT] += (double)n_in * (geom + 12.0);
+        g_timing_bytes[RPX_TIMING_SCAN] += (double)nb * 16.0;
     }
     if (total == 0) return RPX_OK;
     if (total > out->capacity)
@@ -775,9 +1026,9 @@ static rpx_status_t run_generation(const rpx_scene_t *scene, int one_list, int o
         const double rest = gauss ? 100.0 + 6 * 80.0 : 100.0;
         const double rec = gauss ? 188.0 + 6 * 80.0 : 188.0;
         std::lock_guard<std::mutex> g(g_timing_mutex);
-        g_timing_ms[2] += t_emit;
-        g_timing_launches[2] += 1;
-        g_timing_bytes[2] += (double)total * rec + (double)parents_with_children * rest;
+        g_timing_ms[RPX_TIMING_EMIT] += t_emit;
+        g_timing_launches[RPX_TIMING_EMIT] += 1;
+        g_timing_bytes[RPX_TIMING_EMIT] += (double)total * rec + (double)parents_with_children * rest;
     }
     return RPX_OK;
 }

@@ -33,7 +33,7 @@ def test_struct_sizes_match_bindings():
     lib = _lib.load()   # load() itself cross-checks every struct size
     for name, size in _abi.STRUCT_SIZES.items():
         assert lib.rpx_struct_size(name.encode()) == size
-    assert lib.rpx_abi_version() == 1
+    assert lib.rpx_abi_version() == _abi.ABI_VERSION
 
 
 def test_ray_record_layouts():
