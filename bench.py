@@ -44,7 +44,15 @@ def measured_traffic(kernel, rays):
 
 
 def interactions(traced):
-    return int(sum(int(np.count_nonzero(g.end_face_idx != 0xFFFFFFFF)) for g in traced))
+    """Rays of all generations whose nearest-face search hit a face (counted
+    on the device)."""
+    from raypier_optics_amd import metrics
+    total = 0
+    for g in traced:
+        if g.n_rays:
+            end = metrics._fields(g)[3]
+            total += int((end != 0xFFFFFFFF).sum())
+    return total
 
 
 def cpu_baseline(w, sample):
@@ -75,6 +83,56 @@ def copy_calibration(torch, mib):
     del src, dst
 
 
+def timed(torch, lib, run, steps, warmup):
+    """Warm up, then time `steps` calls of run() with the library's kernel
+    timing on; returns wall seconds, interactions per call and per-kernel
+    event totals."""
+    from raypier_optics_amd import _abi
+    n_int = None
+    for _ in range(warmup):
+        traced = run()
+        if n_int is None:
+            n_int = interactions(traced)
+        del traced
+    torch.cuda.synchronize()
+    lib.rpx_timing_enable(1)
+    lib.rpx_timing_read(None, None, None)   # reset totals
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        traced = run()
+        del traced
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    nk = len(_abi.TIMING_KERNELS)
+    ms = (ctypes.c_double * nk)()
+    launches = (ctypes.c_uint64 * nk)()
+    nbytes = (ctypes.c_double * nk)()
+    lib.rpx_timing_read(ms, launches, nbytes)
+    lib.rpx_timing_enable(0)
+    if n_int is None:
+        n_int = interactions(run())
+    return {"seconds": dt, "interactions": n_int, "ms": list(ms), "launches": list(launches),
+            "bytes": list(nbytes)}
+
+
+def roofline(r, measured):
+    """Roofline object for the kernel with the largest share of event time."""
+    from raypier_optics_amd import _abi
+    names = _abi.TIMING_KERNELS
+    ms, launches, nbytes = r["ms"], r["launches"], r["bytes"]
+    k = int(np.argmax(ms))
+    avg_ms = ms[k] / max(launches[k], 1)
+    per_launch = nbytes[k] / max(launches[k], 1)
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": names[k], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(names[k], measured) if measured else None,
+            "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": per_launch,
+            "kernel_time_share": {names[i]: round(ms[i] / max(sum(ms), 1e-9), 4)
+                                  for i in range(len(names)) if launches[i]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,8 +150,9 @@ def main():
         sys.exit(2)
 
     import torch
-    from raypier_optics_amd import workloads, _lib, _abi
+    from raypier_optics_amd import workloads, _lib, metrics
     from raypier_optics_amd.tracer import trace_rays
+    from raypier_optics_amd.split import SplitRayCollection, trace_rays_split
     lib = _lib.load()
     torch.cuda.set_device(0)
     if args.copy_calibration_mib:
@@ -105,42 +164,46 @@ def main():
     rays._device_wavelengths()
     torch.cuda.synchronize()
 
-    n_int = None
-    for _ in range(args.warmup):
-        traced, _ = trace_rays(rays, w.face_lists, recursion_limit=w.recursion_limit, max_length=w.max_length)
-        if n_int is None:
-            n_int = interactions(traced)
-        del traced
-    torch.cuda.synchronize()
+    def run64():
+        return trace_rays(rays, w.face_lists, recursion_limit=w.recursion_limit, max_length=w.max_length)[0]
 
-    lib.rpx_timing_enable(1)
-    lib.rpx_timing_read(None, None, None)   # reset totals
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        traced, _ = trace_rays(rays, w.face_lists, recursion_limit=w.recursion_limit, max_length=w.max_length)
-        del traced
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    KERNELS = _abi.TIMING_KERNELS
-    nk = len(KERNELS)
-    ms = (ctypes.c_double * nk)()
-    launches = (ctypes.c_uint64 * nk)()
-    nbytes = (ctypes.c_double * nk)()
-    lib.rpx_timing_read(ms, launches, nbytes)
-    lib.rpx_timing_enable(0)
-    if n_int is None:
-        traced, _ = trace_rays(rays, w.face_lists, recursion_limit=w.recursion_limit, max_length=w.max_length)
-        n_int = interactions(traced)
+    split_in = SplitRayCollection.from_collection(rays)   # FP32-split input, resident
 
-    ms_per_step = 1000.0 * dt / args.steps
-    value = n_int * args.steps / dt
-    k = int(np.argmax([ms[i] for i in range(nk)]))
-    avg_ms = ms[k] / max(launches[k], 1)
-    bytes_per_launch = nbytes[k] / max(launches[k], 1)
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    kernel_share = {KERNELS[i]: round(ms[i] / max(sum(ms), 1e-9), 4) for i in range(nk) if launches[i]}
+    def run32():
+        return trace_rays_split(split_in, w.face_lists, recursion_limit=w.recursion_limit,
+                                max_length=w.max_length)[0]
 
+    r64 = timed(torch, lib, run64, args.steps, args.warmup)
+    r32 = timed(torch, lib, run32, args.steps, args.warmup)
+
+    # accuracy of the split form: spot RMS per wavelength bin at the detector
+    det = sum(len(fl.faces) for fl in w.face_lists) - 1
+    edges = np.linspace(0.45, 0.65, 9)
+    t64, t32 = run64(), run32()
+    s64, n64 = metrics.spot_rms_by_wavelength(t64, det, w.wavelengths, edges)
+    s32, n32 = metrics.spot_rms_by_wavelength(t32, det, w.wavelengths, edges)
+    p64, _ = metrics.detector_hits(t64, det)
+    p32, _ = metrics.detector_hits(t32, det)
+    hit_err = float((p64 - p32).abs().max()) if p64.shape == p32.shape else None
+    rel = np.abs(s32 - s64) / s64
+    del t64, t32, p64, p32
+
+    value = r64["interactions"] * args.steps / r64["seconds"]
+    ms_per_step = 1000.0 * r64["seconds"] / args.steps
+    value32 = r32["interactions"] * args.steps / r32["seconds"]
+    fp32split = {
+        "value": value32, "unit": "ray-surface interactions/s", "ms_per_step": 1000.0 * r32["seconds"] / args.steps,
+        "speedup_vs_fp64": value32 / value, "interactions_per_step": r32["interactions"],
+        "spot_rms_rel_err_max": float(rel.max()),
+        "spot_rms_rel_err_per_bin": [float(x) for x in rel],
+        "spot_rms_fp64_mm_per_bin": [float(x) for x in s64],
+        "wavelength_bin_edges_um": [float(x) for x in edges],
+        "detector_hits_per_bin": [int(x) for x in n64],
+        "detector_hit_max_abs_err_mm": hit_err,
+        "record_bytes": 112, "arithmetic": "f32 state and amplitudes; f64 root solves, normals, directions, "
+                                          "optical path and phase",
+        "roofline": roofline(r32, measured=None),
+    }
     out = {
         "metric": METRIC,
         "value": value,
@@ -156,14 +219,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": "asphere_doublet_c2: asphere k=-0.6 A4..A10 + cemented N-BK7/N-SF5 doublet, "
                                "MgF2 AR, planar detector; FP64 plain trace",
-                   "rays": args.rays, "interactions_per_step": n_int, "generations": w.recursion_limit,
+                   "rays": args.rays, "interactions_per_step": r64["interactions"],
+                   "generations": w.recursion_limit,
                    "per_ray_wavelength_um": [0.45, 0.65]},
-        "roofline": {"bound": "hbm", "kernel": KERNELS[k], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(KERNELS[k], args.rays),
-                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "kernel_time_share": kernel_share},
-        "fp32split": None,
+        "roofline": roofline(r64, measured=args.rays),
+        "fp32split": fp32split,
         "latency": None,
     }
     if not args.no_cpu:

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RPX_ABI_VERSION 2
+#define RPX_ABI_VERSION 3
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t rpx_status_t;
@@ -319,21 +319,80 @@ rpx_status_t rpx_trace_gausslet(const rpx_scene_t *scene,
                                void *workspace, uint64_t workspace_bytes,
                                void *stream);
 
+/* ---------------------------------------------------- FP32-split form
+ * A reduced-precision ray generation with no counterpart in the reference
+ * (which traces in float64 only).  Ray state is stored in float32 -- vectors,
+ * refractive index, amplitudes, length -- with phase and accumulated optical
+ * path in float64:
+ *   f32[k*capacity + i], k in rpx_split_f32_field  (19 planes)
+ *   f64[k*capacity + i], k in rpx_split_f64_field  (2 planes)
+ *   u32[k*capacity + i], k in rpx_ray_u32_field    (5 planes)
+ * 112 bytes per ray against 188.  Arithmetic: intersection root solves,
+ * surface normals and child directions (reflection, refraction, grating) in
+ * float64 from the float32 inputs; polarisation basis, Fresnel / thin-film
+ * amplitudes and power thresholds in float32; optical path and phase in
+ * float64.  Child order, parent links and the in-place parent update follow
+ * rpx_trace_segment; near thresholds (aperture edges, power thresholds,
+ * total internal reflection) the topology can differ from float64. */
+typedef enum {
+    RPX_S_OX = 0, RPX_S_OY, RPX_S_OZ,
+    RPX_S_DX, RPX_S_DY, RPX_S_DZ,
+    RPX_S_NX, RPX_S_NY, RPX_S_NZ,
+    RPX_S_EX, RPX_S_EY, RPX_S_EZ,
+    RPX_S_N_RE, RPX_S_N_IM,
+    RPX_S_E1_RE, RPX_S_E1_IM,
+    RPX_S_E2_RE, RPX_S_E2_IM,
+    RPX_S_LENGTH,
+    RPX_SPLIT_F32_FIELDS = 19
+} rpx_split_f32_field;
+
+typedef enum {
+    RPX_S_PHASE = 0,
+    RPX_S_ACCUM = 1,
+    RPX_SPLIT_F64_FIELDS = 2
+} rpx_split_f64_field;
+
+typedef struct {
+    float *f32;
+    double *f64;
+    uint32_t *u32;
+    uint64_t capacity;
+} rpx_rays_split_t;
+
+/* float64 SoA <-> FP32-split SoA (device, stream-ordered). */
+rpx_status_t rpx_rays_to_split(const rpx_rays_t *in, uint64_t n, rpx_rays_split_t *out, void *stream);
+rpx_status_t rpx_rays_from_split(const rpx_rays_split_t *in, uint64_t n, rpx_rays_t *out, void *stream);
+
+/* One FP32-split generation; arguments as rpx_trace_segment (workspace size
+ * from rpx_trace_workspace_bytes).  face_list < 0 searches every face list;
+ * otherwise only face (face_list, face_idx), as rpx_trace_one_face_segment. */
+rpx_status_t rpx_trace_segment_split(const rpx_scene_t *scene,
+                                     int32_t face_list, int32_t face_idx,
+                                     const rpx_rays_split_t *in, uint64_t n_in,
+                                     float max_length,
+                                     const double *wavelengths, uint64_t n_wavelengths,
+                                     rpx_rays_split_t *out, uint64_t *n_out,
+                                     uint32_t *face_hits,
+                                     void *workspace, uint64_t workspace_bytes,
+                                     void *stream);
+
 /* Kernel timing with HIP events recorded on the stream each generation runs
  * on (no counterpart in the reference: measurement support).  While enabled,
  * every trace call adds the duration of its kernels to running totals, one
  * slot per kernel (below).  rpx_timing_read copies RPX_TIMING_SLOTS entries
  * of: ms = total duration, launches = number of launches, bytes = bytes the
  * kernel's algorithm reads + writes (per the ray record sizes); then resets
- * the totals.  Ray generations run k_clear + k_generation; gausslet
- * generations run k_intersect + k_scan + k_emit. */
+ * the totals.  Ray generations run k_clear + k_generation (FP32-split:
+ * k_clear + k_generation_split); gausslet generations run k_intersect +
+ * k_scan + k_emit. */
 enum {
     RPX_TIMING_INTERSECT = 0,
     RPX_TIMING_SCAN = 1,
     RPX_TIMING_EMIT = 2,
     RPX_TIMING_GENERATION = 3,
     RPX_TIMING_CLEAR = 4,
-    RPX_TIMING_SLOTS = 5
+    RPX_TIMING_SPLIT = 5,
+    RPX_TIMING_SLOTS = 6
 };
 rpx_status_t rpx_timing_enable(int32_t on);
 rpx_status_t rpx_timing_read(double *ms, uint64_t *launches, double *bytes);

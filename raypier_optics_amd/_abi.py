@@ -33,10 +33,10 @@ gausslet_dtype = np.dtype([('base_ray', ray_dtype),
 
 assert ray_dtype.itemsize == 188 and para_dtype.itemsize == 80 and gausslet_dtype.itemsize == 668
 
-ABI_VERSION = 2          # RPX_ABI_VERSION of include/rpx.h
+ABI_VERSION = 3          # RPX_ABI_VERSION of include/rpx.h
 
 # timing slots (RPX_TIMING_*)
-TIMING_KERNELS = ("k_intersect", "k_scan", "k_emit", "k_generation", "k_clear")
+TIMING_KERNELS = ("k_intersect", "k_scan", "k_emit", "k_generation", "k_clear", "k_generation_split")
 
 REFL_RAY = 1 << 0
 GAUSSLET = 1 << 1
@@ -134,5 +134,15 @@ class Rays(ctypes.Structure):
                 ("capacity", ctypes.c_uint64)]
 
 
+class RaysSplit(ctypes.Structure):
+    """rpx_rays_split_t"""
+    _fields_ = [("f32", ctypes.c_void_p), ("f64", ctypes.c_void_p), ("u32", ctypes.c_void_p),
+                ("capacity", ctypes.c_uint64)]
+
+
+SPLIT_F32 = 19   # RPX_SPLIT_F32_FIELDS (same order as the first 19 float64 fields)
+SPLIT_F64 = 2    # RPX_SPLIT_F64_FIELDS: phase, accumulated_path
+
 STRUCT_SIZES["rpx_scene_desc_t"] = ctypes.sizeof(SceneDesc)
 STRUCT_SIZES["rpx_rays_t"] = ctypes.sizeof(Rays)
+STRUCT_SIZES["rpx_rays_split_t"] = ctypes.sizeof(RaysSplit)

@@ -2,10 +2,13 @@
 
 Usage: python -m raypier_optics_amd.build_lib   (or build() from Python)
 
-Every HIP source is compiled with -ffp-contract=off so that FP64 expressions
-round as written (the FP64 tracer must reproduce the reference's topology
-bit for bit), and with packed-FP32 instructions disabled.
+Every HIP source is compiled with packed-FP32 instructions disabled.  The
+float64 sources use -ffp-contract=off so that FP64 expressions round as
+written (the FP64 tracer must reproduce the reference's topology bit for
+bit).  The FP32-split source is not bound to the reference's rounding: it
+allows fused multiply-adds and the faster float32 divide / square root.
 """
+from concurrent.futures import ThreadPoolExecutor
 import os
 import shutil
 import subprocess
@@ -14,7 +17,11 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librpx.so")
-SOURCES = ["rpx_trace.hip", "rpx_query.hip", "rpx_fields.hip"]
+SOURCES = ["rpx_trace.hip", "rpx_query.hip", "rpx_fields.hip", "rpx_split.hip"]
+# per-source flags that replace -ffp-contract=off
+SOURCE_FLAGS = {
+    "rpx_split.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+}
 ARCH = os.environ.get("RPX_OFFLOAD_ARCH", "gfx950")
 
 
@@ -25,10 +32,10 @@ def hipcc():
     raise RuntimeError("hipcc not found: cannot build librpx.so")
 
 
-def flags():
+def flags(src=None):
     return [
         "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
-        "-ffp-contract=off",
+    ] + SOURCE_FLAGS.get(src, ["-ffp-contract=off"]) + [
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-I", os.path.join(HERE, "..", "include"),
     ]
@@ -36,16 +43,18 @@ def flags():
 
 def build(verbose=False):
     cc = hipcc()
-    objs = []
     build_dir = os.path.join(HERE, "_objs")
     os.makedirs(build_dir, exist_ok=True)
-    for src in SOURCES:
-        obj = os.path.join(build_dir, src.replace(".hip", ".o"))
-        cmd = [cc] + flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
+    objs = [os.path.join(build_dir, src.replace(".hip", ".o")) for src in SOURCES]
+
+    def compile_one(k):
+        cmd = [cc] + flags(SOURCES[k]) + ["-c", os.path.join(CSRC, SOURCES[k]), "-o", objs[k]]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as pool:
+        list(pool.map(compile_one, range(len(SOURCES))))
     tmp = LIB + ".tmp"
     cmd = [cc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:

@@ -1,0 +1,193 @@
+// Generation machinery shared by the FP64 and FP32-split trace kernels:
+// block size and LDS limits, the per-generation workspace, scene staging in
+// LDS, ballot scans and the decoupled look-back that places children.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "rpx_internal.h"
+#include "rpx_host.h"
+
+namespace rpx {
+
+#define RPX_LDS_FACES 64
+#define RPX_LDS_LISTS 16
+#define RPX_BLOCK 256
+
+// ------------------------------------------------------------ workspace
+// Generation control words (device), cleared by k_clear before each
+// single-pass generation and read back by the host afterwards.
+struct GenControl {
+    uint64_t children;       // children written (grand total)
+    uint64_t parents;        // rays with at least one child
+    uint64_t hits;           // rays whose nearest-face search found a face
+    uint64_t error;          // nonzero: a look-back wait ran out of polls
+    uint32_t tile_counter;   // next tile id handed to a starting workgroup
+    uint32_t pad[7];
+};
+
+struct Workspace {
+    uint32_t *tags;          // per ray: child count (bits 0..7) | (face record + 1) << 8
+    uint32_t *block_totals;  // per block: children
+    uint32_t *block_parents; // per block: rays with at least one child
+    uint64_t *block_offsets; // per block, exclusive
+    uint64_t *tile_status;   // per tile: look-back flag (bits 62..63) | value
+    GenControl *ctl;
+};
+
+static uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+
+static Workspace carve(void *ws, uint64_t n_in) {
+    uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
+    char *p = (char *)ws;
+    Workspace w;
+    w.tags = (uint32_t *)p;
+    p += align256(4 * n_in);
+    w.block_totals = (uint32_t *)p;
+    p += align256(4 * nb);
+    w.block_parents = (uint32_t *)p;
+    p += align256(4 * nb);
+    w.block_offsets = (uint64_t *)p;
+    p += align256(8 * nb);
+    w.tile_status = (uint64_t *)p;
+    p += align256(8 * nb);
+    w.ctl = (GenControl *)p;
+    return w;
+}
+
+// Stage face lists and faces in LDS; hand back a view that points there.
+static __device__ __forceinline__ SceneView stage_scene(const SceneView &S, rpx_face_list_t *lds_lists,
+                                                  rpx_face_t *lds_faces) {
+    const int t = threadIdx.x;
+    {
+        const int nw = S.n_faces * (int)(sizeof(rpx_face_t) / 8);
+        const double *src = (const double *)S.faces;
+        double *dst = (double *)lds_faces;
+        for (int k = t; k < nw; k += blockDim.x) dst[k] = src[k];
+    }
+    {
+        const int nw = S.n_face_lists * (int)(sizeof(rpx_face_list_t) / 8);
+        const double *src = (const double *)S.face_lists;
+        double *dst = (double *)lds_lists;
+        for (int k = t; k < nw; k += blockDim.x) dst[k] = src[k];
+    }
+    __syncthreads();
+    SceneView v = S;
+    v.faces = lds_faces;
+    v.face_lists = lds_lists;
+    return v;
+}
+
+// Block-wide exclusive scan of small per-lane counts (0..7) with wavefront
+// ballots: three bit-planes, popcount below the lane, wave totals in LDS.
+static __device__ __forceinline__ uint32_t block_scan_small(uint32_t c, uint32_t *lds_wave, uint32_t *block_total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        uint64_t m = __ballot((c >> b) & 1u);
+        pre += (uint32_t)__popcll(m & lt) << b;
+        tot += (uint32_t)__popcll(m) << b;
+    }
+    if (lane == 0) lds_wave[wave] = tot;
+    __syncthreads();
+    uint32_t base = 0, all = 0;
+    for (int w = 0; w < nwave; ++w) {
+        uint32_t v = lds_wave[w];
+        if (w < wave) base += v;
+        all += v;
+    }
+    *block_total = all;
+    return base + pre;
+}
+
+// --------------------------------------------------- single-pass generation
+// One launch per ray generation: intersect, evaluate the material once, and
+// place the children with a decoupled look-back scan over tiles of RPX_BLOCK
+// rays.  Tiles are handed out by an atomic counter in launch order, so every
+// tile a workgroup waits for belongs to a workgroup that is already running;
+// the wait is also bounded by a poll limit (error flag instead of a hang).
+
+#define RPX_LOOKBACK_POLLS (1u << 22)
+static constexpr uint64_t ST_AGG = 1ull << 62;     // tile aggregate published
+static constexpr uint64_t ST_PREFIX = 2ull << 62;  // inclusive prefix published
+static constexpr uint64_t ST_VALUE = (1ull << 62) - 1;
+
+static __global__ __launch_bounds__(256) void k_clear(uint64_t *tile_status, uint64_t nb, GenControl *ctl) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nb) tile_status[k] = 0;
+    if (k == 0) {
+        ctl->children = 0;
+        ctl->parents = 0;
+        ctl->hits = 0;
+        ctl->error = 0;
+        ctl->tile_counter = 0;
+    }
+}
+
+static __device__ __forceinline__ uint64_t status_load(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+static __device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Run by one whole wavefront.  Publishes the tile's aggregate, sums the
+// predecessors' aggregates 64 tiles at a time until an inclusive prefix is
+// found, publishes this tile's inclusive prefix and returns the exclusive one.
+static __device__ uint64_t lookback(uint64_t *status, uint32_t tile, uint64_t agg, GenControl *ctl) {
+    const int lane = threadIdx.x & 63;
+    if (tile == 0) {
+        if (lane == 0) status_store(&status[0], ST_PREFIX | agg);
+        return 0;
+    }
+    if (lane == 0) status_store(&status[tile], ST_AGG | agg);
+    uint64_t excl = 0;
+    int64_t end = (int64_t)tile - 1;
+    uint32_t polls = 0;
+    for (;;) {
+        const int64_t idx = end - lane;
+        const uint64_t st = (idx >= 0) ? status_load(&status[idx]) : ST_PREFIX;
+        const uint64_t flag = st >> 62;
+        const uint64_t pm = __ballot(flag == 2);
+        const int first = pm ? (__ffsll((unsigned long long)pm) - 1) : 64;
+        const uint64_t window = (first >= 63) ? ~0ull : ((2ull << first) - 1);
+        if (__ballot(flag == 0) & window) {
+            if (++polls > RPX_LOOKBACK_POLLS) {
+                if (lane == 0) atomicAdd((unsigned long long *)&ctl->error, 1ull);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        uint64_t v = (lane <= first) ? (st & ST_VALUE) : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        excl += v;
+        if (first < 64) break;
+        end -= 64;
+    }
+    if (lane == 0) status_store(&status[tile], ST_PREFIX | (excl + agg));
+    return excl;
+}
+
+static inline uint32_t scene_variant(const rpx_scene_t *s) {
+    if ((s->features & ~(uint32_t)V_SPHERES) == 0) return V_SPHERES;
+    if ((s->features & ~(uint32_t)V_LENS) == 0) return V_LENS;
+    return V_ALL;
+}
+
+struct EventSet {
+    hipEvent_t e[5];
+    bool ok = false;
+    EventSet() {
+        ok = true;
+        for (auto &x : e)
+            if (hipEventCreate(&x) != hipSuccess) ok = false;
+    }
+    ~EventSet() {
+        for (auto &x : e) (void)hipEventDestroy(x);
+    }
+};
+
+
+}  // namespace rpx

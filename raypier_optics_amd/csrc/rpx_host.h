@@ -16,6 +16,12 @@ struct rpx_scene {
 // Checks that ops[begin..root] is a well-formed postfix program.
 rpx_status_t rpx_check_shape_program(const rpx_shape_op_t *ops, int n_ops, int root);
 
+// Kernel timing totals (rpx_timing_enable / rpx_timing_read): whether timing
+// is on, and adding one launch of `ms` milliseconds and `bytes` algorithmic
+// bytes to slot `slot` (RPX_TIMING_*).
+bool rpx_timing_is_on();
+void rpx_timing_add(int slot, float ms, double bytes);
+
 // Records the message for rpx_last_error() and returns `code`.
 rpx_status_t rpx_set_error(rpx_status_t code, const std::string &msg);
 

@@ -1,0 +1,268 @@
+// FP32-split ray generation (the reduced-precision trace form) and the
+// conversions between the float64 and FP32-split ray layouts.
+//
+// The generation is the single-pass scheme of the float64 path (one launch:
+// nearest face, material, decoupled look-back placement of the children),
+// with the FP32-split record: the face tests run the float64 face code on the
+// float32 ray promoted to float64, the materials are those of rpx_split.h.
+#include <hip/hip_runtime.h>
+#include <string>
+#include "rpx_split.h"
+#include "rpx_host.h"
+#include "rpx_gen.h"
+
+using namespace rpx;
+
+struct SplitArgs {
+    SceneView S;
+    rpx_rays_split_t in;
+    rpx_rays_split_t out;
+    uint64_t n_in;
+    double max_length;
+    WlTable wavelengths;
+    uint32_t *face_hits;
+    Workspace ws;
+    int32_t one_list, one_face;
+};
+
+// Collects the (at most two) children of one ray as a two-slot shift
+// register (r0 latest), which the compiler keeps in registers.
+struct PairEmitS {
+    RayS r0, r1;
+    int n = 0;
+    RPX_DEV void operator()(const RayS &x) {
+        r1 = r0;
+        r0 = x;
+        ++n;
+    }
+    RPX_DEV const RayS &first() const { return n > 1 ? r1 : r0; }
+};
+
+template <uint32_t F>
+__global__ __launch_bounds__(RPX_BLOCK) void k_generation_split(SplitArgs A) {
+    __shared__ rpx_face_t lds_faces[RPX_LDS_FACES];
+    __shared__ rpx_face_list_t lds_lists[RPX_LDS_LISTS];
+    __shared__ uint32_t lds_hist[RPX_LDS_FACES];
+    __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
+    __shared__ uint32_t lds_tile, lds_parents, lds_hits;
+    __shared__ uint64_t lds_excl;
+    const SceneView S = stage_scene(A.S, lds_lists, lds_faces);
+    for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
+    if (threadIdx.x == 0) {
+        lds_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
+        lds_parents = 0;
+        lds_hits = 0;
+    }
+    __syncthreads();
+    const uint32_t tile = lds_tile;
+    const uint64_t i = (uint64_t)tile * RPX_BLOCK + threadIdx.x;
+    PairEmitS ce;
+    bool hit = false;
+    if (i < A.n_in) {
+        const float *f = A.in.f32;
+        const uint64_t c = A.in.capacity;
+        const V3 origin = v3(f[RPX_S_OX * c + i], f[RPX_S_OY * c + i], f[RPX_S_OZ * c + i]);
+        const V3 dir = v3(f[RPX_S_DX * c + i], f[RPX_S_DY * c + i], f[RPX_S_DZ * c + i]);
+        double length = A.max_length;
+        uint32_t end_face = RPX_NO_FACE;
+        int hit_rec = -1, hit_list = -1;
+        const V3 point = add(origin, mul(dir, A.max_length));
+        const int l0 = A.one_list < 0 ? 0 : A.one_list;
+        const int l1 = A.one_list < 0 ? S.n_face_lists : A.one_list + 1;
+        for (int l = l0; l < l1; ++l) {
+            const rpx_face_list_t &fl = S.face_lists[l];
+            const V3 p1 = transform(fl.inv_trans, origin);
+            const V3 p2 = transform(fl.inv_trans, point);
+            const int k0 = A.one_list < 0 ? fl.first_face : fl.first_face + A.one_face;
+            const int k1 = A.one_list < 0 ? fl.first_face + fl.n_faces : k0 + 1;
+            for (int k = k0; k < k1; ++k) {
+                const rpx_face_t &fc = S.faces[k];
+                const double dist = face_intersect<F>(S, fc, p1, p2, 1);
+                if (fc.tolerance < dist && dist < length) {
+                    length = dist;
+                    end_face = (uint32_t)fc.global_idx;
+                    hit_rec = k;
+                    hit_list = l;
+                }
+            }
+        }
+        A.in.f32[RPX_S_LENGTH * c + i] = (float)length;
+        A.in.u32[RPX_END_FACE_IDX * c + i] = end_face;
+        if (hit_rec >= 0) {
+            hit = true;
+            atomicAdd(&lds_hist[hit_rec], 1u);
+            const rpx_face_t &fc = S.faces[hit_rec];
+            if (fc.material >= 0) {
+                const rpx_face_list_t &fl = S.face_lists[hit_list];
+                RayS r = load_ray_s(A.in, i);
+                r.length = (float)length;
+                r.end_face_idx = end_face;
+                const V3 hp = add(origin, mul(dir, length));
+                const Orientation o = compute_orientation<F>(S, fl, fc, hp);
+                eval_children_split<F>(S.materials[fc.material], r, length, (uint32_t)i, hp, o, A.wavelengths, ce);
+            }
+        }
+    }
+    const uint32_t count = (uint32_t)min(ce.n, 2);
+    uint32_t block_total;
+    const uint32_t local = block_scan_small(count, lds_wave, &block_total);
+    {
+        const uint64_t mp = __ballot(count != 0), mh = __ballot(hit);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&lds_parents, (uint32_t)__popcll(mp));
+            atomicAdd(&lds_hits, (uint32_t)__popcll(mh));
+        }
+    }
+    if (threadIdx.x < 64) {
+        const uint64_t excl = lookback(A.ws.tile_status, tile, block_total, A.ws.ctl);
+        if (threadIdx.x == 0) lds_excl = excl;
+    }
+    __syncthreads();
+    const uint64_t pos = lds_excl + local;
+    if (count > 0 && pos < A.out.capacity) store_ray_s(A.out, pos, ce.first());
+    if (count > 1 && pos + 1 < A.out.capacity) store_ray_s(A.out, pos + 1, ce.r0);
+    if (threadIdx.x == 0) {
+        if (lds_parents) atomicAdd((unsigned long long *)&A.ws.ctl->parents, (unsigned long long)lds_parents);
+        if (lds_hits) atomicAdd((unsigned long long *)&A.ws.ctl->hits, (unsigned long long)lds_hits);
+        if ((uint64_t)tile == (A.n_in - 1) / RPX_BLOCK) A.ws.ctl->children = lds_excl + block_total;
+    }
+    if (A.face_hits) {
+        for (int k = threadIdx.x; k < S.n_list_faces; k += blockDim.x) {
+            uint32_t h = lds_hist[k];
+            if (h) atomicAdd(&A.face_hits[S.faces[k].global_idx], h);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_to_split(rpx_rays_t in, uint64_t n, rpx_rays_split_t out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    static_assert(RPX_SPLIT_F32_FIELDS == RPX_LENGTH + 1, "split float fields mirror the float64 ones");
+    for (int k = 0; k < RPX_SPLIT_F32_FIELDS; ++k) out.f32[k * out.capacity + i] = (float)in.f64[k * in.capacity + i];
+    out.f64[RPX_S_PHASE * out.capacity + i] = in.f64[RPX_PHASE * in.capacity + i];
+    out.f64[RPX_S_ACCUM * out.capacity + i] = in.f64[RPX_ACCUM * in.capacity + i];
+    for (int k = 0; k < RPX_N_U32_FIELDS; ++k) out.u32[k * out.capacity + i] = in.u32[k * in.capacity + i];
+}
+
+__global__ __launch_bounds__(256) void k_from_split(rpx_rays_split_t in, uint64_t n, rpx_rays_t out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < RPX_SPLIT_F32_FIELDS; ++k) out.f64[k * out.capacity + i] = (double)in.f32[k * in.capacity + i];
+    out.f64[RPX_PHASE * out.capacity + i] = in.f64[RPX_S_PHASE * in.capacity + i];
+    out.f64[RPX_ACCUM * out.capacity + i] = in.f64[RPX_S_ACCUM * in.capacity + i];
+    for (int k = 0; k < RPX_N_U32_FIELDS; ++k) out.u32[k * out.capacity + i] = in.u32[k * in.capacity + i];
+}
+
+static rpx_status_t check_split(const rpx_rays_split_t *r, uint64_t n, const char *what) {
+    if (!r) return rpx_set_error(RPX_E_INVALID, std::string(what) + " is null");
+    if (r->capacity < n) return rpx_set_error(RPX_E_INVALID, std::string(what) + " capacity below count");
+    if (n > 0 && (!r->f32 || !r->f64 || !r->u32))
+        return rpx_set_error(RPX_E_INVALID, std::string(what) + " has null fields");
+    return RPX_OK;
+}
+
+static rpx_status_t check_f64(const rpx_rays_t *r, uint64_t n, const char *what) {
+    if (!r) return rpx_set_error(RPX_E_INVALID, std::string(what) + " is null");
+    if (r->capacity < n) return rpx_set_error(RPX_E_INVALID, std::string(what) + " capacity below count");
+    if (n > 0 && (!r->f64 || !r->u32)) return rpx_set_error(RPX_E_INVALID, std::string(what) + " has null fields");
+    return RPX_OK;
+}
+
+extern "C" rpx_status_t rpx_rays_to_split(const rpx_rays_t *in, uint64_t n, rpx_rays_split_t *out, void *stream) {
+    rpx_status_t st = check_f64(in, n, "input rays");
+    if (st) return st;
+    if ((st = check_split(out, n, "output rays"))) return st;
+    if (n == 0) return RPX_OK;
+    hipLaunchKernelGGL(k_to_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *in, n,
+                       *out);
+    RPX_HIP_TRY(hipGetLastError());
+    return RPX_OK;
+}
+
+extern "C" rpx_status_t rpx_rays_from_split(const rpx_rays_split_t *in, uint64_t n, rpx_rays_t *out,
+                                            void *stream) {
+    rpx_status_t st = check_split(in, n, "input rays");
+    if (st) return st;
+    if ((st = check_f64(out, n, "output rays"))) return st;
+    if (n == 0) return RPX_OK;
+    hipLaunchKernelGGL(k_from_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *in,
+                       n, *out);
+    RPX_HIP_TRY(hipGetLastError());
+    return RPX_OK;
+}
+
+static void launch_split(uint32_t variant, dim3 grid, hipStream_t stream, const SplitArgs &A) {
+    if (variant == V_SPHERES)
+        hipLaunchKernelGGL((k_generation_split<V_SPHERES>), grid, dim3(RPX_BLOCK), 0, stream, A);
+    else if (variant == V_LENS)
+        hipLaunchKernelGGL((k_generation_split<V_LENS>), grid, dim3(RPX_BLOCK), 0, stream, A);
+    else
+        hipLaunchKernelGGL((k_generation_split<V_ALL>), grid, dim3(RPX_BLOCK), 0, stream, A);
+}
+
+extern "C" rpx_status_t rpx_trace_segment_split(const rpx_scene_t *scene, int32_t face_list, int32_t face_idx,
+                                                const rpx_rays_split_t *in, uint64_t n_in, float max_length,
+                                                const double *wavelengths, uint64_t n_wavelengths,
+                                                rpx_rays_split_t *out, uint64_t *n_out, uint32_t *face_hits,
+                                                void *workspace, uint64_t workspace_bytes, void *stream_) {
+    if (!scene) return rpx_set_error(RPX_E_INVALID, "scene is null");
+    if (!n_out) return rpx_set_error(RPX_E_INVALID, "n_out is null");
+    *n_out = 0;
+    rpx_status_t st = check_split(in, n_in, "input rays");
+    if (st) return st;
+    if (!out) return rpx_set_error(RPX_E_INVALID, "output rays are null");
+    if (out->capacity > 0 && (!out->f32 || !out->f64 || !out->u32))
+        return rpx_set_error(RPX_E_INVALID, "output rays have null fields");
+    if (n_in == 0) return RPX_OK;
+    if (n_in > 0xFFFFFFFFull) return rpx_set_error(RPX_E_UNSUPPORTED, "more than 2^32-1 rays in one generation");
+    if (!wavelengths && n_wavelengths) return rpx_set_error(RPX_E_INVALID, "wavelength table is null");
+    if (!workspace || workspace_bytes < rpx_trace_workspace_bytes(n_in))
+        return rpx_set_error(RPX_E_INVALID, "workspace too small");
+    if (face_list >= 0 && (face_list >= scene->view.n_face_lists || face_idx < 0 ||
+                           face_idx >= scene->list_sizes[face_list]))
+        return rpx_set_error(RPX_E_INVALID, "face list / face index out of range");
+    hipStream_t stream = (hipStream_t)stream_;
+    SplitArgs A;
+    A.S = scene->view;
+    A.in = *in;
+    A.out = *out;
+    A.n_in = n_in;
+    A.max_length = (double)max_length;
+    A.wavelengths.wl = wavelengths;
+    A.wavelengths.n = wavelengths ? n_wavelengths : 0;
+    A.face_hits = face_hits;
+    A.ws = carve(workspace, n_in);
+    A.one_list = face_list < 0 ? -1 : face_list;
+    A.one_face = face_list < 0 ? -1 : face_idx;
+    const uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
+    static thread_local EventSet ev;
+    const bool timing = rpx_timing_is_on() && ev.ok;
+    if (timing) RPX_HIP_TRY(hipEventRecord(ev.e[0], stream));
+    hipLaunchKernelGGL(k_clear, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, stream, A.ws.tile_status, nb,
+                       A.ws.ctl);
+    RPX_HIP_TRY(hipGetLastError());
+    if (timing) RPX_HIP_TRY(hipEventRecord(ev.e[1], stream));
+    launch_split(scene_variant(scene), dim3((unsigned)nb), stream, A);
+    RPX_HIP_TRY(hipGetLastError());
+    if (timing) RPX_HIP_TRY(hipEventRecord(ev.e[2], stream));
+    GenControl ctl;
+    RPX_HIP_TRY(hipMemcpyAsync(&ctl, A.ws.ctl, sizeof(ctl), hipMemcpyDeviceToHost, stream));
+    RPX_HIP_TRY(hipStreamSynchronize(stream));
+    if (ctl.error) return rpx_set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
+    *n_out = ctl.children;
+    if (timing) {
+        float t_clear = 0, t_gen = 0;
+        RPX_HIP_TRY(hipEventElapsedTime(&t_clear, ev.e[0], ev.e[1]));
+        RPX_HIP_TRY(hipEventElapsedTime(&t_gen, ev.e[1], ev.e[2]));
+        rpx_timing_add(RPX_TIMING_CLEAR, t_clear, (double)nb * 8.0);
+        // algorithmic bytes, FP32-split record: geometry 24 B read and
+        // length + end face 8 B written per input ray, the other 64 B of each
+        // ray that hit, 112 B per child
+        rpx_timing_add(RPX_TIMING_SPLIT, t_gen,
+                       (double)n_in * (24.0 + 8.0) + (double)ctl.hits * 64.0 + (double)ctl.children * 112.0);
+    }
+    if (ctl.children > out->capacity)
+        return rpx_set_error(RPX_E_CAPACITY, "output capacity " + std::to_string(out->capacity) + " below " +
+                                                 std::to_string(ctl.children) + " children");
+    return RPX_OK;
+}

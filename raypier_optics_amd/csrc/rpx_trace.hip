@@ -18,6 +18,7 @@
 #include <mutex>
 #include "rpx_materials.h"
 #include "rpx_host.h"
+#include "rpx_gen.h"
 
 using namespace rpx;
 
@@ -48,6 +49,7 @@ extern "C" int32_t rpx_struct_size(const char *name) {
     if (n == "rpx_zernike_t") return sizeof(rpx_zernike_t);
     if (n == "rpx_rays_t") return sizeof(rpx_rays_t);
     if (n == "rpx_scene_desc_t") return sizeof(rpx_scene_desc_t);
+    if (n == "rpx_rays_split_t") return sizeof(rpx_rays_split_t);
     return -1;
 }
 
@@ -70,10 +72,6 @@ static int material_max_children(int type) {
         return 0;
     }
 }
-
-#define RPX_LDS_FACES 64
-#define RPX_LDS_LISTS 16
-#define RPX_BLOCK 256
 
 rpx_status_t rpx_check_shape_program(const rpx_shape_op_t *ops, int n_ops, int root) {
     if (root < 0) return RPX_OK;
@@ -273,50 +271,9 @@ extern "C" rpx_status_t rpx_scene_destroy(rpx_scene_t *s) {
 
 extern "C" int32_t rpx_scene_max_children(const rpx_scene_t *s) { return s ? s->view.max_children : -1; }
 
-// ------------------------------------------------------------ workspace
-// Generation control words (device), cleared by k_clear before each
-// single-pass generation and read back by the host afterwards.
-struct GenControl {
-    uint64_t children;       // children written (grand total)
-    uint64_t parents;        // rays with at least one child
-    uint64_t hits;           // rays whose nearest-face search found a face
-    uint64_t error;          // nonzero: a look-back wait ran out of polls
-    uint32_t tile_counter;   // next tile id handed to a starting workgroup
-    uint32_t pad[7];
-};
-
-struct Workspace {
-    uint32_t *tags;          // per ray: child count (bits 0..7) | (face record + 1) << 8
-    uint32_t *block_totals;  // per block: children
-    uint32_t *block_parents; // per block: rays with at least one child
-    uint64_t *block_offsets; // per block, exclusive
-    uint64_t *tile_status;   // per tile: look-back flag (bits 62..63) | value
-    GenControl *ctl;
-};
-
-static uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
-
 extern "C" uint64_t rpx_trace_workspace_bytes(uint64_t n_in) {
     uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
     return align256(4 * n_in) + 2 * align256(4 * nb) + 2 * align256(8 * nb) + align256(sizeof(GenControl));
-}
-
-static Workspace carve(void *ws, uint64_t n_in) {
-    uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
-    char *p = (char *)ws;
-    Workspace w;
-    w.tags = (uint32_t *)p;
-    p += align256(4 * n_in);
-    w.block_totals = (uint32_t *)p;
-    p += align256(4 * nb);
-    w.block_parents = (uint32_t *)p;
-    p += align256(4 * nb);
-    w.block_offsets = (uint64_t *)p;
-    p += align256(8 * nb);
-    w.tile_status = (uint64_t *)p;
-    p += align256(8 * nb);
-    w.ctl = (GenControl *)p;
-    return w;
 }
 
 // --------------------------------------------------------------- kernels
@@ -332,57 +289,10 @@ struct TraceArgs {
     int32_t one_list, one_face;  // one-face mode: list and face index inside it (-1 = off)
 };
 
-// Stage face lists and faces in LDS; hand back a view that points there.
-__device__ __forceinline__ SceneView stage_scene(const SceneView &S, rpx_face_list_t *lds_lists,
-                                                  rpx_face_t *lds_faces) {
-    const int t = threadIdx.x;
-    {
-        const int nw = S.n_faces * (int)(sizeof(rpx_face_t) / 8);
-        const double *src = (const double *)S.faces;
-        double *dst = (double *)lds_faces;
-        for (int k = t; k < nw; k += blockDim.x) dst[k] = src[k];
-    }
-    {
-        const int nw = S.n_face_lists * (int)(sizeof(rpx_face_list_t) / 8);
-        const double *src = (const double *)S.face_lists;
-        double *dst = (double *)lds_lists;
-        for (int k = t; k < nw; k += blockDim.x) dst[k] = src[k];
-    }
-    __syncthreads();
-    SceneView v = S;
-    v.faces = lds_faces;
-    v.face_lists = lds_lists;
-    return v;
-}
-
 struct CountEmit {
     int n = 0;
     RPX_DEV void operator()(const Ray &) { ++n; }
 };
-
-// Block-wide exclusive scan of small per-lane counts (0..7) with wavefront
-// ballots: three bit-planes, popcount below the lane, wave totals in LDS.
-__device__ __forceinline__ uint32_t block_scan_small(uint32_t c, uint32_t *lds_wave, uint32_t *block_total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint32_t pre = 0, tot = 0;
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-        uint64_t m = __ballot((c >> b) & 1u);
-        pre += (uint32_t)__popcll(m & lt) << b;
-        tot += (uint32_t)__popcll(m) << b;
-    }
-    if (lane == 0) lds_wave[wave] = tot;
-    __syncthreads();
-    uint32_t base = 0, all = 0;
-    for (int w = 0; w < nwave; ++w) {
-        uint32_t v = lds_wave[w];
-        if (w < wave) base += v;
-        all += v;
-    }
-    *block_total = all;
-    return base + pre;
-}
 
 template <bool GAUSS, uint32_t F>
 __global__ __launch_bounds__(RPX_BLOCK) void k_intersect(TraceArgs A) {
@@ -617,76 +527,6 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
     }
 }
 
-// --------------------------------------------------- single-pass generation
-// One launch per ray generation: intersect, evaluate the material once, and
-// place the children with a decoupled look-back scan over tiles of RPX_BLOCK
-// rays.  Tiles are handed out by an atomic counter in launch order, so every
-// tile a workgroup waits for belongs to a workgroup that is already running;
-// the wait is also bounded by a poll limit (error flag instead of a hang).
-
-#define RPX_LOOKBACK_POLLS (1u << 22)
-static constexpr uint64_t ST_AGG = 1ull << 62;     // tile aggregate published
-static constexpr uint64_t ST_PREFIX = 2ull << 62;  // inclusive prefix published
-static constexpr uint64_t ST_VALUE = (1ull << 62) - 1;
-
-__global__ __launch_bounds__(256) void k_clear(uint64_t *tile_status, uint64_t nb, GenControl *ctl) {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nb) tile_status[k] = 0;
-    if (k == 0) {
-        ctl->children = 0;
-        ctl->parents = 0;
-        ctl->hits = 0;
-        ctl->error = 0;
-        ctl->tile_counter = 0;
-    }
-}
-
-__device__ __forceinline__ uint64_t status_load(const uint64_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Run by one whole wavefront.  Publishes the tile's aggregate, sums the
-// predecessors' aggregates 64 tiles at a time until an inclusive prefix is
-// found, publishes this tile's inclusive prefix and returns the exclusive one.
-__device__ uint64_t lookback(uint64_t *status, uint32_t tile, uint64_t agg, GenControl *ctl) {
-    const int lane = threadIdx.x & 63;
-    if (tile == 0) {
-        if (lane == 0) status_store(&status[0], ST_PREFIX | agg);
-        return 0;
-    }
-    if (lane == 0) status_store(&status[tile], ST_AGG | agg);
-    uint64_t excl = 0;
-    int64_t end = (int64_t)tile - 1;
-    uint32_t polls = 0;
-    for (;;) {
-        const int64_t idx = end - lane;
-        const uint64_t st = (idx >= 0) ? status_load(&status[idx]) : ST_PREFIX;
-        const uint64_t flag = st >> 62;
-        const uint64_t pm = __ballot(flag == 2);
-        const int first = pm ? (__ffsll((unsigned long long)pm) - 1) : 64;
-        const uint64_t window = (first >= 63) ? ~0ull : ((2ull << first) - 1);
-        if (__ballot(flag == 0) & window) {
-            if (++polls > RPX_LOOKBACK_POLLS) {
-                if (lane == 0) atomicAdd((unsigned long long *)&ctl->error, 1ull);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-            continue;
-        }
-        uint64_t v = (lane <= first) ? (st & ST_VALUE) : 0;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        excl += v;
-        if (first < 64) break;
-        end -= 64;
-    }
-    if (lane == 0) status_store(&status[tile], ST_PREFIX | (excl + agg));
-    return excl;
-}
-
 // Collects the (at most two) children of one ray as a two-slot shift
 // register: r0 is the latest child, r1 the one before.  Conditional or
 // indexed slot writes make the compiler keep the pair in scratch memory;
@@ -812,12 +652,6 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_generation(TraceArgs A) {
 }
 
 // ------------------------------------------------------------- variants
-static uint32_t scene_variant(const rpx_scene_t *s) {
-    if ((s->features & ~(uint32_t)V_SPHERES) == 0) return V_SPHERES;
-    if ((s->features & ~(uint32_t)V_LENS) == 0) return V_LENS;
-    return V_ALL;
-}
-
 template <uint32_t F>
 static void launch_intersect_f(bool gauss, dim3 grid, hipStream_t stream, const TraceArgs &A) {
     if (gauss)
@@ -859,6 +693,19 @@ static double g_timing_ms[RPX_TIMING_SLOTS] = {};
 static uint64_t g_timing_launches[RPX_TIMING_SLOTS] = {};
 static double g_timing_bytes[RPX_TIMING_SLOTS] = {};
 
+bool rpx_timing_is_on() {
+    std::lock_guard<std::mutex> g(g_timing_mutex);
+    return g_timing_on;
+}
+
+void rpx_timing_add(int slot, float ms, double bytes) {
+    if (slot < 0 || slot >= RPX_TIMING_SLOTS) return;
+    std::lock_guard<std::mutex> g(g_timing_mutex);
+    g_timing_ms[slot] += ms;
+    g_timing_launches[slot] += 1;
+    g_timing_bytes[slot] += bytes;
+}
+
 extern "C" rpx_status_t rpx_timing_enable(int32_t on) {
     std::lock_guard<std::mutex> g(g_timing_mutex);
     g_timing_on = on != 0;
@@ -877,19 +724,6 @@ extern "C" rpx_status_t rpx_timing_read(double *ms, uint64_t *launches, double *
     }
     return RPX_OK;
 }
-
-struct EventSet {
-    hipEvent_t e[5];
-    bool ok = false;
-    EventSet() {
-        ok = true;
-        for (auto &x : e)
-            if (hipEventCreate(&x) != hipSuccess) ok = false;
-    }
-    ~EventSet() {
-        for (auto &x : e) (void)hipEventDestroy(x);
-    }
-};
 
 // ------------------------------------------------------------- host side
 static rpx_status_t check_rays(const rpx_rays_t *r, uint64_t n, bool gauss, const char *what) {

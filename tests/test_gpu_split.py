@@ -1,0 +1,156 @@
+"""FP32-split trace form against the float64 trace on the same inputs.
+
+The split form computes geometry in float64 from float32 ray state, so its
+results differ from float64 by float32 rounding.  Tolerances (stated per
+test):
+  * spot RMS per wavelength bin at the detector: relative error <= 1e-6;
+  * detector hit points: |dp| <= 2e-5 mm (float32 spacing at 80 mm is
+    7.6e-6 mm; a few roundings accumulate over six generations);
+  * topology: ray counts per generation equal, end faces equal except where a
+    ray grazes an aperture edge or a power threshold (<= 1e-4 of rays);
+  * the physical field E1*E + E2*(d x E) of each child: relative 1e-5 of the
+    record's amplitude (the S/P basis itself may differ where float32 and
+    float64 disagree on the degenerate normal-incidence case);
+  * phase: 2e-3 rad.  A grating adds 1000*((origin - p).t)*m*2*pi/spacing
+    (cmaterials.pyx:1472-1542), so the float32 rounding of the ray origin
+    (~1e-7 mm here) moves the phase by ~4e-4 rad at 600 lines/mm.
+"""
+import numpy as np
+import pytest
+
+from raypier_optics_amd import workloads, metrics
+from raypier_optics_amd.ctracer import FaceList, RayCollection
+from raypier_optics_amd.tracer import trace_rays, trace_ray_sequence
+from raypier_optics_amd.split import SplitRayCollection, trace_rays_split, trace_ray_sequence_split
+from raypier_optics_amd.cfaces import SphericalFace, CircularFace
+from tests.test_gpu_parity import MATERIALS, random_rays
+
+pytestmark = pytest.mark.gpu
+
+
+def physical_field(rec):
+    d = rec["direction"]
+    e = rec["E_vector"]
+    h = np.cross(d, e)
+    return rec["E1_amp"][:, None] * e + rec["E2_amp"][:, None] * h
+
+
+def test_split_roundtrip(gpu):
+    r = random_rays(3000, 5, wl_count=4)
+    rc = RayCollection.from_array(r)
+    rc.wavelengths = np.linspace(0.5, 0.6, 4)
+    back = SplitRayCollection.from_collection(rc).to_collection().copy_as_array()
+    for f in ("origin", "direction", "normal", "E_vector", "length"):
+        assert np.array_equal(back[f], r[f].astype(np.float32).astype(np.float64)), f
+    for f in ("refractive_index", "E1_amp", "E2_amp"):
+        assert np.array_equal(back[f], r[f].astype(np.complex64).astype(np.complex128)), f
+    for f in ("phase", "accumulated_path", "wavelength_idx", "parent_idx", "end_face_idx", "ray_ident",
+              "ray_type_id"):
+        assert np.array_equal(back[f], r[f]), f
+
+
+def _both(w_factory):
+    w64, w32 = w_factory(), w_factory()
+    t64, f64 = trace_rays(w64.collection(), w64.face_lists, recursion_limit=w64.recursion_limit,
+                          max_length=w64.max_length)
+    t32, f32 = trace_rays_split(w32.collection(), w32.face_lists, recursion_limit=w32.recursion_limit,
+                                max_length=w32.max_length)
+    return w64, t64, f64, t32, f32
+
+
+def _compare_topology(t64, t32, frac=1e-4):
+    assert len(t64) == len(t32)
+    for g, (a, b) in enumerate(zip(t64, t32)):
+        assert a.n_rays == b.n_rays, "generation %d: %d vs %d rays" % (g, a.n_rays, b.n_rays)
+        ea = a.end_face_idx
+        eb = b.field("end_face_idx").cpu().numpy().astype(np.uint32)
+        bad = np.count_nonzero(ea != eb)
+        assert bad <= max(1, frac * len(ea)), "generation %d: %d end faces differ" % (g, bad)
+
+
+def test_split_c1_singlet(gpu):
+    w, t64, f64, t32, f32 = _both(workloads.singlet_c1)
+    _compare_topology(t64, t32)
+    det = len(f64) - 1
+    p64, _ = metrics.detector_hits(t64, det)
+    p32, _ = metrics.detector_hits(t32, det)
+    assert p64.shape == p32.shape
+    assert float((p64 - p32).abs().max()) <= 2e-5
+    r64, _ = metrics.spot_rms_by_wavelength(t64, det, w.wavelengths, [0.0, 10.0])
+    r32, _ = metrics.spot_rms_by_wavelength(t32, det, w.wavelengths, [0.0, 10.0])
+    assert abs(r32[0] - r64[0]) <= 1e-6 * r64[0], (r64, r32)
+
+
+def test_split_c2_subset_spot_rms(gpu):
+    w, t64, f64, t32, f32 = _both(lambda: workloads.asphere_doublet_c2(n_rays=262144, seed=3))
+    _compare_topology(t64, t32)
+    det = len(f64) - 1
+    edges = np.linspace(0.45, 0.65, 9)
+    r64, n64 = metrics.spot_rms_by_wavelength(t64, det, w.wavelengths, edges)
+    r32, n32 = metrics.spot_rms_by_wavelength(t32, det, w.wavelengths, edges)
+    assert np.all(n64 > 1000)
+    rel = np.abs(r32 - r64) / r64
+    print("spot RMS per wavelength bin (mm):", r64, "relative error:", rel)
+    assert rel.max() <= 1e-6, rel
+    p64, _ = metrics.detector_hits(t64, det)
+    p32, _ = metrics.detector_hits(t32, det)
+    assert float((p64 - p32).abs().max()) <= 2e-5
+    assert [f.count for f in f64] == [f.count for f in f32] or \
+        max(abs(a.count - b.count) for a, b in zip(f64, f32)) <= 1e-4 * w.rays.shape[0]
+
+
+def test_split_sequence(gpu):
+    w = workloads.singlet_c1(n_rays=3000)
+    lens, det = w.face_lists
+    seq = [(lens, 0), (lens, 1), (det, 0)]
+    t64, _ = trace_ray_sequence(w.collection(), seq, max_length=w.max_length)
+    w2 = workloads.singlet_c1(n_rays=3000)
+    lens2, det2 = w2.face_lists
+    t32, _ = trace_ray_sequence_split(w2.collection(), [(lens2, 0), (lens2, 1), (det2, 0)],
+                                      max_length=w2.max_length)
+    _compare_topology(t64, t32, frac=0)
+    for a, b in zip(t64, t32):
+        ra, rb = a.copy_as_array(), b.to_collection().copy_as_array()
+        assert np.abs(ra["origin"] - rb["origin"]).max() <= 2e-5
+
+
+@pytest.mark.parametrize("k", range(len(MATERIALS)))
+def test_split_materials(gpu, k):
+    """Each material on a curved face (a range of incidence angles and both
+    sides): children of the split trace against float64."""
+    rng = np.random.default_rng(70 + k)
+    n = 4000
+    r = random_rays(n, 80 + k, wl_count=8)
+    r["origin"] = np.stack([rng.uniform(-4, 4, n), rng.uniform(-4, 4, n), np.full(n, -5.0)], axis=1)
+    d = np.stack([rng.uniform(-0.3, 0.3, n), rng.uniform(-0.3, 0.3, n), np.ones(n)], axis=1)
+    r["direction"] = d / np.linalg.norm(d, axis=1)[:, None]
+    e = np.cross(r["direction"], rng.normal(size=(n, 3)))
+    r["E_vector"] = e / np.linalg.norm(e, axis=1)[:, None]
+    r["wavelength_idx"] = rng.integers(0, 8, n)
+    wl = np.linspace(0.45, 0.95, 8)
+    out = []
+    for split in (False, True):
+        fl = FaceList()
+        fl.faces = [SphericalFace(diameter=12.0, z_height=0.0, curvature=20.0 if k % 2 else -20.0,
+                                  material=MATERIALS[k]())]
+        rc = RayCollection.from_array(r)
+        rc.wavelengths = wl
+        tracer = trace_rays_split if split else trace_rays
+        traced, _ = tracer(rc, [fl], recursion_limit=2, max_length=50.0)
+        kids = traced[1] if len(traced) > 1 else None
+        if kids is not None and split:
+            kids = kids.to_collection()
+        out.append(kids.copy_as_array() if kids is not None else np.zeros(0))
+    a, b = out
+    assert len(a) == len(b), "material %d: %d vs %d children" % (k, len(a), len(b))
+    if len(a) == 0:
+        return
+    assert np.array_equal(a["parent_idx"], b["parent_idx"])
+    assert np.array_equal(a["ray_type_id"], b["ray_type_id"])
+    assert np.abs(a["origin"] - b["origin"]).max() <= 1e-5
+    assert np.abs(a["direction"] - b["direction"]).max() <= 1e-6
+    fa, fb = physical_field(a), physical_field(b)
+    scale = np.maximum(np.abs(a["E1_amp"]), np.abs(a["E2_amp"]))[:, None]
+    assert (np.abs(fa - fb) / np.maximum(scale, 1e-30)).max() <= 1e-5
+    assert np.allclose(a["accumulated_path"], b["accumulated_path"], rtol=1e-6, atol=1e-6)
+    assert np.abs(a["phase"] - b["phase"]).max() <= 2e-3
