@@ -133,6 +133,29 @@ def roofline(r, measured):
                                   for i in range(len(names)) if launches[i]}}
 
 
+def serving_latency(n_jobs, qps, seed=2024):
+    """C5 stream: n_jobs random lens jobs (2-8 surfaces, rays log-uniform in
+    [1e3, 2e6], built on the device) arriving as a Poisson process of rate
+    qps into the continuous-batching server."""
+    from raypier_optics_amd import workloads
+    from raypier_optics_amd.serving import TraceServer, run_poisson
+    jobs = [workloads.lens_job_c5(seed + k, device_rays=True) for k in range(n_jobs)]
+    server = TraceServer(hbm_budget_bytes=96 << 30, max_jobs=512, max_rays_in_flight=64 << 20)
+    run_poisson(server, jobs[:4], qps, seed=seed)          # warm: compile, graph capture
+    results, wall = run_poisson(server, jobs, qps, seed=seed + 1)
+    server.close()
+    lat = np.array([r.latency for r in results]) * 1000.0
+    n_int = sum(r.interactions for r in results)
+    bad = sum(1 for r in results if r.status != 0)
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()), "jobs": n_jobs, "qps": qps,
+            "throughput_interactions_per_s": n_int / wall, "jobs_per_s": n_jobs / wall, "wall_s": wall,
+            "rays_per_job_mean": float(np.mean([w.n_rays for w in jobs])), "failed_jobs": bad,
+            "workload": "lens_job_c5: 2-8 sphere/conic/asphere surfaces, N-BK7/N-SF11, rays logU(1e3, 2e6), "
+                        "Poisson arrivals; latency from arrival to result (host submit included)",
+            "hbm_budget_bytes": 96 << 30}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +164,8 @@ def main():
     ap.add_argument("--rays", type=int, default=4194304)
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=128)
+    ap.add_argument("--serve-qps", type=float, default=200.0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run device-to-device copies of this size first (HBM counter calibration)")
     args = ap.parse_args()
@@ -224,7 +249,7 @@ def main():
                    "per_ray_wavelength_um": [0.45, 0.65]},
         "roofline": roofline(r64, measured=args.rays),
         "fp32split": fp32split,
-        "latency": None,
+        "latency": serving_latency(args.serve_jobs, args.serve_qps) if args.serve_jobs > 0 else None,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, min(args.cpu_sample, args.rays))

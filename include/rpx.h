@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RPX_ABI_VERSION 3
+#define RPX_ABI_VERSION 4
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t rpx_status_t;
@@ -375,6 +375,59 @@ rpx_status_t rpx_trace_segment_split(const rpx_scene_t *scene,
                                      uint32_t *face_hits,
                                      void *workspace, uint64_t workspace_bytes,
                                      void *stream);
+
+/* ------------------------------------------------------------- serving
+ * A continuous-batching trace server (no counterpart in the reference, whose
+ * trace loop runs one collection at a time).  Jobs -- a scene, device rays,
+ * a wavelength table -- are submitted as they arrive; rpx_server_step admits
+ * waiting jobs first-come-first-served while their ray buffers fit the HBM
+ * budget, then traces one generation of every active job in one persistent
+ * launch (replayed from a captured HIP graph), and returns the jobs that
+ * finished.  A job's generations follow rpx_trace_segment (float64) exactly;
+ * what it returns is a summary: interactions, generations and the moments of
+ * its detector hits.  Buffers are the server's own (stream-ordered
+ * allocations); the caller's rays are copied at admission and may be freed
+ * after rpx_server_submit returns; the wavelength table must stay valid until
+ * the job finishes. */
+typedef struct rpx_server rpx_server_t;
+
+typedef struct {
+    const rpx_scene_t *scene;
+    const rpx_rays_t *rays;        /* device SoA, float64 */
+    uint64_t n_rays;
+    const double *wavelengths;     /* device */
+    uint64_t n_wavelengths;
+    float max_length;
+    int32_t recursion_limit;       /* as trace_rays: generations traced */
+    int32_t detector_face;         /* global face index whose hits are summed, -1 = none */
+    int32_t pad;
+} rpx_job_desc_t;
+
+typedef struct {
+    uint64_t job_id;
+    int32_t status;                /* RPX_OK, or the error that ended the job */
+    int32_t generations;           /* generations traced */
+    uint64_t interactions;         /* rays of all generations whose search hit a face */
+    uint64_t children;             /* rays created over all generations */
+    double moments[6];             /* detector hits: count, sum x, sum y, sum z,
+                                      sum x^2 + y^2 + z^2 (global frame) */
+    double t_submit, t_admit, t_finish;   /* seconds on the server's monotonic clock */
+} rpx_job_result_t;
+
+rpx_status_t rpx_server_create(uint64_t hbm_budget_bytes, uint32_t max_jobs, uint64_t max_rays_in_flight,
+                               void *stream, rpx_server_t **out);
+rpx_status_t rpx_server_destroy(rpx_server_t *server);
+/* Queue a job; its id is returned in *job_id. */
+rpx_status_t rpx_server_submit(rpx_server_t *server, const rpx_job_desc_t *job, uint64_t *job_id);
+/* Admit, run one batched generation, and copy up to max_finished results of
+ * jobs that completed into `finished` (*n_finished of them); completed jobs
+ * beyond max_finished are returned by later calls. */
+rpx_status_t rpx_server_step(rpx_server_t *server, rpx_job_result_t *finished, uint32_t max_finished,
+                             uint32_t *n_finished);
+/* Jobs waiting for admission and jobs on the device. */
+rpx_status_t rpx_server_counts(const rpx_server_t *server, uint32_t *waiting, uint32_t *active);
+/* Seconds on the server's clock (the clock of t_submit / t_admit / t_finish). */
+double rpx_server_clock(void);
 
 /* Kernel timing with HIP events recorded on the stream each generation runs
  * on (no counterpart in the reference: measurement support).  While enabled,

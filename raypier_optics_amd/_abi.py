@@ -33,7 +33,7 @@ gausslet_dtype = np.dtype([('base_ray', ray_dtype),
 
 assert ray_dtype.itemsize == 188 and para_dtype.itemsize == 80 and gausslet_dtype.itemsize == 668
 
-ABI_VERSION = 3          # RPX_ABI_VERSION of include/rpx.h
+ABI_VERSION = 4          # RPX_ABI_VERSION of include/rpx.h
 
 # timing slots (RPX_TIMING_*)
 TIMING_KERNELS = ("k_intersect", "k_scan", "k_emit", "k_generation", "k_clear", "k_generation_split")
@@ -140,9 +140,27 @@ class RaysSplit(ctypes.Structure):
                 ("capacity", ctypes.c_uint64)]
 
 
+class JobDesc(ctypes.Structure):
+    """rpx_job_desc_t"""
+    _fields_ = [("scene", ctypes.c_void_p), ("rays", ctypes.c_void_p), ("n_rays", ctypes.c_uint64),
+                ("wavelengths", ctypes.c_void_p), ("n_wavelengths", ctypes.c_uint64),
+                ("max_length", ctypes.c_float), ("recursion_limit", ctypes.c_int32),
+                ("detector_face", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class JobResult(ctypes.Structure):
+    """rpx_job_result_t"""
+    _fields_ = [("job_id", ctypes.c_uint64), ("status", ctypes.c_int32), ("generations", ctypes.c_int32),
+                ("interactions", ctypes.c_uint64), ("children", ctypes.c_uint64),
+                ("moments", ctypes.c_double * 6), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_finish", ctypes.c_double)]
+
+
 SPLIT_F32 = 19   # RPX_SPLIT_F32_FIELDS (same order as the first 19 float64 fields)
 SPLIT_F64 = 2    # RPX_SPLIT_F64_FIELDS: phase, accumulated_path
 
 STRUCT_SIZES["rpx_scene_desc_t"] = ctypes.sizeof(SceneDesc)
 STRUCT_SIZES["rpx_rays_t"] = ctypes.sizeof(Rays)
 STRUCT_SIZES["rpx_rays_split_t"] = ctypes.sizeof(RaysSplit)
+STRUCT_SIZES["rpx_job_desc_t"] = ctypes.sizeof(JobDesc)
+STRUCT_SIZES["rpx_job_result_t"] = ctypes.sizeof(JobResult)

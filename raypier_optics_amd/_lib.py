@@ -39,6 +39,12 @@ _SIGNATURES = {
     "rpx_rays_from_split": (c_i32, [P(_abi.RaysSplit), c_u64, P(_abi.Rays), c_void_p]),
     "rpx_trace_segment_split": (c_i32, [c_void_p, c_i32, c_i32, P(_abi.RaysSplit), c_u64, c_f32, c_void_p, c_u64,
                                         P(_abi.RaysSplit), P(c_u64), c_void_p, c_void_p, c_u64, c_void_p]),
+    "rpx_server_create": (c_i32, [c_u64, c_u32, c_u64, c_void_p, P(c_void_p)]),
+    "rpx_server_destroy": (c_i32, [c_void_p]),
+    "rpx_server_submit": (c_i32, [c_void_p, P(_abi.JobDesc), P(c_u64)]),
+    "rpx_server_step": (c_i32, [c_void_p, c_void_p, c_u32, P(c_u32)]),
+    "rpx_server_counts": (c_i32, [c_void_p, P(c_u32), P(c_u32)]),
+    "rpx_server_clock": (c_f64, []),
     "rpx_timing_enable": (c_i32, [c_i32]),
     "rpx_timing_read": (c_i32, [c_void_p, c_void_p, c_void_p]),
     "rpx_query_face": (c_i32, [c_void_p, c_i32, c_i32, c_void_p, c_void_p, c_u64, c_i32, c_void_p, c_void_p]),

@@ -134,10 +134,13 @@ static __device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
 // Run by one whole wavefront.  Publishes the tile's aggregate, sums the
 // predecessors' aggregates 64 tiles at a time until an inclusive prefix is
 // found, publishes this tile's inclusive prefix and returns the exclusive one.
-static __device__ uint64_t lookback(uint64_t *status, uint32_t tile, uint64_t agg, GenControl *ctl) {
+// The scan starts at tile `first` (tiles before it belong to another scan);
+// a wait that runs out of polls adds 1 to *error.
+static __device__ uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t first, uint64_t agg,
+                                    uint64_t *error) {
     const int lane = threadIdx.x & 63;
-    if (tile == 0) {
-        if (lane == 0) status_store(&status[0], ST_PREFIX | agg);
+    if (tile == first) {
+        if (lane == 0) status_store(&status[tile], ST_PREFIX | agg);
         return 0;
     }
     if (lane == 0) status_store(&status[tile], ST_AGG | agg);
@@ -146,24 +149,24 @@ static __device__ uint64_t lookback(uint64_t *status, uint32_t tile, uint64_t ag
     uint32_t polls = 0;
     for (;;) {
         const int64_t idx = end - lane;
-        const uint64_t st = (idx >= 0) ? status_load(&status[idx]) : ST_PREFIX;
+        const uint64_t st = (idx >= (int64_t)first) ? status_load(&status[idx]) : ST_PREFIX;
         const uint64_t flag = st >> 62;
         const uint64_t pm = __ballot(flag == 2);
-        const int first = pm ? (__ffsll((unsigned long long)pm) - 1) : 64;
-        const uint64_t window = (first >= 63) ? ~0ull : ((2ull << first) - 1);
+        const int first_prefix = pm ? (__ffsll((unsigned long long)pm) - 1) : 64;
+        const uint64_t window = (first_prefix >= 63) ? ~0ull : ((2ull << first_prefix) - 1);
         if (__ballot(flag == 0) & window) {
             if (++polls > RPX_LOOKBACK_POLLS) {
-                if (lane == 0) atomicAdd((unsigned long long *)&ctl->error, 1ull);
+                if (lane == 0) atomicAdd((unsigned long long *)error, 1ull);
                 break;
             }
             __builtin_amdgcn_s_sleep(2);
             continue;
         }
-        uint64_t v = (lane <= first) ? (st & ST_VALUE) : 0;
+        uint64_t v = (lane <= first_prefix) ? (st & ST_VALUE) : 0;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
         excl += v;
-        if (first < 64) break;
+        if (first_prefix < 64) break;
         end -= 64;
     }
     if (lane == 0) status_store(&status[tile], ST_PREFIX | (excl + agg));

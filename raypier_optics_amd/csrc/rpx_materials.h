@@ -15,6 +15,21 @@ struct Ray {
     uint32_t wl_idx, parent_idx, end_face_idx, ray_ident, ray_type_id;
 };
 
+// Collects the (at most two) children of one ray as a two-slot shift
+// register: r0 is the latest child, r1 the one before.  Conditional or
+// indexed slot writes make the compiler keep the pair in scratch memory;
+// the unconditional shift keeps both in registers.
+struct PairEmit {
+    Ray r0, r1;
+    int n = 0;
+    RPX_DEV void operator()(const Ray &x) {
+        r1 = r0;
+        r0 = x;
+        ++n;
+    }
+    RPX_DEV const Ray &first() const { return n > 1 ? r1 : r0; }
+};
+
 RPX_DEV Ray load_ray(const rpx_rays_t &r, uint64_t i) {
     const double *f = r.f64;
     const uint64_t c = r.capacity;

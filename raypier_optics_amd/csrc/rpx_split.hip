@@ -114,7 +114,7 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_generation_split(SplitArgs A) {
         }
     }
     if (threadIdx.x < 64) {
-        const uint64_t excl = lookback(A.ws.tile_status, tile, block_total, A.ws.ctl);
+        const uint64_t excl = lookback(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error);
         if (threadIdx.x == 0) lds_excl = excl;
     }
     __syncthreads();

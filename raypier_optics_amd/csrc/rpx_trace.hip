@@ -50,6 +50,8 @@ extern "C" int32_t rpx_struct_size(const char *name) {
     if (n == "rpx_rays_t") return sizeof(rpx_rays_t);
     if (n == "rpx_scene_desc_t") return sizeof(rpx_scene_desc_t);
     if (n == "rpx_rays_split_t") return sizeof(rpx_rays_split_t);
+    if (n == "rpx_job_desc_t") return sizeof(rpx_job_desc_t);
+    if (n == "rpx_job_result_t") return sizeof(rpx_job_result_t);
     return -1;
 }
 
@@ -527,21 +529,6 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
     }
 }
 
-// Collects the (at most two) children of one ray as a two-slot shift
-// register: r0 is the latest child, r1 the one before.  Conditional or
-// indexed slot writes make the compiler keep the pair in scratch memory;
-// the unconditional shift keeps both in registers.
-struct PairEmit {
-    Ray r0, r1;
-    int n = 0;
-    RPX_DEV void operator()(const Ray &x) {
-        r1 = r0;
-        r0 = x;
-        ++n;
-    }
-    RPX_DEV const Ray &first() const { return n > 1 ? r1 : r0; }
-};
-
 template <uint32_t F>
 __global__ __launch_bounds__(RPX_BLOCK) void k_generation(TraceArgs A) {
     __shared__ rpx_face_t lds_faces[RPX_LDS_FACES];
@@ -631,7 +618,7 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_generation(TraceArgs A) {
         }
     }
     if (threadIdx.x < 64) {
-        const uint64_t excl = lookback(A.ws.tile_status, tile, block_total, A.ws.ctl);
+        const uint64_t excl = lookback(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error);
         if (threadIdx.x == 0) lds_excl = excl;
     }
     __syncthreads();

@@ -1,0 +1,155 @@
+"""Continuous-batching trace serving (the C-ABI server of rpx_serve.hip).
+
+Jobs are traced with the float64 generation of trace_rays (core/tracer.py:
+9-47) -- the same faces, materials and child order -- but many jobs share
+each launch: every server step traces one generation of every active job,
+and jobs that arrive meanwhile join at the next step when the HBM budget
+allows.  A job's result is a summary of its trace: generations, interactions
+and the spot of its detector hits.
+
+    server = TraceServer()
+    jid = server.submit(workload)           # workloads.Workload
+    while server.busy():
+        for result in server.step():
+            ...
+"""
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import _abi
+from ._lib import load, check, require_gpu
+from ._scene import compile_scene
+from .workloads import prepare_faces
+
+
+class JobResult(object):
+    """Outcome of one job.  spot_rms is the RMS distance of the detector hits
+    from their centroid (mm); times are on the server clock (seconds)."""
+
+    def __init__(self, r, info=None):
+        self.job_id = int(r.job_id)
+        self.status = int(r.status)
+        self.generations = int(r.generations)
+        self.interactions = int(r.interactions)
+        self.children = int(r.children)
+        m = [float(x) for x in r.moments]
+        self.detector_hits = int(round(m[0]))
+        self.moments = m[:5]
+        if m[0] > 0:
+            c = np.array(m[1:4]) / m[0]
+            self.centroid = c
+            self.spot_rms = math.sqrt(max(m[4] / m[0] - float(c @ c), 0.0))
+        else:
+            self.centroid = None
+            self.spot_rms = float("nan")
+        self.t_submit, self.t_admit, self.t_finish = float(r.t_submit), float(r.t_admit), float(r.t_finish)
+        self.latency = self.t_finish - self.t_submit
+        self.info = info
+
+
+class TraceServer(object):
+    """Owns an rpx_server_t: HBM budget in bytes, at most `max_jobs` jobs on
+    the device, at most `max_rays_in_flight` rays per step."""
+
+    def __init__(self, hbm_budget_bytes=64 << 30, max_jobs=256, max_rays_in_flight=32 << 20):
+        require_gpu()
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.rpx_server_create(int(hbm_budget_bytes), int(max_jobs), int(max_rays_in_flight), None,
+                                         ctypes.byref(h)))
+        self.h = h
+        self._keep = {}         # job id -> objects the device job reads (scene, wavelengths)
+        self._info = {}
+        self._results = (_abi.JobResult * 64)()
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.lib.rpx_server_destroy(self.h)
+        self.h = None
+        self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clock(self):
+        return float(self.lib.rpx_server_clock())
+
+    def submit(self, workload, detector_face=None, info=None):
+        """Queue a workloads.Workload; returns the job id."""
+        import torch
+        all_faces = prepare_faces(workload.face_lists, workload.wavelengths, workload.max_length)
+        for fl in workload.face_lists:
+            fl.sync_transforms()
+        tables = compile_scene(workload.face_lists)
+        handle = tables.handle()
+        rc = workload.collection()
+        rc.reset_length(workload.max_length)   # as trace_rays does (core/tracer.py:20)
+        dev = rc._ensure_device()
+        wl = rc._device_wavelengths()
+        torch.cuda.synchronize()
+        if detector_face is None:
+            detector_face = len(all_faces) - 1
+        s = dev.struct()
+        d = _abi.JobDesc(handle.value, ctypes.addressof(s), rc.n_rays, wl.data_ptr(), len(workload.wavelengths),
+                         float(workload.max_length), int(workload.recursion_limit), int(detector_face), 0)
+        jid = ctypes.c_uint64(0)
+        check(self.lib.rpx_server_submit(self.h, ctypes.byref(d), ctypes.byref(jid)))
+        self._keep[jid.value] = (tables, wl)
+        self._info[jid.value] = info
+        return jid.value
+
+    def step(self):
+        """Admit, trace one batched generation, return finished JobResults."""
+        out = []
+        while True:
+            n = ctypes.c_uint32(0)
+            check(self.lib.rpx_server_step(self.h, self._results, len(self._results), ctypes.byref(n)))
+            for k in range(n.value):
+                r = self._results[k]
+                out.append(JobResult(r, self._info.pop(int(r.job_id), None)))
+                self._keep.pop(int(r.job_id), None)
+            if n.value < len(self._results):
+                return out
+
+    def counts(self):
+        w, a = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        check(self.lib.rpx_server_counts(self.h, ctypes.byref(w), ctypes.byref(a)))
+        return w.value, a.value
+
+    def busy(self):
+        w, a = self.counts()
+        return w + a > 0
+
+
+def run_poisson(server, jobs, qps, seed=0, clock=None):
+    """Submit `jobs` (a list of Workloads) at Poisson arrival times of rate
+    `qps` while stepping the server; returns the JobResults in job order and
+    the wall time.  A job is submitted at the first step boundary after its
+    arrival time; its latency runs from its arrival to its finish."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, len(jobs)))
+    t0 = server.clock()
+    nxt = 0
+    results = {}
+    ids = {}
+    while len(results) < len(jobs):
+        now = server.clock() - t0
+        while nxt < len(jobs) and arrivals[nxt] <= now:
+            ids[server.submit(jobs[nxt], info=nxt)] = nxt
+            nxt += 1
+        if not server.busy():
+            if nxt < len(jobs):   # idle until the next arrival
+                time.sleep(max(0.0, min(arrivals[nxt] - now, 1e-3)))
+                continue
+        for r in server.step():
+            k = ids[r.job_id]
+            r.latency = r.t_finish - (t0 + arrivals[k])
+            results[k] = r
+    wall = server.clock() - t0
+    return [results[k] for k in range(len(jobs))], wall

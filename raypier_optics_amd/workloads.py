@@ -102,17 +102,25 @@ def disk_field(n_rays, radius, z0, max_angle_deg, wl_range, seed):
 
 # ---------------------------------------------------------------- systems
 class Workload(object):
-    def __init__(self, name, rays, wavelengths, face_lists, recursion_limit, max_length, info):
+    def __init__(self, name, rays, wavelengths, face_lists, recursion_limit, max_length, info, source=None):
         self.name = name
-        self.rays = rays                  # host ray_dtype array
+        self.rays = rays                  # host ray_dtype array (None for a device source)
         self.wavelengths = np.asarray(wavelengths, np.float64)
         self.face_lists = face_lists
         self.recursion_limit = recursion_limit
         self.max_length = max_length
         self.info = info
+        self.source = source              # callable building the input RayCollection on the device
+
+    @property
+    def n_rays(self):
+        return len(self.rays) if self.rays is not None else self.info["n_rays"]
 
     def collection(self):
-        rc = RayCollection.from_array(self.rays)
+        if self.source is not None:
+            rc = self.source()
+        else:
+            rc = RayCollection.from_array(self.rays)
         rc.wavelengths = self.wavelengths
         return rc
 
@@ -208,3 +216,133 @@ def asphere_doublet_c2(n_rays=4194304, seed=12345):
     info = {"asphere_R_mm": 30.0, "asphere_CT_mm": 6.0, "doublet_z_mm": zA, "CT1_mm": ct1, "CT2_mm": ct2,
             "coating_um": t_coat, "focus_z_mm": focus, "seed": seed}
     return Workload("asphere_doublet_c2", rays, wl, [asphere, doublet, det], 10, 200.0, info)
+
+
+def _sag(R, h):
+    """Sag of a sphere of optical radius R at height h (sign of R)."""
+    if abs(R) > 1e8:
+        return 0.0
+    return R - math.copysign(math.sqrt(R * R - h * h), R)
+
+
+def device_disk_rays(n, radius, z0, n_wavelengths, seed):
+    """Collimated rays (+z, E along x, E1 = 1) uniform on a disk, built in HBM
+    with a seeded torch generator: ray_ident = index, wavelength index
+    uniform over the table."""
+    import torch
+    from ._device import DeviceRays
+    from ._abi import F64_FIELDS, U32_FIELDS
+    g = torch.Generator(device="cuda")
+    g.manual_seed(int(seed))
+    dev = DeviceRays(max(n, 1))
+    f, u = dev.f64(), dev.u32()
+    f.zero_()
+    rr = radius * torch.sqrt(torch.rand(n, generator=g, device="cuda", dtype=torch.float64))
+    th = 2 * math.pi * torch.rand(n, generator=g, device="cuda", dtype=torch.float64)
+    k = F64_FIELDS.index
+    f[k("ox"), :n] = rr * torch.cos(th)
+    f[k("oy"), :n] = rr * torch.sin(th)
+    f[k("oz"), :n] = z0
+    f[k("dz"), :n] = 1.0
+    f[k("ex"), :n] = 1.0
+    f[k("n_re"), :n] = 1.0
+    f[k("e1_re"), :n] = 1.0
+    f[k("length"), :n] = float("inf")
+    q = U32_FIELDS.index
+    u[q("wavelength_idx"), :n] = torch.randint(0, n_wavelengths, (n,), generator=g, device="cuda",
+                                               dtype=torch.int32)
+    u[q("parent_idx"), :n] = -1
+    u[q("end_face_idx"), :n] = -1
+    u[q("ray_ident"), :n] = torch.arange(n, device="cuda", dtype=torch.int32)
+    u[q("ray_type_id"), :n] = 0
+    return RayCollection.from_device(dev, n)
+
+
+def lens_job_c5(seed, n_rays=None, device_rays=False):
+    """One serving job: a random prescription of 2-8 refracting surfaces
+    (1-4 singlets) drawn from sphere, conic and even-asphere (A4..A10)
+    types, curvatures U(+/-1/20) mm^-1, thicknesses and air gaps U(2, 10) mm
+    (raised where an edge would be thinner than 0.5 mm), each singlet in
+    N-BK7 or N-SF11, uncoated Fresnel interfaces with the default 0.1 power
+    thresholds; collimated rays on a 4 mm-radius disk at three wavelengths
+    (F, d, C lines); opaque detector at the paraxial focus (or 50 mm behind
+    the last surface for a diverging system).  Ray count log-uniform in
+    [1e3, 2e6] unless given.  Conic surfaces are AsphericFaces with zero
+    polynomial terms."""
+    rng = np.random.default_rng(seed)
+    if n_rays is None:
+        n_rays = int(round(math.exp(rng.uniform(math.log(1e3), math.log(2e6)))))
+    n_elem = int(rng.integers(1, 5))
+    air = NondispersiveCurve(1.0)
+    h = 6.0                                   # clear aperture radius
+    shape = CircleShape(radius=h)
+    wl = np.array([0.4861327, 0.5875618, 0.6562725])
+    faces, surfs = [], []
+    z = 0.0
+    prev_back_R, gap = None, 0.0
+    for e in range(n_elem):
+        coefs = N_BK7 if rng.random() < 0.5 else N_SF11
+        g = glass(coefs)
+        n_d = sellmeier_n(coefs, 0.5875618)
+        Rs = []
+        for s in range(2):
+            c = rng.uniform(-1.0 / 20, 1.0 / 20)
+            Rs.append(1.0 / c if abs(c) > 1e-9 else 1e9)
+        if prev_back_R is not None:           # air gap: keep 0.5 mm at the edge
+            z += max(0.0, 0.5 - gap + _sag(prev_back_R, h) - _sag(Rs[0], h))
+        t = rng.uniform(2.0, 10.0)
+        t = max(t, 0.5 + _sag(Rs[0], h) - _sag(Rs[1], h))
+        for s, R in enumerate(Rs):
+            kind = rng.choice(["sphere", "conic", "asphere"])
+            zs = z if s == 0 else z + t
+            front = s == 0
+            # sphere: outside = +z side; asphere (inverted normals): outside = -z side
+            if kind == "sphere":
+                mat = FullDielectricDispersiveMaterial(dispersion_inside=air if front else g,
+                                                       dispersion_outside=g if front else air)
+                f = SphericalFace(diameter=2 * h, z_height=zs, curvature=-R, material=mat)
+            else:
+                k = float(rng.uniform(-1.0, 0.5))
+                A = [0.0] * 4
+                if kind == "asphere":
+                    A = [rng.normal(0, 1e-5), rng.normal(0, 1e-7), rng.normal(0, 1e-10), rng.normal(0, 1e-13)]
+                mat = FullDielectricDispersiveMaterial(dispersion_inside=g if front else air,
+                                                       dispersion_outside=air if front else g)
+                f = AsphericFace(shape=shape, material=mat, conic_const=k, z_height=zs, curvature=R,
+                                 A4=-A[0], A6=-A[1], A8=-A[2], A10=-A[3], invert_normals=True, atol=1e-16)
+            faces.append(f)
+            surfs.append((zs, R, 1.0 if front else n_d, n_d if front else 1.0))
+        prev_back_R = Rs[1]
+        gap = rng.uniform(2.0, 10.0)          # air gap to the next element
+        z += t + gap
+    last_z = surfs[-1][0]
+    y, u, z_prev = 1.0, 0.0, surfs[0][0]
+    for zs, R, n1, n2 in surfs:
+        y = y + u * (zs - z_prev)
+        u = (n1 * u - y * (n2 - n1) / R) / n2
+        z_prev = zs
+    bfl = -y / u if u < 0 else float("inf")
+    det_z = last_z + (bfl if 1.0 < bfl < 500.0 else 50.0)
+    lens = FaceList()
+    lens.faces = faces
+    det = FaceList()
+    det.faces = [CircularFace(diameter=1000.0, z_plane=det_z, material=OpaqueMaterial())]
+    z0 = surfs[0][0] - 5.0
+    info = {"surfaces": len(faces), "detector_z_mm": det_z, "detector_face": len(faces), "seed": seed,
+            "n_rays": n_rays}
+    if device_rays:
+        return Workload("lens_job_c5", None, wl, [lens, det], len(faces) + 3, 1000.0, info,
+                        source=lambda: device_disk_rays(n_rays, 4.0, z0, 3, seed))
+    r = _disk_collimated(n_rays, 4.0, z0, rng)
+    r["wavelength_idx"] = rng.integers(0, 3, n_rays).astype(np.uint32)
+    return Workload("lens_job_c5", r, wl, [lens, det], len(faces) + 3, 1000.0, info)
+
+
+def _disk_collimated(n, radius, z0, rng):
+    rr = radius * np.sqrt(rng.random(n))
+    th = 2 * np.pi * rng.random(n)
+    r = base_rays(n)
+    r["origin"][:, 0] = rr * np.cos(th)
+    r["origin"][:, 1] = rr * np.sin(th)
+    r["origin"][:, 2] = z0
+    return r
