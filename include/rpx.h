@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RPX_ABI_VERSION 4
+#define RPX_ABI_VERSION 5
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t rpx_status_t;
@@ -428,6 +428,47 @@ rpx_status_t rpx_server_step(rpx_server_t *server, rpx_job_result_t *finished, u
 rpx_status_t rpx_server_counts(const rpx_server_t *server, uint32_t *waiting, uint32_t *active);
 /* Seconds on the server's clock (the clock of t_submit / t_admit / t_finish). */
 double rpx_server_clock(void);
+
+/* ------------------------------------------------- sequential adjoint
+ * Spot RMS of a coaxial sequential lens and its gradient by a hand-written
+ * adjoint (no counterpart in the reference).  Surface s is
+ *   z = z[s] + c r^2 / (1 + sqrt(1 - (1 + k) c^2 r^2)) + A4 r^4 + A6 r^6
+ *       + A8 r^8 + A10 r^10,     r^2 = x^2 + y^2,
+ * with param[s] = {c, k, A4, A6, A8, A10} (c = 1/R in mm^-1; AsphericFace
+ * with curvature -R and inverted normals traces the same surface).  Each ray
+ * is traced through surfaces 0..n-1 in order -- intersection by Newton's
+ * method from the vertex plane, refraction by the vector form of Snell's law
+ * with index ratio n_table[wl][medium_before] / n_table[wl][medium_after] --
+ * to the plane z = detector_z.  A ray that leaves a surface's clear aperture,
+ * misses it or is totally reflected is dropped.  The spot RMS is
+ * sqrt(mean |p - mean p|^2) over the (x, y) of the kept rays' detector
+ * points. */
+#define RPX_LENS_MAX_SURFACES 16
+
+typedef struct {
+    int32_t n_surfaces;
+    int32_t n_media;                            /* columns of the index table */
+    double detector_z;
+    double z[RPX_LENS_MAX_SURFACES];
+    double param[RPX_LENS_MAX_SURFACES][6];     /* c, k, A4, A6, A8, A10 */
+    double aperture[RPX_LENS_MAX_SURFACES];     /* clear radius */
+    int32_t medium_before[RPX_LENS_MAX_SURFACES];
+    int32_t medium_after[RPX_LENS_MAX_SURFACES];
+} rpx_lens_t;
+
+uint64_t rpx_lens_workspace_bytes(uint64_t n_rays, int32_t n_surfaces);
+
+/* Forward: moments[0..3] (device) = kept rays, sum x, sum y, sum x^2 + y^2. */
+rpx_status_t rpx_lens_spot(const rpx_lens_t *lens, const rpx_rays_t *rays, uint64_t n_rays,
+                           const double *n_table, uint64_t n_wavelengths,
+                           double *moments, void *workspace, uint64_t workspace_bytes, void *stream);
+
+/* Adjoint: grad[s*6 + q] (device) = d(spot RMS)/d(param[s][q]), given the
+ * moments of rpx_lens_spot for the same inputs.  Intersections are
+ * recomputed, not stored. */
+rpx_status_t rpx_lens_spot_grad(const rpx_lens_t *lens, const rpx_rays_t *rays, uint64_t n_rays,
+                                const double *n_table, uint64_t n_wavelengths, const double *moments,
+                                double *grad, void *workspace, uint64_t workspace_bytes, void *stream);
 
 /* Kernel timing with HIP events recorded on the stream each generation runs
  * on (no counterpart in the reference: measurement support).  While enabled,

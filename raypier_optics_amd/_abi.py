@@ -33,7 +33,7 @@ gausslet_dtype = np.dtype([('base_ray', ray_dtype),
 
 assert ray_dtype.itemsize == 188 and para_dtype.itemsize == 80 and gausslet_dtype.itemsize == 668
 
-ABI_VERSION = 4          # RPX_ABI_VERSION of include/rpx.h
+ABI_VERSION = 5          # RPX_ABI_VERSION of include/rpx.h
 
 # timing slots (RPX_TIMING_*)
 TIMING_KERNELS = ("k_intersect", "k_scan", "k_emit", "k_generation", "k_clear", "k_generation_split")
@@ -156,6 +156,19 @@ class JobResult(ctypes.Structure):
                 ("t_finish", ctypes.c_double)]
 
 
+LENS_MAX_SURFACES = 16
+
+
+class Lens(ctypes.Structure):
+    """rpx_lens_t"""
+    _fields_ = [("n_surfaces", ctypes.c_int32), ("n_media", ctypes.c_int32), ("detector_z", ctypes.c_double),
+                ("z", ctypes.c_double * LENS_MAX_SURFACES),
+                ("param", (ctypes.c_double * 6) * LENS_MAX_SURFACES),
+                ("aperture", ctypes.c_double * LENS_MAX_SURFACES),
+                ("medium_before", ctypes.c_int32 * LENS_MAX_SURFACES),
+                ("medium_after", ctypes.c_int32 * LENS_MAX_SURFACES)]
+
+
 SPLIT_F32 = 19   # RPX_SPLIT_F32_FIELDS (same order as the first 19 float64 fields)
 SPLIT_F64 = 2    # RPX_SPLIT_F64_FIELDS: phase, accumulated_path
 
@@ -164,3 +177,4 @@ STRUCT_SIZES["rpx_rays_t"] = ctypes.sizeof(Rays)
 STRUCT_SIZES["rpx_rays_split_t"] = ctypes.sizeof(RaysSplit)
 STRUCT_SIZES["rpx_job_desc_t"] = ctypes.sizeof(JobDesc)
 STRUCT_SIZES["rpx_job_result_t"] = ctypes.sizeof(JobResult)
+STRUCT_SIZES["rpx_lens_t"] = ctypes.sizeof(Lens)

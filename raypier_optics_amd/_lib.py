@@ -45,6 +45,11 @@ _SIGNATURES = {
     "rpx_server_step": (c_i32, [c_void_p, c_void_p, c_u32, P(c_u32)]),
     "rpx_server_counts": (c_i32, [c_void_p, P(c_u32), P(c_u32)]),
     "rpx_server_clock": (c_f64, []),
+    "rpx_lens_workspace_bytes": (c_u64, [c_u64, c_i32]),
+    "rpx_lens_spot": (c_i32, [P(_abi.Lens), P(_abi.Rays), c_u64, c_void_p, c_u64, c_void_p, c_void_p, c_u64,
+                              c_void_p]),
+    "rpx_lens_spot_grad": (c_i32, [P(_abi.Lens), P(_abi.Rays), c_u64, c_void_p, c_u64, c_void_p, c_void_p,
+                                   c_void_p, c_u64, c_void_p]),
     "rpx_timing_enable": (c_i32, [c_i32]),
     "rpx_timing_read": (c_i32, [c_void_p, c_void_p, c_void_p]),
     "rpx_query_face": (c_i32, [c_void_p, c_i32, c_i32, c_void_p, c_void_p, c_u64, c_i32, c_void_p, c_void_p]),

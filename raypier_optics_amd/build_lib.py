@@ -17,7 +17,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librpx.so")
-SOURCES = ["rpx_trace.hip", "rpx_query.hip", "rpx_fields.hip", "rpx_split.hip", "rpx_serve.hip"]
+SOURCES = ["rpx_trace.hip", "rpx_query.hip", "rpx_fields.hip", "rpx_split.hip", "rpx_serve.hip", "rpx_adjoint.hip"]
 # per-source flags that replace -ffp-contract=off
 SOURCE_FLAGS = {
     "rpx_split.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
@@ -53,7 +53,7 @@ def build(verbose=False):
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
 
-    with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as pool:
+    with ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as pool:
         list(pool.map(compile_one, range(len(SOURCES))))
     tmp = LIB + ".tmp"
     cmd = [cc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs

@@ -52,6 +52,7 @@ extern "C" int32_t rpx_struct_size(const char *name) {
     if (n == "rpx_rays_split_t") return sizeof(rpx_rays_split_t);
     if (n == "rpx_job_desc_t") return sizeof(rpx_job_desc_t);
     if (n == "rpx_job_result_t") return sizeof(rpx_job_result_t);
+    if (n == "rpx_lens_t") return sizeof(rpx_lens_t);
     return -1;
 }
 

@@ -14,7 +14,10 @@ built with the drop-in API.
                   U(0.45, 0.65) um.
 
 Sign conventions follow raypier: SphericalFace.curvature = z_vertex -
-z_centre, and a material's "outside" is the side its face normal points to.
+z_centre (= -R for optical radius R, centre at vertex + R); AsphericFace.
+curvature = -R with the polynomial terms in the usual sag convention
+(z = c r^2/(1+sqrt(1-(1+k)c^2 r^2)) + A4 r^4 + ...); a material's "outside"
+is the side its face normal points to.
 """
 import math
 
@@ -177,15 +180,17 @@ def asphere_doublet_c2(n_rays=4194304, seed=12345):
     mgf2 = NondispersiveCurve(1.38)
     t_coat = 0.55 / (4 * 1.38)
     shape = CircleShape(radius=10.0)
-    # asphere: optical R = +30 mm, k=-0.6, A4..A10 of the prescription;
-    # AsphericFace takes negated A terms with inverted normals (as lens
-    # classes build it), planar back face at CT = 6 mm
+    # asphere: optical R = +30 mm (convex towards the source), k=-0.6, A4..A10
+    # of the prescription in the usual sag convention z = c r^2/(1+sqrt(1-(1+k)
+    # c^2 r^2)) + A4 r^4 + ...; built as raypier's PlanoAsphericLens builds its
+    # A surface (lenses.py:466-477): AsphericFace curvature = -R, inverted
+    # normals; planar back face at CT = 6 mm
     asph_mat = CoatedDispersiveMaterial(dispersion_inside=bk7, dispersion_outside=air, dispersion_coating=mgf2,
                                         coating_thickness=t_coat)
     A = {"A4": -2e-5, "A6": 1e-7, "A8": -3e-10, "A10": 5e-13}
     asphere = FaceList()
-    asphere.faces = [AsphericFace(shape=shape, material=asph_mat, conic_const=-0.6, z_height=0.0, curvature=30.0,
-                                  A4=-A["A4"], A6=-A["A6"], A8=-A["A8"], A10=-A["A10"], invert_normals=True,
+    asphere.faces = [AsphericFace(shape=shape, material=asph_mat, conic_const=-0.6, z_height=0.0, curvature=-30.0,
+                                  A4=A["A4"], A6=A["A6"], A8=A["A8"], A10=A["A10"], invert_normals=True,
                                   atol=1e-16),
                      ShapedPlanarFace(shape=shape, material=asph_mat, z_height=6.0)]
     zA, ct1, ct2 = 16.0, 6.0, 3.0
@@ -277,12 +282,15 @@ def lens_job_c5(seed, n_rays=None, device_rays=False):
     h = 6.0                                   # clear aperture radius
     shape = CircleShape(radius=h)
     wl = np.array([0.4861327, 0.5875618, 0.6562725])
-    faces, surfs = [], []
+    faces, surfs, presc = [], [], []
+    media = [air]
     z = 0.0
     prev_back_R, gap = None, 0.0
     for e in range(n_elem):
         coefs = N_BK7 if rng.random() < 0.5 else N_SF11
         g = glass(coefs)
+        media.append(g)
+        gi = len(media) - 1
         n_d = sellmeier_n(coefs, 0.5875618)
         Rs = []
         for s in range(2):
@@ -308,10 +316,13 @@ def lens_job_c5(seed, n_rays=None, device_rays=False):
                     A = [rng.normal(0, 1e-5), rng.normal(0, 1e-7), rng.normal(0, 1e-10), rng.normal(0, 1e-13)]
                 mat = FullDielectricDispersiveMaterial(dispersion_inside=g if front else air,
                                                        dispersion_outside=air if front else g)
-                f = AsphericFace(shape=shape, material=mat, conic_const=k, z_height=zs, curvature=R,
-                                 A4=-A[0], A6=-A[1], A8=-A[2], A10=-A[3], invert_normals=True, atol=1e-16)
+                f = AsphericFace(shape=shape, material=mat, conic_const=k, z_height=zs, curvature=-R,
+                                 A4=A[0], A6=A[1], A8=A[2], A10=A[3], invert_normals=True, atol=1e-16)
             faces.append(f)
             surfs.append((zs, R, 1.0 if front else n_d, n_d if front else 1.0))
+            presc.append({"kind": str(kind), "z": zs, "c": 1.0 / R, "k": 0.0 if kind == "sphere" else k,
+                          "A": [0.0] * 4 if kind == "sphere" else [float(a) for a in A], "aperture": h,
+                          "before": 0 if front else gi, "after": gi if front else 0})
         prev_back_R = Rs[1]
         gap = rng.uniform(2.0, 10.0)          # air gap to the next element
         z += t + gap
@@ -329,7 +340,7 @@ def lens_job_c5(seed, n_rays=None, device_rays=False):
     det.faces = [CircularFace(diameter=1000.0, z_plane=det_z, material=OpaqueMaterial())]
     z0 = surfs[0][0] - 5.0
     info = {"surfaces": len(faces), "detector_z_mm": det_z, "detector_face": len(faces), "seed": seed,
-            "n_rays": n_rays}
+            "n_rays": n_rays, "prescription": presc, "media": media}
     if device_rays:
         return Workload("lens_job_c5", None, wl, [lens, det], len(faces) + 3, 1000.0, info,
                         source=lambda: device_disk_rays(n_rays, 4.0, z0, 3, seed))
