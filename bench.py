@@ -133,27 +133,63 @@ def roofline(r, measured):
                                   for i in range(len(names)) if launches[i]}}
 
 
-def serving_latency(n_jobs, qps, seed=2024):
+def serving_latency(n_jobs, qps, seed=2024, grad_fraction=0.3):
     """C5 stream: n_jobs random lens jobs (2-8 surfaces, rays log-uniform in
     [1e3, 2e6], built on the device) arriving as a Poisson process of rate
-    qps into the continuous-batching server."""
+    qps into the continuous-batching server; a grad_fraction of them also
+    request d(spot RMS)/d(surface parameters) from the adjoint kernel."""
     from raypier_optics_amd import workloads
     from raypier_optics_amd.serving import TraceServer, run_poisson
     jobs = [workloads.lens_job_c5(seed + k, device_rays=True) for k in range(n_jobs)]
+    grad = np.random.default_rng(seed).random(n_jobs) < grad_fraction
     server = TraceServer(hbm_budget_bytes=96 << 30, max_jobs=512, max_rays_in_flight=64 << 20)
-    run_poisson(server, jobs[:4], qps, seed=seed)          # warm: compile, graph capture
-    results, wall = run_poisson(server, jobs, qps, seed=seed + 1)
+    run_poisson(server, jobs[:4], qps, seed=seed, gradient=grad[:4])     # warm: compile, graph build
+    results, wall = run_poisson(server, jobs, qps, seed=seed + 1, gradient=grad)
     server.close()
     lat = np.array([r.latency for r in results]) * 1000.0
     n_int = sum(r.interactions for r in results)
     bad = sum(1 for r in results if r.status != 0)
-    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()), "jobs": n_jobs, "qps": qps,
-            "throughput_interactions_per_s": n_int / wall, "jobs_per_s": n_jobs / wall, "wall_s": wall,
-            "rays_per_job_mean": float(np.mean([w.n_rays for w in jobs])), "failed_jobs": bad,
-            "workload": "lens_job_c5: 2-8 sphere/conic/asphere surfaces, N-BK7/N-SF11, rays logU(1e3, 2e6), "
-                        "Poisson arrivals; latency from arrival to result (host submit included)",
-            "hbm_budget_bytes": 96 << 30}
+    g = np.array([k for k in range(n_jobs) if grad[k]], dtype=int)
+    out = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "mean_ms": float(lat.mean()), "max_ms": float(lat.max()), "jobs": n_jobs, "qps": qps,
+           "gradient_jobs": int(grad.sum()),
+           "p50_ms_trace_only": float(np.percentile(lat[~grad], 50)) if (~grad).any() else None,
+           "p50_ms_with_gradient": float(np.percentile(lat[g], 50)) if len(g) else None,
+           "throughput_interactions_per_s": n_int / wall, "jobs_per_s": n_jobs / wall, "wall_s": wall,
+           "rays_per_job_mean": float(np.mean([w.n_rays for w in jobs])), "failed_jobs": bad,
+           "workload": "lens_job_c5: 2-8 sphere/conic/asphere surfaces, N-BK7/N-SF11, rays logU(1e3, 2e6), "
+                       "Poisson arrivals, %d%% gradient jobs; latency from arrival to result (host submit and "
+                       "scene compile included)" % int(round(100 * grad_fraction)),
+           "hbm_budget_bytes": 96 << 30}
+    return out
+
+
+def adjoint_timing(n_rays=1 << 21, seed=4, reps=5):
+    """Forward spot RMS and adjoint gradient of an 8-surface lens job with
+    2,097,152 rays: wall time per call (device-resident rays)."""
+    import torch
+    from raypier_optics_amd import workloads
+    from raypier_optics_amd.adjoint import SequentialLens, _Device, _moments, _grad, _sigma
+    w = workloads.lens_job_c5(seed, n_rays=n_rays, device_rays=True)
+    lens = SequentialLens.from_workload(w)
+    d = _Device(lens, w.collection())
+    P = lens.params
+    m = _moments(lens, d, P)
+    _grad(lens, d, P, m)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m = _moments(lens, d, P)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        g = _grad(lens, d, P, m)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fwd, bwd = (t1 - t0) / reps, (t2 - t1) / reps
+    return {"rays": n_rays, "surfaces": lens.n_surfaces, "parameters": 6 * lens.n_surfaces,
+            "forward_ms": 1000 * fwd, "gradient_ms": 1000 * bwd, "kept_rays": int(float(m[0])),
+            "spot_rms_mm": float(_sigma(m)), "ray_surface_gradients_per_s": n_rays * lens.n_surfaces / bwd}
 
 
 def main():
@@ -250,6 +286,7 @@ def main():
         "roofline": roofline(r64, measured=args.rays),
         "fp32split": fp32split,
         "latency": serving_latency(args.serve_jobs, args.serve_qps) if args.serve_jobs > 0 else None,
+        "adjoint": adjoint_timing(),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, min(args.cpu_sample, args.rays))

@@ -127,11 +127,15 @@ class TraceServer(object):
         return w + a > 0
 
 
-def run_poisson(server, jobs, qps, seed=0, clock=None):
+def run_poisson(server, jobs, qps, seed=0, gradient=None):
     """Submit `jobs` (a list of Workloads) at Poisson arrival times of rate
     `qps` while stepping the server; returns the JobResults in job order and
     the wall time.  A job is submitted at the first step boundary after its
-    arrival time; its latency runs from its arrival to its finish."""
+    arrival time; its latency runs from its arrival to its result.  Jobs k
+    with gradient[k] true also get d(spot RMS)/d(surface parameters) from
+    the adjoint kernel once their trace is done (result.grad, result.grad_spot)
+    and their latency includes it."""
+    from .adjoint import SequentialLens, spot_rms_and_grad
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / qps, len(jobs)))
     t0 = server.clock()
@@ -149,6 +153,11 @@ def run_poisson(server, jobs, qps, seed=0, clock=None):
                 continue
         for r in server.step():
             k = ids[r.job_id]
+            r.grad = None
+            if gradient is not None and gradient[k]:
+                w = jobs[k]
+                r.grad_spot, r.grad = spot_rms_and_grad(SequentialLens.from_workload(w), w.collection())
+                r.t_finish = server.clock()
             r.latency = r.t_finish - (t0 + arrivals[k])
             results[k] = r
     wall = server.clock() - t0
