@@ -54,25 +54,41 @@ static Workspace carve(void *ws, uint64_t n_in) {
 }
 
 // Stage face lists and faces in LDS; hand back a view that points there.
-static __device__ __forceinline__ SceneView stage_scene(const SceneView &S, rpx_face_list_t *lds_lists,
-                                                  rpx_face_t *lds_faces) {
-    const int t = threadIdx.x;
-    {
-        const int nw = S.n_faces * (int)(sizeof(rpx_face_t) / 8);
-        const double *src = (const double *)S.faces;
-        double *dst = (double *)lds_faces;
-        for (int k = t; k < nw; k += blockDim.x) dst[k] = src[k];
-    }
-    {
-        const int nw = S.n_face_lists * (int)(sizeof(rpx_face_list_t) / 8);
-        const double *src = (const double *)S.face_lists;
-        double *dst = (double *)lds_lists;
-        for (int k = t; k < nw; k += blockDim.x) dst[k] = src[k];
-    }
+// LDS copy of a scene's small tables.  Faces and face lists always fit
+// (validated at scene creation); materials and shape programs are staged when
+// they fit and otherwise read from global memory.
+#define RPX_LDS_MATERIALS 8
+#define RPX_LDS_SHAPE_OPS 32
+struct SceneLds {
+    rpx_face_t faces[RPX_LDS_FACES];
+    rpx_face_list_t lists[RPX_LDS_LISTS];
+    rpx_material_t materials[RPX_LDS_MATERIALS];
+    rpx_shape_op_t shape_ops[RPX_LDS_SHAPE_OPS];
+};
+
+template <class T>
+static __device__ __forceinline__ void lds_copy(T *dst, const T *src, int n) {
+    static_assert(sizeof(T) % 8 == 0, "tables are copied as doubles");
+    const int nw = n * (int)(sizeof(T) / 8);
+    const double *s = (const double *)src;
+    double *d = (double *)dst;
+    for (int k = threadIdx.x; k < nw; k += blockDim.x) d[k] = s[k];
+}
+
+// Stage the scene in LDS; hand back a view that points there.  Ends with a
+// workgroup barrier.
+static __device__ __forceinline__ SceneView stage_scene(const SceneView &S, SceneLds &L) {
+    lds_copy(L.faces, S.faces, S.n_faces);
+    lds_copy(L.lists, S.face_lists, S.n_face_lists);
+    const bool mats = S.n_materials <= RPX_LDS_MATERIALS, ops = S.n_shape_ops <= RPX_LDS_SHAPE_OPS;
+    if (mats) lds_copy(L.materials, S.materials, S.n_materials);
+    if (ops) lds_copy(L.shape_ops, S.shape_ops, S.n_shape_ops);
     __syncthreads();
     SceneView v = S;
-    v.faces = lds_faces;
-    v.face_lists = lds_lists;
+    v.faces = L.faces;
+    v.face_lists = L.lists;
+    if (mats) v.materials = L.materials;
+    if (ops) v.shape_ops = L.shape_ops;
     return v;
 }
 

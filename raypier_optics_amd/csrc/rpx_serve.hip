@@ -90,10 +90,10 @@ __global__ __launch_bounds__(256) void k_batch_reduce(const BatchJob *jobs, cons
 }
 
 template <uint32_t F>
-__global__ __launch_bounds__(RPX_BLOCK) void k_batch(const BatchJob *jobs, BatchControl *ctl, uint64_t *status,
+__global__ __launch_bounds__(RPX_BLOCK) __attribute__((amdgpu_waves_per_eu(F == V_ALL ? 1 : 3)))
+void k_batch(const BatchJob *jobs, BatchControl *ctl, uint64_t *status,
                                                      double *partials) {
-    __shared__ rpx_face_t lds_faces[RPX_LDS_FACES];
-    __shared__ rpx_face_list_t lds_lists[RPX_LDS_LISTS];
+    __shared__ SceneLds lds_scene;
     __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
     __shared__ uint64_t lds_tile, lds_excl;
     __shared__ int32_t lds_job;
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_batch(const BatchJob *jobs, Batch
         if (tile >= total) return;
         const int j = lds_job;
         const BatchJob &J = jobs[j];
-        const SceneView S = stage_scene(J.S, lds_lists, lds_faces);   // ends with a barrier
+        const SceneView S = stage_scene(J.S, lds_scene);   // ends with a barrier
         const uint64_t i = (tile - J.tile_begin) * RPX_BLOCK + threadIdx.x;
         PairEmit ce;
         bool hit = false;

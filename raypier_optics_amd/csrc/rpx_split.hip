@@ -40,13 +40,12 @@ struct PairEmitS {
 
 template <uint32_t F>
 __global__ __launch_bounds__(RPX_BLOCK) void k_generation_split(SplitArgs A) {
-    __shared__ rpx_face_t lds_faces[RPX_LDS_FACES];
-    __shared__ rpx_face_list_t lds_lists[RPX_LDS_LISTS];
+    __shared__ SceneLds lds_scene;
     __shared__ uint32_t lds_hist[RPX_LDS_FACES];
     __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
     __shared__ uint32_t lds_tile, lds_parents, lds_hits;
     __shared__ uint64_t lds_excl;
-    const SceneView S = stage_scene(A.S, lds_lists, lds_faces);
+    const SceneView S = stage_scene(A.S, lds_scene);
     for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
     if (threadIdx.x == 0) {
         lds_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);

@@ -299,11 +299,10 @@ struct CountEmit {
 
 template <bool GAUSS, uint32_t F>
 __global__ __launch_bounds__(RPX_BLOCK) void k_intersect(TraceArgs A) {
-    __shared__ rpx_face_t lds_faces[RPX_LDS_FACES];
-    __shared__ rpx_face_list_t lds_lists[RPX_LDS_LISTS];
+    __shared__ SceneLds lds_scene;
     __shared__ uint32_t lds_hist[RPX_LDS_FACES];
     __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
-    const SceneView S = stage_scene(A.S, lds_lists, lds_faces);
+    const SceneView S = stage_scene(A.S, lds_scene);
     for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
     __syncthreads();
 
@@ -470,10 +469,9 @@ struct CollectEmit {
 
 template <bool GAUSS, uint32_t F>
 __global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
-    __shared__ rpx_face_t lds_faces[RPX_LDS_FACES];
-    __shared__ rpx_face_list_t lds_lists[RPX_LDS_LISTS];
+    __shared__ SceneLds lds_scene;
     __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
-    const SceneView S = stage_scene(A.S, lds_lists, lds_faces);
+    const SceneView S = stage_scene(A.S, lds_scene);
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t tag = (i < A.n_in) ? A.ws.tags[i] : 0u;
     const uint32_t count = tag & 0xffu;
@@ -531,14 +529,14 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
 }
 
 template <uint32_t F>
-__global__ __launch_bounds__(RPX_BLOCK) void k_generation(TraceArgs A) {
-    __shared__ rpx_face_t lds_faces[RPX_LDS_FACES];
-    __shared__ rpx_face_list_t lds_lists[RPX_LDS_LISTS];
+__global__ __launch_bounds__(RPX_BLOCK) __attribute__((amdgpu_waves_per_eu(F == V_ALL ? 1 : 3)))
+void k_generation(TraceArgs A) {
+    __shared__ SceneLds lds_scene;
     __shared__ uint32_t lds_hist[RPX_LDS_FACES];
     __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
     __shared__ uint32_t lds_tile, lds_parents, lds_hits;
     __shared__ uint64_t lds_excl;
-    const SceneView S = stage_scene(A.S, lds_lists, lds_faces);
+    const SceneView S = stage_scene(A.S, lds_scene);
     for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
     if (threadIdx.x == 0) {
         lds_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
