@@ -3,6 +3,7 @@
 // LDS, ballot scans and the decoupled look-back that places children.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include "rpx_internal.h"
 #include "rpx_host.h"
 
@@ -15,14 +16,21 @@ namespace rpx {
 // ------------------------------------------------------------ workspace
 // Generation control words (device), cleared by k_clear before each
 // single-pass generation and read back by the host afterwards.
+// Hit counters are split into RPX_SHARDS copies, chosen by tile number, so
+// that the workgroups of a generation do not all add to the same address.
+#define RPX_SHARDS 32
 struct GenControl {
     uint64_t children;       // children written (grand total)
-    uint64_t parents;        // rays with at least one child
-    uint64_t hits;           // rays whose nearest-face search found a face
+    uint64_t parents;        // rays with at least one child (gausslet passes)
+    uint64_t hits;           // rays whose nearest-face search found a face (gausslet passes)
     uint64_t error;          // nonzero: a look-back wait ran out of polls
     uint32_t tile_counter;   // next tile id handed to a starting workgroup
     uint32_t pad[7];
+    uint32_t hit_shards[RPX_SHARDS];                  // single pass: rays that hit, per shard
+    uint32_t face_shards[RPX_SHARDS][RPX_LDS_FACES];  // single pass: hits per face record, per shard
 };
+// bytes of GenControl the host reads back after a single-pass generation
+#define RPX_CTL_HEAD (offsetof(GenControl, face_shards))
 
 struct Workspace {
     uint32_t *tags;          // per ray: child count (bits 0..7) | (face record + 1) << 8
@@ -131,6 +139,8 @@ static constexpr uint64_t ST_VALUE = (1ull << 62) - 1;
 static __global__ __launch_bounds__(256) void k_clear(uint64_t *tile_status, uint64_t nb, GenControl *ctl) {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nb) tile_status[k] = 0;
+    if (k < RPX_SHARDS * RPX_LDS_FACES) (&ctl->face_shards[0][0])[k] = 0;
+    if (k < RPX_SHARDS) ctl->hit_shards[k] = 0;
     if (k == 0) {
         ctl->children = 0;
         ctl->parents = 0;
@@ -138,6 +148,22 @@ static __global__ __launch_bounds__(256) void k_clear(uint64_t *tile_status, uin
         ctl->error = 0;
         ctl->tile_counter = 0;
     }
+}
+
+// Adds the sharded per-face-record hit counts of a single-pass generation to
+// face_hits (indexed by global face index).  One workgroup.
+static __global__ __launch_bounds__(64) void k_fold_hits(const GenControl *ctl, const rpx_face_t *faces,
+                                                          int n_list_faces, uint32_t *face_hits) {
+    const int k = threadIdx.x;
+    if (k >= n_list_faces) return;
+    uint32_t h = 0;
+    for (int s = 0; s < RPX_SHARDS; ++s) h += ctl->face_shards[s][k];
+    if (h) atomicAdd(&face_hits[faces[k].global_idx], h);
+}
+
+// Single-pass epilogue of workgroup 0-thread work: hit counts into the shards.
+static __device__ __forceinline__ void add_hits(GenControl *ctl, uint64_t tile, uint32_t hits) {
+    if (hits) atomicAdd(&ctl->hit_shards[tile % RPX_SHARDS], hits);
 }
 
 static __device__ __forceinline__ uint64_t status_load(const uint64_t *p) {

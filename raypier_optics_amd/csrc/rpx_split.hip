@@ -6,6 +6,7 @@
 // with the FP32-split record: the face tests run the float64 face code on the
 // float32 ray promoted to float64, the materials are those of rpx_split.h.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <string>
 #include "rpx_split.h"
 #include "rpx_host.h"
@@ -43,13 +44,12 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_generation_split(SplitArgs A) {
     __shared__ SceneLds lds_scene;
     __shared__ uint32_t lds_hist[RPX_LDS_FACES];
     __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
-    __shared__ uint32_t lds_tile, lds_parents, lds_hits;
+    __shared__ uint32_t lds_tile, lds_hits;
     __shared__ uint64_t lds_excl;
     const SceneView S = stage_scene(A.S, lds_scene);
     for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
     if (threadIdx.x == 0) {
         lds_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
-        lds_parents = 0;
         lds_hits = 0;
     }
     __syncthreads();
@@ -106,11 +106,8 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_generation_split(SplitArgs A) {
     uint32_t block_total;
     const uint32_t local = block_scan_small(count, lds_wave, &block_total);
     {
-        const uint64_t mp = __ballot(count != 0), mh = __ballot(hit);
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&lds_parents, (uint32_t)__popcll(mp));
-            atomicAdd(&lds_hits, (uint32_t)__popcll(mh));
-        }
+        const uint64_t mh = __ballot(hit);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&lds_hits, (uint32_t)__popcll(mh));
     }
     if (threadIdx.x < 64) {
         const uint64_t excl = lookback(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error);
@@ -121,14 +118,14 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_generation_split(SplitArgs A) {
     if (count > 0 && pos < A.out.capacity) store_ray_s(A.out, pos, ce.first());
     if (count > 1 && pos + 1 < A.out.capacity) store_ray_s(A.out, pos + 1, ce.r0);
     if (threadIdx.x == 0) {
-        if (lds_parents) atomicAdd((unsigned long long *)&A.ws.ctl->parents, (unsigned long long)lds_parents);
-        if (lds_hits) atomicAdd((unsigned long long *)&A.ws.ctl->hits, (unsigned long long)lds_hits);
+        add_hits(A.ws.ctl, tile, lds_hits);
         if ((uint64_t)tile == (A.n_in - 1) / RPX_BLOCK) A.ws.ctl->children = lds_excl + block_total;
     }
     if (A.face_hits) {
+        uint32_t *shard = A.ws.ctl->face_shards[tile % RPX_SHARDS];
         for (int k = threadIdx.x; k < S.n_list_faces; k += blockDim.x) {
             uint32_t h = lds_hist[k];
-            if (h) atomicAdd(&A.face_hits[S.faces[k].global_idx], h);
+            if (h) atomicAdd(&shard[k], h);
         }
     }
 }
@@ -237,17 +234,25 @@ extern "C" rpx_status_t rpx_trace_segment_split(const rpx_scene_t *scene, int32_
     static thread_local EventSet ev;
     const bool timing = rpx_timing_is_on() && ev.ok;
     if (timing) RPX_HIP_TRY(hipEventRecord(ev.e[0], stream));
-    hipLaunchKernelGGL(k_clear, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, stream, A.ws.tile_status, nb,
+    hipLaunchKernelGGL(k_clear, dim3((unsigned)((std::max<uint64_t>(nb, RPX_SHARDS * RPX_LDS_FACES) + 255) / 256)),
+                           dim3(256), 0, stream, A.ws.tile_status, nb,
                        A.ws.ctl);
     RPX_HIP_TRY(hipGetLastError());
     if (timing) RPX_HIP_TRY(hipEventRecord(ev.e[1], stream));
     launch_split(scene_variant(scene), dim3((unsigned)nb), stream, A);
     RPX_HIP_TRY(hipGetLastError());
     if (timing) RPX_HIP_TRY(hipEventRecord(ev.e[2], stream));
+    if (face_hits) {
+        hipLaunchKernelGGL(k_fold_hits, dim3(1), dim3(64), 0, stream, A.ws.ctl, scene->view.faces,
+                           scene->view.n_list_faces, face_hits);
+        RPX_HIP_TRY(hipGetLastError());
+    }
     GenControl ctl;
-    RPX_HIP_TRY(hipMemcpyAsync(&ctl, A.ws.ctl, sizeof(ctl), hipMemcpyDeviceToHost, stream));
+    RPX_HIP_TRY(hipMemcpyAsync(&ctl, A.ws.ctl, RPX_CTL_HEAD, hipMemcpyDeviceToHost, stream));
     RPX_HIP_TRY(hipStreamSynchronize(stream));
     if (ctl.error) return rpx_set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
+    ctl.hits = 0;
+    for (int s2 = 0; s2 < RPX_SHARDS; ++s2) ctl.hits += ctl.hit_shards[s2];
     *n_out = ctl.children;
     if (timing) {
         float t_clear = 0, t_gen = 0;

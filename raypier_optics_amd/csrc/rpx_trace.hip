@@ -11,6 +11,7 @@
 //                LDS), then each lane re-evaluates its material and writes its
 //                children at their parent-major position.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <string.h>
 #include <stdio.h>
 #include <string>
@@ -534,13 +535,12 @@ void k_generation(TraceArgs A) {
     __shared__ SceneLds lds_scene;
     __shared__ uint32_t lds_hist[RPX_LDS_FACES];
     __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
-    __shared__ uint32_t lds_tile, lds_parents, lds_hits;
+    __shared__ uint32_t lds_tile, lds_hits;
     __shared__ uint64_t lds_excl;
     const SceneView S = stage_scene(A.S, lds_scene);
     for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
     if (threadIdx.x == 0) {
         lds_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
-        lds_parents = 0;
         lds_hits = 0;
     }
     __syncthreads();
@@ -610,11 +610,8 @@ void k_generation(TraceArgs A) {
     uint32_t block_total;
     const uint32_t local = block_scan_small(count, lds_wave, &block_total);
     {
-        const uint64_t mp = __ballot(count != 0), mh = __ballot(hit);
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&lds_parents, (uint32_t)__popcll(mp));
-            atomicAdd(&lds_hits, (uint32_t)__popcll(mh));
-        }
+        const uint64_t mh = __ballot(hit);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&lds_hits, (uint32_t)__popcll(mh));
     }
     if (threadIdx.x < 64) {
         const uint64_t excl = lookback(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error);
@@ -625,14 +622,14 @@ void k_generation(TraceArgs A) {
     if (count > 0 && pos < A.out.capacity) store_ray(A.out, pos, ce.first());
     if (count > 1 && pos + 1 < A.out.capacity) store_ray(A.out, pos + 1, ce.r0);
     if (threadIdx.x == 0) {
-        if (lds_parents) atomicAdd((unsigned long long *)&A.ws.ctl->parents, (unsigned long long)lds_parents);
-        if (lds_hits) atomicAdd((unsigned long long *)&A.ws.ctl->hits, (unsigned long long)lds_hits);
+        add_hits(A.ws.ctl, tile, lds_hits);
         if ((uint64_t)tile == (A.n_in - 1) / RPX_BLOCK) A.ws.ctl->children = lds_excl + block_total;
     }
     if (A.face_hits) {
+        uint32_t *shard = A.ws.ctl->face_shards[tile % RPX_SHARDS];
         for (int k = threadIdx.x; k < S.n_list_faces; k += blockDim.x) {
             uint32_t h = lds_hist[k];
-            if (h) atomicAdd(&A.face_hits[S.faces[k].global_idx], h);
+            if (h) atomicAdd(&shard[k], h);
         }
     }
 }
@@ -766,17 +763,25 @@ static rpx_status_t run_generation(const rpx_scene_t *scene, int one_list, int o
         if (out->capacity > 0 && (!out->f64 || !out->u32))
             return set_error(RPX_E_INVALID, "output rays have null fields");
         if (timing) HIP_TRY(hipEventRecord(ev.e[0], stream));
-        hipLaunchKernelGGL(k_clear, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, stream, A.ws.tile_status, nb,
+        hipLaunchKernelGGL(k_clear, dim3((unsigned)((std::max<uint64_t>(nb, RPX_SHARDS * RPX_LDS_FACES) + 255) / 256)),
+                           dim3(256), 0, stream, A.ws.tile_status, nb,
                            A.ws.ctl);
         HIP_TRY(hipGetLastError());
         if (timing) HIP_TRY(hipEventRecord(ev.e[1], stream));
         launch_generation(variant, dim3((unsigned)nb), stream, A);
         HIP_TRY(hipGetLastError());
         if (timing) HIP_TRY(hipEventRecord(ev.e[2], stream));
+        if (face_hits) {
+            hipLaunchKernelGGL(k_fold_hits, dim3(1), dim3(64), 0, stream, A.ws.ctl, scene->view.faces,
+                               scene->view.n_list_faces, face_hits);
+            HIP_TRY(hipGetLastError());
+        }
         GenControl ctl;
-        HIP_TRY(hipMemcpyAsync(&ctl, A.ws.ctl, sizeof(ctl), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(&ctl, A.ws.ctl, RPX_CTL_HEAD, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         if (ctl.error) return set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
+        ctl.hits = 0;
+        for (int s = 0; s < RPX_SHARDS; ++s) ctl.hits += ctl.hit_shards[s];
         *n_out = ctl.children;
         if (timing) {
             float t_clear = 0, t_gen = 0;
