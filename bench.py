@@ -192,6 +192,56 @@ def adjoint_timing(n_rays=1 << 21, seed=4, reps=5):
             "spot_rms_mm": float(_sigma(m)), "ray_surface_gradients_per_s": n_rays * lens.n_surfaces / bwd}
 
 
+FIELD_FLOPS_PER_PAIR = 150   # real FP ops of one gausslet-point term with per-gausslet terms hoisted
+
+
+def field_c4(n_gausslets=262144, grid=512):
+    """C4: trace a focused Gaussian beam of n_gausslets gausslets through the
+    asphere singlet, fit the modes, and sum the field on grid x grid points
+    spanning 40 um around focus, in float64 and in the FP32-split form."""
+    import torch
+    from raypier_optics_amd import workloads, cfields
+    from raypier_optics_amd.find_focus import find_ray_focus
+    from raypier_optics_amd.tracer import trace_rays
+    gc, faces, info = workloads.gausslet_c4(n_gausslets=n_gausslets, grid=grid)
+    gc._ensure_device()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    traced, _ = trace_rays(gc, faces, recursion_limit=3, max_length=200.0)
+    torch.cuda.synchronize()
+    t_trace = time.perf_counter() - t0
+    out = traced[2]
+    focus = find_ray_focus(out)
+    pts = workloads.field_grid(focus, grid, info["span_mm"])
+    modes = cfields.gausslet_modes(out)
+    wl = out.wavelengths
+    cfields.sum_gaussian_modes(out, modes, wl, pts[:256], precision=0)      # warm
+    cfields.sum_gaussian_modes(out, modes, wl, pts[:256], precision=1)
+    res = {}
+    for prec in (0, 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e = cfields.sum_gaussian_modes(out, modes, wl, pts, precision=prec, as_tensor=True)
+        torch.cuda.synchronize()
+        res[prec] = (time.perf_counter() - t0, e.cpu().numpy())
+    pairs = out.n_rays * len(pts)
+    e64, e32 = res[0][1], res[1][1]
+    amp = np.abs(e64).max()
+    mask = np.abs(e64) >= 0.1 * amp
+    p64 = pairs / res[0][0]
+    return {"workload": "gausslet_c4: collimated 1.064 um Gaussian beam (2 mm), N-BK7 asphere k=-0.5 A4=1e-5, "
+                        "f=20 mm (NA 0.1); field on a %dx%d grid spanning 40 um at the base-ray focus" % (grid, grid),
+            "gausslets": out.n_rays, "points": len(pts), "pairs": pairs, "gausslet_trace_ms": 1000 * t_trace,
+            "fp64_s": res[0][0], "fp64_pairs_per_s": p64, "fp32split_s": res[1][0],
+            "fp32split_pairs_per_s": pairs / res[1][0], "fp32split_speedup": res[0][0] / res[1][0],
+            "max_abs_dE_over_max_E": float(np.abs(e32 - e64).max() / amp),
+            "phase_err_rad_where_E_ge_0p1max": float(np.abs(np.angle(e32[mask] / e64[mask])).max()),
+            "focus_mm": [float(x) for x in focus],
+            "roofline": {"bound": "valu", "kernel": "k_field_sum", "achieved": p64 * FIELD_FLOPS_PER_PAIR / 1e12,
+                         "peak": 78.6, "unit": "TFLOP/s", "frac": p64 * FIELD_FLOPS_PER_PAIR / 1e12 / 78.6,
+                         "flops_per_pair": FIELD_FLOPS_PER_PAIR}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +251,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=128)
+    ap.add_argument("--field", type=int, default=1, help="run the C4 gausslet field leg (0 to skip)")
     ap.add_argument("--serve-qps", type=float, default=200.0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run device-to-device copies of this size first (HBM counter calibration)")
@@ -287,6 +338,7 @@ def main():
         "fp32split": fp32split,
         "latency": serving_latency(args.serve_jobs, args.serve_qps) if args.serve_jobs > 0 else None,
         "adjoint": adjoint_timing(),
+        "field": field_c4() if args.field else None,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, min(args.cpu_sample, args.rays))

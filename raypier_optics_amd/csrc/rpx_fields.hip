@@ -206,12 +206,116 @@ __global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64
     }
 }
 
+// ------------------------------------------------- field sum FP32-split
+// The same pair term with float32 amplitude arithmetic.  What must stay
+// exact to a small fraction of a radian is done in float64: the local
+// coordinates (x, y, z) of the point in the gausslet's frame (|p - O| is tens
+// of mm) and the large phase phi + Re(kz) z (~1e6 rad at 1e5 wavelengths of
+// optical path), which is reduced modulo 2 pi in float64 before the float32
+// exponential sees it.  The rest -- the Gamma(z) update, the quadratic phase
+// terms (tens of rad), exp, sincos, the complex square root and division --
+// runs in float32; contributions are summed in float64.
+struct Cf2 {
+    float re, im;
+};
+__device__ __forceinline__ Cf2 f2(float r, float i) { Cf2 c; c.re = r; c.im = i; return c; }
+__device__ __forceinline__ Cf2 f2add(Cf2 a, Cf2 b) { return f2(a.re + b.re, a.im + b.im); }
+__device__ __forceinline__ Cf2 f2mul(Cf2 a, Cf2 b) { return f2(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+__device__ __forceinline__ Cf2 f2scale(Cf2 a, float s) { return f2(a.re * s, a.im * s); }
+__device__ __forceinline__ Cf2 f2div(Cf2 n, Cf2 d) {
+    const float inv = 1.0f / (d.re * d.re + d.im * d.im);
+    return f2((n.re * d.re + n.im * d.im) * inv, (n.im * d.re - n.re * d.im) * inv);
+}
+__device__ __forceinline__ Cf2 f2sqrt(Cf2 z) {   // principal root (Kahan)
+    const float x = z.re, y = z.im;
+    if (x == 0.0f && y == 0.0f) return f2(0.0f, y);
+    const float t = sqrtf((fabsf(x) + hypotf(x, y)) * 0.5f);
+    if (x >= 0.0f) return f2(t, y / (2.0f * t));
+    return f2(fabsf(y) / (2.0f * t), copysignf(t, y));
+}
+
+#define FS32_TILE 128
+
+__global__ __launch_bounds__(FS_BLOCK) void k_field_sum_split(const double *gp, uint64_t G, const double *points,
+                                                              uint64_t M, double *out) {
+    // float64 rows: origin(3) E(3) H(3) dir(3) kz.re phase; float32 rows: the rest
+    __shared__ double t64[14][FS32_TILE];
+    __shared__ float t32[14][FS32_TILE];
+    const uint64_t ip = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = ip < M;
+    const double px = active ? points[3 * ip] : 0, py = active ? points[3 * ip + 1] : 0,
+                 pz = active ? points[3 * ip + 2] : 0;
+    double o0r = 0, o0i = 0, o1r = 0, o1i = 0, o2r = 0, o2i = 0;
+    const double TWO_PI = 6.283185307179586;
+    const float RH = 0.70710678f;   // sqrt(1/2): sqrt(i) = RH (1 + i)
+    // float32 row sources in the float64 invariants table (see k_field_prep)
+    const int src32[14] = {13, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27};
+    for (uint64_t base = 0; base < G; base += FS32_TILE) {
+        const int nt = (int)((G - base) < FS32_TILE ? (G - base) : FS32_TILE);
+        __syncthreads();
+        for (int e = threadIdx.x; e < 14 * FS32_TILE; e += FS_BLOCK) {
+            const int q = e / FS32_TILE, g = e - q * FS32_TILE;
+            const bool in = g < nt;
+            t64[q][g] = in ? gp[(uint64_t)(q < 12 ? q : (q == 12 ? 12 : 14)) * G + base + g] : 0.0;
+            t32[q][g] = in ? (float)gp[(uint64_t)src32[q] * G + base + g] : 0.0f;
+        }
+        __syncthreads();
+        if (!active) continue;
+        for (int g = 0; g < nt; ++g) {
+            const double dx = px - t64[0][g], dy = py - t64[1][g], dz = pz - t64[2][g];
+            const double x = dx * t64[3][g] + dy * t64[4][g] + dz * t64[5][g];
+            const double y = dx * t64[6][g] + dy * t64[7][g] + dz * t64[8][g];
+            const double z = dx * t64[9][g] + dy * t64[10][g] + dz * t64[11][g];
+            const double big = t64[13][g] + t64[12][g] * z;           // phase + Re(kz) z
+            const double red = big - TWO_PI * rint(big * (1.0 / TWO_PI));
+            const float zf = (float)z, xf = (float)x, yf = (float)y;
+            const float kzi = t32[0][g];
+            const Cf2 A = f2(t32[1][g], t32[2][g]), B = f2(t32[3][g], t32[4][g]);
+            const Cf2 C = f2(t32[5][g], t32[6][g]), detG0 = f2(t32[7][g], t32[8][g]);
+            const float ira = t32[9][g];
+            const Cf2 E1a = f2(t32[10][g], t32[11][g]), E2a = f2(t32[12][g], t32[13][g]);
+            const Cf2 denom = f2scale(f2add(f2add(f2(1, 0), f2scale(f2add(A, C), zf)), f2scale(detG0, zf * zf)), 2.0f);
+            const Cf2 zd = f2scale(detG0, zf);
+            const Cf2 quad = f2add(f2add(f2mul(f2div(f2add(A, zd), denom), f2(xf * xf, 0)),
+                                         f2div(f2scale(B, 2 * xf * yf), denom)),
+                                   f2mul(f2div(f2add(C, zd), denom), f2(yf * yf, 0)));
+            // arg = phase + kz (z + quad); the part already in `red` is phase + Re(kz) z
+            const Cf2 kz = f2((float)t64[12][g], kzi);
+            const Cf2 kq = f2mul(kz, quad);
+            const float arg_re = (float)red + kq.re;
+            const float arg_im = kzi * zf + kq.im;
+            float sn, cs;
+            sincosf(arg_re, &sn, &cs);
+            const float mag = expf(-arg_im);
+            Cf2 U = f2(mag * cs, mag * sn);
+            const Cf2 q1 = f2add(f2(1, 0), f2scale(A, zf)), q2 = f2add(f2(1, 0), f2scale(C, zf));
+            const Cf2 zb = f2scale(B, zf);
+            const Cf2 w = f2add(f2mul(q1, q2), f2scale(f2mul(zb, zb), -1.0f));
+            const Cf2 sq = f2sqrt(f2(-w.im, w.re));                   // sqrt(i w)
+            U = f2div(U, sq);
+            U = f2mul(U, f2(ira * RH, ira * RH));
+            const Cf2 E1 = f2mul(E1a, U), E2 = f2mul(E2a, U);
+            o0r += (double)(E1.re * (float)t64[3][g] + E2.re * (float)t64[6][g]);
+            o0i += (double)(E1.im * (float)t64[3][g] + E2.im * (float)t64[6][g]);
+            o1r += (double)(E1.re * (float)t64[4][g] + E2.re * (float)t64[7][g]);
+            o1i += (double)(E1.im * (float)t64[4][g] + E2.im * (float)t64[7][g]);
+            o2r += (double)(E1.re * (float)t64[5][g] + E2.re * (float)t64[8][g]);
+            o2i += (double)(E1.im * (float)t64[5][g] + E2.im * (float)t64[8][g]);
+        }
+    }
+    if (active) {
+        out[6 * ip + 0] = o0r; out[6 * ip + 1] = o0i;
+        out[6 * ip + 2] = o1r; out[6 * ip + 3] = o1i;
+        out[6 * ip + 4] = o2r; out[6 * ip + 5] = o2i;
+    }
+}
+
 extern "C" rpx_status_t rpx_sum_gaussian_modes(const rpx_rays_t *rays, uint64_t n_rays, const double *modes,
                                                const double *wavelengths, uint64_t n_wavelengths,
                                                const double *points, uint64_t n_pts, double time_ps, double *out,
                                                int32_t precision, void *stream_) {
     hipStream_t stream = (hipStream_t)stream_;
-    if (precision != 0) return rpx_set_error(RPX_E_UNSUPPORTED, "only the FP64 field sum is built");
+    if (precision != 0 && precision != 1) return rpx_set_error(RPX_E_INVALID, "precision must be 0 or 1");
     if (n_pts == 0) return RPX_OK;
     if (!points || !out) return rpx_set_error(RPX_E_INVALID, "null point/output buffer");
     if (n_rays == 0) {
@@ -230,8 +334,12 @@ extern "C" rpx_status_t rpx_sum_gaussian_modes(const rpx_rays_t *rays, uint64_t 
                        modes, wl, ct, gp);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_field_sum, dim3((unsigned)((n_pts + FS_BLOCK - 1) / FS_BLOCK)), dim3(FS_BLOCK), 0,
-                           stream, gp, n_rays, points, n_pts, out);
+        if (precision == 0)
+            hipLaunchKernelGGL(k_field_sum, dim3((unsigned)((n_pts + FS_BLOCK - 1) / FS_BLOCK)), dim3(FS_BLOCK), 0,
+                               stream, gp, n_rays, points, n_pts, out);
+        else
+            hipLaunchKernelGGL(k_field_sum_split, dim3((unsigned)((n_pts + FS_BLOCK - 1) / FS_BLOCK)),
+                               dim3(FS_BLOCK), 0, stream, gp, n_rays, points, n_pts, out);
         e = hipGetLastError();
     }
     hipError_t e2 = hipFreeAsync(gp, stream);

@@ -724,33 +724,49 @@ class GaussletCollection(_Collection):
     def config_parabasal_rays(self, wavelength_list, radius, working_dist):
         """Place the parabasal rays of a circular Gaussian mode of 1/e^2 radius
         `radius` (mm) with its waist `working_dist` from the base origin
-        (ctracer.pyx:1430-1481)."""
+        (ctracer.pyx:1430-1481), for all gausslets at once.  Each element
+        goes through the same operations as the reference's per-gausslet
+        loop: norm() divides by the magnitude, cross products and sums in
+        the same order."""
         self._ensure_host()
+        n = self._n
+        if n == 0:
+            return
         wl = np.asarray(wavelength_list, dtype=np.float64)
-        for i in range(self._n):
-            g = self._host[i]
-            b = g["base_ray"]
-            base_d = norm(tuple(b["direction"]))
-            o = (0.0, 1.0, 0.0) if base_d[0] > base_d[1] else (1.0, 0.0, 0.0)
-            d1 = norm(cross(base_d, o))
-            d2 = norm(cross(base_d, d1))
-            theta0 = wl[b["wavelength_idx"]] / (math.pi * radius * 1000.0)
-            for j in range(0, 6, 2):
-                angle = (j * 2 * math.pi / 6)
-                oo = addvv(multvs(d1, radius * math.cos(angle)), multvs(d2, radius * math.sin(angle)))
-                oo = addvv(oo, tuple(b["origin"]))
-                oo = addvv(oo, multvs(base_d, working_dist))
-                dd = addvv(multvs(d1, -theta0 * math.sin(angle)), multvs(d2, theta0 * math.cos(angle)))
-                da = addvv(base_d, dd)
-                db = subvv(base_d, dd)
-                g["para_rays"][j]["direction"] = norm(da)
-                g["para_rays"][j]["origin"] = subvv(oo, multvs(da, working_dist))
-                g["para_rays"][j + 1]["direction"] = norm(db)
-                g["para_rays"][j + 1]["origin"] = subvv(oo, multvs(db, working_dist))
-                for k in (j, j + 1):
-                    g["para_rays"][k]["normal"] = b["normal"]
-                    g["para_rays"][k]["length"] = b["length"]
-            self._host[i] = g
+        g = self._host[:n]
+        b = g["base_ray"]
+
+        def nrm(v):
+            m = np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2])
+            return v / m[:, None]
+
+        def crs(a, c):
+            return np.stack([a[:, 1] * c[:, 2] - a[:, 2] * c[:, 1], a[:, 2] * c[:, 0] - a[:, 0] * c[:, 2],
+                             a[:, 0] * c[:, 1] - a[:, 1] * c[:, 0]], axis=1)
+
+        base_d = nrm(np.array(b["direction"], dtype=np.float64))
+        o = np.where((base_d[:, 0] > base_d[:, 1])[:, None], np.array([0.0, 1.0, 0.0]), np.array([1.0, 0.0, 0.0]))
+        d1 = nrm(crs(base_d, o))
+        d2 = nrm(crs(base_d, d1))
+        theta0 = wl[b["wavelength_idx"]] / (math.pi * radius * 1000.0)
+        origin = np.array(b["origin"], dtype=np.float64)
+        para = g["para_rays"]
+        for j in range(0, 6, 2):
+            angle = (j * 2 * math.pi / 6)
+            ca, sa = math.cos(angle), math.sin(angle)
+            oo = d1 * (radius * ca) + d2 * (radius * sa)
+            oo = oo + origin
+            oo = oo + base_d * working_dist
+            dd = d1 * (-theta0 * sa)[:, None] + d2 * (theta0 * ca)[:, None]
+            da = base_d + dd
+            db = base_d - dd
+            para["direction"][:, j] = nrm(da)
+            para["origin"][:, j] = oo - da * working_dist
+            para["direction"][:, j + 1] = nrm(db)
+            para["origin"][:, j + 1] = oo - db * working_dist
+            for k in (j, j + 1):
+                para["normal"][:, k] = b["normal"]
+                para["length"][:, k] = b["length"]
         self._host_changed()
 
 

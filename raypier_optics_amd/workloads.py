@@ -357,3 +357,95 @@ def _disk_collimated(n, radius, z0, rng):
     r["origin"][:, 1] = rr * np.sin(th)
     r["origin"][:, 2] = z0
     return r
+
+
+def collimated_gausslets(n_gausslets, radius, beam_waist, wavelength, z0, blending=1.5, beam_power=1.0,
+                         E_vector=(1.0, 0.0, 0.0), E1_amp=1.0 + 0j, E2_amp=0j):
+    """Collimated Gaussian beam along +z as a grid of gausslets, built the
+    way raypier's CollimatedGaussletSource builds it
+    (gausslet_sources.py:386-457): a lattice with rows skewed by 30 degrees,
+    points selected by the same radial rule, Gaussian amplitude weights,
+    power scaling, and parabasal rays of radius spacing / blending with the
+    waist at the base origin.  The lattice spacing is chosen so that the
+    selection holds at least n_gausslets points; the n_gausslets nearest the
+    axis by that rule (index order on ties) are kept."""
+    from .ctracer import GaussletCollection
+    cosV, sinV = math.cos(math.pi * 30 / 180.), math.sin(math.pi * 30 / 180.)
+
+    def lattice(resolution):
+        spacing = radius / resolution
+        nsteps = int(radius / (spacing * cosV))
+        i = np.arange(-nsteps - 1, nsteps + 2)
+        vi, vj = np.meshgrid(i, i)
+        vi, vj = vi.ravel(), vj.ravel()
+        ri = ((vi + sinV * vj) ** 2 + (cosV * vj ** 2))
+        sel = ri < (radius / spacing) ** 2
+        return spacing, vi[sel], vj[sel], ri[sel]
+
+    res = math.sqrt(n_gausslets * cosV / math.pi)
+    while True:
+        spacing, xi, yj, ri = lattice(res)
+        if len(ri) >= n_gausslets:
+            break
+        res *= 1.01
+    keep = np.sort(np.argsort(ri, kind="stable")[:n_gausslets])
+    xi, yj, ri = xi[keep], yj[keep], ri[keep]
+    direction = np.array([0.0, 0.0, 1.0])
+    v = np.array([1.0, 0.0, 0.0])
+    d1 = np.cross(direction, v)
+    d1 /= np.linalg.norm(d1)
+    d2 = np.cross(direction, d1)
+    d2 /= np.linalg.norm(d2)
+    d2 = cosV * d2 + sinV * d1
+    E = np.cross(np.cross(np.asarray(E_vector, float), direction), direction)
+    E /= np.linalg.norm(E)
+    offsets = (xi[:, None] * d1 + yj[:, None] * d2) * spacing
+    gauss = np.exp(-(ri * (spacing ** 2) / (beam_waist ** 2)))
+    P1 = (E1_amp * np.conj(E1_amp)).real
+    P2 = (E2_amp * np.conj(E1_amp)).real
+    scaling = math.sqrt(beam_power / ((P1 + P2) * (gauss ** 2)).sum())
+    r = base_rays(len(xi))
+    r["origin"] = offsets + np.array([0.0, 0.0, z0])
+    r["direction"] = direction
+    r["E_vector"] = E
+    r["E1_amp"] = scaling * E1_amp * gauss
+    r["E2_amp"] = scaling * E2_amp * gauss
+    r["normal"] = (0.0, 1.0, 0.0)
+    r["ray_type_id"] = 1 << 1
+    r["length"] = np.inf
+    g = GaussletCollection.from_rays(r)
+    wl = np.array([wavelength])
+    g.wavelengths = wl
+    g.config_parabasal_rays(wl, spacing / blending, 0.0)
+    return g, spacing
+
+
+def gausslet_c4(n_gausslets=262144, grid=512, span_mm=0.040):
+    """Gausslet field workload: a collimated Gaussian beam (lambda = 1.064 um,
+    1/e^2 field radius 2 mm, sampled to 4 mm) from z = -80 mm through an
+    N-BK7 plano-convex asphere (optical R = f (n - 1) for f = 20 mm, i.e.
+    NA 0.1 at the beam radius; k = -0.5, A4 = 1e-5; CT 4 mm), uncoated;
+    the gausslets leaving the back face are summed on a grid x grid plane
+    spanning span_mm around the least-squares focus of their base rays.
+    Optical path to the focus ~1e5 wavelengths."""
+    wl = 1.064
+    n = sellmeier_n(N_BK7, wl)
+    R = 20.0 * (n - 1.0)
+    g_bk7 = glass(N_BK7)
+    air = NondispersiveCurve(1.0)
+    shape = CircleShape(radius=6.0)
+    mat = FullDielectricDispersiveMaterial(dispersion_inside=g_bk7, dispersion_outside=air)
+    lens = FaceList()
+    lens.faces = [AsphericFace(shape=shape, material=mat, conic_const=-0.5, z_height=0.0, curvature=-R, A4=1e-5,
+                               invert_normals=True, atol=1e-16),
+                  ShapedPlanarFace(shape=shape, material=mat, z_height=4.0)]
+    gc, spacing = collimated_gausslets(n_gausslets, 4.0, 2.0, wl, -80.0)
+    info = {"n_gausslets": n_gausslets, "spacing_mm": spacing, "R_mm": R, "grid": grid, "span_mm": span_mm}
+    return gc, [lens], info
+
+
+def field_grid(focus, grid, span_mm):
+    """grid x grid points in the z = focus[2] plane, centred on focus."""
+    g = np.linspace(-span_mm / 2, span_mm / 2, grid)
+    gx, gy = np.meshgrid(g + focus[0], g + focus[1])
+    return np.column_stack([gx.ravel(), gy.ravel(), np.full(gx.size, focus[2])])
