@@ -192,6 +192,34 @@ def adjoint_timing(n_rays=1 << 21, seed=4, reps=5):
             "spot_rms_mm": float(_sigma(m)), "ray_surface_gradients_per_s": n_rays * lens.n_surfaces / bwd}
 
 
+def michelson_c3_leg(n_rays=1 << 20, order=1):
+    """C3: the Michelson with a Zernike mirror and a grating (one order),
+    1,048,576 rays to depth 12, float64 and FP32-split; interactions/s."""
+    import torch
+    from raypier_optics_amd import workloads
+    from raypier_optics_amd.tracer import trace_rays
+    from raypier_optics_amd.split import trace_rays_split
+    out = {"workload": "michelson_c3: N-BK7 beamsplitter cube, flat + Zernike (Noll 4-15) PEC mirrors, "
+                       "600 l/mm grating order %+d, relative thresholds 1e-4, depth 12" % order, "rays": n_rays}
+    for name, tracer in (("fp64", trace_rays), ("fp32split", trace_rays_split)):
+        w = workloads.michelson_c3(n_rays=n_rays, order=order)
+        rc = w.collection()
+        rc._ensure_device()
+        rc._device_wavelengths()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        traced, _ = tracer(rc, w.face_lists, recursion_limit=w.recursion_limit, max_length=w.max_length)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        n_int = interactions(traced)
+        out[name] = {"seconds": dt, "interactions": n_int, "interactions_per_s": n_int / dt,
+                     "rays_all_generations": int(sum(g.n_rays for g in traced)), "generations": len(traced)}
+        del traced, rc
+        torch.cuda.empty_cache()
+    out["fp32split_speedup"] = out["fp64"]["seconds"] / out["fp32split"]["seconds"]
+    return out
+
+
 FIELD_FLOPS_PER_PAIR = 150   # real FP ops of one gausslet-point term with per-gausslet terms hoisted
 
 
@@ -252,6 +280,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=128)
     ap.add_argument("--field", type=int, default=1, help="run the C4 gausslet field leg (0 to skip)")
+    ap.add_argument("--michelson", type=int, default=1, help="run the C3 Michelson leg (0 to skip)")
     ap.add_argument("--serve-qps", type=float, default=200.0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run device-to-device copies of this size first (HBM counter calibration)")
@@ -339,6 +368,7 @@ def main():
         "latency": serving_latency(args.serve_jobs, args.serve_qps) if args.serve_jobs > 0 else None,
         "adjoint": adjoint_timing(),
         "field": field_c4() if args.field else None,
+        "michelson": michelson_c3_leg() if args.michelson else None,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, min(args.cpu_sample, args.rays))

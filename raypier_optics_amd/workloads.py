@@ -449,3 +449,109 @@ def field_grid(focus, grid, span_mm):
     g = np.linspace(-span_mm / 2, span_mm / 2, grid)
     gx, gy = np.meshgrid(g + focus[0], g + focus[1])
     return np.column_stack([gx.ravel(), gy.ravel(), np.full(gx.size, focus[2])])
+
+
+def noll_to_nm(j):
+    """(n, m) of Noll index j >= 1 (cosine terms at even j, sine at odd)."""
+    n = int((-1 + math.sqrt(8 * (j - 1) + 1)) / 2)
+    p = j - n * (n + 1) // 2 - 1
+    m = (n % 2) + 2 * ((p + (n + 1) % 2) // 2)
+    return n, (m if j % 2 == 0 else -m)
+
+
+def noll_to_ansi(j):
+    n, m = noll_to_nm(j)
+    return (n * (n + 2) + m) // 2
+
+
+def _placed(faces, columns, translation):
+    """FaceList whose local axes map to the given global columns (x', y',
+    z') and whose local origin sits at `translation`."""
+    from .ctracer import Transform
+    R = np.array(columns, dtype=np.float64).T          # columns -> matrix
+    fl = FaceList()
+    fl.faces = faces
+    t = np.asarray(translation, dtype=np.float64)
+    fl.transform = Transform(rotation=R.tolist(), translation=t.tolist())
+    fl.inverse_transform = Transform(rotation=R.T.tolist(), translation=(-(R.T @ t)).tolist())
+    return fl
+
+
+def michelson_c3(n_rays=1048576, order=1, seed=2025, depth=12, threshold=1e-4, wavelength=0.6328):
+    """Michelson interferometer in the x-y plane, light along +x:
+      * 25 mm N-BK7 beamsplitter cube at the origin -- four extruded side
+        faces and two polygon end faces with FullDielectricMaterial (relative
+        reflection / transmission thresholds `threshold`), the diagonal
+        (-a,-a)-(a,a) a PartiallyReflectiveMaterial(0.5);
+      * arm +x: flat PEC mirror (radius 15 mm) at x = 50;
+      * arm +y: PEC mirror at y = 50 with a Zernike distortion, Noll terms
+        4-15 (ANSI-indexed), coefficients N(0, 0.05 um), unit radius 15 mm;
+      * output port -y: 600 lines/mm reflective grating at y = -50 tilted by
+        10 degrees about z, one diffraction order `order` per trace (the
+        reference's grating material has a single order; orders -1, 0, +1
+        are three traces);
+      * an opaque 200 mm box around everything.
+    Rays: n_rays collimated along +x from x = -40 on a 4 mm disk, random
+    elliptical polarisation (E1 = cos(chi) e^{i phi1}, E2 = sin(chi) e^{i phi2}).
+    The trace runs to `depth` generations."""
+    from .cfaces import ExtrudedPlanarFace, PolygonFace, RectangularFace, DistortionFace
+    from .cmaterials import FullDielectricMaterial, PartiallyReflectiveMaterial, PECMaterial, \
+        DiffractionGratingMaterial
+    from .cdistortions import ZernikeDistortion
+    rng = np.random.default_rng(seed)
+    a, n_glass = 12.5, sellmeier_n(N_BK7, wavelength)
+
+    def glass_mat(n_in, n_out):
+        return FullDielectricMaterial(n_inside=n_in, n_outside=n_out, reflection_threshold=threshold,
+                                      transmission_threshold=threshold)
+    sq = [(-a, a), (a, a), (a, -a), (-a, -a)]                       # clockwise: outward normals
+    cube_faces = []
+    for (x1, y1), (x2, y2) in zip(sq, sq[1:] + sq[:1]):
+        cube_faces.append(ExtrudedPlanarFace(x1=x1, y1=y1, x2=x2, y2=y2, z1=-a, z2=a,
+                                             material=glass_mat(n_glass, 1.0)))
+    cube_faces.append(PolygonFace(z_plane=a, xy_points=sq, material=glass_mat(1.0, n_glass)))
+    cube_faces.append(PolygonFace(z_plane=-a, xy_points=sq, material=glass_mat(n_glass, 1.0)))
+    cube_faces.append(ExtrudedPlanarFace(x1=-a, y1=-a, x2=a, y2=a, z1=-a, z2=a,
+                                         material=PartiallyReflectiveMaterial(reflectivity=0.5)))
+    cube = FaceList()
+    cube.faces = cube_faces
+    mirror1 = _placed([ShapedPlanarFace(shape=CircleShape(radius=15.0), z_height=0.0, material=PECMaterial())],
+                      [(0, 1, 0), (0, 0, 1), (1, 0, 0)], (50.0, 0.0, 0.0))
+    noll = list(range(4, 16))
+    coefs = rng.normal(0.0, 0.05e-3, len(noll))                    # 0.05 um in mm
+    zd = ZernikeDistortion([(noll_to_ansi(j), c) for j, c in zip(noll, coefs)], unit_radius=15.0)
+    base = ShapedPlanarFace(shape=CircleShape(radius=15.0), z_height=0.0)
+    mirror2 = _placed([DistortionFace(base_face=base, distortion=zd, material=PECMaterial())],
+                      [(0, 0, 1), (1, 0, 0), (0, 1, 0)], (0.0, 50.0, 0.0))
+    th = math.radians(10.0)
+    ex = (math.cos(th), math.sin(th), 0.0)
+    grating = _placed([ShapedPlanarFace(shape=CircleShape(radius=20.0), z_height=0.0,
+                                        material=DiffractionGratingMaterial(lines_per_mm=600, order=order,
+                                                                           efficiency=0.33,
+                                                                           origin=(0.0, -50.0, 0.0)))],
+                      [ex, (0, 0, 1), (math.sin(th), -math.cos(th), 0.0)], (0.0, -50.0, 0.0))
+    b = 100.0
+    box_sq = [(-b, b), (b, b), (b, -b), (-b, -b)]
+    box_faces = [ExtrudedPlanarFace(x1=x1, y1=y1, x2=x2, y2=y2, z1=-b, z2=b, material=OpaqueMaterial())
+                 for (x1, y1), (x2, y2) in zip(box_sq, box_sq[1:] + box_sq[:1])]
+    box_faces += [RectangularFace(z_plane=b, length=2 * b, width=2 * b, material=OpaqueMaterial()),
+                  RectangularFace(z_plane=-b, length=2 * b, width=2 * b, material=OpaqueMaterial())]
+    box = FaceList()
+    box.faces = box_faces
+    # rays
+    rr = 4.0 * np.sqrt(rng.random(n_rays))
+    ph = 2 * np.pi * rng.random(n_rays)
+    r = base_rays(n_rays)
+    r["origin"][:, 0] = -40.0
+    r["origin"][:, 1] = rr * np.cos(ph)
+    r["origin"][:, 2] = rr * np.sin(ph)
+    r["direction"] = (1.0, 0.0, 0.0)
+    psi = 2 * np.pi * rng.random(n_rays)
+    r["E_vector"] = np.stack([np.zeros(n_rays), np.cos(psi), np.sin(psi)], axis=1)
+    chi = 0.5 * np.pi * rng.random(n_rays)
+    phi1, phi2 = 2 * np.pi * rng.random(n_rays), 2 * np.pi * rng.random(n_rays)
+    r["E1_amp"] = np.cos(chi) * np.exp(1j * phi1)
+    r["E2_amp"] = np.sin(chi) * np.exp(1j * phi2)
+    info = {"order": order, "zernike_noll_4_15_mm": [float(c) for c in coefs], "n_glass": n_glass,
+            "threshold": threshold, "seed": seed}
+    return Workload("michelson_c3", r, [wavelength], [cube, mirror1, mirror2, grating, box], depth, 500.0, info)
