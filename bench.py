@@ -343,7 +343,7 @@ def main():
         "detector_hit_max_abs_err_mm": hit_err,
         "record_bytes": 112, "arithmetic": "f32 state and amplitudes; f64 root solves, normals, directions, "
                                           "optical path and phase",
-        "roofline": roofline(r32, measured=None),
+        "roofline": roofline(r32, measured=args.rays),
     }
     out = {
         "metric": METRIC,

@@ -197,21 +197,20 @@ __global__ __launch_bounds__(LB) void k_lens_forward(LensArgs A, double *partial
     }
 }
 
-// out[q] = sum over blocks b of partials[b * width + q], fixed order.
+// out[q] = sum over blocks b of partials[b * width + q], fixed order; one
+// workgroup per column q.
 __global__ __launch_bounds__(256) void k_lens_reduce(const double *partials, uint64_t nb, int width, double *out) {
     __shared__ double red[256];
-    for (int q = 0; q < width; ++q) {
-        double v = 0;
-        for (uint64_t b = threadIdx.x; b < nb; b += 256) v += partials[b * width + q];
-        red[threadIdx.x] = v;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (threadIdx.x < (unsigned)w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[q] = red[0];
+    const int q = blockIdx.x;
+    double v = 0;
+    for (uint64_t b = threadIdx.x; b < nb; b += 256) v += partials[b * width + q];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < (unsigned)w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
+    if (threadIdx.x == 0) out[q] = red[0];
 }
 
 __global__ __launch_bounds__(LB) void k_lens_adjoint(LensArgs A, const double *moments, double *partials) {
@@ -337,7 +336,7 @@ extern "C" rpx_status_t rpx_lens_spot(const rpx_lens_t *lens, const rpx_rays_t *
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_lens_forward, dim3((unsigned)nb), dim3(LB), 0, s, A, (double *)workspace);
     RPX_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_lens_reduce, dim3(1), dim3(256), 0, s, (const double *)workspace, nb, 4, moments);
+    hipLaunchKernelGGL(k_lens_reduce, dim3(4), dim3(256), 0, s, (const double *)workspace, nb, 4, moments);
     RPX_HIP_TRY(hipGetLastError());
     return RPX_OK;
 }
@@ -355,7 +354,7 @@ extern "C" rpx_status_t rpx_lens_spot_grad(const rpx_lens_t *lens, const rpx_ray
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_lens_adjoint, dim3((unsigned)nb), dim3(LB), 0, s, A, moments, (double *)workspace);
     RPX_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_lens_reduce, dim3(1), dim3(256), 0, s, (const double *)workspace, nb,
+    hipLaunchKernelGGL(k_lens_reduce, dim3(6 * lens->n_surfaces), dim3(256), 0, s, (const double *)workspace, nb,
                        6 * lens->n_surfaces, grad);
     RPX_HIP_TRY(hipGetLastError());
     return RPX_OK;
