@@ -319,6 +319,26 @@ rpx_status_t rpx_trace_gausslet(const rpx_scene_t *scene,
                                void *workspace, uint64_t workspace_bytes,
                                void *stream);
 
+/* Capture-face selection: select_ray_intersections (ctracer.pyx:1981-2021)
+ * and select_gausslet_intersections (ctracer.pyx:2024-2058) for ONE
+ * collection of ray_col_list.  Each ray (base ray of each gausslet) is tested
+ * along origin -> origin + direction * length against face list `face_list`
+ * as FaceList.intersect_c does (ctracer.pyx:1882-1904).  A ray whose nearest
+ * accepted face has a global index >= 0 is copied to out[out_offset + k], in
+ * input order, with length and end_face_idx updated and wavelength_idx
+ * replaced by wl_map[wavelength_idx]; `in` is not modified.  The caller's
+ * np.unique re-indexing of the wavelength union (ctracer.pyx:2015-2021) is
+ * the wl_map of each collection.  *n_out = rays selected; RPX_E_CAPACITY if
+ * out_offset + *n_out exceeds out->capacity, RPX_E_INVALID if a selected
+ * ray's wavelength_idx is not below n_wl_map.  Workspace:
+ * rpx_trace_workspace_bytes(n_in). */
+rpx_status_t rpx_select_intersections(const rpx_scene_t *scene, int32_t face_list,
+                               const rpx_rays_t *in, uint64_t n_in, int32_t gausslets,
+                               const uint32_t *wl_map, uint64_t n_wl_map,
+                               rpx_rays_t *out, uint64_t out_offset, uint64_t *n_out,
+                               void *workspace, uint64_t workspace_bytes,
+                               void *stream);
+
 /* ---------------------------------------------------- FP32-split form
  * A reduced-precision ray generation with no counterpart in the reference
  * (which traces in float64 only).  Ray state is stored in float32 -- vectors,

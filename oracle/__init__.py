@@ -47,6 +47,8 @@ def load():
         lib.oracle_evaluate_modes.argtypes = [vp, vp, vp, vp, u64, i32, f64, vp]
         lib.oracle_neighbours_gc.argtypes = [vp, u64, vp, vp, vp, vp]
         lib.oracle_sum_gaussian_modes.argtypes = [vp, u64, vp, vp, u64, vp, u64, f64, vp]
+        lib.oracle_select_intersections.restype = u64
+        lib.oracle_select_intersections.argtypes = [vp, i32, vp, u64, i32, vp, vp]
         lib.oracle_sizes.restype = i32
         lib.oracle_sizes.argtypes = [i32]
         _lib = lib
@@ -82,6 +84,18 @@ def trace_segment(tables, rays, max_length, wavelengths, one_list=-1, one_face=-
                                  _p(out), cap, _p(hits))
     assert n <= cap
     return out[:n].copy(), hits
+
+
+def select_intersections(tables, face_list, rays, wl_map, gausslet=False):
+    """Copies of the rays (gausslets) of one collection whose segment meets
+    face list `face_list`, wavelength_idx mapped through `wl_map`."""
+    lib = load()
+    d = _Desc(tables)
+    rays = np.ascontiguousarray(rays)
+    m = np.ascontiguousarray(wl_map, dtype=np.uint32)
+    out = np.zeros(len(rays) + 1, rays.dtype)
+    n = lib.oracle_select_intersections(d.ref, face_list, _p(rays), len(rays), int(bool(gausslet)), _p(m), _p(out))
+    return out[:n].copy()
 
 
 def trace_gausslet(tables, gs, max_length, wavelengths, one_list=-1, one_face=-1, max_children=3):

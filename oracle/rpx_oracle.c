@@ -1230,6 +1230,32 @@ static int facelist_intersect(const rpx_scene_desc_t *S, int l, oray_t *ray, vec
     return out;
 }
 
+/* select_ray_intersections / select_gausslet_intersections for one
+ * collection (ctracer.pyx:1981-2058): a copy of every ray (base ray of every
+ * gausslet) whose segment origin -> origin + direction * length meets face
+ * list `l` is appended to `out`, with the copy's length and end_face_idx
+ * updated by FaceList.intersect_c and wavelength_idx replaced by
+ * wl_map[wavelength_idx] (the np.unique re-indexing of the caller). */
+uint64_t oracle_select_intersections(const rpx_scene_desc_t *S, int l, const void *in, uint64_t n, int gausslet,
+                                     const uint32_t *wl_map, void *out) {
+    uint64_t k = 0;
+    const size_t rec = gausslet ? sizeof(ogausslet_t) : sizeof(oray_t);
+    for (uint64_t i = 0; i < n; i++) {
+        unsigned char buf[sizeof(ogausslet_t)];
+        memcpy(buf, (const unsigned char *)in + i * rec, rec);
+        oray_t *ray = gausslet ? &((ogausslet_t *)buf)->base_ray : (oray_t *)buf;
+        vec_t point = addvv_(ray->origin, multvs_(ray->direction, ray->length));
+        const int rec_hit = facelist_intersect(S, l, ray, point);
+        /* it.face_idx is the idx of the face last accepted (ctracer.pyx:2007-2009) */
+        if (rec_hit >= 0 && S->faces[rec_hit].global_idx >= 0) {
+            ray->wavelength_idx = wl_map[ray->wavelength_idx];
+            memcpy((unsigned char *)out + k * rec, buf, rec);
+            k++;
+        }
+    }
+    return k;
+}
+
 /* trace_segment_c (ctracer.pyx:2062-2118) / trace_one_face_segment_c (:2121-2170).
  * one_list < 0 selects the non-sequential form.  Returns the number of children
  * (written only up to `cap`). */

@@ -35,6 +35,8 @@ _SIGNATURES = {
                                            P(_abi.Rays), P(c_u64), c_void_p, c_void_p, c_u64, c_void_p]),
     "rpx_trace_gausslet": (c_i32, [c_void_p, c_i32, c_i32, P(_abi.Rays), c_u64, c_f64, c_void_p, c_u64,
                                    P(_abi.Rays), P(c_u64), c_void_p, c_void_p, c_u64, c_void_p]),
+    "rpx_select_intersections": (c_i32, [c_void_p, c_i32, P(_abi.Rays), c_u64, c_i32, c_void_p, c_u64,
+                                         P(_abi.Rays), c_u64, P(c_u64), c_void_p, c_u64, c_void_p]),
     "rpx_rays_to_split": (c_i32, [P(_abi.Rays), c_u64, P(_abi.RaysSplit), c_void_p]),
     "rpx_rays_from_split": (c_i32, [P(_abi.RaysSplit), c_u64, P(_abi.Rays), c_void_p]),
     "rpx_trace_segment_split": (c_i32, [c_void_p, c_i32, c_i32, P(_abi.RaysSplit), c_u64, c_f32, c_void_p, c_u64,

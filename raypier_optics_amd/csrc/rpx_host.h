@@ -10,6 +10,7 @@ struct rpx_scene {
     int32_t n_global_faces;
     int32_t list_sizes[16];
     uint32_t features;        // face / material families present (F_*, M_*)
+    bool negative_face_idx;   // a face has global_idx < 0: usable for selection, not for tracing
     std::vector<rpx_shape_op_t> host_shape_ops;   // for validating query roots
 };
 
@@ -24,6 +25,10 @@ void rpx_timing_add(int slot, float ms, double bytes);
 
 // Records the message for rpx_last_error() and returns `code`.
 rpx_status_t rpx_set_error(rpx_status_t code, const std::string &msg);
+
+// RPX_OK if the scene can be traced (every face has a global index >= 0,
+// which indexes the per-face hit counters), else RPX_E_INVALID.
+rpx_status_t rpx_check_trace_scene(const rpx_scene_t *scene);
 
 #define RPX_HIP_TRY(expr)                                                                     \
     do {                                                                                      \

@@ -430,6 +430,7 @@ extern "C" rpx_status_t rpx_server_destroy(rpx_server_t *s) {
 extern "C" rpx_status_t rpx_server_submit(rpx_server_t *s, const rpx_job_desc_t *d, uint64_t *job_id) {
     if (!s || !d || !job_id) return rpx_set_error(RPX_E_INVALID, "null argument");
     if (!d->scene) return rpx_set_error(RPX_E_INVALID, "job scene is null");
+    if (rpx_status_t st = rpx_check_trace_scene(d->scene)) return st;
     if (!d->rays || d->rays->capacity < d->n_rays || (d->n_rays && (!d->rays->f64 || !d->rays->u32)))
         return rpx_set_error(RPX_E_INVALID, "job rays");
     if (d->n_rays > 0xFFFFFFFFull) return rpx_set_error(RPX_E_UNSUPPORTED, "more than 2^32-1 rays in a job");

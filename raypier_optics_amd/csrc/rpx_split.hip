@@ -201,10 +201,11 @@ extern "C" rpx_status_t rpx_trace_segment_split(const rpx_scene_t *scene, int32_
                                                 const double *wavelengths, uint64_t n_wavelengths,
                                                 rpx_rays_split_t *out, uint64_t *n_out, uint32_t *face_hits,
                                                 void *workspace, uint64_t workspace_bytes, void *stream_) {
-    if (!scene) return rpx_set_error(RPX_E_INVALID, "scene is null");
     if (!n_out) return rpx_set_error(RPX_E_INVALID, "n_out is null");
     *n_out = 0;
-    rpx_status_t st = check_split(in, n_in, "input rays");
+    rpx_status_t st = rpx_check_trace_scene(scene);
+    if (st) return st;
+    st = check_split(in, n_in, "input rays");
     if (st) return st;
     if (!out) return rpx_set_error(RPX_E_INVALID, "output rays are null");
     if (out->capacity > 0 && (!out->f32 || !out->f64 || !out->u32))

@@ -119,7 +119,6 @@ static rpx_status_t validate_scene(const rpx_scene_desc_t *d) {
             return set_error(RPX_E_UNSUPPORTED, "unknown face type " + std::to_string(f.type));
         if (f.material >= d->n_materials) return set_error(RPX_E_INVALID, "face material index");
         if (f.shape >= d->n_shape_ops) return set_error(RPX_E_INVALID, "face shape index");
-        if (f.global_idx < 0) return set_error(RPX_E_INVALID, "negative global face index");
         if (f.type == RPX_FACE_POLYGON &&
             (f.aux1 < 1 || f.aux0 < 0 || f.aux0 + f.aux1 > d->n_vertices))
             return set_error(RPX_E_INVALID, "polygon vertex range");
@@ -259,6 +258,9 @@ extern "C" rpx_status_t rpx_scene_create(const rpx_scene_desc_t *d, rpx_scene_t 
     }
     s->features = feat;
     s->n_global_faces = nglob;
+    s->negative_face_idx = false;
+    for (int i = 0; i < nlist; ++i)
+        if (d->faces[i].global_idx < 0) s->negative_face_idx = true;
     s->host_shape_ops.assign(d->shape_ops, d->shape_ops + d->n_shape_ops);
     for (int l = 0; l < d->n_face_lists; ++l) s->list_sizes[l] = d->face_lists[l].n_faces;
     *out = s;
@@ -709,6 +711,14 @@ extern "C" rpx_status_t rpx_timing_read(double *ms, uint64_t *launches, double *
 }
 
 // ------------------------------------------------------------- host side
+rpx_status_t rpx_check_trace_scene(const rpx_scene_t *scene) {
+    if (!scene) return set_error(RPX_E_INVALID, "scene is null");
+    if (scene->negative_face_idx)
+        return set_error(RPX_E_INVALID, "a face has a negative global index (Face.idx): such a scene can be used "
+                                        "for capture selection but not traced");
+    return RPX_OK;
+}
+
 static rpx_status_t check_rays(const rpx_rays_t *r, uint64_t n, bool gauss, const char *what) {
     if (!r) return set_error(RPX_E_INVALID, std::string(what) + " is null");
     if (r->capacity < n) return set_error(RPX_E_INVALID, std::string(what) + " capacity below count");
@@ -722,10 +732,11 @@ static rpx_status_t run_generation(const rpx_scene_t *scene, int one_list, int o
                                    const double *wavelengths, uint64_t n_wavelengths, rpx_rays_t *out,
                                    uint64_t *n_out, uint32_t *face_hits, void *workspace,
                                    uint64_t workspace_bytes, void *stream_) {
-    if (!scene) return set_error(RPX_E_INVALID, "scene is null");
     if (!n_out) return set_error(RPX_E_INVALID, "n_out is null");
     *n_out = 0;
-    rpx_status_t st = check_rays(in, n_in, gauss, "input rays");
+    rpx_status_t st = rpx_check_trace_scene(scene);
+    if (st) return st;
+    st = check_rays(in, n_in, gauss, "input rays");
     if (st) return st;
     if (!out) return set_error(RPX_E_INVALID, "output rays are null");
     if (n_in == 0) return RPX_OK;

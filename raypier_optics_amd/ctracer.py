@@ -1040,3 +1040,75 @@ def trace_one_face_gausslet(rays, face_set, face_idx, all_faces, max_length=100,
     face_set.sync_transforms()
     ctx = _TraceContext([face_set], all_faces)
     return _generation(ctx, rays, max_length, 0, face_idx)
+
+
+# ------------------------------------------------------ capture selection
+def wavelength_maps(wl_list):
+    """(sorted union of the wavelength tables, one uint32 index map per
+    table): wavelength j of table t becomes union[maps[t][j]] -- the
+    np.unique(..., return_inverse=True) re-indexing of ctracer.pyx:2015-2021,
+    split per collection."""
+    wl_list = [np.asarray(w, np.float64).reshape(-1) for w in wl_list]
+    if not wl_list:
+        return np.zeros(0, np.float64), []
+    reduced, inverse = np.unique(np.concatenate(wl_list), return_inverse=True)
+    inverse = np.asarray(inverse).reshape(-1).astype(np.uint32)
+    maps, off = [], 0
+    for w in wl_list:
+        maps.append(inverse[off: off + len(w)].copy())
+        off += len(w)
+    return reduced, maps
+
+
+def _select_intersections(face_set, ray_col_list, gauss):
+    """Rays (gausslets) of every collection whose segment meets `face_set`,
+    concatenated in list order, with wavelengths re-indexed into the sorted
+    union (ctracer.pyx:1981-2058)."""
+    from ._device import DeviceRays
+    from ._scene import compile_scene
+    lib = load()
+    torch = require_gpu()
+    cls = GaussletCollection if gauss else RayCollection
+    ray_col_list[0]   # the reference sizes its output by the first collection
+    wl_list = [np.asarray(rc.wavelengths, np.float64) for rc in ray_col_list]
+    reduced, maps = wavelength_maps(wl_list)
+    inverse = np.concatenate(maps) if maps else np.zeros(0, np.uint32)
+    face_set.sync_transforms()
+    tables = compile_scene([face_set])
+    handle = tables.handle()
+    total = sum(rc.n_rays for rc in ray_col_list)
+    dev_out = DeviceRays(max(total, 1), gausslet=gauss)
+    s_out = dev_out.struct()
+    wl_map = torch.from_numpy(inverse.astype(np.int32)).to("cuda") if len(inverse) else None
+    need = int(lib.rpx_trace_workspace_bytes(max((rc.n_rays for rc in ray_col_list), default=0)))
+    ws = torch.empty(max(need, 8), dtype=torch.uint8, device="cuda")
+    n_sel, off = 0, 0
+    for rc, wl in zip(ray_col_list, wl_list):
+        n = rc.n_rays
+        if n:
+            s_in = rc._ensure_device().struct()
+            m_ptr = ctypes.c_void_p(wl_map.data_ptr() + 4 * off) if wl_map is not None and len(wl) else None
+            k = ctypes.c_uint64(0)
+            check(lib.rpx_select_intersections(handle, 0, ctypes.byref(s_in), n, int(gauss), m_ptr, len(wl),
+                                               ctypes.byref(s_out), n_sel, ctypes.byref(k),
+                                               ctypes.c_void_p(ws.data_ptr()), ws.numel(), stream_handle()))
+            n_sel += k.value
+        off += len(wl)
+    out = cls.from_device(dev_out, n_sel)
+    out.wavelengths = reduced
+    return out
+
+
+def select_ray_intersections(face_set, ray_col_list):
+    """Copies of the rays of every RayCollection in ray_col_list whose own
+    segment (origin to origin + direction * length) meets a face of
+    face_set, with length and end_face_idx set to the nearest such face and
+    wavelength_idx re-indexed into the sorted union of the collections'
+    wavelengths (ctracer.pyx:1981-2021).  The collections are not modified."""
+    return _select_intersections(face_set, ray_col_list, False)
+
+
+def select_gausslet_intersections(face_set, ray_col_list):
+    """select_ray_intersections for GaussletCollections: the base ray decides
+    and is updated; parabasal rays are copied unchanged (ctracer.pyx:2024-2058)."""
+    return _select_intersections(face_set, ray_col_list, True)
