@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RPX_ABI_VERSION 5
+#define RPX_ABI_VERSION 6
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t rpx_status_t;
@@ -338,6 +338,54 @@ rpx_status_t rpx_select_intersections(const rpx_scene_t *scene, int32_t face_lis
                                rpx_rays_t *out, uint64_t out_offset, uint64_t *n_out,
                                void *workspace, uint64_t workspace_bytes,
                                void *stream);
+
+/* ------------------------------------------------ generation consumers
+ * Walks over the generations of one trace (rpx_trace_segment outputs, in
+ * trace order), resident in HBM.  A generation is its SoA buffer and count. */
+typedef struct rpx_generation {
+    rpx_rays_t rays;
+    uint64_t   n_rays;
+} rpx_generation_t;
+
+/* Workspace of rpx_path_lengths / rpx_face_histogram (n_last: rays of the
+ * last generation; 0 for the histogram). */
+uint64_t rpx_lineage_workspace_bytes(int32_t n_generations, uint64_t n_last);
+
+/* MeanOpticalPathLength._calc_result (results.py:112-122) without the final
+ * mean: for every ray of the last generation with end_face_idx ==
+ * target_face, in ray order, totals[k] = sum over the ray and its ancestors
+ * (following parent_idx back to generation 0) of length * Re(n), summed from
+ * the last generation backwards, each product rounded before its addition.
+ * *n_selected = number of such rays; RPX_E_CAPACITY if above capacity;
+ * RPX_E_INVALID if a parent_idx on a walked chain is outside the previous
+ * generation.  The reference's result is the mean of totals. */
+rpx_status_t rpx_path_lengths(const rpx_generation_t *generations, int32_t n_generations,
+                              uint32_t target_face, double *totals, uint64_t capacity,
+                              uint64_t *n_selected, void *workspace, uint64_t workspace_bytes,
+                              void *stream);
+
+/* End-face histogram of every generation, the vote of set_face_sequence
+ * (sources.py:138-155: the most common end_face_idx of each generation).
+ * counts (device, n_generations * (n_bins + 2) uint64):
+ * counts[g*(n_bins+2) + b] = rays of generation g with end_face_idx == b for
+ * b < n_bins; b = n_bins counts end_face_idx == 0xFFFFFFFF (no face);
+ * b = n_bins + 1 counts every other value.  n_bins <= 4094. */
+rpx_status_t rpx_face_histogram(const rpx_generation_t *generations, int32_t n_generations,
+                                uint32_t n_bins, uint64_t *counts, void *workspace,
+                                uint64_t workspace_bytes, void *stream);
+
+/* RayCollection neighbours of a child generation from those of its parent
+ * generation (ctracer.pyx:1084-1106 _eval_neighbours): with rmap[p][t] the
+ * last child (highest index) of parent p whose ray_type_id & REFL_RAY is t,
+ * neighbours[i*K + j] = rmap[c][t_i] for c = parent_neighbours[p_i*K + j] >= 0,
+ * else -1 (K = n_neighbours; row-major int32 arrays on the device).
+ * RPX_E_INVALID if a parent_idx or neighbour index is outside the parent
+ * generation.  Workspace: rpx_neighbours_workspace_bytes(n_parents). */
+uint64_t rpx_neighbours_workspace_bytes(uint64_t n_parents);
+rpx_status_t rpx_propagate_neighbours(const rpx_rays_t *children, uint64_t n_children,
+                                      const int32_t *parent_neighbours, uint64_t n_parents,
+                                      int32_t n_neighbours, int32_t *neighbours,
+                                      void *workspace, uint64_t workspace_bytes, void *stream);
 
 /* ---------------------------------------------------- FP32-split form
  * A reduced-precision ray generation with no counterpart in the reference

@@ -49,6 +49,8 @@ def load():
         lib.oracle_sum_gaussian_modes.argtypes = [vp, u64, vp, vp, u64, vp, u64, f64, vp]
         lib.oracle_select_intersections.restype = u64
         lib.oracle_select_intersections.argtypes = [vp, i32, vp, u64, i32, vp, vp]
+        lib.oracle_eval_neighbours.restype = i32
+        lib.oracle_eval_neighbours.argtypes = [vp, vp, u64, vp, u64, i32, vp, vp]
         lib.oracle_sizes.restype = i32
         lib.oracle_sizes.argtypes = [i32]
         _lib = lib
@@ -96,6 +98,49 @@ def select_intersections(tables, face_list, rays, wl_map, gausslet=False):
     out = np.zeros(len(rays) + 1, rays.dtype)
     n = lib.oracle_select_intersections(d.ref, face_list, _p(rays), len(rays), int(bool(gausslet)), _p(m), _p(out))
     return out[:n].copy()
+
+
+def eval_neighbours(children, parent_nb):
+    """Neighbour table of a child generation (ray_dtype array) from its
+    parent generation's (n_parent, K) table."""
+    lib = load()
+    pnb = np.ascontiguousarray(parent_nb, dtype=np.int32)
+    n_par, k = pnb.shape
+    pidx = np.ascontiguousarray(children["parent_idx"], dtype=np.uint32)
+    rtype = np.ascontiguousarray(children["ray_type_id"], dtype=np.uint32)
+    nb = np.zeros((len(children), k), np.int32)
+    rmap = np.zeros(max(2 * n_par, 1), np.int32)
+    if lib.oracle_eval_neighbours(_p(pidx), _p(rtype), len(children), _p(pnb), n_par, k, _p(rmap), _p(nb)):
+        raise IndexError("parent or neighbour index outside the parent generation")
+    return nb
+
+
+def path_lengths(generations, idx):
+    """MeanOpticalPathLength._calc_result (results.py:112-122) before the
+    mean: `generations` are ray_dtype arrays in trace order."""
+    all_rays = list(reversed(generations))
+    last = all_rays[0]
+    selected_idx = np.argwhere(last["end_face_idx"] == idx).ravel()
+    total = np.zeros(len(selected_idx), "d")
+    for ray in all_rays:
+        selected = ray[selected_idx]
+        total += selected["length"] * selected["refractive_index"].real
+        selected_idx = selected["parent_idx"]
+    return total
+
+
+def face_sequence(end_face_idx_per_generation, face_map):
+    """set_face_sequence (sources.py:143-155) over the end_face_idx arrays of
+    the generations; face_map maps a global face index to its entry."""
+    out = []
+    for face_ids in end_face_idx_per_generation:
+        ids, counts = np.unique(face_ids, return_counts=True)
+        most_common = ids[counts.argmax()]
+        try:
+            out.append(face_map[most_common])
+        except KeyError:
+            break
+    return out
 
 
 def trace_gausslet(tables, gs, max_length, wavelengths, one_list=-1, one_face=-1, max_children=3):

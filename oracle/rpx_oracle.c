@@ -1256,6 +1256,33 @@ uint64_t oracle_select_intersections(const rpx_scene_desc_t *S, int l, const voi
     return k;
 }
 
+/* RayCollection._eval_neighbours (ctracer.pyx:1084-1106): rmap[p][t] is the
+ * last child of parent p with ray_type_id & REFL_RAY == t; child i's
+ * neighbour j is rmap[pnb[p_i][j]][t_i] where pnb[p_i][j] >= 0, else -1.
+ * `rmap` is caller scratch of 2 * n_parent ints.  Returns 0, or -1 for an
+ * index outside the parent generation (the reference raises IndexError). */
+int oracle_eval_neighbours(const uint32_t *parent_idx, const uint32_t *ray_type_id, uint64_t n_child,
+                           const int32_t *pnb, uint64_t n_parent, int k, int32_t *rmap, int32_t *nb) {
+    for (uint64_t q = 0; q < 2 * n_parent; q++) rmap[q] = -1;
+    for (uint64_t q = 0; q < n_child * (uint64_t)k; q++) nb[q] = -1;
+    for (uint64_t i = 0; i < n_child; i++) {
+        if (parent_idx[i] >= n_parent) return -1;
+        rmap[2 * (uint64_t)parent_idx[i] + (ray_type_id[i] & 1u)] = (int32_t)i;
+    }
+    for (uint64_t i = 0; i < n_child; i++) {
+        const uint64_t pidx = parent_idx[i];
+        const uint32_t rtype = ray_type_id[i] & 1u;
+        for (int j = 0; j < k; j++) {
+            const int32_t child_nb = pnb[pidx * (uint64_t)k + j];
+            if (child_nb >= 0) {
+                if ((uint64_t)child_nb >= n_parent) return -1;
+                nb[i * (uint64_t)k + j] = rmap[2 * (uint64_t)child_nb + rtype];
+            }
+        }
+    }
+    return 0;
+}
+
 /* trace_segment_c (ctracer.pyx:2062-2118) / trace_one_face_segment_c (:2121-2170).
  * one_list < 0 selects the non-sequential form.  Returns the number of children
  * (written only up to `cap`). */

@@ -33,7 +33,7 @@ gausslet_dtype = np.dtype([('base_ray', ray_dtype),
 
 assert ray_dtype.itemsize == 188 and para_dtype.itemsize == 80 and gausslet_dtype.itemsize == 668
 
-ABI_VERSION = 5          # RPX_ABI_VERSION of include/rpx.h
+ABI_VERSION = 6          # RPX_ABI_VERSION of include/rpx.h
 
 # timing slots (RPX_TIMING_*)
 TIMING_KERNELS = ("k_intersect", "k_scan", "k_emit", "k_generation", "k_clear", "k_generation_split")
@@ -178,3 +178,11 @@ STRUCT_SIZES["rpx_rays_split_t"] = ctypes.sizeof(RaysSplit)
 STRUCT_SIZES["rpx_job_desc_t"] = ctypes.sizeof(JobDesc)
 STRUCT_SIZES["rpx_job_result_t"] = ctypes.sizeof(JobResult)
 STRUCT_SIZES["rpx_lens_t"] = ctypes.sizeof(Lens)
+
+
+class Generation(ctypes.Structure):
+    """rpx_generation_t"""
+    _fields_ = [("rays", Rays), ("n_rays", ctypes.c_uint64)]
+
+
+STRUCT_SIZES["rpx_generation_t"] = ctypes.sizeof(Generation)

@@ -52,6 +52,13 @@ _SIGNATURES = {
                               c_void_p]),
     "rpx_lens_spot_grad": (c_i32, [P(_abi.Lens), P(_abi.Rays), c_u64, c_void_p, c_u64, c_void_p, c_void_p,
                                    c_void_p, c_u64, c_void_p]),
+    "rpx_lineage_workspace_bytes": (c_u64, [c_i32, c_u64]),
+    "rpx_path_lengths": (c_i32, [P(_abi.Generation), c_i32, c_u32, c_void_p, c_u64, P(c_u64), c_void_p, c_u64,
+                                 c_void_p]),
+    "rpx_face_histogram": (c_i32, [P(_abi.Generation), c_i32, c_u32, c_void_p, c_void_p, c_u64, c_void_p]),
+    "rpx_neighbours_workspace_bytes": (c_u64, [c_u64]),
+    "rpx_propagate_neighbours": (c_i32, [P(_abi.Rays), c_u64, c_void_p, c_u64, c_i32, c_void_p, c_void_p, c_u64,
+                                         c_void_p]),
     "rpx_timing_enable": (c_i32, [c_i32]),
     "rpx_timing_read": (c_i32, [c_void_p, c_void_p, c_void_p]),
     "rpx_query_face": (c_i32, [c_void_p, c_i32, c_i32, c_void_p, c_void_p, c_u64, c_i32, c_void_p, c_void_p]),

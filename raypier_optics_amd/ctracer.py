@@ -8,6 +8,7 @@ trace loop (raypier/core/ctracer.pyx:2062-2385).  Small-object conveniences
 import ctypes
 import itertools
 import math
+import time
 
 import numpy as np
 
@@ -524,7 +525,9 @@ class RayCollection(_Collection):
 
     def __init__(self, max_size=0):
         super().__init__(max_size)
-        self._neighbours = None
+        self._neighbours = None     # host (n, K) int32 array, or None
+        self._nb_dev = None         # device copy / result of propagation, or None
+        self._mtime = 0.0
 
     def _ray_field(self, name):
         return self._base_field(name)
@@ -553,13 +556,71 @@ class RayCollection(_Collection):
     def base_rays(self):
         return self
 
+    # neighbours ---------------------------------------------------------
+    # Neighbour indices (n_rays, K) int32 (-1 = none) are given for source
+    # rays and derived for every traced generation from its parent's, on the
+    # GPU, when first asked for; a derived table is re-derived when an
+    # ancestor's table is newer (ctracer.pyx:1068-1131).
+    @classmethod
+    def _wrap_device(cls, dev, n):
+        obj = super()._wrap_device(dev, n)
+        obj._neighbours = None
+        obj._nb_dev = None
+        obj._mtime = 0.0
+        return obj
+
+    @property
+    def parent(self):
+        return self._parent
+
+    @parent.setter
+    def parent(self, rc):
+        _Collection.parent.fset(self, rc)
+        self._nb_dev = None
+
+    def _get_mtime(self, guard):
+        if self._parent is None or guard == id(self):
+            return self._mtime
+        pmtime = self._parent._get_mtime(guard)
+        if pmtime > self._mtime:
+            self._neighbours = None
+            self._nb_dev = None
+            return pmtime
+        return self._mtime
+
+    def _has_neighbours(self):
+        return self._neighbours is not None or self._nb_dev is not None
+
+    def _neighbours_device(self):
+        """The neighbour table as an int32 (n_rays, K) cuda tensor, or None."""
+        if self._parent is not None:
+            pmtime = self._parent._get_mtime(id(self))
+            if not (self._mtime >= pmtime and self._has_neighbours()):
+                pnb = self._parent._neighbours_device()
+                if pnb is not None:
+                    self._nb_dev = _propagate_neighbours(self, pnb)
+                    self._neighbours = None
+                self._mtime = time.monotonic()
+        if self._nb_dev is None and self._neighbours is not None:
+            torch = require_gpu()
+            nb = np.ascontiguousarray(self._neighbours, dtype=np.int32)
+            self._nb_dev = torch.from_numpy(nb.reshape(nb.shape[0], -1)).to("cuda")
+        return self._nb_dev
+
     @property
     def neighbours(self):
-        return None if self._neighbours is None else np.asarray(self._neighbours)
+        if self._parent is None and self._nb_dev is None:
+            return None if self._neighbours is None else np.asarray(self._neighbours)
+        dev = self._neighbours_device()
+        if self._neighbours is None and dev is not None:
+            self._neighbours = dev.cpu().numpy()
+        return np.asarray(self._neighbours)
 
     @neighbours.setter
     def neighbours(self, nb):
-        self._neighbours = nb
+        self._neighbours = np.asarray(nb, dtype=np.int32)
+        self._nb_dev = None
+        self._mtime = time.monotonic()
 
     @classmethod
     def from_array(cls, data):
@@ -1040,6 +1101,25 @@ def trace_one_face_gausslet(rays, face_set, face_idx, all_faces, max_length=100,
     face_set.sync_transforms()
     ctx = _TraceContext([face_set], all_faces)
     return _generation(ctx, rays, max_length, 0, face_idx)
+
+
+def _propagate_neighbours(children, parent_nb):
+    """Neighbours of `children` from its parent generation's table (device
+    int32 (n_parent, K)): RayCollection._eval_neighbours, ctracer.pyx:1084-1106."""
+    torch = require_gpu()
+    lib = load()
+    n, K = children.n_rays, int(parent_nb.shape[1]) if parent_nb.dim() == 2 else 0
+    out = torch.full((n, K), -1, dtype=torch.int32, device="cuda")
+    if n == 0 or K == 0:
+        return out
+    parent_nb = parent_nb.contiguous()
+    s = children._ensure_device().struct()
+    n_par = int(parent_nb.shape[0])
+    ws = torch.empty(int(lib.rpx_neighbours_workspace_bytes(n_par)), dtype=torch.uint8, device="cuda")
+    check(lib.rpx_propagate_neighbours(ctypes.byref(s), n, ctypes.c_void_p(parent_nb.data_ptr()), n_par, K,
+                                       ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                                       stream_handle()))
+    return out
 
 
 # ------------------------------------------------------ capture selection
