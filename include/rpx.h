@@ -387,6 +387,79 @@ rpx_status_t rpx_propagate_neighbours(const rpx_rays_t *children, uint64_t n_chi
                                       int32_t n_neighbours, int32_t *neighbours,
                                       void *workspace, uint64_t workspace_bytes, void *stream);
 
+/* --------------------------------------------------------------- sources
+ * Hexagonal lattices of source rays generated in HBM.  Lattice indices vi, vj
+ * run over [-nsteps-1, nsteps+1] in numpy.meshgrid order (label = row * side
+ * + column with side = 2*nsteps + 3, the row giving vj); selected points
+ * keep label order.  The host computes the scalars and vectors exactly as the
+ * reference does (numpy) and the kernels evaluate the per-point expressions
+ * in the reference's order.
+ *   RPX_LATTICE_RAY_FIELD (HexagonalRayFieldSource sources.py:1035-1107,
+ *   CollimatedGaussletSource gausslet_sources.py:386-457):
+ *     r = (vi + sin30*vj)^2 + cos30*vj^2, selected if r < limit;
+ *     origin = (vi*d1 + vj*d2) * spacing + origin; direction, E_vector given.
+ *   RPX_LATTICE_POINT_SOURCE (GaussianPointSource gausslet_sources.py:516-566,
+ *   make_hexagonal_grid core/gausslets.py:55-80):
+ *     kx = (vi + sin30*vj)*spacing, ky = (cos30*vj)*spacing,
+ *     r = kx^2 + ky^2, selected if r < limit; kz = sqrt(k_abs_sq - r);
+ *     direction = normalised ((kx*d1 + ky*d2) + kz*direction) / k_abs;
+ *     E_vector = direction x d2; origin = origin.
+ *   Both: weight g = exp(-((r * weight_num) / weight_den)); E1/E2 = the
+ *   complex amplitude times g; normal (0,1,0); n = 1; other fields 0;
+ *   ray_ident = output index if ray_ident_index, else 0. */
+#define RPX_LATTICE_RAY_FIELD    0
+#define RPX_LATTICE_POINT_SOURCE 1
+
+typedef struct rpx_lattice {
+    int32_t  kind;
+    int32_t  nsteps;
+    double   sin30, cos30;
+    double   limit;
+    double   spacing;
+    double   origin[3];
+    double   d1[3], d2[3];
+    double   direction[3];
+    double   E_vector[3];
+    double   weight_num, weight_den;
+    double   E1[2], E2[2];           /* (re, im) */
+    double   k_abs, k_abs_sq;
+    double   power;                  /* P of the weight power sum, 0 = skip */
+    uint32_t ray_type_id;
+    int32_t  ray_ident_index;
+} rpx_lattice_t;
+
+uint64_t rpx_lattice_workspace_bytes(int32_t nsteps);
+/* Numbers the selected points in the workspace; *n_points = their count;
+ * *weight_power (optional) = sum over them of power * g^2 (the beam power
+ * normalisation of CollimatedGaussletSource, gausslet_sources.py:430-432,
+ * summed in a fixed order). */
+rpx_status_t rpx_lattice_select(const rpx_lattice_t *lattice, uint64_t *n_points, double *weight_power,
+                                void *workspace, uint64_t workspace_bytes, void *stream);
+/* Writes the selected points (after rpx_lattice_select on the same
+ * workspace and lattice) into `out`; with out->para the six parabasal rays
+ * copy the base ray (GaussletCollection.from_rays, ctracer.pyx:1321-1345);
+ * with `neighbours` (device int32, n_points x 6) the lattice neighbours
+ * (sources.py:1063-1085), -1 where unselected or off the lattice. */
+rpx_status_t rpx_lattice_emit(const rpx_lattice_t *lattice, rpx_rays_t *out, int32_t *neighbours,
+                              void *workspace, uint64_t workspace_bytes, void *stream);
+
+/* GaussletCollection.config_parabasal_rays (ctracer.pyx:1430-1481) on
+ * gausslets in HBM.  wavelengths: device table indexed by wavelength_idx;
+ * angle_cos_sin (host): cos and sin of 0, 2pi/3 and 4pi/3 as the reference's
+ * libm computes them, interleaved (c0, s0, c1, s1, c2, s2).  Workspace: 8 B.
+ * RPX_E_INVALID if a wavelength_idx is outside the table. */
+rpx_status_t rpx_config_parabasal_rays(rpx_rays_t *gausslets, uint64_t n, const double *wavelengths,
+                                       uint64_t n_wavelengths, double radius, double working_dist,
+                                       const double *angle_cos_sin, void *workspace,
+                                       uint64_t workspace_bytes, void *stream);
+
+/* GaussletCollection.project_to_plane (ctracer.pyx:1391-1420): every base
+ * ray (and parabasal ray, when rays->para) moves along its direction to the
+ * plane through `origin` with normal `direction` (host 3-vectors); the base
+ * ray's accumulated_path grows by Re(n) times the distance moved. */
+rpx_status_t rpx_project_to_plane(rpx_rays_t *rays, uint64_t n, const double *origin,
+                                  const double *direction, void *stream);
+
 /* ---------------------------------------------------- FP32-split form
  * A reduced-precision ray generation with no counterpart in the reference
  * (which traces in float64 only).  Ray state is stored in float32 -- vectors,

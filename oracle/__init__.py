@@ -51,6 +51,9 @@ def load():
         lib.oracle_select_intersections.argtypes = [vp, i32, vp, u64, i32, vp, vp]
         lib.oracle_eval_neighbours.restype = i32
         lib.oracle_eval_neighbours.argtypes = [vp, vp, u64, vp, u64, i32, vp, vp]
+        lib.oracle_config_parabasal_rays.restype = i32
+        lib.oracle_config_parabasal_rays.argtypes = [vp, u64, vp, u64, f64, f64]
+        lib.oracle_project_to_plane.argtypes = [vp, u64, vp, vp]
         lib.oracle_sizes.restype = i32
         lib.oracle_sizes.argtypes = [i32]
         _lib = lib
@@ -141,6 +144,25 @@ def face_sequence(end_face_idx_per_generation, face_map):
         except KeyError:
             break
     return out
+
+
+def config_parabasal_rays(gausslets, wavelength_list, radius, working_dist):
+    """A copy of the gausslet_dtype array with its parabasal rays configured."""
+    lib = load()
+    g = np.ascontiguousarray(gausslets).copy()
+    wl = np.ascontiguousarray(wavelength_list, dtype=np.float64)
+    if lib.oracle_config_parabasal_rays(_p(g), len(g), _p(wl), len(wl), float(radius), float(working_dist)):
+        raise IndexError("wavelength_idx outside the wavelength list")
+    return g
+
+
+def project_to_plane(gausslets, origin, direction):
+    lib = load()
+    g = np.ascontiguousarray(gausslets).copy()
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    d = np.ascontiguousarray(direction, dtype=np.float64)
+    lib.oracle_project_to_plane(_p(g), len(g), _p(o), _p(d))
+    return g
 
 
 def trace_gausslet(tables, gs, max_length, wavelengths, one_list=-1, one_face=-1, max_children=3):

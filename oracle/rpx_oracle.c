@@ -1283,6 +1283,58 @@ int oracle_eval_neighbours(const uint32_t *parent_idx, const uint32_t *ray_type_
     return 0;
 }
 
+/* GaussletCollection.config_parabasal_rays (ctracer.pyx:1430-1481). */
+int oracle_config_parabasal_rays(ogausslet_t *gs, uint64_t n, const double *wavelength_list, uint64_t n_wl,
+                                 double radius, double working_dist) {
+    for (uint64_t i = 0; i < n; i++) {
+        ogausslet_t *gc = gs + i;
+        vec_t base_d = norm_(gc->base_ray.direction);
+        vec_t o;
+        if (base_d.x > base_d.y) o = V(0.0, 1.0, 0.0);
+        else o = V(1.0, 0.0, 0.0);
+        vec_t d1 = norm_(cross_(base_d, o));
+        vec_t d2 = norm_(cross_(base_d, d1));
+        if (gc->base_ray.wavelength_idx >= n_wl) return -1;
+        double theta0 = wavelength_list[gc->base_ray.wavelength_idx] / (M_PI * radius * 1000.0);
+        for (int j = 0; j < 6; j += 2) {
+            double angle = (j * 2 * M_PI / 6);
+            o = addvv_(multvs_(d1, radius * cos(angle)), multvs_(d2, radius * sin(angle)));
+            o = addvv_(o, gc->base_ray.origin);
+            o = addvv_(o, multvs_(base_d, working_dist));
+            vec_t d = addvv_(multvs_(d1, -theta0 * sin(angle)), multvs_(d2, theta0 * cos(angle)));
+            vec_t da = addvv_(base_d, d);
+            vec_t db = subvv_(base_d, d);
+            gc->para[j].direction = norm_(da);
+            gc->para[j].origin = subvv_(o, multvs_(da, working_dist));
+            gc->para[j + 1].direction = norm_(db);
+            gc->para[j + 1].origin = subvv_(o, multvs_(db, working_dist));
+            gc->para[j].normal = gc->base_ray.normal;
+            gc->para[j + 1].normal = gc->base_ray.normal;
+            gc->para[j].length = gc->base_ray.length;
+            gc->para[j + 1].length = gc->base_ray.length;
+        }
+    }
+    return 0;
+}
+
+/* GaussletCollection.project_to_plane (ctracer.pyx:1391-1420). */
+void oracle_project_to_plane(ogausslet_t *gs, uint64_t n, const double *origin, const double *direction) {
+    vec_t o = V(origin[0], origin[1], origin[2]);
+    vec_t d = norm_(V(direction[0], direction[1], direction[2]));
+    for (uint64_t i = 0; i < n; i++) {
+        ogausslet_t *gc = gs + i;
+        oray_t *ray = &gc->base_ray;
+        double a = dotprod_(subvv_(o, ray->origin), d) / dotprod_(ray->direction, d);
+        ray->origin = addvv_(ray->origin, multvs_(ray->direction, a));
+        ray->accumulated_path += ray->n_re * a;
+        for (int j = 0; j < 6; j++) {
+            opara_t *para = &gc->para[j];  /* ctracer.pyx:1416-1419 */
+            a = dotprod_(subvv_(o, para->origin), d) / dotprod_(para->direction, d);
+            para->origin = addvv_(para->origin, multvs_(para->direction, a));
+        }
+    }
+}
+
 /* trace_segment_c (ctracer.pyx:2062-2118) / trace_one_face_segment_c (:2121-2170).
  * one_list < 0 selects the non-sequential form.  Returns the number of children
  * (written only up to `cap`). */

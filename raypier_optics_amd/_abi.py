@@ -186,3 +186,21 @@ class Generation(ctypes.Structure):
 
 
 STRUCT_SIZES["rpx_generation_t"] = ctypes.sizeof(Generation)
+
+LATTICE_RAY_FIELD = 0
+LATTICE_POINT_SOURCE = 1
+_d3 = ctypes.c_double * 3
+
+
+class Lattice(ctypes.Structure):
+    """rpx_lattice_t"""
+    _fields_ = [("kind", ctypes.c_int32), ("nsteps", ctypes.c_int32), ("sin30", ctypes.c_double),
+                ("cos30", ctypes.c_double), ("limit", ctypes.c_double), ("spacing", ctypes.c_double),
+                ("origin", _d3), ("d1", _d3), ("d2", _d3), ("direction", _d3), ("E_vector", _d3),
+                ("weight_num", ctypes.c_double), ("weight_den", ctypes.c_double),
+                ("E1", ctypes.c_double * 2), ("E2", ctypes.c_double * 2), ("k_abs", ctypes.c_double),
+                ("k_abs_sq", ctypes.c_double), ("power", ctypes.c_double), ("ray_type_id", ctypes.c_uint32),
+                ("ray_ident_index", ctypes.c_int32)]
+
+
+STRUCT_SIZES["rpx_lattice_t"] = ctypes.sizeof(Lattice)

@@ -59,6 +59,12 @@ _SIGNATURES = {
     "rpx_neighbours_workspace_bytes": (c_u64, [c_u64]),
     "rpx_propagate_neighbours": (c_i32, [P(_abi.Rays), c_u64, c_void_p, c_u64, c_i32, c_void_p, c_void_p, c_u64,
                                          c_void_p]),
+    "rpx_lattice_workspace_bytes": (c_u64, [c_i32]),
+    "rpx_lattice_select": (c_i32, [P(_abi.Lattice), P(c_u64), P(c_f64), c_void_p, c_u64, c_void_p]),
+    "rpx_lattice_emit": (c_i32, [P(_abi.Lattice), P(_abi.Rays), c_void_p, c_void_p, c_u64, c_void_p]),
+    "rpx_config_parabasal_rays": (c_i32, [P(_abi.Rays), c_u64, c_void_p, c_u64, c_f64, c_f64, P(c_f64), c_void_p,
+                                          c_u64, c_void_p]),
+    "rpx_project_to_plane": (c_i32, [P(_abi.Rays), c_u64, P(c_f64), P(c_f64), c_void_p]),
     "rpx_timing_enable": (c_i32, [c_i32]),
     "rpx_timing_read": (c_i32, [c_void_p, c_void_p, c_void_p]),
     "rpx_query_face": (c_i32, [c_void_p, c_i32, c_i32, c_void_p, c_void_p, c_u64, c_i32, c_void_p, c_void_p]),

@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librpx.so")
 SOURCES = ["rpx_trace.hip", "rpx_query.hip", "rpx_fields.hip", "rpx_split.hip", "rpx_serve.hip", "rpx_adjoint.hip",
-           "rpx_select.hip", "rpx_lineage.hip"]
+           "rpx_select.hip", "rpx_lineage.hip",
+           "rpx_sources.hip"]
 # per-source flags that replace -ffp-contract=off
 SOURCE_FLAGS = {
     "rpx_split.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],

@@ -50,6 +50,7 @@ extern "C" int32_t rpx_struct_size(const char *name) {
     if (n == "rpx_zernike_t") return sizeof(rpx_zernike_t);
     if (n == "rpx_rays_t") return sizeof(rpx_rays_t);
     if (n == "rpx_generation_t") return sizeof(rpx_generation_t);
+    if (n == "rpx_lattice_t") return sizeof(rpx_lattice_t);
     if (n == "rpx_scene_desc_t") return sizeof(rpx_scene_desc_t);
     if (n == "rpx_rays_split_t") return sizeof(rpx_rays_split_t);
     if (n == "rpx_job_desc_t") return sizeof(rpx_job_desc_t);

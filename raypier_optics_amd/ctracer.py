@@ -765,70 +765,38 @@ class GaussletCollection(_Collection):
         self._host_changed()
 
     def project_to_plane(self, origin, direction):
-        """Move every ray to its intersection with a plane (ctracer.pyx:1391-1420)."""
-        self._ensure_host()
-        o = np.asarray(origin, dtype=np.float64)
-        d = np.asarray(norm(direction), dtype=np.float64)
-        g = self._host[: self._n]
-        b = g["base_ray"]
-        a = ((o - b["origin"]) @ d) / (b["direction"] @ d)
-        b["origin"] = b["origin"] + b["direction"] * a[:, None]
-        b["accumulated_path"] = b["accumulated_path"] + b["refractive_index"].real * a
-        g["base_ray"] = b
-        p = g["para_rays"]
-        ap = ((o - p["origin"]) @ d) / (p["direction"] @ d)
-        p["origin"] = p["origin"] + p["direction"] * ap[..., None]
-        g["para_rays"] = p
-        self._host[: self._n] = g
-        self._host_changed()
+        """Move every base and parabasal ray along its direction to the plane
+        through `origin` with normal `direction` (ctracer.pyx:1391-1420), on
+        the GPU."""
+        lib = load()
+        if self._n == 0:
+            return
+        s = self._ensure_device().struct()
+        o = (ctypes.c_double * 3)(*[float(x) for x in origin])
+        d = (ctypes.c_double * 3)(*[float(x) for x in direction])
+        check(lib.rpx_project_to_plane(ctypes.byref(s), self._n, o, d, stream_handle()))
+        self._device_changed()
 
     def config_parabasal_rays(self, wavelength_list, radius, working_dist):
         """Place the parabasal rays of a circular Gaussian mode of 1/e^2 radius
         `radius` (mm) with its waist `working_dist` from the base origin
-        (ctracer.pyx:1430-1481), for all gausslets at once.  Each element
-        goes through the same operations as the reference's per-gausslet
-        loop: norm() divides by the magnitude, cross products and sums in
-        the same order."""
-        self._ensure_host()
-        n = self._n
-        if n == 0:
+        (ctracer.pyx:1430-1481), for all gausslets at once on the GPU."""
+        torch = require_gpu()
+        lib = load()
+        if self._n == 0:
             return
-        wl = np.asarray(wavelength_list, dtype=np.float64)
-        g = self._host[:n]
-        b = g["base_ray"]
-
-        def nrm(v):
-            m = np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2])
-            return v / m[:, None]
-
-        def crs(a, c):
-            return np.stack([a[:, 1] * c[:, 2] - a[:, 2] * c[:, 1], a[:, 2] * c[:, 0] - a[:, 0] * c[:, 2],
-                             a[:, 0] * c[:, 1] - a[:, 1] * c[:, 0]], axis=1)
-
-        base_d = nrm(np.array(b["direction"], dtype=np.float64))
-        o = np.where((base_d[:, 0] > base_d[:, 1])[:, None], np.array([0.0, 1.0, 0.0]), np.array([1.0, 0.0, 0.0]))
-        d1 = nrm(crs(base_d, o))
-        d2 = nrm(crs(base_d, d1))
-        theta0 = wl[b["wavelength_idx"]] / (math.pi * radius * 1000.0)
-        origin = np.array(b["origin"], dtype=np.float64)
-        para = g["para_rays"]
+        wl = torch.from_numpy(np.ascontiguousarray(wavelength_list, dtype=np.float64)).to("cuda")
+        s = self._ensure_device().struct()
+        trig = []
         for j in range(0, 6, 2):
             angle = (j * 2 * math.pi / 6)
-            ca, sa = math.cos(angle), math.sin(angle)
-            oo = d1 * (radius * ca) + d2 * (radius * sa)
-            oo = oo + origin
-            oo = oo + base_d * working_dist
-            dd = d1 * (-theta0 * sa)[:, None] + d2 * (theta0 * ca)[:, None]
-            da = base_d + dd
-            db = base_d - dd
-            para["direction"][:, j] = nrm(da)
-            para["origin"][:, j] = oo - da * working_dist
-            para["direction"][:, j + 1] = nrm(db)
-            para["origin"][:, j + 1] = oo - db * working_dist
-            for k in (j, j + 1):
-                para["normal"][:, k] = b["normal"]
-                para["length"][:, k] = b["length"]
-        self._host_changed()
+            trig += [math.cos(angle), math.sin(angle)]
+        trig = (ctypes.c_double * 6)(*trig)
+        ws = torch.empty(8, dtype=torch.uint8, device="cuda")
+        check(lib.rpx_config_parabasal_rays(ctypes.byref(s), self._n, ctypes.c_void_p(wl.data_ptr()), wl.numel(),
+                                            float(radius), float(working_dist), trig, ctypes.c_void_p(ws.data_ptr()),
+                                            ws.numel(), stream_handle()))
+        self._device_changed()
 
 
 # ---------------------------------------------------------- plugin bases
