@@ -679,6 +679,30 @@ rpx_status_t rpx_evaluate_modes(const double *x, const double *y,
                                 uint64_t n, int32_t row, double blending,
                                 double *modes, void *stream);
 
+/* project_to_sphere (core/fields.py:51-80): each ray whose line meets the
+ * sphere (centre, radius) moves along its direction to the intersection
+ * with the smaller (most negative) path parameter, its accumulated_path
+ * growing by that parameter times Re(n); rays whose line misses are dropped.
+ * The kept rays go to out[0 .. *n_out) in input order.  Workspace:
+ * rpx_trace_workspace_bytes(n). */
+rpx_status_t rpx_project_to_sphere(const rpx_rays_t *in, uint64_t n, const double *centre, double radius,
+                                   rpx_rays_t *out, uint64_t *n_out, void *workspace,
+                                   uint64_t workspace_bytes, void *stream);
+
+/* evaluate_neighbours (core/fields.py:83-111) and evaluate_modes
+ * (cfields.pyx:156-228) for a ray field: every ray whose six neighbours
+ * (device int32, n x 6, indices into `rays`) are all >= 0 projects them onto
+ * the plane through its origin normal to its direction, takes their offsets
+ * and slopes in its (E, E x d) frame and fits its Gaussian mode.  Those rays
+ * go to out[0 .. *n_out) in input order, their modes (complex A, B, C as 6
+ * doubles) to modes[6k ..] and, when neighbour_xy is given, the neighbour
+ * coordinates x[6], y[6], dx[6], dy[6] to neighbour_xy[24k ..].
+ * RPX_E_INVALID if a neighbour index is >= n.
+ * Workspace: rpx_trace_workspace_bytes(n). */
+rpx_status_t rpx_ray_field_modes(const rpx_rays_t *rays, uint64_t n, const int32_t *neighbours,
+                                 double blending, rpx_rays_t *out, double *modes, double *neighbour_xy,
+                                 uint64_t *n_out, void *workspace, uint64_t workspace_bytes, void *stream);
+
 /* sum_gaussian_modes (cfields.pyx:52-115).
  *   rays      base rays (SoA, only the ray part is read)
  *   modes     device complex128[n_rays][3]

@@ -10,8 +10,11 @@
 //                 through LDS in tiles and every lane sums them in gausslet
 //                 order, so each point's sum has the reference's order.
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <string>
 #include "rpx_materials.h"
 #include "rpx_host.h"
+#include "rpx_gen.h"
 
 using namespace rpx;
 
@@ -108,6 +111,206 @@ extern "C" rpx_status_t rpx_evaluate_modes(const double *x, const double *y, con
                        dy, n, row, blending, modes);
     RPX_HIP_TRY(hipGetLastError());
     return RPX_OK;
+}
+
+// ------------------------------------------------------ ray-field modes
+// The RayField path of eval_Efield_from_rays (core/fields.py:196-218):
+//   k_project_sphere    project_to_sphere (core/fields.py:51-80): rays whose
+//                       line meets the sphere move back to the nearer
+//                       intersection; the others are dropped (compacted in
+//                       ray order).
+//   k_ray_field_modes   evaluate_neighbours (core/fields.py:83-111) fused with
+//                       the mode fit: rays with six neighbours >= 0 project
+//                       each neighbour onto their own origin plane, fit the
+//                       mode, and are compacted with it in ray order.
+// numpy's sum over a length-3 axis accumulates from +0.0: ((0 + a) + b) + c
+// (the leading zero only changes the sign of an exactly zero result).
+__device__ __forceinline__ double np_sum3(double a, double b, double c) { return ((0.0 + a) + b) + c; }
+
+__global__ __launch_bounds__(RPX_BLOCK) void k_project_sphere(rpx_rays_t in, uint64_t n, V3 centre, double radius_sq,
+                                                              rpx_rays_t out, Workspace w) {
+    __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
+    __shared__ uint32_t lds_tile;
+    __shared__ uint64_t lds_excl;
+    if (threadIdx.x == 0) lds_tile = atomicAdd(&w.ctl->tile_counter, 1u);
+    __syncthreads();
+    const uint32_t tile = lds_tile;
+    const uint64_t i = (uint64_t)tile * RPX_BLOCK + threadIdx.x;
+    bool sel = false;
+    double alpha = 0.0;
+    if (i < n) {
+        const double *f = in.f64;
+        const uint64_t c = in.capacity;
+        const V3 o = sub(v3(f[RPX_OX * c + i], f[RPX_OY * c + i], f[RPX_OZ * c + i]), centre);
+        const V3 d = v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]);
+        const double cc = np_sum3(o.x * o.x, o.y * o.y, o.z * o.z) - radius_sq;
+        const double b = 2 * np_sum3(d.x * o.x, d.y * o.y, d.z * o.z);
+        const double disc = b * b - 4 * cc;
+        sel = disc >= 0.0;
+        if (sel) {
+            const double sq = sqrt(disc);
+            const double root1 = (-b + sq) / 2, root2 = (-b - sq) / 2;
+            alpha = root2 < root1 ? root2 : root1;
+        }
+    }
+    uint32_t block_total;
+    const uint32_t local = block_scan_small(sel ? 1u : 0u, lds_wave, &block_total);
+    if (threadIdx.x < 64) {
+        const uint64_t excl = lookback(w.tile_status, tile, 0, block_total, &w.ctl->error);
+        if (threadIdx.x == 0) lds_excl = excl;
+    }
+    __syncthreads();
+    const uint64_t pos = lds_excl + local;
+    if (sel && pos < out.capacity) {
+        Ray r = load_ray(in, i);
+        r.origin = add(r.origin, mul(r.direction, alpha));
+        r.accum = r.accum + alpha * r.n.re;
+        store_ray(out, pos, r);
+    }
+    if (threadIdx.x == 0 && (uint64_t)tile == (n - 1) / RPX_BLOCK) w.ctl->children = lds_excl + block_total;
+}
+
+__global__ __launch_bounds__(RPX_BLOCK) void k_ray_field_modes(rpx_rays_t in, uint64_t n, const int32_t *nb,
+                                                               double blending, rpx_rays_t out, double *modes,
+                                                               double *xy, Workspace w) {
+    __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
+    __shared__ uint32_t lds_tile;
+    __shared__ uint64_t lds_excl;
+    if (threadIdx.x == 0) lds_tile = atomicAdd(&w.ctl->tile_counter, 1u);
+    __syncthreads();
+    const uint32_t tile = lds_tile;
+    const uint64_t i = (uint64_t)tile * RPX_BLOCK + threadIdx.x;
+    bool sel = false;
+    double m[6], x[6], y[6], dx[6], dy[6];
+    if (i < n) {
+        int32_t k6[6];
+        sel = true;
+        for (int j = 0; j < 6; ++j) {
+            k6[j] = nb[i * 6 + j];
+            if (k6[j] < 0) sel = false;
+        }
+        if (sel) {
+            for (int j = 0; j < 6; ++j)
+                if ((uint64_t)k6[j] >= n) {
+                    atomicAdd((unsigned long long *)&w.ctl->hits, 1ull);
+                    sel = false;
+                }
+        }
+        if (sel) {
+            const double *f = in.f64;
+            const uint64_t c = in.capacity;
+            const V3 o = v3(f[RPX_OX * c + i], f[RPX_OY * c + i], f[RPX_OZ * c + i]);
+            const V3 d = v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]);
+            const V3 E = v3(f[RPX_EX * c + i], f[RPX_EY * c + i], f[RPX_EZ * c + i]);
+            const V3 H = cross(E, d);  // numpy.cross(E, direction)
+            for (int j = 0; j < 6; ++j) {
+                const uint64_t q = (uint64_t)k6[j];
+                const V3 no = v3(f[RPX_OX * c + q], f[RPX_OY * c + q], f[RPX_OZ * c + q]);
+                const V3 nd = v3(f[RPX_DX * c + q], f[RPX_DY * c + q], f[RPX_DZ * c + q]);
+                const V3 off = sub(no, o);
+                const double dz = np_sum3(nd.x * d.x, nd.y * d.y, nd.z * d.z);
+                const double alpha = (-np_sum3(off.x * d.x, off.y * d.y, off.z * d.z)) / dz;
+                const V3 pr = add(off, mul(nd, alpha));
+                x[j] = np_sum3(pr.x * E.x, pr.y * E.y, pr.z * E.z);
+                y[j] = np_sum3(pr.x * H.x, pr.y * H.y, pr.z * H.z);
+                dx[j] = np_sum3(nd.x * E.x, nd.y * E.y, nd.z * E.z) / dz;
+                dy[j] = np_sum3(nd.x * H.x, nd.y * H.y, nd.z * H.z) / dz;
+            }
+            fit_one_mode(x, y, dx, dy, 6, blending, m);
+        }
+    }
+    uint32_t block_total;
+    const uint32_t local = block_scan_small(sel ? 1u : 0u, lds_wave, &block_total);
+    if (threadIdx.x < 64) {
+        const uint64_t excl = lookback(w.tile_status, tile, 0, block_total, &w.ctl->error);
+        if (threadIdx.x == 0) lds_excl = excl;
+    }
+    __syncthreads();
+    const uint64_t pos = lds_excl + local;
+    if (sel && pos < out.capacity) {
+        store_ray(out, pos, load_ray(in, i));
+        for (int q = 0; q < 6; ++q) modes[6 * pos + q] = m[q];
+        if (xy) {
+            for (int q = 0; q < 6; ++q) {
+                xy[24 * pos + q] = x[q];
+                xy[24 * pos + 6 + q] = y[q];
+                xy[24 * pos + 12 + q] = dx[q];
+                xy[24 * pos + 18 + q] = dy[q];
+            }
+        }
+    }
+    if (threadIdx.x == 0 && (uint64_t)tile == (n - 1) / RPX_BLOCK) w.ctl->children = lds_excl + block_total;
+}
+
+static rpx_status_t compact_launch_checks(const rpx_rays_t *in, uint64_t n, const rpx_rays_t *out, uint64_t ws_bytes,
+                                          void *ws) {
+    if (!in || in->capacity < n) return rpx_set_error(RPX_E_INVALID, "input rays");
+    if (n > 0 && (!in->f64 || !in->u32)) return rpx_set_error(RPX_E_INVALID, "input rays have null fields");
+    if (!out || (out->capacity > 0 && (!out->f64 || !out->u32))) return rpx_set_error(RPX_E_INVALID, "output rays");
+    if (n > 0xFFFFFFFFull) return rpx_set_error(RPX_E_UNSUPPORTED, "more than 2^32-1 rays");
+    if (n > 0 && (!ws || ws_bytes < rpx_trace_workspace_bytes(n))) return rpx_set_error(RPX_E_INVALID, "workspace too small");
+    return RPX_OK;
+}
+
+static rpx_status_t finish_compaction(const Workspace &w, uint64_t capacity, uint64_t *n_out, hipStream_t stream,
+                                      const char *what) {
+    GenControl ctl;
+    RPX_HIP_TRY(hipMemcpyAsync(&ctl, w.ctl, RPX_CTL_HEAD, hipMemcpyDeviceToHost, stream));
+    RPX_HIP_TRY(hipStreamSynchronize(stream));
+    if (ctl.error) return rpx_set_error(RPX_E_HIP, std::string(what) + ": a look-back wait exceeded its poll limit");
+    *n_out = ctl.children;
+    if (ctl.hits)
+        return rpx_set_error(RPX_E_INVALID, std::to_string(ctl.hits) + " rays have a neighbour index outside the rays");
+    if (ctl.children > capacity)
+        return rpx_set_error(RPX_E_CAPACITY, std::string(what) + ": output capacity below " +
+                                                 std::to_string(ctl.children));
+    return RPX_OK;
+}
+
+extern "C" rpx_status_t rpx_project_to_sphere(const rpx_rays_t *in, uint64_t n, const double *centre, double radius,
+                                              rpx_rays_t *out, uint64_t *n_out, void *workspace,
+                                              uint64_t workspace_bytes, void *stream_) {
+    if (!n_out) return rpx_set_error(RPX_E_INVALID, "n_out is null");
+    *n_out = 0;
+    if (!centre) return rpx_set_error(RPX_E_INVALID, "centre is null");
+    if (rpx_status_t st = compact_launch_checks(in, n, out, workspace_bytes, workspace)) return st;
+    if (n == 0) return RPX_OK;
+    hipStream_t stream = (hipStream_t)stream_;
+    Workspace w = carve(workspace, n);
+    const uint64_t nb = (n + RPX_BLOCK - 1) / RPX_BLOCK;
+    hipLaunchKernelGGL(k_clear, dim3((unsigned)((std::max<uint64_t>(nb, RPX_SHARDS * RPX_LDS_FACES) + 255) / 256)),
+                       dim3(256), 0, stream, w.tile_status, nb, w.ctl);
+    RPX_HIP_TRY(hipGetLastError());
+    V3 c;
+    c.x = centre[0];
+    c.y = centre[1];
+    c.z = centre[2];
+    hipLaunchKernelGGL(k_project_sphere, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, *in, n, c, radius * radius,
+                       *out, w);
+    RPX_HIP_TRY(hipGetLastError());
+    return finish_compaction(w, out->capacity, n_out, stream, "project_to_sphere");
+}
+
+extern "C" rpx_status_t rpx_ray_field_modes(const rpx_rays_t *rays, uint64_t n, const int32_t *neighbours,
+                                            double blending, rpx_rays_t *out, double *modes, double *neighbour_xy,
+                                            uint64_t *n_out, void *workspace, uint64_t workspace_bytes,
+                                            void *stream_) {
+    if (!n_out) return rpx_set_error(RPX_E_INVALID, "n_out is null");
+    *n_out = 0;
+    if (rpx_status_t st = compact_launch_checks(rays, n, out, workspace_bytes, workspace)) return st;
+    if (n == 0) return RPX_OK;
+    if (!neighbours) return rpx_set_error(RPX_E_INVALID, "neighbours are null");
+    if (out->capacity > 0 && !modes) return rpx_set_error(RPX_E_INVALID, "modes are null");
+    hipStream_t stream = (hipStream_t)stream_;
+    Workspace w = carve(workspace, n);
+    const uint64_t nb = (n + RPX_BLOCK - 1) / RPX_BLOCK;
+    hipLaunchKernelGGL(k_clear, dim3((unsigned)((std::max<uint64_t>(nb, RPX_SHARDS * RPX_LDS_FACES) + 255) / 256)),
+                       dim3(256), 0, stream, w.tile_status, nb, w.ctl);
+    RPX_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_ray_field_modes, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, *rays, n, neighbours,
+                       blending, *out, modes, neighbour_xy, w);
+    RPX_HIP_TRY(hipGetLastError());
+    return finish_compaction(w, out->capacity, n_out, stream, "ray field modes");
 }
 
 // ------------------------------------------------------------ field sum FP64
