@@ -339,6 +339,21 @@ rpx_status_t rpx_select_intersections(const rpx_scene_t *scene, int32_t face_lis
                                void *workspace, uint64_t workspace_bytes,
                                void *stream);
 
+/* Gausslet capture of ResampleGaussletMaterial (cmaterials.pyx:1766-1831):
+ * after a gausslet generation (`in`, updated in place), copies every gausslet
+ * whose end_face_idx is one of face_ids (<= 16) to out[out_offset + k] in
+ * input order -- the base ray as the generation left it, the parabasal rays
+ * with their lengths from para_length (device, row j at
+ * para_length[j * para_length_stride + i], taken before the generation, as
+ * the reference captures before trace_parabasal_rays moves them; NULL: as in
+ * `in`).  *n_out = gausslets captured.  Workspace:
+ * rpx_trace_workspace_bytes(n_in). */
+rpx_status_t rpx_capture_gausslets(const rpx_rays_t *in, uint64_t n_in, const int32_t *face_ids,
+                                   int32_t n_face_ids, const double *para_length,
+                                   uint64_t para_length_stride, rpx_rays_t *out, uint64_t out_offset,
+                                   uint64_t *n_out, void *workspace, uint64_t workspace_bytes,
+                                   void *stream);
+
 /* ------------------------------------------------ generation consumers
  * Walks over the generations of one trace (rpx_trace_segment outputs, in
  * trace order), resident in HBM.  A generation is its SoA buffer and count. */

@@ -69,6 +69,8 @@ _SIGNATURES = {
                                       c_void_p]),
     "rpx_ray_field_modes": (c_i32, [P(_abi.Rays), c_u64, c_void_p, c_f64, P(_abi.Rays), c_void_p, c_void_p,
                                     P(c_u64), c_void_p, c_u64, c_void_p]),
+    "rpx_capture_gausslets": (c_i32, [P(_abi.Rays), c_u64, P(c_i32), c_i32, c_void_p, c_u64, P(_abi.Rays), c_u64,
+                                      P(c_u64), c_void_p, c_u64, c_void_p]),
     "rpx_timing_enable": (c_i32, [c_i32]),
     "rpx_timing_read": (c_i32, [c_void_p, c_void_p, c_void_p]),
     "rpx_query_face": (c_i32, [c_void_p, c_i32, c_i32, c_void_p, c_void_p, c_u64, c_i32, c_void_p, c_void_p]),

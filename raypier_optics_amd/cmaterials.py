@@ -109,6 +109,48 @@ class OpaqueMaterial(InterfaceMaterial):
         rec["type"] = _abi.MAT_OPAQUE
 
 
+class ResampleGaussletMaterial(InterfaceMaterial):
+    """A pseudo-material that captures the incident gausslets and, at the
+    end of the generation, launches the gausslets returned by
+    eval_func(captured_rays) instead of refracting or reflecting them
+    (cmaterials.pyx:1766-1831).  On the device it produces no children; the
+    capture is a compaction of the generation's gausslets that ended on its
+    faces (rpx_capture_gausslets)."""
+
+    def __init__(self, size=2, eval_func=None, **kwds):
+        super().__init__(**kwds)
+        from .ctracer import GaussletCollection
+        self.capture_count = 0
+        self.captured_rays = GaussletCollection(size)
+        self._evaluation_func = None
+        if eval_func is not None:
+            self.eval_func = eval_func
+
+    @property
+    def eval_func(self):
+        return self._evaluation_func
+
+    @eval_func.setter
+    def eval_func(self, obj):
+        if not callable(obj):
+            raise ValueError("The eval_func property must be a callable.")
+        self._evaluation_func = obj
+
+    def is_decomp_material(self):
+        return True
+
+    def _rpx_fill_material(self, rec, b):
+        rec["type"] = _abi.MAT_OPAQUE
+
+    def eval_decomposed_rays(self, new_rays):
+        """eval_decomposed_rays_c (cmaterials.pyx:1824-1835)."""
+        self.captured_rays._wavelengths = new_rays._wavelengths
+        self.captured_rays._wl_dev = None
+        gc = self._evaluation_func(self.captured_rays)
+        new_rays.extend(gc)
+        self.captured_rays.clear_ray_list()
+
+
 class TransparentMaterial(InterfaceMaterial):
     """Passes the ray unchanged apart from the S/P projection (cmaterials.pyx:254-278)."""
 

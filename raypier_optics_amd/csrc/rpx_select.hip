@@ -157,3 +157,104 @@ extern "C" rpx_status_t rpx_select_intersections(const rpx_scene_t *scene, int32
                                              std::to_string(out_offset + ctl.children) + " rays");
     return RPX_OK;
 }
+
+// ------------------------------------------------------------- capture
+// ResampleGaussletMaterial.eval_child_ray_c (cmaterials.pyx:1811-1822) appends
+// every gausslet whose base ray ends on one of its faces to captured_rays,
+// after FaceList.intersect_c set the base ray's length and end face but before
+// trace_parabasal_rays (ctracer.pyx:2263-2268) moves the parabasal lengths.
+// Run after the generation: the base ray comes from `in`, the six parabasal
+// lengths from `para_length` (row j at para_length[j * stride + i]), taken
+// before the generation.  Output in input order at out[out_offset + k].
+#define RPX_CAPTURE_MAX_FACES 16
+struct CaptureArgs {
+    rpx_rays_t in;
+    rpx_rays_t out;
+    uint64_t n_in;
+    uint64_t out_offset;
+    const double *para_length;
+    uint64_t stride;
+    uint32_t faces[RPX_CAPTURE_MAX_FACES];
+    int32_t n_faces;
+    Workspace ws;
+};
+
+__global__ __launch_bounds__(RPX_BLOCK) void k_capture(CaptureArgs A) {
+    __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
+    __shared__ uint32_t lds_tile;
+    __shared__ uint64_t lds_excl;
+    if (threadIdx.x == 0) lds_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
+    __syncthreads();
+    const uint32_t tile = lds_tile;
+    const uint64_t i = (uint64_t)tile * RPX_BLOCK + threadIdx.x;
+    bool sel = false;
+    if (i < A.n_in) {
+        const uint32_t ef = A.in.u32[RPX_END_FACE_IDX * A.in.capacity + i];
+        for (int k = 0; k < A.n_faces; ++k) sel = sel || ef == A.faces[k];
+    }
+    uint32_t block_total;
+    const uint32_t local = block_scan_small(sel ? 1u : 0u, lds_wave, &block_total);
+    if (threadIdx.x < 64) {
+        const uint64_t excl = lookback(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error);
+        if (threadIdx.x == 0) lds_excl = excl;
+    }
+    __syncthreads();
+    const uint64_t pos = A.out_offset + lds_excl + local;
+    if (sel && pos < A.out.capacity) {
+        store_ray(A.out, pos, load_ray(A.in, i));
+        const uint64_t ci = A.in.capacity, co = A.out.capacity;
+        for (int j = 0; j < RPX_N_PARA; ++j) {
+            const uint64_t b = (uint64_t)j * RPX_PARA_FIELDS;
+            for (int q = 0; q < 9; ++q) A.out.para[(b + q) * co + pos] = A.in.para[(b + q) * ci + i];
+            A.out.para[(b + 9) * co + pos] =
+                A.para_length ? A.para_length[(uint64_t)j * A.stride + i] : A.in.para[(b + 9) * ci + i];
+        }
+    }
+    if (threadIdx.x == 0 && (uint64_t)tile == (A.n_in - 1) / RPX_BLOCK) A.ws.ctl->children = lds_excl + block_total;
+}
+
+extern "C" rpx_status_t rpx_capture_gausslets(const rpx_rays_t *in, uint64_t n_in, const int32_t *face_ids,
+                                              int32_t n_face_ids, const double *para_length,
+                                              uint64_t para_length_stride, rpx_rays_t *out, uint64_t out_offset,
+                                              uint64_t *n_out, void *workspace, uint64_t workspace_bytes,
+                                              void *stream_) {
+    if (!n_out) return rpx_set_error(RPX_E_INVALID, "n_out is null");
+    *n_out = 0;
+    if (!in || in->capacity < n_in) return rpx_set_error(RPX_E_INVALID, "input gausslets");
+    if (n_in > 0 && (!in->f64 || !in->u32 || !in->para)) return rpx_set_error(RPX_E_INVALID, "input gausslets");
+    if (!out || (out->capacity > 0 && (!out->f64 || !out->u32 || !out->para)))
+        return rpx_set_error(RPX_E_INVALID, "output gausslets");
+    if (n_face_ids < 0 || n_face_ids > RPX_CAPTURE_MAX_FACES || (n_face_ids > 0 && !face_ids))
+        return rpx_set_error(RPX_E_INVALID, "face ids (at most 16)");
+    if (para_length && para_length_stride < n_in) return rpx_set_error(RPX_E_INVALID, "para length stride");
+    if (n_in == 0 || n_face_ids == 0) return RPX_OK;
+    if (n_in > 0xFFFFFFFFull) return rpx_set_error(RPX_E_UNSUPPORTED, "more than 2^32-1 gausslets");
+    if (!workspace || workspace_bytes < rpx_trace_workspace_bytes(n_in))
+        return rpx_set_error(RPX_E_INVALID, "workspace too small");
+    hipStream_t stream = (hipStream_t)stream_;
+    CaptureArgs A;
+    A.in = *in;
+    A.out = *out;
+    A.n_in = n_in;
+    A.out_offset = out_offset;
+    A.para_length = para_length;
+    A.stride = para_length_stride;
+    A.n_faces = n_face_ids;
+    for (int k = 0; k < RPX_CAPTURE_MAX_FACES; ++k) A.faces[k] = k < n_face_ids ? (uint32_t)face_ids[k] : 0u;
+    A.ws = carve(workspace, n_in);
+    const uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
+    hipLaunchKernelGGL(k_clear, dim3((unsigned)((std::max<uint64_t>(nb, RPX_SHARDS * RPX_LDS_FACES) + 255) / 256)),
+                       dim3(256), 0, stream, A.ws.tile_status, nb, A.ws.ctl);
+    RPX_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_capture, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
+    RPX_HIP_TRY(hipGetLastError());
+    GenControl ctl;
+    RPX_HIP_TRY(hipMemcpyAsync(&ctl, A.ws.ctl, RPX_CTL_HEAD, hipMemcpyDeviceToHost, stream));
+    RPX_HIP_TRY(hipStreamSynchronize(stream));
+    if (ctl.error) return rpx_set_error(RPX_E_HIP, "capture scan: a look-back wait exceeded its poll limit");
+    *n_out = ctl.children;
+    if (out_offset + ctl.children > out->capacity)
+        return rpx_set_error(RPX_E_CAPACITY, "output capacity " + std::to_string(out->capacity) + " below " +
+                                                 std::to_string(out_offset + ctl.children) + " gausslets");
+    return RPX_OK;
+}

@@ -697,11 +697,31 @@ class GaussletCollection(_Collection):
         return [self[i] for i in range(self._n)]
 
     def extend(self, gc):
-        data = gc.copy_as_array()
-        self._grow(self._n + len(data))
-        self._host[self._n: self._n + len(data)] = data
-        self._n += len(data)
-        self._host_changed()
+        """Append the gausslets of gc (ctracer.pyx:1242-1259).  Collections
+        in HBM are joined on the device."""
+        if self._where == "host" and gc._where == "host":
+            data = gc.copy_as_array()
+            self._grow(self._n + len(data))
+            self._host[self._n: self._n + len(data)] = data
+            self._n += len(data)
+            self._host_changed()
+            return
+        if gc.n_rays == 0:
+            return
+        from ._device import DeviceRays
+        n1, n2 = self._n, gc.n_rays
+        out = DeviceRays(n1 + n2, gausslet=True)
+        for src, lo, n in ((self, 0, n1), (gc, n1, n2)):
+            if n == 0:
+                continue
+            d = src._ensure_device()
+            out.f64()[:, lo: lo + n] = d.f64()[:, :n]
+            out.u32()[:, lo: lo + n] = d.u32()[:, :n]
+            out.para_view()[:, :, lo: lo + n] = d.para_view()[:, :, :n]
+        self._dev = out
+        self._n = n1 + n2
+        self._host = np.zeros(0, self._dtype)
+        self._device_changed()
 
     @property
     def base_rays(self):
@@ -1006,7 +1026,55 @@ def _wavelength_table(rays, ctx):
     return rays._device_wavelengths()
 
 
-def _generation(ctx, rays, max_length, one_list=-1, one_face=-1, flush=True):
+def _capture(rays, face_ids, para_length, captured):
+    """Append to `captured` (GaussletCollection) the gausslets of `rays`
+    whose base ray ended on one of face_ids, with the parabasal lengths
+    they had before the generation (rpx_capture_gausslets)."""
+    from ._device import DeviceRays
+    torch = require_gpu()
+    lib = load()
+    n = rays.n_rays
+    ids = (ctypes.c_int32 * len(face_ids))(*face_ids)
+    out = DeviceRays(max(n, 1), gausslet=True)
+    ws = torch.empty(max(int(lib.rpx_trace_workspace_bytes(n)), 8), dtype=torch.uint8, device="cuda")
+    k = ctypes.c_uint64(0)
+    s_in, s_out = rays._ensure_device().struct(), out.struct()
+    check(lib.rpx_capture_gausslets(ctypes.byref(s_in), n, ids, len(face_ids), ctypes.c_void_p(para_length.data_ptr()),
+                                    int(para_length.shape[1]), ctypes.byref(s_out), 0, ctypes.byref(k),
+                                    ctypes.c_void_p(ws.data_ptr()), ws.numel(), stream_handle()))
+    if k.value:
+        captured.extend(GaussletCollection.from_device(out, k.value))
+    return k.value
+
+
+def _decompose(ctx, rays, out, para_length, decomp_faces, max_length):
+    """The decomposition step of trace_gausslet_c (ctracer.pyx:2274-2280):
+    gausslets that ended on a decomposition face were captured by its
+    material; every decomposition face hit (Face.count) makes its material
+    append its eval_func's gausslets to the generation, and its count is
+    cleared."""
+    idx = [f.idx for f in decomp_faces]
+    hits = ctx.hits[idx].cpu().numpy().astype(np.int64)
+    ctx.hits[idx] = 0
+    mats = []
+    for f in decomp_faces:
+        if not any(f.material is m for m in mats):
+            mats.append(f.material)
+    for m in mats:
+        ids = [f.idx for f in decomp_faces if f.material is m]
+        got = _capture(rays, ids, para_length, m.captured_rays)
+        m.capture_count += got
+    extended = False
+    for f, h in zip(decomp_faces, hits):
+        if f.count + int(h):
+            f.material.eval_decomposed_rays(out)
+            extended = True
+        f.count = 0
+    if extended:
+        out.reset_length(max_length)
+
+
+def _generation(ctx, rays, max_length, one_list=-1, one_face=-1, flush=True, decomp_faces=()):
     """Trace one generation on the GPU; returns the child collection."""
     from ._device import DeviceRays
     lib = load()
@@ -1018,6 +1086,8 @@ def _generation(ctx, rays, max_length, one_list=-1, one_face=-1, flush=True):
         out.parent = rays
         return out
     dev_in = rays._ensure_device()
+    decomp = list(decomp_faces) if gauss else []
+    para_length = dev_in.para_view()[:, 9, :n].clone() if decomp else None
     wl = _wavelength_table(rays, ctx)
     cap = max(ctx.max_children * n, 1)
     dev_out = DeviceRays(cap, gausslet=gauss)
@@ -1035,10 +1105,12 @@ def _generation(ctx, rays, max_length, one_list=-1, one_face=-1, flush=True):
         code = lib.rpx_trace_one_face_segment(ctx.handle, one_list, one_face, *args, float(max_length), *tail)
     check(code)
     rays._device_changed()
-    if flush:
-        ctx.flush_counts()
     out = cls.from_device(dev_out, n_out.value)
     out.parent = rays
+    if decomp:
+        _decompose(ctx, rays, out, para_length, decomp, max_length)
+    if flush:
+        ctx.flush_counts()
     return out
 
 
@@ -1062,13 +1134,13 @@ def trace_gausslet(rays, face_sets, all_faces, max_length=100, decomp_faces=[]):
     for fs in face_sets:
         fs.sync_transforms()
     ctx = _TraceContext(face_sets, all_faces)
-    return _generation(ctx, rays, max_length)
+    return _generation(ctx, rays, max_length, decomp_faces=decomp_faces)
 
 
 def trace_one_face_gausslet(rays, face_set, face_idx, all_faces, max_length=100, decomp_faces=[]):
     face_set.sync_transforms()
     ctx = _TraceContext([face_set], all_faces)
-    return _generation(ctx, rays, max_length, 0, face_idx)
+    return _generation(ctx, rays, max_length, 0, face_idx, decomp_faces=decomp_faces)
 
 
 def _propagate_neighbours(children, parent_nb):

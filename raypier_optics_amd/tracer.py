@@ -31,12 +31,13 @@ def trace_rays(input_rays, face_lists, recursion_limit=100, max_length=100.0):
     for fs in face_sets:
         fs.sync_transforms()
     ctx = _TraceContext(face_sets, all_faces)
+    decomp_faces = [f for f in all_faces if f.material.is_decomp_material()]
     traced_rays = []
     rays = input_rays
     count = 0
     while rays.n_rays > 0 and count < recursion_limit:
         traced_rays.append(rays)
-        rays = _generation(ctx, rays, max_length, flush=False)
+        rays = _generation(ctx, rays, max_length, flush=False, decomp_faces=decomp_faces)
         count += 1
     ctx.flush_counts()
     return traced_rays, all_faces
@@ -49,6 +50,7 @@ def trace_ray_sequence(input_rays, face_sequence, recursion_limit=100, max_lengt
     face_lists = [fl for fl, fidx in face_sequence]
     face_idx_list = [fidx for fl, fidx in face_sequence]
     all_faces = _prepare(input_rays, face_lists, max_length)
+    decomp_faces = [f for f in all_faces if f.material.is_decomp_material()]
     contexts = {}
     rays = input_rays
     count = 0
@@ -57,7 +59,7 @@ def trace_ray_sequence(input_rays, face_sequence, recursion_limit=100, max_lengt
         key = id(fl)
         if key not in contexts:
             contexts[key] = _TraceContext([fl], all_faces)
-        rays = _generation(contexts[key], rays, max_length, 0, fidx)
+        rays = _generation(contexts[key], rays, max_length, 0, fidx, decomp_faces=decomp_faces)
         if (count > recursion_limit) or (rays.n_rays == 0):
             break
         traced_rays.append(rays)
