@@ -694,6 +694,37 @@ rpx_status_t rpx_evaluate_modes(const double *x, const double *y,
                                 uint64_t n, int32_t row, double blending,
                                 double *modes, void *stream);
 
+/* ------------------------------------------------ beam decomposition
+ * calc_mode_curvature (cfields.pyx:231-293): for each of n points, the mode
+ * basis x', y', z' (n x 3 each, row-major) from the wavefront slopes dx, dy
+ * and the curvature coefficients A, B, C from the second derivatives. */
+rpx_status_t rpx_calc_mode_curvature(const double *dx, const double *dy, const double *dx2,
+                                     const double *dy2, const double *dxdy, uint64_t n,
+                                     double *A, double *B, double *C,
+                                     double *x, double *y, double *z, void *stream);
+
+/* apply_mode_curvature (cfields.pyx:397-424): adds the curvature
+ * coefficients A, B, C (device, one per gausslet) to the parabasal
+ * directions of n gausslets in HBM. */
+rpx_status_t rpx_apply_mode_curvature(rpx_rays_t *gausslets, uint64_t n, const double *A,
+                                      const double *B, const double *C, void *stream);
+
+/* build_interaction_matrix (cfields.pyx:295-395): the field of mode i (at
+ * (rx, ry) with curvature A, B, C, basis x, y, z and width spacing/blending)
+ * at test point j (tx, ty), for every pair within max_spacing, in (i, j)
+ * order: data[2k], data[2k+1] = E_ij (complex), ij[k] = i, ij[capacity+k] = j.
+ * *n_out = number of pairs; RPX_E_CAPACITY if above capacity.  (The
+ * reference's test_spacing only sizes its output buffer.)  Workspace:
+ * rpx_trace_workspace_bytes(n * l). */
+rpx_status_t rpx_build_interaction_matrix(const double *rx, const double *ry, uint64_t n,
+                                          const double *tx, const double *ty, uint64_t l,
+                                          const double *A, const double *B, const double *C,
+                                          const double *x, const double *y, const double *z,
+                                          double wavelength, double spacing, double max_spacing,
+                                          double blending, double *data, uint64_t *ij,
+                                          uint64_t capacity, uint64_t *n_out, void *workspace,
+                                          uint64_t workspace_bytes, void *stream);
+
 /* project_to_sphere (core/fields.py:51-80): each ray whose line meets the
  * sphere (centre, radius) moves along its direction to the intersection
  * with the smaller (most negative) path parameter, its accumulated_path

@@ -54,6 +54,10 @@ def load():
         lib.oracle_config_parabasal_rays.restype = i32
         lib.oracle_config_parabasal_rays.argtypes = [vp, u64, vp, u64, f64, f64]
         lib.oracle_project_to_plane.argtypes = [vp, u64, vp, vp]
+        lib.oracle_calc_mode_curvature.argtypes = [vp] * 5 + [u64] + [vp] * 6
+        lib.oracle_apply_mode_curvature.argtypes = [vp, u64, vp, vp, vp]
+        lib.oracle_build_interaction_matrix.restype = u64
+        lib.oracle_build_interaction_matrix.argtypes = [vp, vp, u64, vp, vp, u64] + [vp] * 6 + [f64] * 4 + [vp] * 3
         lib.oracle_sizes.restype = i32
         lib.oracle_sizes.argtypes = [i32]
         _lib = lib
@@ -163,6 +167,39 @@ def project_to_plane(gausslets, origin, direction):
     d = np.ascontiguousarray(direction, dtype=np.float64)
     lib.oracle_project_to_plane(_p(g), len(g), _p(o), _p(d))
     return g
+
+
+def calc_mode_curvature(rx, ry, dx, dy, dx2, dy2, dxdy):
+    lib = load()
+    ins = [np.ascontiguousarray(v, np.float64) for v in (dx, dy, dx2, dy2, dxdy)]
+    n = len(ins[0])
+    A, B, C = (np.zeros(n) for _ in range(3))
+    X, Y, Z = (np.zeros((n, 3)) for _ in range(3))
+    lib.oracle_calc_mode_curvature(*[_p(v) for v in ins], n, _p(A), _p(B), _p(C), _p(X), _p(Y), _p(Z))
+    return A, B, C, X, Y, Z
+
+
+def apply_mode_curvature(gausslets, A, B, C):
+    """A copy of the gausslet_dtype array with the curvature applied."""
+    lib = load()
+    g = np.ascontiguousarray(gausslets).copy()
+    A, B, C = (np.ascontiguousarray(v, np.float64) for v in (A, B, C))
+    lib.oracle_apply_mode_curvature(_p(g), len(g), _p(A), _p(B), _p(C))
+    return g
+
+
+def build_interaction_matrix(rx, ry, tx, ty, A, B, C, x, y, z, wavelength, spacing, max_spacing, blending):
+    lib = load()
+    rx, ry, tx, ty, A, B, C = (np.ascontiguousarray(v, np.float64) for v in (rx, ry, tx, ty, A, B, C))
+    x, y, z = (np.ascontiguousarray(v, np.float64).reshape(-1, 3) for v in (x, y, z))
+    n, l = len(rx), len(tx)
+    data = np.zeros(max(n * l, 1), np.complex128)
+    xi = np.zeros(max(n * l, 1), np.uint64)
+    xj = np.zeros(max(n * l, 1), np.uint64)
+    m = lib.oracle_build_interaction_matrix(_p(rx), _p(ry), n, _p(tx), _p(ty), l, _p(A), _p(B), _p(C), _p(x), _p(y),
+                                            _p(z), wavelength, spacing, max_spacing, blending, _p(data), _p(xi),
+                                            _p(xj))
+    return data[:m].copy(), (xi[:m].copy(), xj[:m].copy())
 
 
 def trace_gausslet(tables, gs, max_length, wavelengths, one_list=-1, one_face=-1, max_children=3):

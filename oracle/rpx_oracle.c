@@ -1636,6 +1636,99 @@ void oracle_sum_gaussian_modes(const oray_t *rays, uint64_t nray, const double *
     }
 }
 
+/* calc_mode_curvature (cfields.pyx:231-293). */
+void oracle_calc_mode_curvature(const double *dx, const double *dy, const double *dx2, const double *dy2,
+                                const double *dxdy, uint64_t N, double *A, double *B, double *C, double *x,
+                                double *y, double *z) {
+    vec_t a, b, c;
+    for (uint64_t i = 0; i < N; i++) {
+        double Fz;
+        c.x = -dx[i];
+        c.y = -dy[i];
+        c.z = 1.0;
+        c = norm_(c);
+        b.x = 0.0;
+        b.y = 0.0;
+        b.z = 1.0;
+        a = norm_(cross_(b, c));
+        b = cross_(c, a);
+        x[3 * i] = a.x; x[3 * i + 1] = a.y; x[3 * i + 2] = a.z;
+        y[3 * i] = b.x; y[3 * i + 1] = b.y; y[3 * i + 2] = b.z;
+        z[3 * i] = c.x; z[3 * i + 1] = c.y; z[3 * i + 2] = c.z;
+        Fz = c.x * dx[i] + c.y * dy[i] + c.z;
+        A[i] = -(a.x * a.x * dx2[i] + 2 * a.x * a.y * dxdy[i] + a.y * a.y * dy2[i]) / Fz;
+        B[i] = -(a.x * b.x * dx2[i] + (a.x * b.y + b.x * a.y) * dxdy[i] + a.y * b.y * dy2[i]) / Fz;
+        C[i] = -(b.x * b.x * dx2[i] + 2 * b.x * b.y * dxdy[i] + b.y * b.y * dy2[i]) / Fz;
+    }
+}
+
+/* apply_mode_curvature (cfields.pyx:397-424). */
+void oracle_apply_mode_curvature(ogausslet_t *gs, uint64_t n, const double *A, const double *B, const double *C) {
+    for (uint64_t i = 0; i < n; i++) {
+        ogausslet_t *g = gs + i;
+        oray_t *ray = &g->base_ray;
+        vec_t z = ray->direction, x = ray->E_vector, y = cross_(x, z);
+        for (int j = 0; j < 6; j++) {
+            opara_t *para = &g->para[j];
+            vec_t r = subvv_(para->origin, ray->origin), shift;
+            double px = dotprod_(r, x), py = dotprod_(r, y);
+            shift = multvs_(x, (-A[i] * px - B[i] * py));
+            shift = addvv_(shift, multvs_(y, (-C[i] * py - B[i] * px)));
+            para->direction = norm_(addvv_(shift, para->direction));
+        }
+    }
+}
+
+static cplx calc_mode_U(cplx A, cplx B, cplx C, cplx detG0, vec_t pt, vec_t E, vec_t H, vec_t direction, cplx k,
+                        double phase, double inv_root_area) {  /* cfields.pyx:118-151 */
+    const cplx rootI = csqrt(CMPLX(0.0, 1.0));
+    cplx denom, AA, CC, U;
+    double x = dotprod_(pt, E), y = dotprod_(pt, H), z = dotprod_(pt, direction);
+    denom = (RE(1) + (RE(z) * (A + C)) + RE(z * z) * detG0) * RE(2);
+    AA = (A + RE(z) * detG0) / denom;
+    CC = (C + RE(z) * detG0) / denom;
+    U = cexp(CMPLX(0.0, 1.0) * (RE(phase) + k * (RE(z) + AA * RE(x * x) + B * RE(2 * x * y) / denom + CC * RE(y * y))));
+    U /= csqrt(((RE(1) + RE(z) * A) * (RE(1) + RE(z) * C) - (RE(z) * B) * (RE(z) * B)) * CMPLX(0.0, 1.0));
+    U *= RE(inv_root_area) * rootI;
+    return U;
+}
+
+/* build_interaction_matrix (cfields.pyx:295-395) without its output size
+ * estimate: returns the number of pairs; data (complex), xi, xj sized by
+ * the caller for n * l pairs. */
+uint64_t oracle_build_interaction_matrix(const double *rx, const double *ry, uint64_t N, const double *tx,
+                                         const double *ty, uint64_t L, const double *A, const double *B,
+                                         const double *C, const double *x, const double *y, const double *z,
+                                         double wavelength, double spacing, double max_spacing, double blending,
+                                         double *data, uint64_t *xi, uint64_t *xj) {
+    uint64_t ct = 0;
+    double phase = 0.0, k, inv_root_area, impart;
+    k = 2000.0 * M_PI / wavelength;
+    impart = 2 * (blending * blending) / (k * spacing * spacing);
+    inv_root_area = 1.0;
+    for (uint64_t i = 0; i < N; i++) {
+        vec_t o = V(rx[i], ry[i], 0.0);
+        vec_t a = V(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+        vec_t b = V(y[3 * i], y[3 * i + 1], y[3 * i + 2]);
+        vec_t c = V(z[3 * i], z[3 * i + 1], z[3 * i + 2]);
+        cplx _A = CMPLX(A[i], impart), _B = CMPLX(B[i], 0), _C = CMPLX(C[i], impart);
+        cplx detG0 = (_A * _C);
+        for (uint64_t j = 0; j < L; j++) {
+            vec_t pt = V(tx[j], ty[j], 0.0);
+            pt = subvv_(pt, o);
+            if (((pt.x * pt.x) + (pt.y * pt.y)) <= (max_spacing * max_spacing)) {
+                cplx u = calc_mode_U(_A, _B, _C, detG0, pt, a, b, c, RE(k), phase, inv_root_area);
+                xi[ct] = i;
+                xj[ct] = j;
+                data[2 * ct] = creal(u);
+                data[2 * ct + 1] = cimag(u);
+                ct += 1;
+            }
+        }
+    }
+    return ct;
+}
+
 int oracle_sizes(int which) {
     switch (which) {
     case 0: return (int)sizeof(oray_t);

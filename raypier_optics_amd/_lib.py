@@ -71,6 +71,10 @@ _SIGNATURES = {
                                     P(c_u64), c_void_p, c_u64, c_void_p]),
     "rpx_capture_gausslets": (c_i32, [P(_abi.Rays), c_u64, P(c_i32), c_i32, c_void_p, c_u64, P(_abi.Rays), c_u64,
                                       P(c_u64), c_void_p, c_u64, c_void_p]),
+    "rpx_calc_mode_curvature": (c_i32, [c_void_p] * 5 + [c_u64] + [c_void_p] * 7),
+    "rpx_apply_mode_curvature": (c_i32, [P(_abi.Rays), c_u64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "rpx_build_interaction_matrix": (c_i32, [c_void_p, c_void_p, c_u64, c_void_p, c_void_p, c_u64] + [c_void_p] * 6 +
+                                     [c_f64] * 4 + [c_void_p, c_void_p, c_u64, P(c_u64), c_void_p, c_u64, c_void_p]),
     "rpx_timing_enable": (c_i32, [c_i32]),
     "rpx_timing_read": (c_i32, [c_void_p, c_void_p, c_void_p]),
     "rpx_query_face": (c_i32, [c_void_p, c_i32, c_i32, c_void_p, c_void_p, c_u64, c_i32, c_void_p, c_void_p]),

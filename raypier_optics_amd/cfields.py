@@ -7,6 +7,7 @@ import ctypes
 
 import numpy as np
 
+from . import _abi
 from ._lib import load, check, require_gpu, stream_handle
 
 
@@ -70,3 +71,63 @@ def gausslet_modes(gausslets, blending=1.0):
     s = dev.struct()
     check(load().rpx_gausslet_modes(ctypes.byref(s), n, float(blending), _ptr(out), stream_handle()))
     return out[:n]
+
+
+# ------------------------------------------------------ beam decomposition
+def calc_mode_curvature(rx, ry, dx, dy, dx2, dy2, dxdy):
+    """(A, B, C, x', y', z'): Gaussian mode curvature coefficients and local
+    basis at each point from the wavefront's derivatives
+    (cfields.pyx:231-293)."""
+    torch = require_gpu()
+    n = len(rx)
+    ins = [_dev(np.ascontiguousarray(v, np.float64).reshape(-1)) for v in (dx, dy, dx2, dy2, dxdy)]
+    A, B, C = (torch.empty(max(n, 1), dtype=torch.float64, device="cuda") for _ in range(3))
+    X, Y, Z = (torch.empty((max(n, 1), 3), dtype=torch.float64, device="cuda") for _ in range(3))
+    check(load().rpx_calc_mode_curvature(*[_ptr(t) for t in ins], n, _ptr(A), _ptr(B), _ptr(C), _ptr(X), _ptr(Y),
+                                         _ptr(Z), stream_handle()))
+    return tuple(t[:n].cpu().numpy() for t in (A, B, C, X, Y, Z))
+
+
+def apply_mode_curvature(gc, A, B, C):
+    """Add the curvature coefficients to the parabasal directions of a
+    GaussletCollection (cfields.pyx:397-424)."""
+    n = gc.n_rays
+    if n == 0:
+        return
+    dA, dB, dC = (_dev(np.ascontiguousarray(v, np.float64).reshape(-1)[:n]) for v in (A, B, C))
+    s = gc._ensure_device().struct()
+    check(load().rpx_apply_mode_curvature(ctypes.byref(s), n, _ptr(dA), _ptr(dB), _ptr(dC), stream_handle()))
+    gc._device_changed()
+
+
+def build_interaction_matrix(rx, ry, tx, ty, A, B, C, x, y, z, wavelength, spacing, max_spacing, blending,
+                             test_spacing):
+    """(data, (i, j)) COO entries of the field of mode i at test point j
+    for every pair closer than max_spacing (cfields.pyx:295-395)."""
+    torch = require_gpu()
+    lib = load()
+    n, l = len(rx), len(tx)
+    f = lambda v: _dev(np.ascontiguousarray(v, np.float64).reshape(-1))
+    d_rx, d_ry, d_tx, d_ty, d_A, d_B, d_C = (f(v) for v in (rx, ry, tx, ty, A, B, C))
+    d_x, d_y, d_z = (f(np.asarray(v).reshape(-1, 3)) for v in (x, y, z))
+    # the reference's own output estimate, grown on demand
+    cap = max(n * (1 + int((7 * max_spacing * max_spacing) / (test_spacing * test_spacing))), 1)
+    ws = torch.empty(max(int(lib.rpx_trace_workspace_bytes(n * l)), 8), dtype=torch.uint8, device="cuda")
+    while True:
+        data = torch.empty(2 * cap, dtype=torch.float64, device="cuda")
+        ij = torch.empty(2 * cap, dtype=torch.int64, device="cuda")
+        k = ctypes.c_uint64(0)
+        code = lib.rpx_build_interaction_matrix(_ptr(d_rx), _ptr(d_ry), n, _ptr(d_tx), _ptr(d_ty), l, _ptr(d_A),
+                                                _ptr(d_B), _ptr(d_C), _ptr(d_x), _ptr(d_y), _ptr(d_z),
+                                                float(wavelength), float(spacing), float(max_spacing),
+                                                float(blending), _ptr(data), _ptr(ij), cap, ctypes.byref(k),
+                                                _ptr(ws), ws.numel(), stream_handle())
+        if code == _abi.RPX_E_CAPACITY:
+            cap = int(k.value)
+            continue
+        check(code)
+        break
+    m = int(k.value)
+    vals = data[: 2 * m].cpu().numpy().view(np.complex128)
+    idx = ij.cpu().numpy().view(np.uint64)
+    return vals.copy(), (idx[:m].copy(), idx[cap: cap + m].copy())

@@ -313,6 +313,199 @@ extern "C" rpx_status_t rpx_ray_field_modes(const rpx_rays_t *rays, uint64_t n, 
     return finish_compaction(w, out->capacity, n_out, stream, "ray field modes");
 }
 
+// ----------------------------------------------- decomposition mode maths
+// calc_mode_curvature (cfields.pyx:231-293): mode basis and curvature
+// coefficients from the wavefront's first and second derivatives.
+__global__ __launch_bounds__(256) void k_mode_curvature(const double *dx, const double *dy, const double *dx2,
+                                                        const double *dy2, const double *dxdy, uint64_t n,
+                                                        double *A, double *B, double *C, double *x, double *y,
+                                                        double *z) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 c = norm(v3(-dx[i], -dy[i], 1.0));
+    V3 b = v3(0.0, 0.0, 1.0);
+    const V3 a = norm(cross(b, c));
+    b = cross(c, a);
+    x[3 * i] = a.x; x[3 * i + 1] = a.y; x[3 * i + 2] = a.z;
+    y[3 * i] = b.x; y[3 * i + 1] = b.y; y[3 * i + 2] = b.z;
+    z[3 * i] = c.x; z[3 * i + 1] = c.y; z[3 * i + 2] = c.z;
+    const double Fz = c.x * dx[i] + c.y * dy[i] + c.z;
+    A[i] = -(a.x * a.x * dx2[i] + 2 * a.x * a.y * dxdy[i] + a.y * a.y * dy2[i]) / Fz;
+    B[i] = -(a.x * b.x * dx2[i] + (a.x * b.y + b.x * a.y) * dxdy[i] + a.y * b.y * dy2[i]) / Fz;
+    C[i] = -(b.x * b.x * dx2[i] + 2 * b.x * b.y * dxdy[i] + b.y * b.y * dy2[i]) / Fz;
+}
+
+extern "C" rpx_status_t rpx_calc_mode_curvature(const double *dx, const double *dy, const double *dx2,
+                                                const double *dy2, const double *dxdy, uint64_t n, double *A,
+                                                double *B, double *C, double *x, double *y, double *z,
+                                                void *stream) {
+    if (n == 0) return RPX_OK;
+    if (!dx || !dy || !dx2 || !dy2 || !dxdy || !A || !B || !C || !x || !y || !z)
+        return rpx_set_error(RPX_E_INVALID, "mode curvature buffers");
+    hipLaunchKernelGGL(k_mode_curvature, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, dx, dy,
+                       dx2, dy2, dxdy, n, A, B, C, x, y, z);
+    RPX_HIP_TRY(hipGetLastError());
+    return RPX_OK;
+}
+
+// apply_mode_curvature (cfields.pyx:397-424): tilts each parabasal direction
+// by the curvature coefficients in the base ray's (E, E x d) frame.
+__global__ __launch_bounds__(256) void k_apply_mode_curvature(rpx_rays_t g, uint64_t n, const double *A,
+                                                              const double *B, const double *C) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *f = g.f64;
+    const uint64_t c = g.capacity;
+    const V3 o = v3(f[RPX_OX * c + i], f[RPX_OY * c + i], f[RPX_OZ * c + i]);
+    const V3 zz = v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]);
+    const V3 xx = v3(f[RPX_EX * c + i], f[RPX_EY * c + i], f[RPX_EZ * c + i]);
+    const V3 yy = cross(xx, zz);
+    const double a = A[i], b = B[i], cc = C[i];
+    for (int j = 0; j < RPX_N_PARA; ++j) {
+        const uint64_t q = (uint64_t)j * RPX_PARA_FIELDS;
+        const V3 po = v3(g.para[(q + 0) * c + i], g.para[(q + 1) * c + i], g.para[(q + 2) * c + i]);
+        const V3 pd = v3(g.para[(q + 3) * c + i], g.para[(q + 4) * c + i], g.para[(q + 5) * c + i]);
+        const V3 r = sub(po, o);
+        const double px = dot(r, xx), py = dot(r, yy);
+        V3 shift = mul(xx, (-a * px - b * py));
+        shift = add(shift, mul(yy, (-cc * py - b * px)));
+        const V3 nd = norm(add(shift, pd));
+        g.para[(q + 3) * c + i] = nd.x;
+        g.para[(q + 4) * c + i] = nd.y;
+        g.para[(q + 5) * c + i] = nd.z;
+    }
+}
+
+extern "C" rpx_status_t rpx_apply_mode_curvature(rpx_rays_t *gausslets, uint64_t n, const double *A, const double *B,
+                                                 const double *C, void *stream) {
+    if (n == 0) return RPX_OK;
+    if (!gausslets || gausslets->capacity < n || !gausslets->f64 || !gausslets->para || !A || !B || !C)
+        return rpx_set_error(RPX_E_INVALID, "mode curvature buffers");
+    hipLaunchKernelGGL(k_apply_mode_curvature, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       *gausslets, n, A, B, C);
+    RPX_HIP_TRY(hipGetLastError());
+    return RPX_OK;
+}
+
+// calc_mode_U (cfields.pyx:118-151), statement by statement.
+RPX_DEV Cx calc_mode_u(Cx A, Cx B, Cx C, Cx detG0, V3 pt, V3 E, V3 H, V3 dir, Cx k, double phase,
+                       double inv_root_area) {
+    const Cx I = cx(0.0, 1.0);
+    const Cx rootI = cx(sqrt(0.5), sqrt(0.5));
+    const double x = dot(pt, E), y = dot(pt, H), z = dot(pt, dir);
+    const Cx denom = cmul(cadd(cadd(cx(1, 0), cmul(cx(z, 0), cadd(A, C))), cmul(cx(z * z, 0), detG0)), cx(2, 0));
+    const Cx zd = cmul(cx(z, 0), detG0);
+    const Cx AA = cdiv(cadd(A, zd), denom);
+    const Cx CC = cdiv(cadd(C, zd), denom);
+    Cx inner = cadd(cx(z, 0), cmul(AA, cx(x * x, 0)));
+    inner = cadd(inner, cdiv(cmul(B, cx(2 * x * y, 0)), denom));
+    inner = cadd(inner, cmul(CC, cx(y * y, 0)));
+    Cx U = cexp_(cmul(I, cadd(cx(phase, 0), cmul(k, inner))));
+    const Cx q1 = cadd(cx(1, 0), cmul(cx(z, 0), A));
+    const Cx q2 = cadd(cx(1, 0), cmul(cx(z, 0), C));
+    const Cx zb = cmul(cx(z, 0), B);
+    U = cdiv(U, csqrt_(cmul(csub(cmul(q1, q2), cmul(zb, zb)), I)));
+    return cmul(U, cmul(cx(inv_root_area, 0), rootI));
+}
+
+// build_interaction_matrix (cfields.pyx:295-395): the COO entries (i, j,
+// E_ij) for every mode i and test point j within max_spacing, in (i, j)
+// order -- one lane per pair of the flattened index i * L + j, placed by the
+// look-back.
+struct InteractionArgs {
+    const double *rx, *ry, *tx, *ty, *A, *B, *C, *x, *y, *z;
+    uint64_t n, l;
+    double k, impart, max2;
+    double *data;
+    uint64_t *ij;
+    uint64_t capacity;
+    Workspace ws;
+};
+
+__global__ __launch_bounds__(RPX_BLOCK) void k_interaction(InteractionArgs P) {
+    __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
+    __shared__ uint32_t lds_tile;
+    __shared__ uint64_t lds_excl;
+    if (threadIdx.x == 0) lds_tile = atomicAdd(&P.ws.ctl->tile_counter, 1u);
+    __syncthreads();
+    const uint32_t tile = lds_tile;
+    const uint64_t total = P.n * P.l;
+    const uint64_t t = (uint64_t)tile * RPX_BLOCK + threadIdx.x;
+    bool sel = false;
+    uint64_t i = 0, j = 0;
+    V3 pt = v3(0, 0, 0);
+    if (t < total) {
+        i = t / P.l;
+        j = t - i * P.l;
+        pt = sub(v3(P.tx[j], P.ty[j], 0.0), v3(P.rx[i], P.ry[i], 0.0));
+        sel = ((pt.x * pt.x) + (pt.y * pt.y)) <= P.max2;
+    }
+    Cx U = cx(0, 0);
+    if (sel) {
+        const Cx A = cx(P.A[i], P.impart), B = cx(P.B[i], 0.0), C = cx(P.C[i], P.impart);
+        const Cx detG0 = cmul(A, C);
+        const V3 a = v3(P.x[3 * i], P.x[3 * i + 1], P.x[3 * i + 2]);
+        const V3 b = v3(P.y[3 * i], P.y[3 * i + 1], P.y[3 * i + 2]);
+        const V3 c = v3(P.z[3 * i], P.z[3 * i + 1], P.z[3 * i + 2]);
+        U = calc_mode_u(A, B, C, detG0, pt, a, b, c, cx(P.k, 0.0), 0.0, 1.0);
+    }
+    uint32_t block_total;
+    const uint32_t local = block_scan_small(sel ? 1u : 0u, lds_wave, &block_total);
+    if (threadIdx.x < 64) {
+        const uint64_t excl = lookback(P.ws.tile_status, tile, 0, block_total, &P.ws.ctl->error);
+        if (threadIdx.x == 0) lds_excl = excl;
+    }
+    __syncthreads();
+    const uint64_t pos = lds_excl + local;
+    if (sel && pos < P.capacity) {
+        P.data[2 * pos] = U.re;
+        P.data[2 * pos + 1] = U.im;
+        P.ij[pos] = i;
+        P.ij[P.capacity + pos] = j;
+    }
+    if (threadIdx.x == 0 && (uint64_t)tile == (total - 1) / RPX_BLOCK) P.ws.ctl->children = lds_excl + block_total;
+}
+
+extern "C" rpx_status_t rpx_build_interaction_matrix(const double *rx, const double *ry, uint64_t n,
+                                                     const double *tx, const double *ty, uint64_t l,
+                                                     const double *A, const double *B, const double *C,
+                                                     const double *x, const double *y, const double *z,
+                                                     double wavelength, double spacing, double max_spacing,
+                                                     double blending, double *data, uint64_t *ij,
+                                                     uint64_t capacity, uint64_t *n_out, void *workspace,
+                                                     uint64_t workspace_bytes, void *stream_) {
+    if (!n_out) return rpx_set_error(RPX_E_INVALID, "n_out is null");
+    *n_out = 0;
+    const uint64_t total = n * l;
+    if (total == 0) return RPX_OK;
+    if (l != 0 && total / l != n) return rpx_set_error(RPX_E_UNSUPPORTED, "n * l overflows");
+    if (total > 0xFFFFFF00ull) return rpx_set_error(RPX_E_UNSUPPORTED, "more than 2^32-256 mode / point pairs");
+    if (!rx || !ry || !tx || !ty || !A || !B || !C || !x || !y || !z)
+        return rpx_set_error(RPX_E_INVALID, "interaction matrix inputs");
+    if (capacity > 0 && (!data || !ij)) return rpx_set_error(RPX_E_INVALID, "interaction matrix outputs");
+    if (!workspace || workspace_bytes < rpx_trace_workspace_bytes(total))
+        return rpx_set_error(RPX_E_INVALID, "workspace too small");
+    hipStream_t stream = (hipStream_t)stream_;
+    InteractionArgs P;
+    P.rx = rx; P.ry = ry; P.tx = tx; P.ty = ty; P.A = A; P.B = B; P.C = C; P.x = x; P.y = y; P.z = z;
+    P.n = n;
+    P.l = l;
+    P.k = 2000.0 * M_PI / wavelength;
+    P.impart = 2 * (blending * blending) / (P.k * spacing * spacing);
+    P.max2 = max_spacing * max_spacing;
+    P.data = data;
+    P.ij = ij;
+    P.capacity = capacity;
+    P.ws = carve(workspace, total);
+    const uint64_t nb = (total + RPX_BLOCK - 1) / RPX_BLOCK;
+    hipLaunchKernelGGL(k_clear, dim3((unsigned)((std::max<uint64_t>(nb, RPX_SHARDS * RPX_LDS_FACES) + 255) / 256)),
+                       dim3(256), 0, stream, P.ws.tile_status, nb, P.ws.ctl);
+    RPX_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_interaction, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, P);
+    RPX_HIP_TRY(hipGetLastError());
+    return finish_compaction(P.ws, capacity, n_out, stream, "interaction matrix");
+}
+
 // ------------------------------------------------------------ field sum FP64
 // Per-gausslet invariants, SoA with stride G: origin(3) E(3) H(3) dir(3)
 // kz(2) phase A(2) B(2) C(2) detG0(2) inv_root_area E1(2) E2(2) = 30.

@@ -134,3 +134,102 @@ def test_empty_decomposition_and_two_faces(gpu):
     assert len(calls) == 2 and calls[0] > 0 and calls[1] == 0
     assert mat.capture_count == calls[0]
     assert all(f.count == 0 for f in plane.faces)
+
+
+# ------------------------------------------------ decomposition mode maths
+from raypier_optics_amd import cfields, fields
+from raypier_optics_amd import gausslets as rg
+from oracle import gausslets as og
+
+
+def _slopes(n, seed):
+    rng = np.random.default_rng(seed)
+    rx, ry = rng.uniform(-2, 2, n), rng.uniform(-2, 2, n)
+    return (rx, ry, rng.uniform(-0.1, 0.1, n), rng.uniform(-0.1, 0.1, n), rng.uniform(-0.05, 0.05, n),
+            rng.uniform(-0.05, 0.05, n), rng.uniform(-0.01, 0.01, n))
+
+
+def test_calc_mode_curvature_parity(gpu):
+    args = _slopes(5000, 3)
+    got = cfields.calc_mode_curvature(*args)
+    ref = oracle.calc_mode_curvature(*args)
+    for g, r in zip(got, ref):
+        assert g.tobytes() == r.tobytes()
+
+
+def test_apply_mode_curvature_parity(gpu):
+    gc, faces, info = workloads.gausslet_c4(n_gausslets=4000, grid=16)
+    before = gc.copy_as_array()
+    rng = np.random.default_rng(4)
+    A, B, C = (rng.uniform(-0.05, 0.05, gc.n_rays) for _ in range(3))
+    cfields.apply_mode_curvature(gc, A, B, C)
+    assert gc.copy_as_array().tobytes() == oracle.apply_mode_curvature(before, A, B, C).tobytes()
+
+
+def test_build_interaction_matrix_parity(gpu):
+    rx, ry = rg.make_hexagonal_grid(1.5, spacing=0.1)
+    tx, ty = rg.make_hexagonal_grid(1.5, spacing=0.05)
+    n = len(rx)
+    rng = np.random.default_rng(5)
+    dx, dy = rng.uniform(-0.05, 0.05, n), rng.uniform(-0.05, 0.05, n)
+    A, B, C, x, y, z = oracle.calc_mode_curvature(rx, ry, dx, dy, rng.uniform(-0.1, 0.1, n),
+                                                  rng.uniform(-0.1, 0.1, n), rng.uniform(-0.02, 0.02, n))
+    args = (rx, ry, tx, ty, A, B, C, x, y, z, 0.8, 0.1, 0.25, 1.5)
+    data, (i, j) = cfields.build_interaction_matrix(*args, test_spacing=0.3)   # estimate too small: regrow
+    rdata, (ri, rj) = oracle.build_interaction_matrix(*args)
+    assert len(ri) > 10 * n
+    assert np.array_equal(i, ri) and np.array_equal(j, rj)
+    assert np.allclose(data, rdata, rtol=1e-12, atol=0)
+
+
+def _beam():
+    """The C4 source: a collimated Gaussian beam of gausslets at z = -80
+    (a wavefront the decomposition grid resolves)."""
+    gc, faces, info = workloads.gausslet_c4(n_gausslets=3000, grid=16)
+    return gc
+
+
+@pytest.mark.parametrize("curvature", [None, 1e5])
+def test_decompose_position(gpu, curvature):
+    beam = _beam()
+    plane_z = -79.0
+    kw = dict(origin=(0.0, 0.0, plane_z), direction=(0.0, 0.0, 1.0), axis1=(1.0, 0.0, 0.0), radius=3.0,
+              resolution=12.0, curvature=curvature, blending=1.5)
+    got, E_field, uphase = rg.decompose_position(beam, **kw)
+    ref, rE, ruphase = og.decompose_position(beam.copy_as_array(), np.asarray(beam.wavelengths), **kw)
+    g = got.copy_as_array()
+    assert len(g) == len(ref) > 150
+    scale = np.abs(rE).max()
+    assert np.abs(E_field - rE).max() <= 1e-9 * scale
+    assert np.abs(uphase - ruphase).max() <= 1e-6
+    b, rb = g["base_ray"], ref["base_ray"]
+    assert b["origin"].tobytes() == rb["origin"].tobytes()
+    assert np.abs(b["direction"] - rb["direction"]).max() <= 1e-9
+    amp = np.abs(rb["E1_amp"]).max()
+    assert np.abs(b["E1_amp"] - rb["E1_amp"]).max() <= 1e-6 * amp
+    assert np.abs(g["para_rays"]["direction"] - ref["para_rays"]["direction"]).max() <= 1e-9
+    # the decomposed beam carries the input's field across the launch disc
+    pts = np.stack([np.linspace(-2.0, 2.0, 21), np.zeros(21), np.full(21, plane_z + 0.5)], axis=1)
+    e_in = fields.eval_Efield_from_gausslets(beam, pts)
+    e_out = fields.eval_Efield_from_gausslets(got, pts)
+    err = np.abs(np.abs(e_out[:, 0]) - np.abs(e_in[:, 0])).max() / np.abs(e_in[:, 0]).max()
+    print("decomposed field vs input: %.3g of max" % err)
+    assert err <= 0.05
+
+
+def test_decompose_angle(gpu):
+    """A Gaussian spot sampled on a grid, decomposed into directions: the
+    host steps are the reference's numpy/scipy calls, the parabasal set-up
+    runs on the device and agrees bit for bit."""
+    n = 32
+    s = 2.0                                             # sample spacing, microns
+    g = (np.arange(n) - n / 2) * s
+    X, Y = np.meshgrid(g, g)
+    E = np.zeros((n, n, 3), np.complex128)
+    E[:, :, 0] = np.exp(-(X ** 2 + Y ** 2) / (12.0 ** 2))
+    args = ((0.0, 0.0, 0.0), np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0]), E, s, 10.0, 1.0)
+    got, spec = rg.decompose_angle(*args)
+    ref, rspec = og.decompose_angle(*args)
+    assert spec.tobytes() == rspec.tobytes()
+    assert got.n_rays == len(ref) > 20
+    assert got.copy_as_array().tobytes() == ref.tobytes()
