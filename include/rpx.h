@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RPX_ABI_VERSION 6
+#define RPX_ABI_VERSION 7
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t rpx_status_t;
@@ -228,6 +228,9 @@ typedef struct rpx_zernike_coef {
     int32_t r_ref, rp_ref, ror_ref;   /* value refs in the grad program   */
     int32_t sag_ref;                  /* value ref in the sag program     */
     double  value;
+    double  nval;             /* sqrt(n+1) (m == 0) or sqrt(2(n+1)), times value:
+                                 the normalised coefficient N of
+                                 cdistortions.pyx:434-436, computed once */
 } rpx_zernike_coef_t;
 
 typedef struct rpx_zernike {
@@ -236,7 +239,7 @@ typedef struct rpx_zernike {
     int32_t coef_begin;       /* index into zernike_coefs                  */
     int32_t sag_begin, sag_len;     /* range in zernike_insns              */
     int32_t grad_begin, grad_len;
-    int32_t n_slots;          /* 3 * k_max                                 */
+    int32_t n_slots;          /* workspace slots the programs use (<= 3 * k_max) */
     int32_t pad;
 } rpx_zernike_t;
 

@@ -142,6 +142,42 @@ class _ZProgram(object):
         return s
 
 
+def compact_slots(insns, final_refs):
+    """Renumber the workspace slots of one straight-line program so that a
+    slot is reused once its value is dead (linear scan over the program;
+    the slots in final_refs stay live to the end).  Returns (program, map
+    from old to new slot, slots used).  Every source is written before it
+    is read (memoised recursion), so the renumbering keeps every value."""
+    last_use = {}
+    for k, ins in enumerate(insns):
+        for s in ins["src"]:
+            if s >= 0:
+                last_use[int(s)] = k
+    for s in final_refs:
+        last_use[s] = len(insns)
+    mapping, free, used = {}, [], 0
+    out = []
+    for k, ins in enumerate(insns):
+        rec = ins.copy()
+        rec["src"] = [mapping[int(s)] if s >= 0 else int(s) for s in ins["src"]]
+        for s in sorted(set(int(x) for x in ins["src"] if x >= 0)):
+            if last_use.get(s) == k:
+                free.append(mapping[s])
+        free.sort(reverse=True)
+        dst = int(ins["dst"])
+        if free:
+            phys = free.pop()
+        else:
+            phys = used
+            used += 1
+        mapping[dst] = phys
+        rec["dst"] = phys
+        if last_use.get(dst, -1) < k:     # written and never read: free at once
+            free.append(phys)
+        out.append(rec)
+    return out, mapping, used
+
+
 def compile_zernike(coef_list, unit_radius):
     """Compile sorted (j, value) pairs into (zernike record fields, coef
     records, sag instructions, grad instructions)."""
@@ -156,13 +192,26 @@ def compile_zernike(coef_list, unit_radius):
     for (j, v), (n, m, k) in zip(coef_list, nmk):
         rec = np.zeros((), _abi.zernike_coef_dtype)
         rec["n"], rec["m"], rec["value"] = n, m, v
+        # N = sqrt(n+1) or sqrt(2(n+1)), then N *= value, as the reference
+        # evaluates it per call (correctly rounded on host and device alike)
+        N = math.sqrt(float(n + 1)) if m == 0 else math.sqrt(float(2 * (n + 1)))
+        rec["nval"] = N * float(v)
         rec["sag_ref"] = sag.R(k, n, abs(m))
         # z_offset_and_gradient_c calls R, R', R/r in this order (:496-498)
         rec["r_ref"] = grad.R(k, n, abs(m))
         rec["rp_ref"] = grad.Rp(k, n, abs(m))
         rec["ror_ref"] = grad.Ror(k, n, abs(m))
         coefs.append(rec)
-    return kmax, coefs, sag.insns, grad.insns
+    sag_p, smap, s_used = compact_slots(sag.insns, [int(c["sag_ref"]) for c in coefs if c["sag_ref"] >= 0])
+    grad_p, gmap, g_used = compact_slots(grad.insns, [int(c[f]) for c in coefs for f in ("r_ref", "rp_ref", "ror_ref")
+                                                      if c[f] >= 0])
+    for c in coefs:
+        if c["sag_ref"] >= 0:
+            c["sag_ref"] = smap[int(c["sag_ref"])]
+        for f in ("r_ref", "rp_ref", "ror_ref"):
+            if c[f] >= 0:
+                c[f] = gmap[int(c[f])]
+    return kmax, coefs, sag_p, grad_p, max(s_used, g_used, 1)
 
 
 class ZernikeDistortion(Distortion):
@@ -212,7 +261,7 @@ class ZernikeDistortion(Distortion):
         raise NotImplementedError()
 
     def _rpx_compile(self, rec, builder):
-        kmax, coefs, sag, grad = compile_zernike(self._coefs, self.unit_radius)
+        kmax, coefs, sag, grad, n_slots = compile_zernike(self._coefs, self.unit_radius)
         cb, ib = builder.add_zernike_program(coefs, sag + grad)
         rec["unit_radius"] = self.unit_radius
         rec["n_coefs"] = len(coefs)
@@ -221,7 +270,7 @@ class ZernikeDistortion(Distortion):
         rec["sag_len"] = len(sag)
         rec["grad_begin"] = ib + len(sag)
         rec["grad_len"] = len(grad)
-        rec["n_slots"] = 3 * kmax
+        rec["n_slots"] = n_slots
 
     def z_offset(self, x, y):
         from ._queries import zernike_eval

@@ -89,20 +89,22 @@ RPX_DEV bool shape_inside(const SceneView &S, int root, double x, double y) {
 // ------------------------------------------------------------- Zernike
 // Straight-line form of the memoised radial recursion of
 // cdistortions.pyx:149-316 (see rpx_zernike_insn_t).
-RPX_DEV double zref(const double *slot, int ref) {
-    return ref >= 0 ? slot[ref] : (ref == -1 ? 0.0 : 1.0);
+// The workspace is addressed as slot[ref * stride]: stride 1 for a private
+// array, RPX_BLOCK for the LDS array below (one column per lane).
+RPX_DEV double zref(const double *slot, int ref, int stride) {
+    return ref >= 0 ? slot[ref * stride] : (ref == -1 ? 0.0 : 1.0);
 }
 
-RPX_DEV void zernike_run(const SceneView &S, int begin, int len, double r, double *slot) {
+RPX_DEV void zernike_run(const SceneView &S, int begin, int len, double r, double *slot, int stride) {
     for (int i = begin; i < begin + len; ++i) {
         const rpx_zernike_insn_t &in = S.zinsns[i];
-        double a = zref(slot, in.src[0]), b = zref(slot, in.src[1]), c = zref(slot, in.src[2]);
+        double a = zref(slot, in.src[0], stride), b = zref(slot, in.src[1], stride), c = zref(slot, in.src[2], stride);
         double v;
         if (in.op == 0) {
             v = r * (a + b);
             v -= c;
         } else if (in.op == 1) {
-            double d = zref(slot, in.src[3]), e = zref(slot, in.src[4]);
+            double d = zref(slot, in.src[3], stride), e = zref(slot, in.src[4], stride);
             v = a + b;
             v += r * (c + d);
             v -= e;
@@ -110,27 +112,78 @@ RPX_DEV void zernike_run(const SceneView &S, int begin, int len, double r, doubl
             v = a + b;
             v -= c;
         }
-        slot[in.dst] = v;
+        slot[in.dst * stride] = v;
     }
+}
+
+// Programs are straight-line chains (each instruction reads what the last
+// ones wrote), so their workspace sits in LDS when it fits: a private array
+// would put every step through scratch memory.  Column threadIdx.x of
+// RPX_ZSLOTS_LDS rows; kernels that run the distortion code launch blocks of
+// at most RPX_BLOCK threads.
+#define RPX_ZSLOTS_LDS 24
+RPX_DEV double *zernike_lds() {
+    __shared__ double zs[RPX_ZSLOTS_LDS * 256];
+    return zs + threadIdx.x;
+}
+
+// cos(k theta) and sin(k theta) for theta = atan2(y, x), from cos = x/r and
+// sin = y/r by the angle-addition recurrence (equal to the reference's
+// cos(m * theta) to a few ulp for the orders a distortion uses); at r = 0
+// the reference's atan2 branch value is kept.
+struct Harmonic {
+    double c1, s1;      // cos theta, sin theta
+    double theta;       // used only at r = 0
+    bool origin;
+};
+
+RPX_DEV Harmonic harmonic(double x, double y, double r) {
+    Harmonic h;
+    h.origin = r == 0.0;
+    h.theta = h.origin ? atan2(y, x) : 0.0;
+    h.c1 = h.origin ? 1.0 : x / r;
+    h.s1 = h.origin ? 0.0 : y / r;
+    return h;
+}
+
+RPX_DEV void harmonic_k(const Harmonic &h, int k, double *ck, double *sk) {
+    if (h.origin) {
+        *ck = cos(k * h.theta);
+        *sk = sin(k * h.theta);
+        return;
+    }
+    double c = 1.0, s = 0.0;
+    for (int i = 0; i < k; ++i) {
+        const double cn = c * h.c1 - s * h.s1;
+        const double sn = s * h.c1 + c * h.s1;
+        c = cn;
+        s = sn;
+    }
+    *ck = c;
+    *sk = s;
 }
 
 // ZernikeDistortion.z_offset_c (cdistortions.pyx:416-450)
 RPX_DEV double zernike_sag(const SceneView &S, int zi, double x, double y) {
     const rpx_zernike_t &Z = S.zernikes[zi];
-    double slot[RPX_MAX_ZERNIKE_SLOTS];
+    double priv[RPX_MAX_ZERNIKE_SLOTS];
+    const bool lds = Z.n_slots <= RPX_ZSLOTS_LDS;
+    double *slot = lds ? zernike_lds() : priv;
+    const int stride = lds ? 256 : 1;
     x /= Z.unit_radius;
     y /= Z.unit_radius;
     double r = sqrt(x * x + y * y);
-    double theta = atan2(y, x);
-    zernike_run(S, Z.sag_begin, Z.sag_len, r, slot);
+    const Harmonic hm = harmonic(x, y, r);
+    zernike_run(S, Z.sag_begin, Z.sag_len, r, slot, stride);
     double z = 0.0;
     for (int i = 0; i < Z.n_coefs; ++i) {
         const rpx_zernike_coef_t &c = S.zcoefs[Z.coef_begin + i];
-        double N = (c.m == 0) ? sqrt((double)(c.n + 1)) : sqrt((double)(2 * (c.n + 1)));
-        N *= c.value;
-        double PH = (c.m >= 0) ? cos(c.m * theta) : -sin(c.m * theta);
-        double R = zref(slot, c.sag_ref);
-        z += N * R * PH;
+        double ck, sk;
+        harmonic_k(hm, c.m >= 0 ? c.m : -c.m, &ck, &sk);
+        // cos(m theta), or -sin(m theta) = sin(|m| theta) for m < 0
+        const double PH = (c.m >= 0) ? ck : sk;
+        double R = zref(slot, c.sag_ref, stride);
+        z += c.nval * R * PH;
     }
     return z;
 }
@@ -139,27 +192,33 @@ RPX_DEV double zernike_sag(const SceneView &S, int zi, double x, double y) {
 // returns (dz/dx, dz/dy, z).
 RPX_DEV V3 zernike_grad(const SceneView &S, int zi, double x, double y) {
     const rpx_zernike_t &Z = S.zernikes[zi];
-    double slot[RPX_MAX_ZERNIKE_SLOTS];
+    double priv[RPX_MAX_ZERNIKE_SLOTS];
+    const bool lds = Z.n_slots <= RPX_ZSLOTS_LDS;
+    double *slot = lds ? zernike_lds() : priv;
+    const int stride = lds ? 256 : 1;
     x /= Z.unit_radius;
     y /= Z.unit_radius;
     double r = sqrt(x * x + y * y);
-    double theta = atan2(y, x);
-    zernike_run(S, Z.grad_begin, Z.grad_len, r, slot);
+    const Harmonic hm = harmonic(x, y, r);
+    zernike_run(S, Z.grad_begin, Z.grad_len, r, slot, stride);
     V3 out = v3(0.0, 0.0, 0.0);
-    double st = sin(theta), ct = cos(theta);
+    const double st = hm.origin ? sin(hm.theta) : hm.s1, ct = hm.origin ? cos(hm.theta) : hm.c1;
     for (int i = 0; i < Z.n_coefs; ++i) {
         const rpx_zernike_coef_t &c = S.zcoefs[Z.coef_begin + i];
+        const int am = c.m >= 0 ? c.m : -c.m;
+        double ck, sk;
+        harmonic_k(hm, am, &ck, &sk);
         double PH, PHp;
         if (c.m >= 0) {
-            PH = cos(c.m * theta);
-            PHp = (double)(-c.m) * sin(c.m * theta);
+            PH = ck;                       // cos(m theta)
+            PHp = (double)(-c.m) * sk;     // -m sin(m theta)
         } else {
-            PH = -sin(c.m * theta);
-            PHp = (double)(-c.m) * cos(c.m * theta);
+            PH = sk;                       // -sin(m theta) = sin(|m| theta)
+            PHp = (double)(-c.m) * ck;     // -m cos(m theta) = |m| cos(|m| theta)
         }
-        double R = zref(slot, c.r_ref), Rp = zref(slot, c.rp_ref), Ror = zref(slot, c.ror_ref);
-        double N = (c.m == 0) ? sqrt((double)(c.n + 1)) : sqrt((double)(2 * (c.n + 1)));
-        N *= c.value;
+        double R = zref(slot, c.r_ref, stride), Rp = zref(slot, c.rp_ref, stride),
+               Ror = zref(slot, c.ror_ref, stride);
+        const double N = c.nval;
         out.z += N * R * PH;
         out.x += N * (Rp * ct * PH + Ror * (-st) * PHp);
         out.y += N * (Rp * st * PH + Ror * (ct)*PHp);

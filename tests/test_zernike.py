@@ -228,7 +228,7 @@ def test_program_independent_of_operand_order():
     4-15 set the recursion's cache contents do not depend on it: swapping the
     two recursive operands gives the same values."""
     coefs = [(j, 0.01 * (j - 2)) for j in range(3, 15)]
-    kmax, cs, sag, grad = compile_zernike(sorted(coefs), 1.0)
+    kmax, cs, sag, grad, _ = compile_zernike(sorted(coefs), 1.0)
     import raypier_optics_amd.cdistortions as cd
     orig_R, orig_Rp = cd._ZProgram.R, cd._ZProgram.Rp
 
@@ -252,7 +252,7 @@ def test_program_independent_of_operand_order():
         return s
     try:
         cd._ZProgram.R = R_swapped
-        kmax2, cs2, sag2, grad2 = compile_zernike(sorted(coefs), 1.0)
+        kmax2, cs2, sag2, grad2, _ = compile_zernike(sorted(coefs), 1.0)
     finally:
         cd._ZProgram.R = orig_R
     for r in np.linspace(0, 1, 9):
