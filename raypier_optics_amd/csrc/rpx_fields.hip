@@ -532,7 +532,7 @@ __global__ void k_field_prep(rpx_rays_t r, uint64_t n, const double *modes, WlTa
     double v[RPX_GP] = {ray.origin.x, ray.origin.y, ray.origin.z, E.x, E.y, E.z, H.x, H.y, H.z,
                         ray.direction.x, ray.direction.y, ray.direction.z, kz.re, kz.im, phase,
                         A.re, A.im, B.re, B.im, C.re, C.im, detG0.re, detG0.im, ira,
-                        ray.E1.re, ray.E1.im, ray.E2.re, ray.E2.im, 0.0, 0.0};
+                        ray.E1.re, ray.E1.im, ray.E2.re, ray.E2.im, A.re + C.re, A.im + C.im};
     for (int k2 = 0; k2 < RPX_GP; ++k2) gp[(uint64_t)k2 * n + i] = v[k2];
 }
 
@@ -547,7 +547,6 @@ __global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64
     V3 pt0 = active ? v3(points[3 * ip], points[3 * ip + 1], points[3 * ip + 2]) : v3(0, 0, 0);
     double o0r = 0, o0i = 0, o1r = 0, o1i = 0, o2r = 0, o2i = 0;
     const Cx rootI = cx(sqrt(0.5), sqrt(0.5));
-    const Cx I = cx(0.0, 1.0);
     for (uint64_t base = 0; base < G; base += FS_TILE) {
         const int nt = (int)((G - base) < FS_TILE ? (G - base) : FS_TILE);
         __syncthreads();
@@ -568,28 +567,42 @@ __global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64
             Cx C = cx(tile[19][g], tile[20][g]), detG0 = cx(tile[21][g], tile[22][g]);
             double ira = tile[23][g];
             Cx E1a = cx(tile[24][g], tile[25][g]), E2a = cx(tile[26][g], tile[27][g]);
-            // calc_mode_U (cfields.pyx:118-151)
+            // calc_mode_U (cfields.pyx:118-151), real-by-complex products
+            // written out and the three divisions by `denom` as one complex
+            // reciprocal (equal to the reference's expression to a few ulp)
+            const double ApCr = tile[28][g], ApCi = tile[29][g];
             V3 pt = sub(pt0, O);
-            double x = dot(pt, E), y = dot(pt, H), z = dot(pt, D);
-            Cx denom = cmul(cadd(cadd(cx(1, 0), cmul(cx(z, 0), cadd(A, C))), cmul(cx(z * z, 0), detG0)), cx(2, 0));
-            Cx zd = cmul(cx(z, 0), detG0);
-            Cx AA = cdiv(cadd(A, zd), denom);
-            Cx CC = cdiv(cadd(C, zd), denom);
-            Cx inner = cadd(cx(z, 0), cmul(AA, cx(x * x, 0)));
-            inner = cadd(inner, cdiv(cmul(B, cx(2 * x * y, 0)), denom));
-            inner = cadd(inner, cmul(CC, cx(y * y, 0)));
-            Cx arg = cadd(cx(phase, 0), cmul(kz, inner));
-            Cx U = cexp_(cmul(I, arg));
-            Cx q1 = cadd(cx(1, 0), cmul(cx(z, 0), A));
-            Cx q2 = cadd(cx(1, 0), cmul(cx(z, 0), C));
-            Cx zb = cmul(cx(z, 0), B);
-            Cx s = csqrt_(cmul(csub(cmul(q1, q2), cmul(zb, zb)), I));
-            U = cdiv(U, s);
-            U = cmul(U, cmul(cx(ira, 0), rootI));
+            const double x = dot(pt, E), y = dot(pt, H), z = dot(pt, D);
+            const double z2 = z * z;
+            const double dre = (1.0 + z * ApCr + z2 * detG0.re) * 2.0;
+            const double dim = (z * ApCi + z2 * detG0.im) * 2.0;
+            const double dinv = 1.0 / (dre * dre + dim * dim);
+            const Cx rd = cx(dre * dinv, -dim * dinv);
+            const Cx AA = cmul(cx(A.re + z * detG0.re, A.im + z * detG0.im), rd);
+            const Cx CC = cmul(cx(C.re + z * detG0.re, C.im + z * detG0.im), rd);
+            const Cx Bd = cmul(B, rd);
+            const double x2 = x * x, y2 = y * y, xy2 = 2.0 * x * y;
+            const double in_re = z + AA.re * x2 + Bd.re * xy2 + CC.re * y2;
+            const double in_im = AA.im * x2 + Bd.im * xy2 + CC.im * y2;
+            // i * (phase + kz * inner)
+            const double ar = phase + (kz.re * in_re - kz.im * in_im);
+            const double ai = kz.re * in_im + kz.im * in_re;
+            double sn, cs;
+            sincos(ar, &sn, &cs);
+            const double ex = exp(-ai);
+            // 1 / csqrt(((1 + zA)(1 + zC) - (zB)^2) * I)
+            const Cx q1 = cx(1.0 + z * A.re, z * A.im), q2 = cx(1.0 + z * C.re, z * C.im);
+            const Cx zb = cx(z * B.re, z * B.im);
+            const Cx w0 = csub(cmul(q1, q2), cmul(zb, zb));
+            const Cx sq = csqrt_(cx(-w0.im, w0.re));
+            const double sinv = ex * ira / (sq.re * sq.re + sq.im * sq.im);
+            // U = exp(i arg) / sq * ira * rootI
+            const Cx u0 = cmul(cx(cs, sn), cx(sq.re, -sq.im));
+            const Cx U = cmul(cx(u0.re * sinv, u0.im * sinv), rootI);
             Cx E1 = cmul(E1a, U), E2 = cmul(E2a, U);
-            Cx t0 = cadd(cmul(E1, cx(E.x, 0)), cmul(E2, cx(H.x, 0)));
-            Cx t1 = cadd(cmul(E1, cx(E.y, 0)), cmul(E2, cx(H.y, 0)));
-            Cx t2 = cadd(cmul(E1, cx(E.z, 0)), cmul(E2, cx(H.z, 0)));
+            Cx t0 = cx(E1.re * E.x + E2.re * H.x, E1.im * E.x + E2.im * H.x);
+            Cx t1 = cx(E1.re * E.y + E2.re * H.y, E1.im * E.y + E2.im * H.y);
+            Cx t2 = cx(E1.re * E.z + E2.re * H.z, E1.im * E.z + E2.im * H.z);
             o0r += t0.re; o0i += t0.im;
             o1r += t1.re; o1i += t1.im;
             o2r += t2.re; o2i += t2.im;
