@@ -304,6 +304,18 @@ RPX_DEV double asph_grad(const Asph &A, double alpha) {
     return out;
 }
 
+// A base-ray hit on an aspheric face is accepted only inside the aperture,
+// where the surface lies within z_height +/- p[12]; the converged root
+// satisfies the sag equation to within the Newton tolerance.  A segment whose
+// z-range misses that band cannot be accepted.
+RPX_DEV bool asph_outside_band(const rpx_face_t &f, V3 p1, V3 p2, int is_base) {
+    if (!is_base || f.p[12] < 0.0) return false;
+    double len = sep(p1, p2);
+    double m = f.p[12] + 1e-9 * (1.0 + fabs(f.p[1]) + f.p[12]) + 10.0 * f.p[10] * len * (1.0 + f.p[13]);
+    double zlo = fmin(p1.z, p2.z), zhi = fmax(p1.z, p2.z);
+    return zhi < f.p[1] - m || zlo > f.p[1] + m;
+}
+
 // ------------------------------------------------------------- intersect
 // Base-face intersection used by DistortionFace (no recursion into a second
 // distortion level: the host refuses nested distortions).
@@ -432,16 +444,7 @@ RPX_DEV double plain_intersect(const SceneView &S, const rpx_face_t &f, V3 p1, V
         if constexpr ((F & F_ASPHERE) == 0) return -1;  // cfaces.pyx:1909-1976
         double atol = f.p[10];
         double tl = atol * atol;
-        if (is_base && f.p[12] >= 0.0) {
-            // A base-ray hit is accepted only inside the aperture, where the
-            // surface lies within z_height +/- p[12]; the converged root
-            // satisfies the sag equation to within the Newton tolerance.  A
-            // segment whose z-range misses that band cannot be accepted.
-            double len = sep(p1, p2);
-            double m = f.p[12] + 1e-9 * (1.0 + fabs(f.p[1]) + f.p[12]) + 10.0 * atol * len * (1.0 + f.p[13]);
-            double zlo = fmin(p1.z, p2.z), zhi = fmax(p1.z, p2.z);
-            if (zhi < f.p[1] - m || zlo > f.p[1] + m) return -1;
-        }
+        if (asph_outside_band(f, p1, p2, is_base)) return -1;
         V3 d = sub(p2, p1);
         V3 a = p1;
         a.z -= f.p[1];

@@ -46,10 +46,14 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_generation_split(SplitArgs A) {
     __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
     __shared__ uint32_t lds_tile, lds_hits;
     __shared__ uint64_t lds_excl;
+    // the tile number is fetched first, so that its round trip overlaps the
+    // scene staging
+    uint32_t my_tile = 0;
+    if (threadIdx.x == 0) my_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
     const SceneView S = stage_scene(A.S, lds_scene);
     for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
     if (threadIdx.x == 0) {
-        lds_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
+        lds_tile = my_tile;
         lds_hits = 0;
     }
     __syncthreads();
