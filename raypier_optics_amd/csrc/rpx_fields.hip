@@ -508,8 +508,10 @@ extern "C" rpx_status_t rpx_build_interaction_matrix(const double *rx, const dou
 
 // ------------------------------------------------------------ field sum FP64
 // Per-gausslet invariants, SoA with stride G: origin(3) E(3) H(3) dir(3)
-// kz(2) phase A(2) B(2) C(2) detG0(2) inv_root_area E1(2) E2(2) = 30.
-#define RPX_GP 30
+// kz(2) phase A(2) B(2) C(2) detG0(2) inv_root_area E1(2) E2(2) A+C(2)
+// V(6) = 36, where V = (E1 E + E2 H) inv_root_area sqrt(i) is the complex
+// vector each unit of U adds to the field (cfields.pyx:108-114, 147-148).
+#define RPX_GP 36
 
 __global__ void k_field_prep(rpx_rays_t r, uint64_t n, const double *modes, WlTable wl, double ct, double *gp) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -529,30 +531,38 @@ __global__ void k_field_prep(rpx_rays_t r, uint64_t n, const double *modes, WlTa
     B.im *= invk;
     C.im *= invk;
     Cx detG0 = csub(cmul(A, C), cmul(B, B));
+    const Cx w = cmul(cx(ira, 0.0), cx(sqrt(0.5), sqrt(0.5)));
+    const Cx Vx = cmul(cadd(cmul(ray.E1, cx(E.x, 0)), cmul(ray.E2, cx(H.x, 0))), w);
+    const Cx Vy = cmul(cadd(cmul(ray.E1, cx(E.y, 0)), cmul(ray.E2, cx(H.y, 0))), w);
+    const Cx Vz = cmul(cadd(cmul(ray.E1, cx(E.z, 0)), cmul(ray.E2, cx(H.z, 0))), w);
     double v[RPX_GP] = {ray.origin.x, ray.origin.y, ray.origin.z, E.x, E.y, E.z, H.x, H.y, H.z,
                         ray.direction.x, ray.direction.y, ray.direction.z, kz.re, kz.im, phase,
                         A.re, A.im, B.re, B.im, C.re, C.im, detG0.re, detG0.im, ira,
-                        ray.E1.re, ray.E1.im, ray.E2.re, ray.E2.im, A.re + C.re, A.im + C.im};
+                        ray.E1.re, ray.E1.im, ray.E2.re, ray.E2.im, A.re + C.re, A.im + C.im,
+                        Vx.re, Vx.im, Vy.re, Vy.im, Vz.re, Vz.im};
     for (int k2 = 0; k2 < RPX_GP; ++k2) gp[(uint64_t)k2 * n + i] = v[k2];
 }
 
 #define FS_BLOCK 256
 #define FS_TILE 64
 
+// Rows of the invariants table the float64 pair reads: 0-22 and 28-35.
+#define FS_ROWS 31
+__device__ __forceinline__ int fs_row(int q) { return q < 23 ? q : q + 5; }
+
 __global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64_t G, const double *points,
                                                         uint64_t M, double *out) {
-    __shared__ double tile[RPX_GP][FS_TILE];
+    __shared__ double tile[FS_ROWS][FS_TILE];
     const uint64_t ip = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = ip < M;
     V3 pt0 = active ? v3(points[3 * ip], points[3 * ip + 1], points[3 * ip + 2]) : v3(0, 0, 0);
     double o0r = 0, o0i = 0, o1r = 0, o1i = 0, o2r = 0, o2i = 0;
-    const Cx rootI = cx(sqrt(0.5), sqrt(0.5));
     for (uint64_t base = 0; base < G; base += FS_TILE) {
         const int nt = (int)((G - base) < FS_TILE ? (G - base) : FS_TILE);
         __syncthreads();
-        for (int e = threadIdx.x; e < RPX_GP * FS_TILE; e += FS_BLOCK) {
+        for (int e = threadIdx.x; e < FS_ROWS * FS_TILE; e += FS_BLOCK) {
             const int q = e / FS_TILE, g = e - q * FS_TILE;
-            tile[q][g] = (g < nt) ? gp[(uint64_t)q * G + base + g] : 0.0;
+            tile[q][g] = (g < nt) ? gp[(uint64_t)fs_row(q) * G + base + g] : 0.0;
         }
         __syncthreads();
         if (!active) continue;
@@ -565,18 +575,16 @@ __global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64
             double phase = tile[14][g];
             Cx A = cx(tile[15][g], tile[16][g]), B = cx(tile[17][g], tile[18][g]);
             Cx C = cx(tile[19][g], tile[20][g]), detG0 = cx(tile[21][g], tile[22][g]);
-            double ira = tile[23][g];
-            Cx E1a = cx(tile[24][g], tile[25][g]), E2a = cx(tile[26][g], tile[27][g]);
+            const double ApCr = tile[23][g], ApCi = tile[24][g];
             // calc_mode_U (cfields.pyx:118-151), real-by-complex products
             // written out and the three divisions by `denom` as one complex
             // reciprocal (equal to the reference's expression to a few ulp)
-            const double ApCr = tile[28][g], ApCi = tile[29][g];
             V3 pt = sub(pt0, O);
             const double x = dot(pt, E), y = dot(pt, H), z = dot(pt, D);
             const double z2 = z * z;
             const double dre = (1.0 + z * ApCr + z2 * detG0.re) * 2.0;
             const double dim = (z * ApCi + z2 * detG0.im) * 2.0;
-            const double dinv = 1.0 / (dre * dre + dim * dim);
+            const double dinv = rcp_nr(dre * dre + dim * dim);
             const Cx rd = cx(dre * dinv, -dim * dinv);
             const Cx AA = cmul(cx(A.re + z * detG0.re, A.im + z * detG0.im), rd);
             const Cx CC = cmul(cx(C.re + z * detG0.re, C.im + z * detG0.im), rd);
@@ -588,24 +596,36 @@ __global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64
             const double ar = phase + (kz.re * in_re - kz.im * in_im);
             const double ai = kz.re * in_im + kz.im * in_re;
             double sn, cs;
-            sincos(ar, &sn, &cs);
+            sincos_cw(ar, &sn, &cs);
             const double ex = exp(-ai);
-            // 1 / csqrt(((1 + zA)(1 + zC) - (zB)^2) * I)
+            // 1 / csqrt(w), w = ((1 + zA)(1 + zC) - (zB)^2) * I: with h = |w|
+            // and sq the principal root (Kahan's branches, t = sqrt((|Re w| +
+            // h) / 2)), |sq|^2 = h and 1/sq = conj(sq) / h; 1/h and 1/t come
+            // from reciprocal square roots, so the pair needs no division
             const Cx q1 = cx(1.0 + z * A.re, z * A.im), q2 = cx(1.0 + z * C.re, z * C.im);
             const Cx zb = cx(z * B.re, z * B.im);
             const Cx w0 = csub(cmul(q1, q2), cmul(zb, zb));
-            const Cx sq = csqrt_(cx(-w0.im, w0.re));
-            const double sinv = ex * ira / (sq.re * sq.re + sq.im * sq.im);
-            // U = exp(i arg) / sq * ira * rootI
-            const Cx u0 = cmul(cx(cs, sn), cx(sq.re, -sq.im));
-            const Cx U = cmul(cx(u0.re * sinv, u0.im * sinv), rootI);
-            Cx E1 = cmul(E1a, U), E2 = cmul(E2a, U);
-            Cx t0 = cx(E1.re * E.x + E2.re * H.x, E1.im * E.x + E2.im * H.x);
-            Cx t1 = cx(E1.re * E.y + E2.re * H.y, E1.im * E.y + E2.im * H.y);
-            Cx t2 = cx(E1.re * E.z + E2.re * H.z, E1.im * E.z + E2.im * H.z);
-            o0r += t0.re; o0i += t0.im;
-            o1r += t1.re; o1i += t1.im;
-            o2r += t2.re; o2i += t2.im;
+            const double wr = -w0.im, wi = w0.re;
+            const double hh = wr * wr + wi * wi;
+            const double rh = rsq_nr(hh);                          // 1/h
+            const double m = (fabs(wr) + hh * rh) * 0.5;
+            const double rt = rsq_nr(m);                           // 1/t
+            const double t = m * rt;
+            const double qy = wi * rt * 0.5;                       // wi / (2t)
+            const Cx sq = wr >= 0.0 ? cx(t, qy) : cx(fabs(qy), copysign(t, wi));
+            const double sinv = ex * rh;
+            // U = exp(i arg) / sq (inv_root_area sqrt(i) is folded into V)
+            const double ur = (cs * sq.re + sn * sq.im) * sinv;
+            const double ui = (sn * sq.re - cs * sq.im) * sinv;
+            // out += U V, V = (E1 E + E2 H) inv_root_area sqrt(i)
+            const double vxr = tile[25][g], vxi = tile[26][g], vyr = tile[27][g], vyi = tile[28][g];
+            const double vzr = tile[29][g], vzi = tile[30][g];
+            o0r = fma(ur, vxr, fma(-ui, vxi, o0r));
+            o0i = fma(ur, vxi, fma(ui, vxr, o0i));
+            o1r = fma(ur, vyr, fma(-ui, vyi, o1r));
+            o1i = fma(ur, vyi, fma(ui, vyr, o1i));
+            o2r = fma(ur, vzr, fma(-ui, vzi, o2r));
+            o2i = fma(ur, vzi, fma(ui, vzr, o2i));
         }
     }
     if (active) {

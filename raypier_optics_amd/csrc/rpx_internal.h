@@ -107,6 +107,66 @@ RPX_DEV Cx cexp_(Cx z) {
     return cx(e * c, e * s);
 }
 
+// 1/d from the hardware reciprocal estimate and two Newton-Raphson steps
+// (e = 1 - d x, x += x e), within an ulp of the rounded quotient for the
+// normal, finite d of the field sum.
+RPX_DEV double rcp_nr(double d) {
+    double x = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, x, 1.0);
+    x = fma(x, e, x);
+    e = fma(-d, x, 1.0);
+    return fma(x, e, x);
+}
+
+// 1/sqrt(a) from the hardware estimate and two Newton-Raphson steps
+// (y += y (1 - a y^2) / 2), for the normal, finite, positive a of the field sum.
+RPX_DEV double rsq_nr(double a) {
+    double y = __builtin_amdgcn_rsq(a);
+    double e = fma(-(a * y), y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-(a * y), y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
+// sin(x) and cos(x): Cody-Waite reduction r = x - n pi/2 with pi/2 split as
+// C1 (33 significant bits, so n C1 is exact for |n| < 2^20, and x - n C1 is
+// exact) + C2, then the Taylor series of sin to r^15 and of cos to r^16 on
+// |r| <= pi/4 (truncation below 5e-17) and the quadrant of n.  Arguments
+// beyond 2^20 pi/2 go to the library routine.
+RPX_DEV void sincos_cw(double x, double *s, double *c) {
+    const double TWO_OVER_PI = 0.6366197723675814;
+    const double C1 = 1.5707963267341256;       // 0x1.921fb544p+0
+    const double C2 = 6.077100506506192e-11;    // pi/2 - C1, rounded
+    const double nd = rint(x * TWO_OVER_PI);
+    if (!(fabs(nd) < 1048576.0)) {
+        sincos(x, s, c);
+        return;
+    }
+    const double r = fma(-nd, C2, fma(-nd, C1, x));
+    const double u = r * r;
+    double ps = -7.647163731819816e-13;          // -1/15!
+    ps = fma(ps, u, 1.6059043836821613e-10);     // 1/13!
+    ps = fma(ps, u, -2.505210838544172e-08);     // -1/11!
+    ps = fma(ps, u, 2.7557319223985893e-06);     // 1/9!
+    ps = fma(ps, u, -1.984126984126984e-04);     // -1/7!
+    ps = fma(ps, u, 8.333333333333333e-03);      // 1/5!
+    ps = fma(ps, u, -1.6666666666666666e-01);    // -1/3!
+    const double sr = fma(r * u, ps, r);
+    double pc = 4.779477332387385e-14;           // 1/16!
+    pc = fma(pc, u, -1.1470745597729725e-11);    // -1/14!
+    pc = fma(pc, u, 2.08767569878681e-09);       // 1/12!
+    pc = fma(pc, u, -2.755731922398589e-07);     // -1/10!
+    pc = fma(pc, u, 2.48015873015873e-05);       // 1/8!
+    pc = fma(pc, u, -1.388888888888889e-03);     // -1/6!
+    pc = fma(pc, u, 4.1666666666666664e-02);     // 1/4!
+    pc = fma(pc, u, -0.5);                       // -1/2!
+    const double cr = fma(u, pc, 1.0);
+    const int q = (int)nd & 3;
+    const double sv = (q & 1) ? cr : sr, cv = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -sv : sv;
+    *c = ((q + 1) & 2) ? -cv : cv;
+}
+
 // ---------------------------------------------------------- feature masks
 // Face and material families; generation kernels are instantiated for a few
 // masks and a scene runs the smallest one that covers what it uses.
