@@ -546,6 +546,12 @@ __global__ void k_field_prep(rpx_rays_t r, uint64_t n, const double *modes, WlTa
 #define FS_BLOCK 256
 #define FS_TILE 64
 
+// Complex product with fused multiply-adds (the field sum's 1e-9 bound
+// allows contraction; the trace code in this file keeps separate roundings).
+__device__ __forceinline__ Cx cmul_fma(Cx a, Cx b) {
+    return cx(fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re));
+}
+
 // Rows of the invariants table the float64 pair reads: 0-22 and 28-35.
 #define FS_ROWS 31
 __device__ __forceinline__ int fs_row(int q) { return q < 23 ? q : q + 5; }
@@ -577,24 +583,27 @@ __global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64
             Cx C = cx(tile[19][g], tile[20][g]), detG0 = cx(tile[21][g], tile[22][g]);
             const double ApCr = tile[23][g], ApCi = tile[24][g];
             // calc_mode_U (cfields.pyx:118-151), real-by-complex products
-            // written out and the three divisions by `denom` as one complex
-            // reciprocal (equal to the reference's expression to a few ulp)
+            // written out, the three divisions by `denom` as one complex
+            // reciprocal, products and sums fused (equal to the reference's
+            // expression to a few ulp)
             V3 pt = sub(pt0, O);
-            const double x = dot(pt, E), y = dot(pt, H), z = dot(pt, D);
+            const double x = fma(pt.x, E.x, fma(pt.y, E.y, pt.z * E.z));
+            const double y = fma(pt.x, H.x, fma(pt.y, H.y, pt.z * H.z));
+            const double z = fma(pt.x, D.x, fma(pt.y, D.y, pt.z * D.z));
             const double z2 = z * z;
-            const double dre = (1.0 + z * ApCr + z2 * detG0.re) * 2.0;
-            const double dim = (z * ApCi + z2 * detG0.im) * 2.0;
-            const double dinv = rcp_nr(dre * dre + dim * dim);
+            const double dre = 2.0 * fma(z2, detG0.re, fma(z, ApCr, 1.0));
+            const double dim = 2.0 * fma(z2, detG0.im, z * ApCi);
+            const double dinv = rcp_nr(fma(dre, dre, dim * dim));
             const Cx rd = cx(dre * dinv, -dim * dinv);
-            const Cx AA = cmul(cx(A.re + z * detG0.re, A.im + z * detG0.im), rd);
-            const Cx CC = cmul(cx(C.re + z * detG0.re, C.im + z * detG0.im), rd);
-            const Cx Bd = cmul(B, rd);
+            const Cx AA = cmul_fma(cx(fma(z, detG0.re, A.re), fma(z, detG0.im, A.im)), rd);
+            const Cx CC = cmul_fma(cx(fma(z, detG0.re, C.re), fma(z, detG0.im, C.im)), rd);
+            const Cx Bd = cmul_fma(B, rd);
             const double x2 = x * x, y2 = y * y, xy2 = 2.0 * x * y;
-            const double in_re = z + AA.re * x2 + Bd.re * xy2 + CC.re * y2;
-            const double in_im = AA.im * x2 + Bd.im * xy2 + CC.im * y2;
+            const double in_re = fma(AA.re, x2, fma(Bd.re, xy2, fma(CC.re, y2, z)));
+            const double in_im = fma(AA.im, x2, fma(Bd.im, xy2, CC.im * y2));
             // i * (phase + kz * inner)
-            const double ar = phase + (kz.re * in_re - kz.im * in_im);
-            const double ai = kz.re * in_im + kz.im * in_re;
+            const double ar = phase + fma(kz.re, in_re, -(kz.im * in_im));
+            const double ai = fma(kz.re, in_im, kz.im * in_re);
             double sn, cs;
             sincos_cw(ar, &sn, &cs);
             const double ex = exp(-ai);
@@ -602,11 +611,11 @@ __global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64
             // and sq the principal root (Kahan's branches, t = sqrt((|Re w| +
             // h) / 2)), |sq|^2 = h and 1/sq = conj(sq) / h; 1/h and 1/t come
             // from reciprocal square roots, so the pair needs no division
-            const Cx q1 = cx(1.0 + z * A.re, z * A.im), q2 = cx(1.0 + z * C.re, z * C.im);
+            const Cx q1 = cx(fma(z, A.re, 1.0), z * A.im), q2 = cx(fma(z, C.re, 1.0), z * C.im);
             const Cx zb = cx(z * B.re, z * B.im);
-            const Cx w0 = csub(cmul(q1, q2), cmul(zb, zb));
+            const Cx w0 = csub(cmul_fma(q1, q2), cmul_fma(zb, zb));
             const double wr = -w0.im, wi = w0.re;
-            const double hh = wr * wr + wi * wi;
+            const double hh = fma(wr, wr, wi * wi);
             const double rh = rsq_nr(hh);                          // 1/h
             const double m = (fabs(wr) + hh * rh) * 0.5;
             const double rt = rsq_nr(m);                           // 1/t
@@ -615,8 +624,8 @@ __global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64
             const Cx sq = wr >= 0.0 ? cx(t, qy) : cx(fabs(qy), copysign(t, wi));
             const double sinv = ex * rh;
             // U = exp(i arg) / sq (inv_root_area sqrt(i) is folded into V)
-            const double ur = (cs * sq.re + sn * sq.im) * sinv;
-            const double ui = (sn * sq.re - cs * sq.im) * sinv;
+            const double ur = fma(cs, sq.re, sn * sq.im) * sinv;
+            const double ui = fma(sn, sq.re, -(cs * sq.im)) * sinv;
             // out += U V, V = (E1 E + E2 H) inv_root_area sqrt(i)
             const double vxr = tile[25][g], vxi = tile[26][g], vyr = tile[27][g], vyi = tile[28][g];
             const double vzr = tile[29][g], vzi = tile[30][g];
