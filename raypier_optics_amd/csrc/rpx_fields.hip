@@ -650,95 +650,92 @@ __global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64
 // coordinates (x, y, z) of the point in the gausslet's frame (|p - O| is tens
 // of mm) and the large phase phi + Re(kz) z (~1e6 rad at 1e5 wavelengths of
 // optical path), which is reduced modulo 2 pi in float64 before the float32
-// exponential sees it.  The rest -- the Gamma(z) update, the quadratic phase
-// terms (tens of rad), exp, sincos, the complex square root and division --
-// runs in float32; contributions are summed in float64.
-struct Cf2 {
-    float re, im;
-};
-__device__ __forceinline__ Cf2 f2(float r, float i) { Cf2 c; c.re = r; c.im = i; return c; }
-__device__ __forceinline__ Cf2 f2add(Cf2 a, Cf2 b) { return f2(a.re + b.re, a.im + b.im); }
-__device__ __forceinline__ Cf2 f2mul(Cf2 a, Cf2 b) { return f2(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
-__device__ __forceinline__ Cf2 f2scale(Cf2 a, float s) { return f2(a.re * s, a.im * s); }
-__device__ __forceinline__ Cf2 f2div(Cf2 n, Cf2 d) {
-    const float inv = 1.0f / (d.re * d.re + d.im * d.im);
-    return f2((n.re * d.re + n.im * d.im) * inv, (n.im * d.re - n.re * d.im) * inv);
-}
-__device__ __forceinline__ Cf2 f2sqrt(Cf2 z) {   // principal root (Kahan)
-    const float x = z.re, y = z.im;
-    if (x == 0.0f && y == 0.0f) return f2(0.0f, y);
-    const float t = sqrtf((fabsf(x) + hypotf(x, y)) * 0.5f);
-    if (x >= 0.0f) return f2(t, y / (2.0f * t));
-    return f2(fabsf(y) / (2.0f * t), copysignf(t, y));
-}
-
+// exponential sees it.  The rest -- the Gamma(z) update (one complex
+// reciprocal), the quadratic phase terms (tens of rad), exp, sincos and
+// 1/sqrt(i w) from hardware reciprocal square roots -- runs in float32,
+// with V = (E1 E + E2 H) inv_root_area sqrt(i) from the invariants table;
+// contributions are summed in float64.
 #define FS32_TILE 128
 
 __global__ __launch_bounds__(FS_BLOCK) void k_field_sum_split(const double *gp, uint64_t G, const double *points,
                                                               uint64_t M, double *out) {
-    // float64 rows: origin(3) E(3) H(3) dir(3) kz.re phase; float32 rows: the rest
+    // float64 rows: origin(3) E(3) H(3) dir(3) kz.re phase; float32 rows:
+    // Im kz, A(2) B(2) C(2) detG0(2) A+C(2) V(6)
     __shared__ double t64[14][FS32_TILE];
-    __shared__ float t32[14][FS32_TILE];
+    __shared__ float t32[17][FS32_TILE];
     const uint64_t ip = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = ip < M;
     const double px = active ? points[3 * ip] : 0, py = active ? points[3 * ip + 1] : 0,
                  pz = active ? points[3 * ip + 2] : 0;
     double o0r = 0, o0i = 0, o1r = 0, o1i = 0, o2r = 0, o2i = 0;
     const double TWO_PI = 6.283185307179586;
-    const float RH = 0.70710678f;   // sqrt(1/2): sqrt(i) = RH (1 + i)
     // float32 row sources in the float64 invariants table (see k_field_prep)
-    const int src32[14] = {13, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27};
+    const int src32[17] = {13, 15, 16, 17, 18, 19, 20, 21, 22, 28, 29, 30, 31, 32, 33, 34, 35};
     for (uint64_t base = 0; base < G; base += FS32_TILE) {
         const int nt = (int)((G - base) < FS32_TILE ? (G - base) : FS32_TILE);
         __syncthreads();
-        for (int e = threadIdx.x; e < 14 * FS32_TILE; e += FS_BLOCK) {
+        for (int e = threadIdx.x; e < 17 * FS32_TILE; e += FS_BLOCK) {
             const int q = e / FS32_TILE, g = e - q * FS32_TILE;
             const bool in = g < nt;
-            t64[q][g] = in ? gp[(uint64_t)(q < 12 ? q : (q == 12 ? 12 : 14)) * G + base + g] : 0.0;
+            if (q < 14) t64[q][g] = in ? gp[(uint64_t)(q < 12 ? q : (q == 12 ? 12 : 14)) * G + base + g] : 0.0;
             t32[q][g] = in ? (float)gp[(uint64_t)src32[q] * G + base + g] : 0.0f;
         }
         __syncthreads();
         if (!active) continue;
         for (int g = 0; g < nt; ++g) {
             const double dx = px - t64[0][g], dy = py - t64[1][g], dz = pz - t64[2][g];
-            const double x = dx * t64[3][g] + dy * t64[4][g] + dz * t64[5][g];
-            const double y = dx * t64[6][g] + dy * t64[7][g] + dz * t64[8][g];
-            const double z = dx * t64[9][g] + dy * t64[10][g] + dz * t64[11][g];
-            const double big = t64[13][g] + t64[12][g] * z;           // phase + Re(kz) z
-            const double red = big - TWO_PI * rint(big * (1.0 / TWO_PI));
+            const double x = fma(dx, t64[3][g], fma(dy, t64[4][g], dz * t64[5][g]));
+            const double y = fma(dx, t64[6][g], fma(dy, t64[7][g], dz * t64[8][g]));
+            const double z = fma(dx, t64[9][g], fma(dy, t64[10][g], dz * t64[11][g]));
+            const double big = fma(t64[12][g], z, t64[13][g]);        // phase + Re(kz) z
+            const double red = fma(-TWO_PI, rint(big * (1.0 / TWO_PI)), big);
             const float zf = (float)z, xf = (float)x, yf = (float)y;
             const float kzi = t32[0][g];
-            const Cf2 A = f2(t32[1][g], t32[2][g]), B = f2(t32[3][g], t32[4][g]);
-            const Cf2 C = f2(t32[5][g], t32[6][g]), detG0 = f2(t32[7][g], t32[8][g]);
-            const float ira = t32[9][g];
-            const Cf2 E1a = f2(t32[10][g], t32[11][g]), E2a = f2(t32[12][g], t32[13][g]);
-            const Cf2 denom = f2scale(f2add(f2add(f2(1, 0), f2scale(f2add(A, C), zf)), f2scale(detG0, zf * zf)), 2.0f);
-            const Cf2 zd = f2scale(detG0, zf);
-            const Cf2 quad = f2add(f2add(f2mul(f2div(f2add(A, zd), denom), f2(xf * xf, 0)),
-                                         f2div(f2scale(B, 2 * xf * yf), denom)),
-                                   f2mul(f2div(f2add(C, zd), denom), f2(yf * yf, 0)));
+            const float Ar = t32[1][g], Ai = t32[2][g], Br = t32[3][g], Bi = t32[4][g];
+            const float Cr = t32[5][g], Ci = t32[6][g], Dr = t32[7][g], Di = t32[8][g];
+            // Gamma(z) update with one complex reciprocal of denom
+            const float z2 = zf * zf;
+            const float dre = 2.0f * fmaf(z2, Dr, fmaf(zf, t32[9][g], 1.0f));
+            const float dim = 2.0f * fmaf(z2, Di, zf * t32[10][g]);
+            const float dinv = __builtin_amdgcn_rcpf(fmaf(dre, dre, dim * dim));
+            const float rr = dre * dinv, ri = -dim * dinv;
+            const float aar = fmaf(zf, Dr, Ar), aai = fmaf(zf, Di, Ai);
+            const float ccr = fmaf(zf, Dr, Cr), cci = fmaf(zf, Di, Ci);
+            const float x2 = xf * xf, y2 = yf * yf, xy2 = 2.0f * xf * yf;
+            // quad = (AA x^2 + B 2xy + CC y^2) / denom
+            const float nr = fmaf(aar, x2, fmaf(Br, xy2, ccr * y2));
+            const float ni = fmaf(aai, x2, fmaf(Bi, xy2, cci * y2));
+            const float qr = fmaf(nr, rr, -(ni * ri)), qi = fmaf(nr, ri, ni * rr);
             // arg = phase + kz (z + quad); the part already in `red` is phase + Re(kz) z
-            const Cf2 kz = f2((float)t64[12][g], kzi);
-            const Cf2 kq = f2mul(kz, quad);
-            const float arg_re = (float)red + kq.re;
-            const float arg_im = kzi * zf + kq.im;
+            const float kzr = (float)t64[12][g];
+            const float arg_re = (float)red + fmaf(kzr, qr, -(kzi * qi));
+            const float arg_im = fmaf(kzi, zf, fmaf(kzr, qi, kzi * qr));
             float sn, cs;
             sincosf(arg_re, &sn, &cs);
             const float mag = expf(-arg_im);
-            Cf2 U = f2(mag * cs, mag * sn);
-            const Cf2 q1 = f2add(f2(1, 0), f2scale(A, zf)), q2 = f2add(f2(1, 0), f2scale(C, zf));
-            const Cf2 zb = f2scale(B, zf);
-            const Cf2 w = f2add(f2mul(q1, q2), f2scale(f2mul(zb, zb), -1.0f));
-            const Cf2 sq = f2sqrt(f2(-w.im, w.re));                   // sqrt(i w)
-            U = f2div(U, sq);
-            U = f2mul(U, f2(ira * RH, ira * RH));
-            const Cf2 E1 = f2mul(E1a, U), E2 = f2mul(E2a, U);
-            o0r += (double)(E1.re * (float)t64[3][g] + E2.re * (float)t64[6][g]);
-            o0i += (double)(E1.im * (float)t64[3][g] + E2.im * (float)t64[6][g]);
-            o1r += (double)(E1.re * (float)t64[4][g] + E2.re * (float)t64[7][g]);
-            o1i += (double)(E1.im * (float)t64[4][g] + E2.im * (float)t64[7][g]);
-            o2r += (double)(E1.re * (float)t64[5][g] + E2.re * (float)t64[8][g]);
-            o2i += (double)(E1.im * (float)t64[5][g] + E2.im * (float)t64[8][g]);
+            // 1 / sqrt(i w) as conj(sq) / |w| from reciprocal square roots
+            const float q1r = fmaf(zf, Ar, 1.0f), q1i = zf * Ai, q2r = fmaf(zf, Cr, 1.0f), q2i = zf * Ci;
+            const float zbr = zf * Br, zbi = zf * Bi;
+            const float w_re = fmaf(q1r, q2r, -(q1i * q2i)) - fmaf(zbr, zbr, -(zbi * zbi));
+            const float w_im = fmaf(q1r, q2i, q1i * q2r) - 2.0f * zbr * zbi;
+            const float wr = -w_im, wi = w_re;
+            const float hh = fmaf(wr, wr, wi * wi);
+            const float rh = __builtin_amdgcn_rsqf(hh);                  // 1/|w|
+            const float m = (fabsf(wr) + hh * rh) * 0.5f;
+            const float rt = __builtin_amdgcn_rsqf(m);                   // 1/t
+            const float t = m * rt, qy = wi * rt * 0.5f;
+            const float sqr = wr >= 0.0f ? t : fabsf(qy), sqi = wr >= 0.0f ? qy : copysignf(t, wi);
+            const float sinv = mag * rh;
+            const float ur = fmaf(cs, sqr, sn * sqi) * sinv, ui = fmaf(sn, sqr, -(cs * sqi)) * sinv;
+            // out += U V, V = (E1 E + E2 H) inv_root_area sqrt(i)
+            const float vxr = t32[11][g], vxi = t32[12][g], vyr = t32[13][g], vyi = t32[14][g];
+            const float vzr = t32[15][g], vzi = t32[16][g];
+            o0r += (double)fmaf(ur, vxr, -(ui * vxi));
+            o0i += (double)fmaf(ur, vxi, ui * vxr);
+            o1r += (double)fmaf(ur, vyr, -(ui * vyi));
+            o1i += (double)fmaf(ur, vyi, ui * vyr);
+            o2r += (double)fmaf(ur, vzr, -(ui * vzi));
+            o2i += (double)fmaf(ur, vzi, ui * vzr);
         }
     }
     if (active) {
