@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RPX_ABI_VERSION 7
+#define RPX_ABI_VERSION 8
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t rpx_status_t;
@@ -215,13 +215,21 @@ typedef struct rpx_material {
  *   op 2 (R/r):      (a + b) - c
  * Then, per coefficient i (sorted by ANSI j), Z terms use value(r_ref[i]),
  * value(rp_ref[i]), value(ror_ref[i]).  Two programs exist per distortion:
- * `sag` (z_offset_c: radial values only) and `grad` (z_offset_and_gradient_c). */
+ * `sag` (z_offset_c: radial values only) and `grad` (z_offset_and_gradient_c).
+ * Every op is an integer-coefficient polynomial operation in r, so each value a
+ * coefficient uses is a polynomial in r with integer coefficients; the
+ * builder expands the programs exactly and stores those polynomials in the
+ * coefficient records, which is what the kernels evaluate (Horner in r).
+ * The programs are validated and kept as the provenance of the polynomials. */
 typedef struct rpx_zernike_insn {
     int32_t op;
     int32_t dst;
     int32_t src[5];
     int32_t pad;
 } rpx_zernike_insn_t;
+
+#define RPX_ZPOLY 10        /* polynomial coefficients per value (n <= 9) */
+enum { RPX_ZP_SAG = 0, RPX_ZP_R = 1, RPX_ZP_RP = 2, RPX_ZP_ROR = 3 };
 
 typedef struct rpx_zernike_coef {
     int32_t n, m;             /* m signed (ANSI)                          */
@@ -231,6 +239,10 @@ typedef struct rpx_zernike_coef {
     double  nval;             /* sqrt(n+1) (m == 0) or sqrt(2(n+1)), times value:
                                  the normalised coefficient N of
                                  cdistortions.pyx:434-436, computed once */
+    int32_t deg[4];           /* degree of each polynomial below (0..9)   */
+    double  poly[4][RPX_ZPOLY];   /* value(sag_ref), value(r_ref), value(rp_ref),
+                                     value(ror_ref) as polynomials in r,
+                                     ascending powers                      */
 } rpx_zernike_coef_t;
 
 typedef struct rpx_zernike {

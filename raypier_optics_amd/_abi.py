@@ -33,7 +33,7 @@ gausslet_dtype = np.dtype([('base_ray', ray_dtype),
 
 assert ray_dtype.itemsize == 188 and para_dtype.itemsize == 80 and gausslet_dtype.itemsize == 668
 
-ABI_VERSION = 7          # RPX_ABI_VERSION of include/rpx.h
+ABI_VERSION = 8          # RPX_ABI_VERSION of include/rpx.h
 
 # timing slots (RPX_TIMING_*)
 TIMING_KERNELS = ("k_intersect", "k_scan", "k_emit", "k_generation", "k_clear", "k_generation_split")
@@ -96,9 +96,11 @@ material_dtype = np.dtype([('type', np.int32), ('order', np.int32), ('pad', np.i
 
 zernike_insn_dtype = np.dtype([('op', np.int32), ('dst', np.int32), ('src', np.int32, (5,)), ('pad', np.int32)])
 
+ZPOLY = 10              # RPX_ZPOLY
+ZP_SAG, ZP_R, ZP_RP, ZP_ROR = 0, 1, 2, 3
 zernike_coef_dtype = np.dtype([('n', np.int32), ('m', np.int32), ('r_ref', np.int32), ('rp_ref', np.int32),
                                ('ror_ref', np.int32), ('sag_ref', np.int32), ('value', np.float64),
-                               ('nval', np.float64)])
+                               ('nval', np.float64), ('deg', np.int32, (4,)), ('poly', np.float64, (4, ZPOLY))])
 
 zernike_dtype = np.dtype([('unit_radius', np.float64), ('n_coefs', np.int32), ('coef_begin', np.int32),
                           ('sag_begin', np.int32), ('sag_len', np.int32), ('grad_begin', np.int32),

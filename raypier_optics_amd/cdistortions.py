@@ -178,6 +178,49 @@ def compact_slots(insns, final_refs):
     return out, mapping, used
 
 
+def _padd(a, b):
+    n = max(len(a), len(b))
+    return [(a[k] if k < len(a) else 0) + (b[k] if k < len(b) else 0) for k in range(n)]
+
+
+def _pneg(a):
+    return [-x for x in a]
+
+
+def _pshift(a):          # r * a
+    return [0] + list(a)
+
+
+def program_polynomials(insns):
+    """Run a straight-line program on integer polynomials in r (lists of
+    Python ints, ascending powers): every op is r*(a+b)-c, (a+b)+r*(c+d)-e or
+    (a+b)-c, so each slot holds a polynomial with integer coefficients, the
+    exact expansion of what the recursion computes.  Returns value(ref)."""
+    slot = {}
+
+    def val(ref):
+        ref = int(ref)
+        if ref == CONST_ZERO:
+            return [0]
+        if ref == CONST_ONE:
+            return [1]
+        return slot[ref]
+
+    for ins in insns:
+        a, b, c, d, e = (val(x) for x in ins["src"])
+        op = int(ins["op"])
+        if op == 0:
+            v = _padd(_pshift(_padd(a, b)), _pneg(c))
+        elif op == 1:
+            v = _padd(_padd(_padd(a, b), _pshift(_padd(c, d))), _pneg(e))
+        else:
+            v = _padd(_padd(a, b), _pneg(c))
+        while len(v) > 1 and v[-1] == 0:
+            v.pop()
+        slot[int(ins["dst"])] = v
+    return val
+
+
 def compile_zernike(coef_list, unit_radius):
     """Compile sorted (j, value) pairs into (zernike record fields, coef
     records, sag instructions, grad instructions)."""
@@ -202,6 +245,16 @@ def compile_zernike(coef_list, unit_radius):
         rec["rp_ref"] = grad.Rp(k, n, abs(m))
         rec["ror_ref"] = grad.Ror(k, n, abs(m))
         coefs.append(rec)
+    # the values as polynomials in r (what the kernels evaluate)
+    sag_val, grad_val = program_polynomials(sag.insns), program_polynomials(grad.insns)
+    for c in coefs:
+        for k, (f, val) in enumerate((("sag_ref", sag_val), ("r_ref", grad_val), ("rp_ref", grad_val),
+                                      ("ror_ref", grad_val))):
+            p = val(c[f])
+            if len(p) > _abi.ZPOLY or any(abs(x) >= 2 ** 53 for x in p):
+                raise ValueError("Zernike polynomial out of range")
+            c["deg"][k] = len(p) - 1
+            c["poly"][k][:len(p)] = [float(x) for x in p]
     sag_p, smap, s_used = compact_slots(sag.insns, [int(c["sag_ref"]) for c in coefs if c["sag_ref"] >= 0])
     grad_p, gmap, g_used = compact_slots(grad.insns, [int(c[f]) for c in coefs for f in ("r_ref", "rp_ref", "ror_ref")
                                                       if c[f] >= 0])

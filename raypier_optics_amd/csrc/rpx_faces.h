@@ -87,44 +87,24 @@ RPX_DEV bool shape_inside(const SceneView &S, int root, double x, double y) {
 }
 
 // ------------------------------------------------------------- Zernike
-// Straight-line form of the memoised radial recursion of
-// cdistortions.pyx:149-316 (see rpx_zernike_insn_t).
-// The workspace is addressed as slot[ref * stride]: stride 1 for a private
-// array, RPX_BLOCK for the LDS array below (one column per lane).
-RPX_DEV double zref(const double *slot, int ref, int stride) {
-    return ref >= 0 ? slot[ref * stride] : (ref == -1 ? 0.0 : 1.0);
-}
+// The memoised radial recursion of cdistortions.pyx:149-316 is expanded at
+// scene build into integer polynomials in r (rpx_zernike_coef_t.poly, the
+// exact values of the compiled recursion, slot sharing included); a
+// coefficient's R, R', R/r are evaluated here by Horner's rule over the
+// zero-padded RPX_ZPOLY coefficients (the loads are independent, so they
+// issue together).  When every active lane evaluates the same distortion the
+// tables are read through the constant address space, i.e. with scalar loads
+// into SGPRs; otherwise (per-lane faces: normals, parabasal rays) with vector
+// loads.  The scene tables are not written while kernels run.
+typedef __attribute__((address_space(4))) const rpx_zernike_t ZernikeK;
+typedef __attribute__((address_space(4))) const rpx_zernike_coef_t ZCoefK;
 
-RPX_DEV void zernike_run(const SceneView &S, int begin, int len, double r, double *slot, int stride) {
-    for (int i = begin; i < begin + len; ++i) {
-        const rpx_zernike_insn_t &in = S.zinsns[i];
-        double a = zref(slot, in.src[0], stride), b = zref(slot, in.src[1], stride), c = zref(slot, in.src[2], stride);
-        double v;
-        if (in.op == 0) {
-            v = r * (a + b);
-            v -= c;
-        } else if (in.op == 1) {
-            double d = zref(slot, in.src[3], stride), e = zref(slot, in.src[4], stride);
-            v = a + b;
-            v += r * (c + d);
-            v -= e;
-        } else {
-            v = a + b;
-            v -= c;
-        }
-        slot[in.dst * stride] = v;
-    }
-}
-
-// Programs are straight-line chains (each instruction reads what the last
-// ones wrote), so their workspace sits in LDS when it fits: a private array
-// would put every step through scratch memory.  Column threadIdx.x of
-// RPX_ZSLOTS_LDS rows; kernels that run the distortion code launch blocks of
-// at most RPX_BLOCK threads.
-#define RPX_ZSLOTS_LDS 24
-RPX_DEV double *zernike_lds() {
-    __shared__ double zs[RPX_ZSLOTS_LDS * 256];
-    return zs + threadIdx.x;
+template <class CT>
+RPX_DEV double zpoly(CT &c, int k, double r) {
+    double v = c.poly[k][RPX_ZPOLY - 1];
+#pragma unroll
+    for (int i = RPX_ZPOLY - 2; i >= 0; --i) v = fma(v, r, c.poly[k][i]);
+    return v;
 }
 
 // cos(k theta) and sin(k theta) for theta = atan2(y, x), from cos = x/r and
@@ -164,47 +144,43 @@ RPX_DEV void harmonic_k(const Harmonic &h, int k, double *ck, double *sk) {
 }
 
 // ZernikeDistortion.z_offset_c (cdistortions.pyx:416-450)
-RPX_DEV double zernike_sag(const SceneView &S, int zi, double x, double y) {
-    const rpx_zernike_t &Z = S.zernikes[zi];
-    double priv[RPX_MAX_ZERNIKE_SLOTS];
-    const bool lds = Z.n_slots <= RPX_ZSLOTS_LDS;
-    double *slot = lds ? zernike_lds() : priv;
-    const int stride = lds ? 256 : 1;
+template <class ZT, class CT>
+RPX_DEV double zernike_sag_t(ZT &Z, CT *coefs, double x, double y) {
     x /= Z.unit_radius;
     y /= Z.unit_radius;
     double r = sqrt(x * x + y * y);
     const Harmonic hm = harmonic(x, y, r);
-    zernike_run(S, Z.sag_begin, Z.sag_len, r, slot, stride);
     double z = 0.0;
     for (int i = 0; i < Z.n_coefs; ++i) {
-        const rpx_zernike_coef_t &c = S.zcoefs[Z.coef_begin + i];
+        CT &c = coefs[Z.coef_begin + i];
         double ck, sk;
         harmonic_k(hm, c.m >= 0 ? c.m : -c.m, &ck, &sk);
         // cos(m theta), or -sin(m theta) = sin(|m| theta) for m < 0
         const double PH = (c.m >= 0) ? ck : sk;
-        double R = zref(slot, c.sag_ref, stride);
-        z += c.nval * R * PH;
+        z += c.nval * zpoly(c, RPX_ZP_SAG, r) * PH;
     }
     return z;
 }
 
+RPX_DEV double zernike_sag(const SceneView &S, int zi, double x, double y) {
+    const int z0 = __builtin_amdgcn_readfirstlane(zi);
+    if (__ballot(zi != z0) == 0)
+        return zernike_sag_t(((ZernikeK *)S.zernikes)[z0], (ZCoefK *)S.zcoefs, x, y);
+    return zernike_sag_t(S.zernikes[zi], S.zcoefs, x, y);
+}
+
 // ZernikeDistortion.z_offset_and_gradient_c (cdistortions.pyx:456-514):
 // returns (dz/dx, dz/dy, z).
-RPX_DEV V3 zernike_grad(const SceneView &S, int zi, double x, double y) {
-    const rpx_zernike_t &Z = S.zernikes[zi];
-    double priv[RPX_MAX_ZERNIKE_SLOTS];
-    const bool lds = Z.n_slots <= RPX_ZSLOTS_LDS;
-    double *slot = lds ? zernike_lds() : priv;
-    const int stride = lds ? 256 : 1;
+template <class ZT, class CT>
+RPX_DEV V3 zernike_grad_t(ZT &Z, CT *coefs, double x, double y) {
     x /= Z.unit_radius;
     y /= Z.unit_radius;
     double r = sqrt(x * x + y * y);
     const Harmonic hm = harmonic(x, y, r);
-    zernike_run(S, Z.grad_begin, Z.grad_len, r, slot, stride);
     V3 out = v3(0.0, 0.0, 0.0);
     const double st = hm.origin ? sin(hm.theta) : hm.s1, ct = hm.origin ? cos(hm.theta) : hm.c1;
     for (int i = 0; i < Z.n_coefs; ++i) {
-        const rpx_zernike_coef_t &c = S.zcoefs[Z.coef_begin + i];
+        CT &c = coefs[Z.coef_begin + i];
         const int am = c.m >= 0 ? c.m : -c.m;
         double ck, sk;
         harmonic_k(hm, am, &ck, &sk);
@@ -216,8 +192,7 @@ RPX_DEV V3 zernike_grad(const SceneView &S, int zi, double x, double y) {
             PH = sk;                       // -sin(m theta) = sin(|m| theta)
             PHp = (double)(-c.m) * ck;     // -m cos(m theta) = |m| cos(|m| theta)
         }
-        double R = zref(slot, c.r_ref, stride), Rp = zref(slot, c.rp_ref, stride),
-               Ror = zref(slot, c.ror_ref, stride);
+        const double R = zpoly(c, RPX_ZP_R, r), Rp = zpoly(c, RPX_ZP_RP, r), Ror = zpoly(c, RPX_ZP_ROR, r);
         const double N = c.nval;
         out.z += N * R * PH;
         out.x += N * (Rp * ct * PH + Ror * (-st) * PHp);
@@ -226,6 +201,13 @@ RPX_DEV V3 zernike_grad(const SceneView &S, int zi, double x, double y) {
     out.x /= Z.unit_radius;
     out.y /= Z.unit_radius;
     return out;
+}
+
+RPX_DEV V3 zernike_grad(const SceneView &S, int zi, double x, double y) {
+    const int z0 = __builtin_amdgcn_readfirstlane(zi);
+    if (__ballot(zi != z0) == 0)
+        return zernike_grad_t(((ZernikeK *)S.zernikes)[z0], (ZCoefK *)S.zcoefs, x, y);
+    return zernike_grad_t(S.zernikes[zi], S.zcoefs, x, y);
 }
 
 // ----------------------------------------------------------- conic / asphere

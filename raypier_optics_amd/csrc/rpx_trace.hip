@@ -163,6 +163,8 @@ static rpx_status_t validate_scene(const rpx_scene_desc_t *d) {
             int refs[4] = {c.r_ref, c.rp_ref, c.ror_ref, c.sag_ref};
             for (int r : refs)
                 if (r < -2 || r >= z.n_slots) return set_error(RPX_E_INVALID, "Zernike value ref");
+            for (int q = 0; q < 4; ++q)
+                if (c.deg[q] < 0 || c.deg[q] >= RPX_ZPOLY) return set_error(RPX_E_INVALID, "Zernike polynomial degree");
         }
         int ranges[2][2] = {{z.sag_begin, z.sag_len}, {z.grad_begin, z.grad_len}};
         for (auto &rg : ranges)

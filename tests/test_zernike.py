@@ -260,3 +260,74 @@ def test_program_independent_of_operand_order():
         for c1, c2 in zip(cs, cs2):
             for ref in ("r_ref", "rp_ref", "ror_ref"):
                 assert v1(int(c1[ref])) == v2(int(c2[ref]))
+
+
+# ------------------------------------------------------ polynomial tables
+def horner(c, k, r):
+    d = int(c["deg"][k])
+    v = float(c["poly"][k][d])
+    for i in range(d - 1, -1, -1):
+        v = v * r + float(c["poly"][k][i])
+    return v
+
+
+@pytest.mark.parametrize("coefs", [
+    [(j, 0.01 * (j - 2)) for j in range(3, 15)],                 # Noll 4-15
+    [(7, 1.0)], [(4, 0.3), (13, -0.2)], [(12, 1.0), (24, 0.5), (4, -0.1)],
+    [(j, 0.1 / (1 + j)) for j in range(0, 45)],                   # every term to n = 8
+])
+def test_polynomial_tables_equal_programs(coefs):
+    """The kernels evaluate each value a coefficient uses from its integer
+    polynomial in r: at every r it equals the compiled program's value (the
+    literal recursion, slot sharing included) to rounding, and the sag and
+    gradient built from it agree with the oracle's recursion to 1e-12."""
+    t, zi = tables_for(ZernikeDistortion(coefs, unit_radius=7.5))
+    Z = t.zernikes[zi]
+    coefs_t = t.zcoefs[Z["coef_begin"]: Z["coef_begin"] + Z["n_coefs"]]
+    sag_p = t.zinsns[Z["sag_begin"]: Z["sag_begin"] + Z["sag_len"]]
+    grad_p = t.zinsns[Z["grad_begin"]: Z["grad_begin"] + Z["grad_len"]]
+    for r in np.linspace(0.0, 1.05, 23):
+        vs = run_program(sag_p, int(Z["n_slots"]), r)
+        vg = run_program(grad_p, int(Z["n_slots"]), r)
+        for c in coefs_t:
+            scale = np.abs(c["poly"]).sum() + 1.0
+            for k, (ref, v) in enumerate((("sag_ref", vs), ("r_ref", vg), ("rp_ref", vg), ("ror_ref", vg))):
+                assert abs(horner(c, k, r) - v(int(c[ref]))) <= 4e-16 * scale * max(1.0, r) ** 9
+    # the whole sag / gradient from the tables against the oracle
+    rng = np.random.default_rng(11)
+    x, y = rng.uniform(-7, 7, 25), rng.uniform(-7, 7, 25)
+    x[0] = y[0] = 0.0
+    s_ref, g_ref = oracle.zernike(t, zi, x, y)
+    ur = float(Z["unit_radius"])
+    for X, Y, s0, g0 in zip(x, y, s_ref, g_ref):
+        X, Y = X / ur, Y / ur
+        r, th = math.hypot(X, Y), math.atan2(Y, X)
+        z = gx = gy = 0.0
+        for c in coefs_t:
+            m = int(c["m"])
+            N = float(c["nval"])
+            if m >= 0:
+                PH, PHp = math.cos(m * th), -m * math.sin(m * th)
+            else:
+                PH, PHp = -math.sin(m * th), -m * math.cos(m * th)
+            R, Rp, Ror = horner(c, 1, r), horner(c, 2, r), horner(c, 3, r)
+            z += N * horner(c, 0, r) * PH
+            gx += N * (Rp * math.cos(th) * PH - Ror * math.sin(th) * PHp)
+            gy += N * (Rp * math.sin(th) * PH + Ror * math.cos(th) * PHp)
+        tol = 1e-12 * (1.0 + np.abs(g_ref).max() + np.abs(s_ref).max())
+        assert abs(z - s0) <= tol
+        assert abs(gx / ur - g0[0]) <= tol and abs(gy / ur - g0[1]) <= tol
+
+
+def test_polynomial_tables_textbook():
+    """Closed forms: R_2^0 = 2r^2 - 1, R_3^1 = 3r^3 - 2r, R_4^0 = 6r^4 - 6r^2 + 1
+    and their derivatives (n < 5, where the recursion has no quirk)."""
+    kmax, cs, sag, grad, _ = compile_zernike([(4, 1.0), (8, 1.0), (12, 1.0)], 1.0)
+    want = {(2, 0): [-1, 0, 2], (3, 1): [0, -2, 0, 3], (4, 0): [1, 0, -6, 0, 6]}
+    for c in cs:
+        p = want[(int(c["n"]), int(c["m"]))]
+        d = len(p) - 1
+        assert int(c["deg"][1]) == d and list(c["poly"][1][:d + 1]) == p
+        assert list(c["poly"][0][:d + 1]) == p
+        dp = [k * p[k] for k in range(1, d + 1)]
+        assert int(c["deg"][2]) == d - 1 and list(c["poly"][2][:d]) == dp
