@@ -10,6 +10,7 @@
 //                 through LDS in tiles and every lane sums them in gausslet
 //                 order, so each point's sum has the reference's order.
 #include <hip/hip_runtime.h>
+#include <mutex>
 #include <algorithm>
 #include <string>
 #include "rpx_materials.h"
@@ -556,15 +557,19 @@ __device__ __forceinline__ Cx cmul_fma(Cx a, Cx b) {
 #define FS_ROWS 31
 __device__ __forceinline__ int fs_row(int q) { return q < 23 ? q : q + 5; }
 
-__global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64_t G, const double *points,
-                                                        uint64_t M, double *out) {
+// Block (x, y): points x*FS_BLOCK.., gausslets [y*chunk, min(G, (y+1)*chunk)),
+// summed in gausslet order into out + y*6*M.
+__global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64_t G, uint64_t chunk,
+                                                        const double *points, uint64_t M, double *out) {
     __shared__ double tile[FS_ROWS][FS_TILE];
     const uint64_t ip = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t g0 = (uint64_t)blockIdx.y * chunk, g1 = min(G, g0 + chunk);
+    out += (uint64_t)blockIdx.y * 6 * M;
     const bool active = ip < M;
     V3 pt0 = active ? v3(points[3 * ip], points[3 * ip + 1], points[3 * ip + 2]) : v3(0, 0, 0);
     double o0r = 0, o0i = 0, o1r = 0, o1i = 0, o2r = 0, o2i = 0;
-    for (uint64_t base = 0; base < G; base += FS_TILE) {
-        const int nt = (int)((G - base) < FS_TILE ? (G - base) : FS_TILE);
+    for (uint64_t base = g0; base < g1; base += FS_TILE) {
+        const int nt = (int)((g1 - base) < FS_TILE ? (g1 - base) : FS_TILE);
         __syncthreads();
         for (int e = threadIdx.x; e < FS_ROWS * FS_TILE; e += FS_BLOCK) {
             const int q = e / FS_TILE, g = e - q * FS_TILE;
@@ -657,13 +662,15 @@ __global__ __launch_bounds__(FS_BLOCK) void k_field_sum(const double *gp, uint64
 // contributions are summed in float64.
 #define FS32_TILE 128
 
-__global__ __launch_bounds__(FS_BLOCK) void k_field_sum_split(const double *gp, uint64_t G, const double *points,
-                                                              uint64_t M, double *out) {
+__global__ __launch_bounds__(FS_BLOCK) void k_field_sum_split(const double *gp, uint64_t G, uint64_t chunk,
+                                                              const double *points, uint64_t M, double *out) {
     // float64 rows: origin(3) E(3) H(3) dir(3) kz.re phase; float32 rows:
     // Im kz, A(2) B(2) C(2) detG0(2) A+C(2) V(6)
     __shared__ double t64[14][FS32_TILE];
     __shared__ float t32[17][FS32_TILE];
     const uint64_t ip = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t g0 = (uint64_t)blockIdx.y * chunk, g1 = min(G, g0 + chunk);
+    out += (uint64_t)blockIdx.y * 6 * M;
     const bool active = ip < M;
     const double px = active ? points[3 * ip] : 0, py = active ? points[3 * ip + 1] : 0,
                  pz = active ? points[3 * ip + 2] : 0;
@@ -671,8 +678,8 @@ __global__ __launch_bounds__(FS_BLOCK) void k_field_sum_split(const double *gp, 
     const double TWO_PI = 6.283185307179586;
     // float32 row sources in the float64 invariants table (see k_field_prep)
     const int src32[17] = {13, 15, 16, 17, 18, 19, 20, 21, 22, 28, 29, 30, 31, 32, 33, 34, 35};
-    for (uint64_t base = 0; base < G; base += FS32_TILE) {
-        const int nt = (int)((G - base) < FS32_TILE ? (G - base) : FS32_TILE);
+    for (uint64_t base = g0; base < g1; base += FS32_TILE) {
+        const int nt = (int)((g1 - base) < FS32_TILE ? (g1 - base) : FS32_TILE);
         __syncthreads();
         for (int e = threadIdx.x; e < 17 * FS32_TILE; e += FS_BLOCK) {
             const int q = e / FS32_TILE, g = e - q * FS32_TILE;
@@ -745,6 +752,63 @@ __global__ __launch_bounds__(FS_BLOCK) void k_field_sum_split(const double *gp, 
     }
 }
 
+// out[e] = sum over chunks c (in order) of part[c*n + e].
+__global__ void k_field_reduce(const double *part, uint32_t n_chunks, uint64_t n, double *out) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double s = part[e];
+    for (uint32_t c = 1; c < n_chunks; ++c) s += part[(uint64_t)c * n + e];
+    out[e] = s;
+}
+
+// Workgroups of FS_BLOCK threads of `kernel` resident at once on the current
+// device (occupancy per CU times CUs), cached per kernel and device.
+static uint64_t field_resident(const void *kernel) {
+    static std::mutex mu;
+    static struct { const void *k; int dev; uint64_t n; } cache[8];
+    static int n_cache = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        for (int i = 0; i < n_cache; ++i)
+            if (cache[i].k == kernel && cache[i].dev == dev) return cache[i].n;
+    }
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, FS_BLOCK, 0) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) {
+        (void)hipGetLastError();
+        cus = 1;
+    }
+    const uint64_t n = (uint64_t)per_cu * (uint64_t)cus;
+    std::lock_guard<std::mutex> g(mu);
+    if (n_cache < 8) {
+        cache[n_cache].k = kernel;
+        cache[n_cache].dev = dev;
+        cache[n_cache].n = n;
+        ++n_cache;
+    }
+    return n;
+}
+
+// Gausslets per chunk for the 2-D field-sum grid: enough chunks that the grid
+// is about four rounds of resident workgroups (fewer point blocks than that
+// leave CUs idle, and a grid of 1.3 rounds idles most of them in its second
+// round), a whole number of tiles per chunk, and a partial-sum buffer of at
+// most 512 MB.
+static uint64_t field_chunk(const void *kernel, uint64_t G, uint64_t M, uint64_t tile) {
+    const uint64_t pb = (M + FS_BLOCK - 1) / FS_BLOCK;
+    const uint64_t tiles = (G + tile - 1) / tile;
+    uint64_t c = (4 * field_resident(kernel) + pb - 1) / pb;
+    const uint64_t cap = (512ull << 20) / (48 * M);
+    c = std::max<uint64_t>(1, std::min(c, std::min<uint64_t>(tiles, std::max<uint64_t>(cap, 1))));
+    c = std::min<uint64_t>(c, 65535);
+    return ((tiles + c - 1) / c) * tile;
+}
+
 extern "C" rpx_status_t rpx_sum_gaussian_modes(const rpx_rays_t *rays, uint64_t n_rays, const double *modes,
                                                const double *wavelengths, uint64_t n_wavelengths,
                                                const double *points, uint64_t n_pts, double time_ps, double *out,
@@ -768,16 +832,37 @@ extern "C" rpx_status_t rpx_sum_gaussian_modes(const rpx_rays_t *rays, uint64_t 
     hipLaunchKernelGGL(k_field_prep, dim3((unsigned)((n_rays + 255) / 256)), dim3(256), 0, stream, *rays, n_rays,
                        modes, wl, ct, gp);
     hipError_t e = hipGetLastError();
+    double *part = nullptr;
     if (e == hipSuccess) {
-        if (precision == 0)
-            hipLaunchKernelGGL(k_field_sum, dim3((unsigned)((n_pts + FS_BLOCK - 1) / FS_BLOCK)), dim3(FS_BLOCK), 0,
-                               stream, gp, n_rays, points, n_pts, out);
-        else
-            hipLaunchKernelGGL(k_field_sum_split, dim3((unsigned)((n_pts + FS_BLOCK - 1) / FS_BLOCK)),
-                               dim3(FS_BLOCK), 0, stream, gp, n_rays, points, n_pts, out);
-        e = hipGetLastError();
+        // gausslet chunks on grid y, partial sums reduced in chunk order
+        const void *kern = precision == 0 ? (const void *)k_field_sum : (const void *)k_field_sum_split;
+        const uint64_t chunk = field_chunk(kern, n_rays, n_pts, precision == 0 ? FS_TILE : FS32_TILE);
+        const uint64_t n_chunks = (n_rays + chunk - 1) / chunk;
+        double *dst = out;
+        if (n_chunks > 1) {
+            e = hipMallocAsync((void **)&part, sizeof(double) * 6 * n_pts * n_chunks, stream);
+            dst = part;
+        }
+        const dim3 grid((unsigned)((n_pts + FS_BLOCK - 1) / FS_BLOCK), (unsigned)n_chunks);
+        if (e == hipSuccess) {
+            if (precision == 0)
+                hipLaunchKernelGGL(k_field_sum, grid, dim3(FS_BLOCK), 0, stream, gp, n_rays, chunk, points, n_pts, dst);
+            else
+                hipLaunchKernelGGL(k_field_sum_split, grid, dim3(FS_BLOCK), 0, stream, gp, n_rays, chunk, points,
+                                   n_pts, dst);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && n_chunks > 1) {
+            hipLaunchKernelGGL(k_field_reduce, dim3((unsigned)((6 * n_pts + 255) / 256)), dim3(256), 0, stream,
+                               (const double *)part, (uint32_t)n_chunks, 6 * n_pts, out);
+            e = hipGetLastError();
+        }
     }
     hipError_t e2 = hipFreeAsync(gp, stream);
+    if (part) {
+        hipError_t e3 = hipFreeAsync(part, stream);
+        if (e2 == hipSuccess) e2 = e3;
+    }
     if (e != hipSuccess) return rpx_set_error(RPX_E_HIP, std::string("field sum launch: ") + hipGetErrorString(e));
     if (e2 != hipSuccess) return rpx_set_error(RPX_E_HIP, std::string("field sum free: ") + hipGetErrorString(e2));
     return RPX_OK;
