@@ -322,6 +322,30 @@ def test_gausslet_modes_and_field(gpu):
     assert np.abs(got - ref).max() <= 1e-9 * scale
 
 
+@pytest.mark.parametrize("n_g", [1, 40, 64, 65, 400])
+def test_field_sum_gausslet_chunks(gpu, n_g):
+    """sum_gaussian_modes over subsets of the gausslets: up to one LDS tile
+    the kernel writes the sum directly, beyond it the gausslets are split
+    into chunks whose partial sums are reduced in chunk order.  Both paths,
+    both precisions, against the oracle (float64 1e-9, FP32-split 1e-5 of
+    max|E|, the bounds of the field tests)."""
+    w, g = gausslet_workload(400)
+    traced, _ = trace_rays(g, w.face_lists, recursion_limit=3, max_length=w.max_length)
+    arr = traced[2].copy_as_array()[:n_g]
+    x, y, dx, dy = oracle.neighbours_gc(arr)
+    modes = oracle.evaluate_modes(x, y, dx, dy, 1.0)
+    rays = RayCollection.from_array(arr["base_ray"].copy())
+    z_f = w.info["detector_z_mm"]
+    gx, gy = np.meshgrid(np.linspace(-0.02, 0.02, 9), np.linspace(-0.02, 0.02, 7))
+    pts = np.column_stack([gx.ravel(), gy.ravel(), np.full(gx.size, z_f)])
+    ref = oracle.sum_gaussian_modes(arr["base_ray"], modes, w.wavelengths, pts)
+    scale = np.abs(ref).max()
+    assert scale > 0
+    for prec, tol in ((0, 1e-9), (1, 1e-5)):
+        got = cfields.sum_gaussian_modes(rays, modes, w.wavelengths, pts, precision=prec)
+        assert np.abs(got - ref).max() <= tol * scale, prec
+
+
 @pytest.mark.parametrize("order", (-1, 0, 1))
 def test_trace_c3_michelson(gpu, order):
     """Michelson (beamsplitter cube, flat and Zernike-distorted PEC mirrors,
