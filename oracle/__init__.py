@@ -7,6 +7,13 @@ cpu_baseline leg.  The product library never links or calls it.
 
 Functions take the scene tables compiled by raypier_optics_amd._scene and
 numpy arrays of the packed ray_dtype / gausslet_dtype records.
+
+Parity unpinned (checked only against this restatement): the libm boundary,
+the phase conventions of the coated r and t, which rays the asphere Newton
+abandons and the DistortionFace iteration count, the Zernike derivative values
+for n >= 5, the assumed left-to-right evaluation order, Sellmeier formula 5;
+in oracle/sources.py and oracle/fields.py, numpy's reduction order where it is
+mirrored and `exp` in the source amplitude weights.
 """
 import ctypes
 import os

@@ -12,8 +12,17 @@
  *
  * Scene tables use the layouts of include/rpx.h (plain data).  The Zernike
  * distortion here runs the reference's memoised recursion literally, over its
- * NaN-initialised workspace; the compiled instruction lists of the scene table
- * are ignored.
+ * NaN-initialised workspace; the compiled instruction lists and polynomial
+ * tables of the scene table are ignored.
+ *
+ * Parity unpinned (no known answer, closed form or invariant checks these;
+ * they rest on this restatement alone): the libm boundary (pow, sin, cos,
+ * atan2, exp, hypot, csqrt, cexp; the device is compared to 1e-9); the phase
+ * conventions of the coated r and t (their magnitudes are pinned); which
+ * rays the asphere Newton abandons and the DistortionFace iteration count;
+ * the Zernike derivative values for n >= 5 (the reference's slot-sharing
+ * quirk); the left-to-right evaluation order assumed for compound
+ * expressions; Sellmeier formula 5.
  */
 #include <complex.h>
 #include <math.h>
