@@ -299,6 +299,22 @@ RPX_DEV bool asph_outside_band(const rpx_face_t &f, V3 p1, V3 p2, int is_base) {
 }
 
 // ------------------------------------------------------------- intersect
+// Face record k of a table in global memory, for an index that is the same
+// in every lane: read through the constant address space, i.e. with scalar
+// loads into SGPRs (word by word, so that the host pass of the compiler,
+// which cannot copy-construct from that address space, accepts it).  Scene
+// tables are not written while kernels run.
+RPX_DEV rpx_face_t face_uniform(const rpx_face_t *faces, int k) {
+    typedef __attribute__((address_space(4))) const uint64_t WordK;
+    static_assert(sizeof(rpx_face_t) % 8 == 0, "face records are copied as 64-bit words");
+    const WordK *w = (const WordK *)(faces + k);
+    rpx_face_t f;
+    uint64_t *d = (uint64_t *)&f;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(rpx_face_t) / 8); ++i) d[i] = w[i];
+    return f;
+}
+
 // Base-face intersection used by DistortionFace (no recursion into a second
 // distortion level: the host refuses nested distortions).
 template <uint32_t F>

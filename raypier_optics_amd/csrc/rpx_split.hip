@@ -40,7 +40,8 @@ struct PairEmitS {
 };
 
 template <uint32_t F>
-__global__ __launch_bounds__(RPX_BLOCK) void k_generation_split(SplitArgs A) {
+__global__ __launch_bounds__(RPX_BLOCK) __attribute__((amdgpu_waves_per_eu(F == V_ALL ? 1 : 4)))
+void k_generation_split(SplitArgs A) {
     __shared__ SceneLds lds_scene;
     __shared__ uint32_t lds_hist[RPX_LDS_FACES];
     __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
@@ -79,14 +80,22 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_generation_split(SplitArgs A) {
             const int k0 = A.one_list < 0 ? fl.first_face : fl.first_face + A.one_face;
             const int k1 = A.one_list < 0 ? fl.first_face + fl.n_faces : k0 + 1;
             for (int k = k0; k < k1; ++k) {
-                const rpx_face_t &fc = S.faces[k];
-                const double dist = face_intersect<F>(S, fc, p1, p2, 1);
-                if (fc.tolerance < dist && dist < length) {
-                    length = dist;
-                    end_face = (uint32_t)fc.global_idx;
-                    hit_rec = k;
-                    hit_list = l;
-                }
+                auto test = [&](const rpx_face_t &fc) {
+                    const double dist = face_intersect<F>(S, fc, p1, p2, 1);
+                    if (fc.tolerance < dist && dist < length) {
+                        length = dist;
+                        end_face = (uint32_t)fc.global_idx;
+                        hit_rec = k;
+                        hit_list = l;
+                    }
+                };
+                // k is the same in every lane: the lens and sphere variants
+                // read the face record with scalar loads (SGPRs, not VGPRs,
+                // hold its parameters through the root solve), which lets
+                // them run 4 waves per SIMD; the all-features variant reads
+                // the LDS copy
+                if constexpr (F == V_ALL) test(S.faces[k]);
+                else test(face_uniform(A.S.faces, k));
             }
         }
         A.in.f32[RPX_S_LENGTH * c + i] = (float)length;
