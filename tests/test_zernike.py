@@ -167,8 +167,10 @@ def run_program(insns, n_slots, r):
 
 
 def interpret(t, zi, x, y):
-    """The device algorithm (rpx_faces.h zernike_sag / zernike_grad) in Python
-    floats: libm calls are the same C functions the oracle uses."""
+    """The compiled programs of the scene tables, run in Python floats with
+    the reference's summation (libm calls are the same C functions the oracle
+    uses): the form the polynomial tables the kernels evaluate are expanded
+    from."""
     Z = t.zernikes[zi]
     coefs = t.zcoefs[Z["coef_begin"]: Z["coef_begin"] + Z["n_coefs"]]
     sag_p = t.zinsns[Z["sag_begin"]: Z["sag_begin"] + Z["sag_len"]]
