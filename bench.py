@@ -286,8 +286,8 @@ def main():
                     help="run device-to-device copies of this size first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding)",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: tracing shards by contiguous blocks of input rays and the field "
+              "sum by evaluation points, but no multi-GPU path is built", file=sys.stderr)
         sys.exit(2)
 
     import torch

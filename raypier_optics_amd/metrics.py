@@ -4,8 +4,8 @@ A detector hit is a ray of any generation whose end_face_idx is the
 detector's face index; its point is origin + direction * length, evaluated
 in float64.  `spot_rms_by_wavelength` bins the hits by wavelength and
 returns, per bin, the RMS distance of the hit points from their centroid
-(the spot RMS radius) -- the figure by which the FP32-split form is judged
-against float64.
+(the spot RMS radius) -- the figure in which the FP32-split form's error
+against float64 is stated.
 """
 import numpy as np
 
