@@ -718,7 +718,7 @@ __global__ __launch_bounds__(FS_BLOCK) void k_field_sum_split(const double *gp, 
             const float arg_re = (float)red + fmaf(kzr, qr, -(kzi * qi));
             const float arg_im = fmaf(kzi, zf, fmaf(kzr, qi, kzi * qr));
             float sn, cs;
-            sincosf(arg_re, &sn, &cs);
+            sincos_cwf(arg_re, &sn, &cs);
             const float mag = expf(-arg_im);
             // 1 / sqrt(i w) as conj(sq) / |w| from reciprocal square roots
             const float q1r = fmaf(zf, Ar, 1.0f), q1i = zf * Ai, q2r = fmaf(zf, Cr, 1.0f), q2i = zf * Ci;

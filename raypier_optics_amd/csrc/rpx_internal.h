@@ -167,6 +167,38 @@ RPX_DEV void sincos_cw(double x, double *s, double *c) {
     *c = ((q + 1) & 2) ? -cv : cv;
 }
 
+// Float32 sin(x) and cos(x) for |x| < 2^13: r = x - n pi/2 by two fused
+// steps (pi/2 = P1 + P2 in float32; fmaf rounds each step once), then the
+// Taylor series of sin to r^11 and of cos to r^12 on |r| <= pi/4
+// (truncation below 2e-9) and the quadrant of n.  Larger arguments go to the
+// library routine.
+RPX_DEV void sincos_cwf(float x, float *s, float *c) {
+    const float nf = rintf(x * 0.6366197466850281f);
+    if (!(fabsf(nf) < 8192.0f)) {
+        sincosf(x, s, c);
+        return;
+    }
+    const float r = fmaf(-nf, -4.371138828673793e-08f, fmaf(-nf, 1.5707963705062866f, x));
+    const float u = r * r;
+    float ps = -2.5052107943679403e-08f;         // -1/11!
+    ps = fmaf(ps, u, 2.7557318844628753e-06f);   // 1/9!
+    ps = fmaf(ps, u, -1.9841270113829523e-04f);  // -1/7!
+    ps = fmaf(ps, u, 8.333333767950535e-03f);    // 1/5!
+    ps = fmaf(ps, u, -1.666666716337204e-01f);   // -1/3!
+    const float sr = fmaf(r * u, ps, r);
+    float pc = 2.0876755879584152e-09f;          // 1/12!
+    pc = fmaf(pc, u, -2.755731998149713e-07f);   // -1/10!
+    pc = fmaf(pc, u, 2.4801587642286904e-05f);   // 1/8!
+    pc = fmaf(pc, u, -1.3888889225199819e-03f);  // -1/6!
+    pc = fmaf(pc, u, 4.16666679084301e-02f);     // 1/4!
+    pc = fmaf(pc, u, -0.5f);                     // -1/2!
+    const float cr = fmaf(u, pc, 1.0f);
+    const int q = (int)nf & 3;
+    const float sv = (q & 1) ? cr : sr, cv = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -sv : sv;
+    *c = ((q + 1) & 2) ? -cv : cv;
+}
+
 // ---------------------------------------------------------- feature masks
 // Face and material families; generation kernels are instantiated for a few
 // masks and a scene runs the smallest one that covers what it uses.
