@@ -322,8 +322,8 @@ def test_gausslet_modes_and_field(gpu):
     assert np.abs(got - ref).max() <= 1e-9 * scale
 
 
-@pytest.mark.parametrize("n_g", [1, 40, 64, 65, 400])
-def test_field_sum_gausslet_chunks(gpu, n_g):
+@pytest.mark.parametrize("n_g,n_x", [(1, 9), (40, 9), (64, 9), (65, 9), (400, 9), (400, 1)])
+def test_field_sum_gausslet_chunks(gpu, n_g, n_x):
     """sum_gaussian_modes over subsets of the gausslets: up to one LDS tile
     the kernel writes the sum directly, beyond it the gausslets are split
     into chunks whose partial sums are reduced in chunk order.  Both paths,
@@ -336,7 +336,7 @@ def test_field_sum_gausslet_chunks(gpu, n_g):
     modes = oracle.evaluate_modes(x, y, dx, dy, 1.0)
     rays = RayCollection.from_array(arr["base_ray"].copy())
     z_f = w.info["detector_z_mm"]
-    gx, gy = np.meshgrid(np.linspace(-0.02, 0.02, 9), np.linspace(-0.02, 0.02, 7))
+    gx, gy = np.meshgrid(np.linspace(-0.02, 0.02, n_x), np.linspace(-0.02, 0.02, 7 if n_x > 1 else 1))
     pts = np.column_stack([gx.ravel(), gy.ravel(), np.full(gx.size, z_f)])
     ref = oracle.sum_gaussian_modes(arr["base_ray"], modes, w.wavelengths, pts)
     scale = np.abs(ref).max()
