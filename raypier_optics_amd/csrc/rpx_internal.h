@@ -214,7 +214,10 @@ enum : uint32_t {
     M_GRATING = 1u << 8,
     V_ALL = 0x1ffu,
     V_LENS = F_PLANE | F_SPHERE | F_CONIC | F_ASPHERE | M_SIMPLE | M_FRESNEL | M_COATED,
-    V_SPHERES = F_PLANE | F_SPHERE | M_SIMPLE | M_FRESNEL | M_COATED
+    V_SPHERES = F_PLANE | F_SPHERE | M_SIMPLE | M_FRESNEL | M_COATED,
+    // flat optics (cubes, plane and distorted mirrors, gratings); the
+    // FP32-split generation kernel only, other kernels run V_ALL for it
+    V_FLAT = F_PLANE | F_DISTORT | M_SIMPLE | M_FRESNEL | M_GRATING
 };
 
 // ------------------------------------------------------------------- scene

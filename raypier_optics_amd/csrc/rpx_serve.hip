@@ -263,6 +263,7 @@ struct StepGraph {
 
 struct rpx_server {
     hipStream_t stream = nullptr;
+    hipMemPool_t pool = nullptr;   // job buffers; keeps up to the HBM budget cached
     uint64_t budget = 0, used = 0;
     uint32_t max_jobs = 0;
     uint64_t max_tiles = 0;
@@ -285,7 +286,7 @@ struct rpx_server {
 
 static rpx_status_t alloc_buffer(rpx_server_t *s, Buffer &b, uint64_t cap) {
     b.cap = std::max<uint64_t>(cap, 1);
-    RPX_HIP_TRY(hipMallocAsync(&b.p, b.bytes(), s->stream));
+    RPX_HIP_TRY(hipMallocFromPoolAsync(&b.p, b.bytes(), s->pool, s->stream));
     return RPX_OK;
 }
 
@@ -375,6 +376,22 @@ extern "C" rpx_status_t rpx_server_create(uint64_t hbm_budget_bytes, uint32_t ma
     if (e != hipSuccess) return fail(rpx_set_error(RPX_E_HIP, hipGetErrorString(e)));
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return fail(rpx_set_error(RPX_E_NO_DEVICE, "no HIP device"));
+    // A pool of the server's own whose release threshold is the budget: the
+    // buffers freed each step stay mapped for the next step's allocations
+    // instead of going back to the driver at every synchronisation.
+    hipMemPoolProps props = {};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    e = hipMemPoolCreate(&s->pool, &props);
+    if (e != hipSuccess) {
+        s->pool = nullptr;
+        return fail(rpx_set_error(RPX_E_HIP, hipGetErrorString(e)));
+    }
+    uint64_t keep = hbm_budget_bytes;
+    e = hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold, &keep);
+    if (e != hipSuccess) return fail(rpx_set_error(RPX_E_HIP, hipGetErrorString(e)));
     if (hipMalloc(&s->d_jobs, sizeof(BatchJob) * max_jobs) != hipSuccess ||
         hipMalloc(&s->d_ctl, sizeof(BatchControl)) != hipSuccess ||
         hipMalloc(&s->d_status, sizeof(uint64_t) * s->max_tiles) != hipSuccess ||
@@ -422,6 +439,7 @@ extern "C" rpx_status_t rpx_server_destroy(rpx_server_t *s) {
     (void)hipHostFree(s->h_ctl_back);
     (void)hipHostFree(s->h_counts);
     (void)hipHostFree(s->h_moments);
+    if (s->pool) (void)hipMemPoolDestroy(s->pool);   // every job buffer was freed and the stream synchronised
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return RPX_OK;

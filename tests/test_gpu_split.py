@@ -154,3 +154,41 @@ def test_split_materials(gpu, k):
     assert (np.abs(fa - fb) / np.maximum(scale, 1e-30)).max() <= 1e-5
     assert np.allclose(a["accumulated_path"], b["accumulated_path"], rtol=1e-6, atol=1e-6)
     assert np.abs(a["phase"] - b["phase"]).max() <= 2e-3
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_split_c3_michelson(gpu, order):
+    """Flat optics (beamsplitter cube, plane and Zernike-distorted PEC
+    mirrors, 600 l/mm grating) to depth 12: every generation of the split
+    trace against float64.  Tolerances: the topology bound above; origins
+    1e-4 mm (float32 spacing is 7.6e-6 mm at 100 mm and a ray here crosses
+    the scene up to twelve times); directions 1e-6; the physical field 1e-5
+    of the amplitude.  Phase: 2e-3 rad without a grating order; with order
+    +1 the grating adds 1000*((origin - p).t)*2*pi/spacing, 3770 rad per mm
+    at 600 lines/mm, and at y = -50 mm a float32 origin coordinate is
+    rounded by up to 1.9e-6 mm, so three coordinates bound the phase error
+    at 1.2e-2 rad."""
+    w = lambda: workloads.michelson_c3(n_rays=4096, order=order)
+    _, t64, f64, t32, f32 = _both(w)
+    _compare_topology(t64, t32)
+    assert len(t64) >= 6
+    worst = {"origin": 0.0, "direction": 0.0, "field": 0.0, "phase": 0.0}
+    for g, (a, b) in enumerate(zip(t64, t32)):
+        ra, rb = a.copy_as_array(), b.to_collection().copy_as_array()
+        same = ra["end_face_idx"] == rb["end_face_idx"]
+        if g > 0:
+            same &= ra["parent_idx"] == rb["parent_idx"]
+        ra, rb = ra[same], rb[same]
+        if len(ra) == 0:
+            continue
+        worst["origin"] = max(worst["origin"], float(np.abs(ra["origin"] - rb["origin"]).max()))
+        worst["direction"] = max(worst["direction"], float(np.abs(ra["direction"] - rb["direction"]).max()))
+        fa, fb = physical_field(ra), physical_field(rb)
+        scale = np.maximum(np.abs(ra["E1_amp"]), np.abs(ra["E2_amp"]))[:, None]
+        worst["field"] = max(worst["field"], float((np.abs(fa - fb) / np.maximum(scale, 1e-30)).max()))
+        worst["phase"] = max(worst["phase"], float(np.abs(ra["phase"] - rb["phase"]).max()))
+    print("order %d worst split - float64:" % order, worst)
+    assert worst["origin"] <= 1e-4
+    assert worst["direction"] <= 1e-6
+    assert worst["field"] <= 1e-5
+    assert worst["phase"] <= (1.2e-2 if order else 2e-3)
