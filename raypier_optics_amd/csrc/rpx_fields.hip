@@ -256,8 +256,7 @@ static rpx_status_t compact_launch_checks(const rpx_rays_t *in, uint64_t n, cons
 static rpx_status_t finish_compaction(const Workspace &w, uint64_t capacity, uint64_t *n_out, hipStream_t stream,
                                       const char *what) {
     GenControl ctl;
-    RPX_HIP_TRY(hipMemcpyAsync(&ctl, w.ctl, RPX_CTL_HEAD, hipMemcpyDeviceToHost, stream));
-    RPX_HIP_TRY(hipStreamSynchronize(stream));
+    RPX_HIP_TRY(read_control(ctl, w.ctl, stream));
     if (ctl.error) return rpx_set_error(RPX_E_HIP, std::string(what) + ": a look-back wait exceeded its poll limit");
     *n_out = ctl.children;
     if (ctl.hits)

@@ -2,6 +2,7 @@
 // block size and LDS limits, the per-generation workspace, scene staging in
 // LDS, ballot scans and the decoupled look-back that places children.
 #pragma once
+#include <cstring>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include "rpx_internal.h"
@@ -213,6 +214,19 @@ static __device__ uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t fi
     }
     if (lane == 0) status_store(&status[tile], ST_PREFIX | (excl + agg));
     return excl;
+}
+
+// Reads the head of a generation's control block back to the host and waits
+// for it.  The copy lands in a pinned per-thread block (a pageable
+// destination would cost a staged copy), held until the process ends.
+static inline hipError_t read_control(GenControl &out, const GenControl *d_ctl, hipStream_t stream) {
+    thread_local GenControl *pin = nullptr;
+    if (!pin && hipHostMalloc((void **)&pin, sizeof(GenControl), 0) != hipSuccess) pin = nullptr;
+    GenControl *dst = pin ? pin : &out;
+    hipError_t e = hipMemcpyAsync(dst, d_ctl, RPX_CTL_HEAD, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess && dst != &out) memcpy(&out, dst, RPX_CTL_HEAD);
+    return e;
 }
 
 static inline uint32_t scene_variant(const rpx_scene_t *s) {

@@ -156,8 +156,7 @@ extern "C" rpx_status_t rpx_path_lengths(const rpx_generation_t *generations, in
                        n_generations, target_face, totals, capacity, w.tile_status, w.ctl);
     RPX_HIP_TRY(hipGetLastError());
     GenControl ctl;
-    RPX_HIP_TRY(hipMemcpyAsync(&ctl, w.ctl, RPX_CTL_HEAD, hipMemcpyDeviceToHost, stream));
-    RPX_HIP_TRY(hipStreamSynchronize(stream));
+    RPX_HIP_TRY(read_control(ctl, w.ctl, stream));
     if (ctl.error) return rpx_set_error(RPX_E_HIP, "path walk: a look-back wait exceeded its poll limit");
     *n_selected = ctl.children;
     if (ctl.hits)

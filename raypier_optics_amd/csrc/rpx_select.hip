@@ -145,8 +145,7 @@ extern "C" rpx_status_t rpx_select_intersections(const rpx_scene_t *scene, int32
     else launch_select_f<V_ALL>(gauss, dim3((unsigned)nb), stream, A);
     RPX_HIP_TRY(hipGetLastError());
     GenControl ctl;
-    RPX_HIP_TRY(hipMemcpyAsync(&ctl, A.ws.ctl, RPX_CTL_HEAD, hipMemcpyDeviceToHost, stream));
-    RPX_HIP_TRY(hipStreamSynchronize(stream));
+    RPX_HIP_TRY(read_control(ctl, A.ws.ctl, stream));
     if (ctl.error) return rpx_set_error(RPX_E_HIP, "selection scan: a look-back wait exceeded its poll limit");
     *n_out = ctl.children;
     if (ctl.hits)
@@ -249,8 +248,7 @@ extern "C" rpx_status_t rpx_capture_gausslets(const rpx_rays_t *in, uint64_t n_i
     hipLaunchKernelGGL(k_capture, dim3((unsigned)nb), dim3(RPX_BLOCK), 0, stream, A);
     RPX_HIP_TRY(hipGetLastError());
     GenControl ctl;
-    RPX_HIP_TRY(hipMemcpyAsync(&ctl, A.ws.ctl, RPX_CTL_HEAD, hipMemcpyDeviceToHost, stream));
-    RPX_HIP_TRY(hipStreamSynchronize(stream));
+    RPX_HIP_TRY(read_control(ctl, A.ws.ctl, stream));
     if (ctl.error) return rpx_set_error(RPX_E_HIP, "capture scan: a look-back wait exceeded its poll limit");
     *n_out = ctl.children;
     if (out_offset + ctl.children > out->capacity)

@@ -290,8 +290,7 @@ extern "C" rpx_status_t rpx_lattice_emit(const rpx_lattice_t *L, rpx_rays_t *out
     const uint64_t nl = n_labels(L->nsteps);
     LatticeWs w = carve_lattice(workspace, L->nsteps);
     GenControl ctl;
-    RPX_HIP_TRY(hipMemcpyAsync(&ctl, w.ctl, RPX_CTL_HEAD, hipMemcpyDeviceToHost, stream));
-    RPX_HIP_TRY(hipStreamSynchronize(stream));
+    RPX_HIP_TRY(read_control(ctl, w.ctl, stream));
     if (ctl.children > out->capacity)
         return rpx_set_error(RPX_E_CAPACITY, "output capacity " + std::to_string(out->capacity) + " below " +
                                                  std::to_string(ctl.children) + " lattice points");

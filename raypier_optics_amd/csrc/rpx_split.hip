@@ -272,8 +272,7 @@ extern "C" rpx_status_t rpx_trace_segment_split(const rpx_scene_t *scene, int32_
         RPX_HIP_TRY(hipGetLastError());
     }
     GenControl ctl;
-    RPX_HIP_TRY(hipMemcpyAsync(&ctl, A.ws.ctl, RPX_CTL_HEAD, hipMemcpyDeviceToHost, stream));
-    RPX_HIP_TRY(hipStreamSynchronize(stream));
+    RPX_HIP_TRY(read_control(ctl, A.ws.ctl, stream));
     if (ctl.error) return rpx_set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
     ctl.hits = 0;
     for (int s2 = 0; s2 < RPX_SHARDS; ++s2) ctl.hits += ctl.hit_shards[s2];

@@ -796,8 +796,7 @@ static rpx_status_t run_generation(const rpx_scene_t *scene, int one_list, int o
             HIP_TRY(hipGetLastError());
         }
         GenControl ctl;
-        HIP_TRY(hipMemcpyAsync(&ctl, A.ws.ctl, RPX_CTL_HEAD, hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
+        HIP_TRY(read_control(ctl, A.ws.ctl, stream));
         if (ctl.error) return set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
         ctl.hits = 0;
         for (int s = 0; s < RPX_SHARDS; ++s) ctl.hits += ctl.hit_shards[s];
