@@ -151,12 +151,20 @@ static __global__ __launch_bounds__(256) void k_clear(uint64_t *tile_status, uin
     }
 }
 
-// Adds the sharded per-face-record hit counts of a single-pass generation to
-// face_hits (indexed by global face index).  One workgroup.
+// Ends a single-pass generation.  Adds its sharded per-face-record hit
+// counts to face_hits (indexed by global face index) when face_hits is set,
+// and copies the control head to head_out (host memory mapped for the
+// device) when that is set.  One workgroup.
 static __global__ __launch_bounds__(64) void k_fold_hits(const GenControl *ctl, const rpx_face_t *faces,
-                                                          int n_list_faces, uint32_t *face_hits) {
+                                                          int n_list_faces, uint32_t *face_hits,
+                                                          GenControl *head_out) {
     const int k = threadIdx.x;
-    if (k >= n_list_faces) return;
+    if (head_out) {
+        const uint32_t *src = (const uint32_t *)ctl;
+        uint32_t *dst = (uint32_t *)head_out;
+        for (int i = k; i < (int)(RPX_CTL_HEAD / 4); i += 64) dst[i] = src[i];
+    }
+    if (!face_hits || k >= n_list_faces) return;
     uint32_t h = 0;
     for (int s = 0; s < RPX_SHARDS; ++s) h += ctl->face_shards[s][k];
     if (h) atomicAdd(&face_hits[faces[k].global_idx], h);
@@ -216,16 +224,52 @@ static __device__ uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t fi
     return excl;
 }
 
-// Reads the head of a generation's control block back to the host and waits
-// for it.  The copy lands in a pinned per-thread block (a pageable
-// destination would cost a staged copy), held until the process ends.
+// A pinned per-thread block, mapped for every device, that control heads are
+// read back into (a pageable destination would cost a staged copy); held
+// until the process ends.  Both pointers are null if it cannot be allocated.
+struct PinnedControl {
+    GenControl *host = nullptr, *dev = nullptr;
+};
+static inline const PinnedControl &pinned_control() {
+    thread_local PinnedControl p;
+    thread_local bool tried = false;
+    if (!tried) {
+        tried = true;
+        GenControl *h = nullptr;
+        void *d = nullptr;
+        if (hipHostMalloc((void **)&h, sizeof(GenControl), hipHostMallocMapped | hipHostMallocPortable) ==
+                hipSuccess &&
+            hipHostGetDevicePointer(&d, h, 0) == hipSuccess) {
+            p.host = h;
+            p.dev = (GenControl *)d;
+        }
+    }
+    return p;
+}
+
+// Reads the head of a control block back to the host and waits for it.
 static inline hipError_t read_control(GenControl &out, const GenControl *d_ctl, hipStream_t stream) {
-    thread_local GenControl *pin = nullptr;
-    if (!pin && hipHostMalloc((void **)&pin, sizeof(GenControl), 0) != hipSuccess) pin = nullptr;
-    GenControl *dst = pin ? pin : &out;
+    GenControl *dst = pinned_control().host ? pinned_control().host : &out;
     hipError_t e = hipMemcpyAsync(dst, d_ctl, RPX_CTL_HEAD, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e == hipSuccess && dst != &out) memcpy(&out, dst, RPX_CTL_HEAD);
+    return e;
+}
+
+// Ends a single-pass generation: one k_fold_hits launch folds the face hits
+// (when face_hits is set) and writes the control head into the pinned block,
+// which the host reads once the stream has drained.
+static inline hipError_t finish_generation(GenControl &out, const GenControl *d_ctl, const rpx_face_t *faces,
+                                           int n_list_faces, uint32_t *face_hits, hipStream_t stream) {
+    const PinnedControl &p = pinned_control();
+    if (p.host || face_hits) {
+        hipLaunchKernelGGL(k_fold_hits, dim3(1), dim3(64), 0, stream, d_ctl, faces, n_list_faces, face_hits, p.dev);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (!p.host) return read_control(out, d_ctl, stream);
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) memcpy(&out, p.host, RPX_CTL_HEAD);
     return e;
 }
 

@@ -266,13 +266,9 @@ extern "C" rpx_status_t rpx_trace_segment_split(const rpx_scene_t *scene, int32_
     launch_split(split_variant(scene), dim3((unsigned)nb), stream, A);
     RPX_HIP_TRY(hipGetLastError());
     if (timing) RPX_HIP_TRY(hipEventRecord(ev.e[2], stream));
-    if (face_hits) {
-        hipLaunchKernelGGL(k_fold_hits, dim3(1), dim3(64), 0, stream, A.ws.ctl, scene->view.faces,
-                           scene->view.n_list_faces, face_hits);
-        RPX_HIP_TRY(hipGetLastError());
-    }
     GenControl ctl;
-    RPX_HIP_TRY(read_control(ctl, A.ws.ctl, stream));
+    RPX_HIP_TRY(finish_generation(ctl, A.ws.ctl, scene->view.faces, scene->view.n_list_faces, face_hits,
+                                  stream));
     if (ctl.error) return rpx_set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
     ctl.hits = 0;
     for (int s2 = 0; s2 < RPX_SHARDS; ++s2) ctl.hits += ctl.hit_shards[s2];

@@ -790,13 +790,9 @@ static rpx_status_t run_generation(const rpx_scene_t *scene, int one_list, int o
         launch_generation(variant, dim3((unsigned)nb), stream, A);
         HIP_TRY(hipGetLastError());
         if (timing) HIP_TRY(hipEventRecord(ev.e[2], stream));
-        if (face_hits) {
-            hipLaunchKernelGGL(k_fold_hits, dim3(1), dim3(64), 0, stream, A.ws.ctl, scene->view.faces,
-                               scene->view.n_list_faces, face_hits);
-            HIP_TRY(hipGetLastError());
-        }
         GenControl ctl;
-        HIP_TRY(read_control(ctl, A.ws.ctl, stream));
+        HIP_TRY(finish_generation(ctl, A.ws.ctl, scene->view.faces, scene->view.n_list_faces, face_hits,
+                                      stream));
         if (ctl.error) return set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
         ctl.hits = 0;
         for (int s = 0; s < RPX_SHARDS; ++s) ctl.hits += ctl.hit_shards[s];
