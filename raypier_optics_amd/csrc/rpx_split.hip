@@ -202,7 +202,8 @@ extern "C" rpx_status_t rpx_rays_from_split(const rpx_rays_split_t *in, uint64_t
 
 // The scene's variant, or V_FLAT for a scene that needs V_ALL only for its
 // distortions and gratings: at 3 waves per SIMD (with some spilling) it runs
-// C3's Michelson 5 % faster than V_ALL at 2.
+// a Michelson of a beamsplitter cube, a plane and a Zernike mirror and a
+// grating (1,048,576 rays to depth 12) 5 % faster than V_ALL at 2.
 static uint32_t split_variant(const rpx_scene_t *s) {
     const uint32_t v = scene_variant(s);
     return (v == V_ALL && (s->features & ~(uint32_t)V_FLAT) == 0) ? (uint32_t)V_FLAT : v;
