@@ -118,3 +118,24 @@ def test_server_admits_while_running(gpu):
     res = {r.job_id: r for r in done}
     assert res[js].t_admit < res[jb].t_finish
     _same(ref_small, res[js], "small job")
+
+
+def test_two_servers_at_once(gpu):
+    """Two servers live together, each with its own stream and memory pool,
+    stepped in turn: every job's result is bit-identical between them."""
+    jobs = _jobs(seeds=range(20, 26))
+    servers = [TraceServer(hbm_budget_bytes=64 << 20, max_jobs=4, max_rays_in_flight=1 << 22) for _ in range(2)]
+    try:
+        ids = [[s.submit(w) for w in jobs] for s in servers]
+        res = [{}, {}]
+        while any(s.busy() for s in servers):
+            for k, s in enumerate(servers):
+                if s.busy():
+                    res[k].update({r.job_id: r for r in s.step()})
+    finally:
+        for s in servers:
+            s.close()
+    for k in range(len(jobs)):
+        a, b = res[0][ids[0][k]], res[1][ids[1][k]]
+        _same(a, b, "job %d" % k)
+        assert a.detector_hits > 0
