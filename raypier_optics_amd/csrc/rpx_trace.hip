@@ -908,38 +908,75 @@ extern "C" rpx_status_t rpx_trace_gausslet(const rpx_scene_t *scene, int32_t fac
 // ------------------------------------------------------ AoS <-> SoA
 // Packed ray_t: 21 doubles (168 B) then 5 uint32 (20 B) = 188 B = 47 words.
 // gausslet_t appends 6 x 10 doubles = 480 B -> 167 words.
-__global__ void k_pack(rpx_rays_t soa, uint64_t n, uint32_t *aos, int words_per) {
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n * (uint64_t)words_per) return;
-    const uint64_t i = g / words_per;
-    const int w = (int)(g - i * words_per);
+// Both directions stage RPX_PACK_RAYS whole records per workgroup in LDS:
+// the AoS side moves as one contiguous run of words, and each SoA plane as
+// RPX_PACK_RAYS consecutive values (doubles as single 8-byte accesses).
+// The record stride (47 or 167 words) is odd, so lanes reading one field
+// of consecutive records hit distinct LDS banks.
+#define RPX_PACK_RAYS 64
+
+template <int WPR>
+__global__ __launch_bounds__(256) void k_pack(rpx_rays_t soa, uint64_t n, uint32_t *aos) {
+    __shared__ uint32_t lds[RPX_PACK_RAYS * WPR];
+    const uint64_t r0 = (uint64_t)blockIdx.x * RPX_PACK_RAYS;
+    const int nr = (int)((n - r0) < RPX_PACK_RAYS ? (n - r0) : RPX_PACK_RAYS);
     const uint64_t c = soa.capacity;
-    uint32_t v;
-    if (w < 42) {
-        v = ((const uint32_t *)(soa.f64 + (uint64_t)(w >> 1) * c + i))[w & 1];
-    } else if (w < 47) {
-        v = soa.u32[(uint64_t)(w - 42) * c + i];
-    } else {
-        const int pw = w - 47;
-        v = ((const uint32_t *)(soa.para + (uint64_t)(pw >> 1) * c + i))[pw & 1];
+    for (int k = threadIdx.x; k < 21 * RPX_PACK_RAYS; k += 256) {
+        const int f = k / RPX_PACK_RAYS, r = k - f * RPX_PACK_RAYS;
+        if (r < nr) {
+            const uint64_t v = __double_as_longlong(soa.f64[(uint64_t)f * c + r0 + r]);
+            lds[r * WPR + 2 * f] = (uint32_t)v;
+            lds[r * WPR + 2 * f + 1] = (uint32_t)(v >> 32);
+        }
     }
-    aos[g] = v;
+    for (int k = threadIdx.x; k < 5 * RPX_PACK_RAYS; k += 256) {
+        const int f = k / RPX_PACK_RAYS, r = k - f * RPX_PACK_RAYS;
+        if (r < nr) lds[r * WPR + 42 + f] = soa.u32[(uint64_t)f * c + r0 + r];
+    }
+    if constexpr (WPR > 47) {
+        for (int k = threadIdx.x; k < 60 * RPX_PACK_RAYS; k += 256) {
+            const int f = k / RPX_PACK_RAYS, r = k - f * RPX_PACK_RAYS;
+            if (r < nr) {
+                const uint64_t v = __double_as_longlong(soa.para[(uint64_t)f * c + r0 + r]);
+                lds[r * WPR + 47 + 2 * f] = (uint32_t)v;
+                lds[r * WPR + 47 + 2 * f + 1] = (uint32_t)(v >> 32);
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t *dst = aos + r0 * WPR;
+    for (int k = threadIdx.x; k < nr * WPR; k += 256) dst[k] = lds[k];
 }
 
-__global__ void k_unpack(const uint32_t *aos, uint64_t n, rpx_rays_t soa, int words_per) {
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n * (uint64_t)words_per) return;
-    const uint64_t i = g / words_per;
-    const int w = (int)(g - i * words_per);
+template <int WPR>
+__global__ __launch_bounds__(256) void k_unpack(const uint32_t *aos, uint64_t n, rpx_rays_t soa) {
+    __shared__ uint32_t lds[RPX_PACK_RAYS * WPR];
+    const uint64_t r0 = (uint64_t)blockIdx.x * RPX_PACK_RAYS;
+    const int nr = (int)((n - r0) < RPX_PACK_RAYS ? (n - r0) : RPX_PACK_RAYS);
+    const uint32_t *src = aos + r0 * WPR;
+    for (int k = threadIdx.x; k < nr * WPR; k += 256) lds[k] = src[k];
+    __syncthreads();
     const uint64_t c = soa.capacity;
-    const uint32_t v = aos[g];
-    if (w < 42) {
-        ((uint32_t *)(soa.f64 + (uint64_t)(w >> 1) * c + i))[w & 1] = v;
-    } else if (w < 47) {
-        soa.u32[(uint64_t)(w - 42) * c + i] = v;
-    } else {
-        const int pw = w - 47;
-        ((uint32_t *)(soa.para + (uint64_t)(pw >> 1) * c + i))[pw & 1] = v;
+    for (int k = threadIdx.x; k < 21 * RPX_PACK_RAYS; k += 256) {
+        const int f = k / RPX_PACK_RAYS, r = k - f * RPX_PACK_RAYS;
+        if (r < nr) {
+            const uint64_t v = (uint64_t)lds[r * WPR + 2 * f] | ((uint64_t)lds[r * WPR + 2 * f + 1] << 32);
+            soa.f64[(uint64_t)f * c + r0 + r] = __longlong_as_double((long long)v);
+        }
+    }
+    for (int k = threadIdx.x; k < 5 * RPX_PACK_RAYS; k += 256) {
+        const int f = k / RPX_PACK_RAYS, r = k - f * RPX_PACK_RAYS;
+        if (r < nr) soa.u32[(uint64_t)f * c + r0 + r] = lds[r * WPR + 42 + f];
+    }
+    if constexpr (WPR > 47) {
+        for (int k = threadIdx.x; k < 60 * RPX_PACK_RAYS; k += 256) {
+            const int f = k / RPX_PACK_RAYS, r = k - f * RPX_PACK_RAYS;
+            if (r < nr) {
+                const uint64_t v =
+                    (uint64_t)lds[r * WPR + 47 + 2 * f] | ((uint64_t)lds[r * WPR + 47 + 2 * f + 1] << 32);
+                soa.para[(uint64_t)f * c + r0 + r] = __longlong_as_double((long long)v);
+            }
+        }
     }
 }
 
@@ -949,11 +986,14 @@ extern "C" rpx_status_t rpx_pack_rays(const rpx_rays_t *soa, uint64_t n, void *a
     if (st) return st;
     if (n == 0) return RPX_OK;
     if (!aos) return set_error(RPX_E_INVALID, "AoS buffer is null");
-    const int words = gausslet ? 167 : 47;
-    const uint64_t total = n * words;
-    const uint64_t nb = (total + 255) / 256;
-    hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, *soa, n, (uint32_t *)aos,
-                       words);
+    const uint64_t nb = (n + RPX_PACK_RAYS - 1) / RPX_PACK_RAYS;
+    if (nb > 0x7fffffffull) return set_error(RPX_E_INVALID, "too many rays for one pack launch");
+    if (gausslet)
+        hipLaunchKernelGGL(k_pack<167>, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, *soa, n,
+                           (uint32_t *)aos);
+    else
+        hipLaunchKernelGGL(k_pack<47>, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, *soa, n,
+                           (uint32_t *)aos);
     HIP_TRY(hipGetLastError());
     return RPX_OK;
 }
@@ -964,11 +1004,14 @@ extern "C" rpx_status_t rpx_unpack_rays(const void *aos, uint64_t n, rpx_rays_t 
     if (st) return st;
     if (n == 0) return RPX_OK;
     if (!aos) return set_error(RPX_E_INVALID, "AoS buffer is null");
-    const int words = gausslet ? 167 : 47;
-    const uint64_t total = n * words;
-    const uint64_t nb = (total + 255) / 256;
-    hipLaunchKernelGGL(k_unpack, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, (const uint32_t *)aos,
-                       n, *soa, words);
+    const uint64_t nb = (n + RPX_PACK_RAYS - 1) / RPX_PACK_RAYS;
+    if (nb > 0x7fffffffull) return set_error(RPX_E_INVALID, "too many rays for one unpack launch");
+    if (gausslet)
+        hipLaunchKernelGGL(k_unpack<167>, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream,
+                           (const uint32_t *)aos, n, *soa);
+    else
+        hipLaunchKernelGGL(k_unpack<47>, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream,
+                           (const uint32_t *)aos, n, *soa);
     HIP_TRY(hipGetLastError());
     return RPX_OK;
 }
