@@ -690,7 +690,9 @@ rpx_status_t rpx_eval_child_rays(const rpx_scene_t *scene, int32_t material,
                                  rpx_rays_t *out, uint32_t *counts, void *stream);
 
 /* AoS <-> SoA: RayCollection.copy_as_array / from_array (ctracer.pyx:1048,
- * 1142) for the packed 188-byte ray_t and 668-byte gausslet_t layouts. */
+ * 1142) for the packed 188-byte ray_t and 668-byte gausslet_t layouts.  The
+ * AoS buffer is read or written by a kernel: device memory, or pinned host
+ * memory mapped for the device. */
 rpx_status_t rpx_pack_rays(const rpx_rays_t *soa, uint64_t n, void *aos,
                            int32_t gausslet, void *stream);
 rpx_status_t rpx_unpack_rays(const void *aos, uint64_t n, rpx_rays_t *soa,
