@@ -908,34 +908,35 @@ extern "C" rpx_status_t rpx_trace_gausslet(const rpx_scene_t *scene, int32_t fac
 // ------------------------------------------------------ AoS <-> SoA
 // Packed ray_t: 21 doubles (168 B) then 5 uint32 (20 B) = 188 B = 47 words.
 // gausslet_t appends 6 x 10 doubles = 480 B -> 167 words.
-// Both directions stage RPX_PACK_RAYS whole records per workgroup in LDS:
-// the AoS side moves as one contiguous run of words, and each SoA plane as
-// RPX_PACK_RAYS consecutive values (doubles as single 8-byte accesses).
-// The record stride (47 or 167 words) is odd, so lanes reading one field
-// of consecutive records hit distinct LDS banks.
-#define RPX_PACK_RAYS 64
+// Both directions stage pack_records(WPR) whole records per workgroup in
+// LDS: the AoS side moves as one contiguous run of words, and each SoA plane
+// as that many consecutive values (doubles as single 8-byte accesses).  The
+// record stride (47 or 167 words) is odd, so lanes reading one field of
+// consecutive records hit distinct LDS banks.  Gausslet records take 32 per
+// workgroup (21 KB of LDS) so that more workgroups fit a CU.
+constexpr int pack_records(int wpr) { return wpr > 47 ? 32 : 64; }
 
-template <int WPR>
+template <int WPR, int R = pack_records(WPR)>
 __global__ __launch_bounds__(256) void k_pack(rpx_rays_t soa, uint64_t n, uint32_t *aos) {
-    __shared__ uint32_t lds[RPX_PACK_RAYS * WPR];
-    const uint64_t r0 = (uint64_t)blockIdx.x * RPX_PACK_RAYS;
-    const int nr = (int)((n - r0) < RPX_PACK_RAYS ? (n - r0) : RPX_PACK_RAYS);
+    __shared__ uint32_t lds[R * WPR];
+    const uint64_t r0 = (uint64_t)blockIdx.x * R;
+    const int nr = (int)((n - r0) < R ? (n - r0) : R);
     const uint64_t c = soa.capacity;
-    for (int k = threadIdx.x; k < 21 * RPX_PACK_RAYS; k += 256) {
-        const int f = k / RPX_PACK_RAYS, r = k - f * RPX_PACK_RAYS;
+    for (int k = threadIdx.x; k < 21 * R; k += 256) {
+        const int f = k / R, r = k - f * R;
         if (r < nr) {
             const uint64_t v = __double_as_longlong(soa.f64[(uint64_t)f * c + r0 + r]);
             lds[r * WPR + 2 * f] = (uint32_t)v;
             lds[r * WPR + 2 * f + 1] = (uint32_t)(v >> 32);
         }
     }
-    for (int k = threadIdx.x; k < 5 * RPX_PACK_RAYS; k += 256) {
-        const int f = k / RPX_PACK_RAYS, r = k - f * RPX_PACK_RAYS;
+    for (int k = threadIdx.x; k < 5 * R; k += 256) {
+        const int f = k / R, r = k - f * R;
         if (r < nr) lds[r * WPR + 42 + f] = soa.u32[(uint64_t)f * c + r0 + r];
     }
     if constexpr (WPR > 47) {
-        for (int k = threadIdx.x; k < 60 * RPX_PACK_RAYS; k += 256) {
-            const int f = k / RPX_PACK_RAYS, r = k - f * RPX_PACK_RAYS;
+        for (int k = threadIdx.x; k < 60 * R; k += 256) {
+            const int f = k / R, r = k - f * R;
             if (r < nr) {
                 const uint64_t v = __double_as_longlong(soa.para[(uint64_t)f * c + r0 + r]);
                 lds[r * WPR + 47 + 2 * f] = (uint32_t)v;
@@ -948,29 +949,29 @@ __global__ __launch_bounds__(256) void k_pack(rpx_rays_t soa, uint64_t n, uint32
     for (int k = threadIdx.x; k < nr * WPR; k += 256) dst[k] = lds[k];
 }
 
-template <int WPR>
+template <int WPR, int R = pack_records(WPR)>
 __global__ __launch_bounds__(256) void k_unpack(const uint32_t *aos, uint64_t n, rpx_rays_t soa) {
-    __shared__ uint32_t lds[RPX_PACK_RAYS * WPR];
-    const uint64_t r0 = (uint64_t)blockIdx.x * RPX_PACK_RAYS;
-    const int nr = (int)((n - r0) < RPX_PACK_RAYS ? (n - r0) : RPX_PACK_RAYS);
+    __shared__ uint32_t lds[R * WPR];
+    const uint64_t r0 = (uint64_t)blockIdx.x * R;
+    const int nr = (int)((n - r0) < R ? (n - r0) : R);
     const uint32_t *src = aos + r0 * WPR;
     for (int k = threadIdx.x; k < nr * WPR; k += 256) lds[k] = src[k];
     __syncthreads();
     const uint64_t c = soa.capacity;
-    for (int k = threadIdx.x; k < 21 * RPX_PACK_RAYS; k += 256) {
-        const int f = k / RPX_PACK_RAYS, r = k - f * RPX_PACK_RAYS;
+    for (int k = threadIdx.x; k < 21 * R; k += 256) {
+        const int f = k / R, r = k - f * R;
         if (r < nr) {
             const uint64_t v = (uint64_t)lds[r * WPR + 2 * f] | ((uint64_t)lds[r * WPR + 2 * f + 1] << 32);
             soa.f64[(uint64_t)f * c + r0 + r] = __longlong_as_double((long long)v);
         }
     }
-    for (int k = threadIdx.x; k < 5 * RPX_PACK_RAYS; k += 256) {
-        const int f = k / RPX_PACK_RAYS, r = k - f * RPX_PACK_RAYS;
+    for (int k = threadIdx.x; k < 5 * R; k += 256) {
+        const int f = k / R, r = k - f * R;
         if (r < nr) soa.u32[(uint64_t)f * c + r0 + r] = lds[r * WPR + 42 + f];
     }
     if constexpr (WPR > 47) {
-        for (int k = threadIdx.x; k < 60 * RPX_PACK_RAYS; k += 256) {
-            const int f = k / RPX_PACK_RAYS, r = k - f * RPX_PACK_RAYS;
+        for (int k = threadIdx.x; k < 60 * R; k += 256) {
+            const int f = k / R, r = k - f * R;
             if (r < nr) {
                 const uint64_t v =
                     (uint64_t)lds[r * WPR + 47 + 2 * f] | ((uint64_t)lds[r * WPR + 47 + 2 * f + 1] << 32);
@@ -986,7 +987,8 @@ extern "C" rpx_status_t rpx_pack_rays(const rpx_rays_t *soa, uint64_t n, void *a
     if (st) return st;
     if (n == 0) return RPX_OK;
     if (!aos) return set_error(RPX_E_INVALID, "AoS buffer is null");
-    const uint64_t nb = (n + RPX_PACK_RAYS - 1) / RPX_PACK_RAYS;
+    const uint64_t recs = (uint64_t)pack_records(gausslet ? 167 : 47);
+    const uint64_t nb = (n + recs - 1) / recs;
     if (nb > 0x7fffffffull) return set_error(RPX_E_INVALID, "too many rays for one pack launch");
     if (gausslet)
         hipLaunchKernelGGL(k_pack<167>, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, *soa, n,
@@ -1004,7 +1006,8 @@ extern "C" rpx_status_t rpx_unpack_rays(const void *aos, uint64_t n, rpx_rays_t 
     if (st) return st;
     if (n == 0) return RPX_OK;
     if (!aos) return set_error(RPX_E_INVALID, "AoS buffer is null");
-    const uint64_t nb = (n + RPX_PACK_RAYS - 1) / RPX_PACK_RAYS;
+    const uint64_t recs = (uint64_t)pack_records(gausslet ? 167 : 47);
+    const uint64_t nb = (n + recs - 1) / recs;
     if (nb > 0x7fffffffull) return set_error(RPX_E_INVALID, "too many rays for one unpack launch");
     if (gausslet)
         hipLaunchKernelGGL(k_unpack<167>, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream,
