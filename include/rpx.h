@@ -781,6 +781,27 @@ rpx_status_t rpx_sum_gaussian_modes(const rpx_rays_t *rays, uint64_t n_rays,
                                     double time_ps, double *out,
                                     int32_t precision, void *stream);
 
+/* rpx_sum_gaussian_modes without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * per-gausslet invariants table and the chunk partial sums live in a
+ * file-scope device array of RPX_FIELD_SCRATCH_BYTES per GPU, held until the
+ * process ends; nothing is allocated or freed and no memset is issued.
+ * Arguments and results are those of rpx_sum_gaussian_modes (the same sums,
+ * in the same order, whenever its partial sums fit beside the table; else
+ * fewer gausslet chunks).  n_rays above rpx_field_noalloc_max_rays() returns
+ * RPX_E_CAPACITY.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (two streams, two threads) race and must be ordered by
+ * the caller. */
+#define RPX_FIELD_SCRATCH_BYTES (128ull << 20)
+uint64_t     rpx_field_noalloc_max_rays(void);
+rpx_status_t rpx_sum_gaussian_modes_noalloc(const rpx_rays_t *rays, uint64_t n_rays,
+                                            const double *modes,
+                                            const double *wavelengths, uint64_t n_wavelengths,
+                                            const double *points, uint64_t n_pts,
+                                            double time_ps, double *out,
+                                            int32_t precision, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,8 @@ MAT_NONE, MAT_OPAQUE, MAT_TRANSPARENT, MAT_PEC, MAT_PARTIAL, MAT_DIELECTRIC, MAT
 
 RPX_OK, RPX_E_INVALID, RPX_E_UNSUPPORTED, RPX_E_CAPACITY, RPX_E_HIP, RPX_E_NO_DEVICE = range(6)
 
+RPX_FIELD_SCRATCH_BYTES = 128 << 20   # static scratch of rpx_sum_gaussian_modes_noalloc
+
 FACE_NPARAM = 16
 MAX_DISP_COEFS = 18
 MAX_ZERNIKE_SLOTS = 96

@@ -89,6 +89,9 @@ _SIGNATURES = {
                                    c_void_p]),
     "rpx_sum_gaussian_modes": (c_i32, [P(_abi.Rays), c_u64, c_void_p, c_void_p, c_u64, c_void_p, c_u64, c_f64,
                                        c_void_p, c_i32, c_void_p]),
+    "rpx_field_noalloc_max_rays": (c_u64, []),
+    "rpx_sum_gaussian_modes_noalloc": (c_i32, [P(_abi.Rays), c_u64, c_void_p, c_void_p, c_u64, c_void_p, c_u64,
+                                               c_f64, c_void_p, c_i32, c_void_p]),
 }
 
 EXPORTED = sorted(_SIGNATURES)

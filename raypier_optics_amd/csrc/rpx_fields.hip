@@ -808,55 +808,73 @@ static uint64_t field_chunk(const void *kernel, uint64_t G, uint64_t M, uint64_t
     return ((tiles + c - 1) / c) * tile;
 }
 
+static rpx_status_t field_sum_checks(const rpx_rays_t *rays, uint64_t n_rays, const double *modes,
+                                     const double *points, uint64_t n_pts, double *out, int32_t precision) {
+    if (precision != 0 && precision != 1) return rpx_set_error(RPX_E_INVALID, "precision must be 0 or 1");
+    if (n_pts == 0) return RPX_OK;
+    if (!points || !out) return rpx_set_error(RPX_E_INVALID, "null point/output buffer");
+    if (n_rays == 0) return RPX_OK;
+    if (!rays || !rays->f64 || !rays->u32 || rays->capacity < n_rays || !modes)
+        return rpx_set_error(RPX_E_INVALID, "ray/mode buffers");
+    return RPX_OK;
+}
+
+// Gausslet chunks on grid y for this precision (see field_chunk).
+static uint64_t field_sum_chunk(uint64_t n_rays, uint64_t n_pts, int32_t precision) {
+    const void *kern = precision == 0 ? (const void *)k_field_sum : (const void *)k_field_sum_split;
+    return field_chunk(kern, n_rays, n_pts, precision == 0 ? FS_TILE : FS32_TILE);
+}
+
+// The field-sum launches: invariants into gp (RPX_GP * n_rays doubles), the
+// pair kernel over gausslet chunks of `chunk` (partial sums into `part`,
+// 6 * n_pts doubles per chunk, when there is more than one chunk), then the
+// chunk-ordered reduction into out.  n_rays > 0 and n_pts > 0.
+static hipError_t field_sum_launch(const rpx_rays_t *rays, uint64_t n_rays, const double *modes, WlTable wl,
+                                   const double *points, uint64_t n_pts, double time_ps, double *out,
+                                   int32_t precision, uint64_t chunk, double *gp, double *part,
+                                   hipStream_t stream) {
+    const double ct = 0.299792458 * time_ps;
+    hipLaunchKernelGGL(k_field_prep, dim3((unsigned)((n_rays + 255) / 256)), dim3(256), 0, stream, *rays, n_rays,
+                       modes, wl, ct, gp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint64_t n_chunks = (n_rays + chunk - 1) / chunk;
+    double *dst = n_chunks > 1 ? part : out;
+    const dim3 grid((unsigned)((n_pts + FS_BLOCK - 1) / FS_BLOCK), (unsigned)n_chunks);
+    if (precision == 0)
+        hipLaunchKernelGGL(k_field_sum, grid, dim3(FS_BLOCK), 0, stream, gp, n_rays, chunk, points, n_pts, dst);
+    else
+        hipLaunchKernelGGL(k_field_sum_split, grid, dim3(FS_BLOCK), 0, stream, gp, n_rays, chunk, points, n_pts,
+                           dst);
+    e = hipGetLastError();
+    if (e != hipSuccess || n_chunks == 1) return e;
+    hipLaunchKernelGGL(k_field_reduce, dim3((unsigned)((6 * n_pts + 255) / 256)), dim3(256), 0, stream,
+                       (const double *)part, (uint32_t)n_chunks, 6 * n_pts, out);
+    return hipGetLastError();
+}
+
 extern "C" rpx_status_t rpx_sum_gaussian_modes(const rpx_rays_t *rays, uint64_t n_rays, const double *modes,
                                                const double *wavelengths, uint64_t n_wavelengths,
                                                const double *points, uint64_t n_pts, double time_ps, double *out,
                                                int32_t precision, void *stream_) {
     hipStream_t stream = (hipStream_t)stream_;
-    if (precision != 0 && precision != 1) return rpx_set_error(RPX_E_INVALID, "precision must be 0 or 1");
+    if (rpx_status_t st = field_sum_checks(rays, n_rays, modes, points, n_pts, out, precision)) return st;
     if (n_pts == 0) return RPX_OK;
-    if (!points || !out) return rpx_set_error(RPX_E_INVALID, "null point/output buffer");
     if (n_rays == 0) {
         RPX_HIP_TRY(hipMemsetAsync(out, 0, sizeof(double) * 6 * n_pts, stream));
         return RPX_OK;
     }
-    if (!rays || !rays->f64 || !rays->u32 || rays->capacity < n_rays || !modes)
-        return rpx_set_error(RPX_E_INVALID, "ray/mode buffers");
     WlTable wl;
     wl.wl = wavelengths;
     wl.n = wavelengths ? n_wavelengths : 0;
-    double *gp = nullptr;
+    const uint64_t chunk = field_sum_chunk(n_rays, n_pts, precision);
+    const uint64_t n_chunks = (n_rays + chunk - 1) / chunk;
+    double *gp = nullptr, *part = nullptr;
     RPX_HIP_TRY(hipMallocAsync((void **)&gp, sizeof(double) * RPX_GP * n_rays, stream));
-    const double ct = 0.299792458 * time_ps;
-    hipLaunchKernelGGL(k_field_prep, dim3((unsigned)((n_rays + 255) / 256)), dim3(256), 0, stream, *rays, n_rays,
-                       modes, wl, ct, gp);
-    hipError_t e = hipGetLastError();
-    double *part = nullptr;
-    if (e == hipSuccess) {
-        // gausslet chunks on grid y, partial sums reduced in chunk order
-        const void *kern = precision == 0 ? (const void *)k_field_sum : (const void *)k_field_sum_split;
-        const uint64_t chunk = field_chunk(kern, n_rays, n_pts, precision == 0 ? FS_TILE : FS32_TILE);
-        const uint64_t n_chunks = (n_rays + chunk - 1) / chunk;
-        double *dst = out;
-        if (n_chunks > 1) {
-            e = hipMallocAsync((void **)&part, sizeof(double) * 6 * n_pts * n_chunks, stream);
-            dst = part;
-        }
-        const dim3 grid((unsigned)((n_pts + FS_BLOCK - 1) / FS_BLOCK), (unsigned)n_chunks);
-        if (e == hipSuccess) {
-            if (precision == 0)
-                hipLaunchKernelGGL(k_field_sum, grid, dim3(FS_BLOCK), 0, stream, gp, n_rays, chunk, points, n_pts, dst);
-            else
-                hipLaunchKernelGGL(k_field_sum_split, grid, dim3(FS_BLOCK), 0, stream, gp, n_rays, chunk, points,
-                                   n_pts, dst);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess && n_chunks > 1) {
-            hipLaunchKernelGGL(k_field_reduce, dim3((unsigned)((6 * n_pts + 255) / 256)), dim3(256), 0, stream,
-                               (const double *)part, (uint32_t)n_chunks, 6 * n_pts, out);
-            e = hipGetLastError();
-        }
-    }
+    hipError_t e = hipSuccess;
+    if (n_chunks > 1) e = hipMallocAsync((void **)&part, sizeof(double) * 6 * n_pts * n_chunks, stream);
+    if (e == hipSuccess)
+        e = field_sum_launch(rays, n_rays, modes, wl, points, n_pts, time_ps, out, precision, chunk, gp, part, stream);
     hipError_t e2 = hipFreeAsync(gp, stream);
     if (part) {
         hipError_t e3 = hipFreeAsync(part, stream);
@@ -864,5 +882,99 @@ extern "C" rpx_status_t rpx_sum_gaussian_modes(const rpx_rays_t *rays, uint64_t 
     }
     if (e != hipSuccess) return rpx_set_error(RPX_E_HIP, std::string("field sum launch: ") + hipGetErrorString(e));
     if (e2 != hipSuccess) return rpx_set_error(RPX_E_HIP, std::string("field sum free: ") + hipGetErrorString(e2));
+    return RPX_OK;
+}
+
+// --------------------------------------------- field sum without allocation
+// rpx_sum_gaussian_modes_noalloc keeps the invariants table and the chunk
+// partial sums in this file-scope device array instead of stream-ordered
+// allocations.  Each device holds its own copy until the process ends; host
+// code uses the per-device address from hipGetSymbolAddress, never the
+// array's name (which is a host-side stand-in).
+static __device__ __align__(256) double scratch[RPX_FIELD_SCRATCH_BYTES / sizeof(double)];
+
+#define RPX_MAX_DEVICES 64
+
+// Device address of `scratch` on the current device, looked up once per
+// device; nullptr (with the error recorded) if the lookup fails.
+static double *field_scratch(rpx_status_t *st) {
+    static std::mutex mu;
+    static double *ptrs[RPX_MAX_DEVICES] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        *st = rpx_set_error(RPX_E_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    if (dev < 0 || dev >= RPX_MAX_DEVICES) {
+        *st = rpx_set_error(RPX_E_INVALID, "device ordinal beyond the no-alloc scratch table");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> g(mu);
+    if (!ptrs[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            *st = rpx_set_error(RPX_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        ptrs[dev] = (double *)p;
+    }
+    return ptrs[dev];
+}
+
+// out[i] = 0 for i < n (a kernel rather than a memset node, so that the call
+// can be captured into a graph).
+__global__ void k_fill_zero(double *out, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = 0.0;
+}
+
+extern "C" uint64_t rpx_field_noalloc_max_rays(void) {
+    return RPX_FIELD_SCRATCH_BYTES / (sizeof(double) * RPX_GP);
+}
+
+extern "C" rpx_status_t rpx_sum_gaussian_modes_noalloc(const rpx_rays_t *rays, uint64_t n_rays, const double *modes,
+                                                       const double *wavelengths, uint64_t n_wavelengths,
+                                                       const double *points, uint64_t n_pts, double time_ps,
+                                                       double *out, int32_t precision, void *stream_) {
+    hipStream_t stream = (hipStream_t)stream_;
+    if (rpx_status_t st = field_sum_checks(rays, n_rays, modes, points, n_pts, out, precision)) return st;
+    if (n_pts == 0) return RPX_OK;
+    if (n_rays > rpx_field_noalloc_max_rays())
+        return rpx_set_error(RPX_E_CAPACITY, "rpx_sum_gaussian_modes_noalloc: " + std::to_string(n_rays) +
+                                                 " gausslets exceed the scratch limit of " +
+                                                 std::to_string(rpx_field_noalloc_max_rays()));
+    rpx_status_t st = RPX_OK;
+    double *base = field_scratch(&st);
+    if (!base) return st;
+    if (n_rays == 0) {
+        hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((6 * n_pts + 255) / 256)), dim3(256), 0, stream, out,
+                           6 * n_pts);
+        RPX_HIP_TRY(hipGetLastError());
+        return RPX_OK;
+    }
+    WlTable wl;
+    wl.wl = wavelengths;
+    wl.n = wavelengths ? n_wavelengths : 0;
+    // the invariants table first, the partial sums after it; the chunk count
+    // of the allocating entry point when its partial sums fit (then the two
+    // entry points sum in the same order), else as many chunks as fit, else
+    // one chunk summed straight into out
+    const uint64_t gp_doubles = ((RPX_GP * n_rays + 31) / 32) * 32;
+    const uint64_t left = RPX_FIELD_SCRATCH_BYTES / sizeof(double) - gp_doubles;
+    const uint64_t tile = precision == 0 ? FS_TILE : FS32_TILE;
+    uint64_t chunk = field_sum_chunk(n_rays, n_pts, precision);
+    uint64_t n_chunks = (n_rays + chunk - 1) / chunk;
+    if (n_chunks > 1 && 6 * n_pts * n_chunks > left) {
+        const uint64_t fit = left / (6 * n_pts);
+        const uint64_t tiles = (n_rays + tile - 1) / tile;
+        chunk = fit < 2 ? n_rays : ((tiles + fit - 1) / fit) * tile;
+        n_chunks = (n_rays + chunk - 1) / chunk;
+    }
+    double *part = n_chunks > 1 ? base + gp_doubles : nullptr;
+    hipError_t e =
+        field_sum_launch(rays, n_rays, modes, wl, points, n_pts, time_ps, out, precision, chunk, base, part, stream);
+    if (e != hipSuccess) return rpx_set_error(RPX_E_HIP, std::string("field sum launch: ") + hipGetErrorString(e));
     return RPX_OK;
 }
