@@ -490,6 +490,17 @@ rpx_status_t rpx_config_parabasal_rays(rpx_rays_t *gausslets, uint64_t n, const 
 rpx_status_t rpx_project_to_plane(rpx_rays_t *rays, uint64_t n, const double *origin,
                                   const double *direction, void *stream);
 
+/* GaussletCollection.lagrange_invariant (ctracer.pyx:1347-1389): per
+ * gausslet, 1000 sqrt(v/6) / wavelength, where v sums the squared
+ * differences h_i.u_j - h_j.u_i over the parabasal pairs (0,5) (1,2) (3,4)
+ * (1,4) (0,3) (2,5), h being a parabasal ray's offset from the base origin
+ * and u its direction, both in the frame (E/|E|, E/|E| x direction).
+ * out: device float64[n].  Workspace: 8 B.  RPX_E_INVALID if a
+ * wavelength_idx is outside the table. */
+rpx_status_t rpx_lagrange_invariant(const rpx_rays_t *gausslets, uint64_t n, const double *wavelengths,
+                                    uint64_t n_wavelengths, double *out, void *workspace,
+                                    uint64_t workspace_bytes, void *stream);
+
 /* ---------------------------------------------------- FP32-split form
  * A reduced-precision ray generation with no counterpart in the reference
  * (which traces in float64 only).  Ray state is stored in float32 -- vectors,

@@ -8,12 +8,10 @@ cpu_baseline leg.  The product library never links or calls it.
 Functions take the scene tables compiled by raypier_optics_amd._scene and
 numpy arrays of the packed ray_dtype / gausslet_dtype records.
 
-Parity unpinned (checked only against this restatement): the libm boundary,
-the phase conventions of the coated r and t, which rays the asphere Newton
-abandons and the DistortionFace iteration count, the Zernike derivative values
-for n >= 5, the assumed left-to-right evaluation order, Sellmeier formula 5;
-in oracle/sources.py and oracle/fields.py, numpy's reduction order where it is
-mirrored and `exp` in the source amplitude weights.
+Parity unpinned (checked only against this restatement): the libm
+boundary, the DistortionFace result for rays that have not converged in its
+20 fixed-point steps, and the Zernike derivative values for n >= 5; in
+oracle/sources.py and oracle/fields.py, `exp` in the source amplitude weights.
 """
 import ctypes
 import os
@@ -61,6 +59,8 @@ def load():
         lib.oracle_config_parabasal_rays.restype = i32
         lib.oracle_config_parabasal_rays.argtypes = [vp, u64, vp, u64, f64, f64]
         lib.oracle_project_to_plane.argtypes = [vp, u64, vp, vp]
+        lib.oracle_lagrange_invariant.restype = i32
+        lib.oracle_lagrange_invariant.argtypes = [vp, u64, vp, u64, vp]
         lib.oracle_calc_mode_curvature.argtypes = [vp] * 5 + [u64] + [vp] * 6
         lib.oracle_apply_mode_curvature.argtypes = [vp, u64, vp, vp, vp]
         lib.oracle_build_interaction_matrix.restype = u64
@@ -174,6 +174,17 @@ def project_to_plane(gausslets, origin, direction):
     d = np.ascontiguousarray(direction, dtype=np.float64)
     lib.oracle_project_to_plane(_p(g), len(g), _p(o), _p(d))
     return g
+
+
+def lagrange_invariant(gausslets, wavelengths):
+    """GaussletCollection.lagrange_invariant of a gausslet_dtype array."""
+    lib = load()
+    g = np.ascontiguousarray(gausslets)
+    wl = np.ascontiguousarray(wavelengths, dtype=np.float64)
+    out = np.zeros(len(g))
+    if lib.oracle_lagrange_invariant(_p(g), len(g), _p(wl), len(wl), _p(out)):
+        raise IndexError("wavelength_idx outside the wavelength list")
+    return out
 
 
 def calc_mode_curvature(rx, ry, dx, dy, dx2, dy2, dxdy):

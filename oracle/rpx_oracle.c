@@ -17,12 +17,10 @@
  *
  * Parity unpinned (no known answer, closed form or invariant checks these;
  * they rest on this restatement alone): the libm boundary (pow, sin, cos,
- * atan2, exp, hypot, csqrt, cexp; the device is compared to 1e-9); the phase
- * conventions of the coated r and t (their magnitudes are pinned); which
- * rays the asphere Newton abandons and the DistortionFace iteration count;
- * the Zernike derivative values for n >= 5 (the reference's slot-sharing
- * quirk); the left-to-right evaluation order assumed for compound
- * expressions; Sellmeier formula 5.
+ * atan2, exp, hypot, csqrt, cexp; the device is compared to 1e-9); the
+ * DistortionFace result for rays whose fixed-point iteration has not
+ * converged in its 20 steps; the Zernike derivative values for n >= 5 (the
+ * reference's slot-sharing quirk).
  */
 #include <complex.h>
 #include <math.h>
@@ -1342,6 +1340,38 @@ void oracle_project_to_plane(ogausslet_t *gs, uint64_t n, const double *origin, 
             para->origin = addvv_(para->origin, multvs_(para->direction, a));
         }
     }
+}
+
+/* GaussletCollection.lagrange_invariant (ctracer.pyx:1347-1389).  Returns -1
+ * if a wavelength_idx is outside the table. */
+int oracle_lagrange_invariant(const ogausslet_t *gs, uint64_t n, const double *wavelen, uint64_t n_wl, double *out) {
+    for (uint64_t k = 0; k < n; k++) {
+        const ogausslet_t *g = gs + k;
+        const oray_t *r = &g->base_ray;
+        vec_t h[6], u[6];
+        vec_t axis1 = norm_(r->E_vector);
+        vec_t axis2 = cross_(axis1, r->direction);
+        for (int i = 0; i < 6; i++) {
+            const opara_t *p = &g->para[i];
+            vec_t o = subvv_(p->origin, r->origin);
+            h[i].x = dotprod_(o, axis1);
+            h[i].y = dotprod_(o, axis2);
+            h[i].z = 0.0;
+            u[i].x = dotprod_(p->direction, axis1);
+            u[i].y = dotprod_(p->direction, axis2);
+            u[i].z = 0.0;
+        }
+        double v = 0, t;
+        t = dotprod_(h[0], u[5]) - dotprod_(h[5], u[0]); v += t * t;
+        t = dotprod_(h[1], u[2]) - dotprod_(h[2], u[1]); v += t * t;
+        t = dotprod_(h[3], u[4]) - dotprod_(h[4], u[3]); v += t * t;
+        t = dotprod_(h[1], u[4]) - dotprod_(h[4], u[1]); v += t * t;
+        t = dotprod_(h[0], u[3]) - dotprod_(h[3], u[0]); v += t * t;
+        t = dotprod_(h[2], u[5]) - dotprod_(h[5], u[2]); v += t * t;
+        if (r->wavelength_idx >= n_wl) return -1;
+        out[k] = 1000 * sqrt(v / 6) / wavelen[r->wavelength_idx];
+    }
+    return 0;
 }
 
 /* trace_segment_c (ctracer.pyx:2062-2118) / trace_one_face_segment_c (:2121-2170).

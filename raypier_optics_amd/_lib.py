@@ -65,6 +65,7 @@ _SIGNATURES = {
     "rpx_config_parabasal_rays": (c_i32, [P(_abi.Rays), c_u64, c_void_p, c_u64, c_f64, c_f64, P(c_f64), c_void_p,
                                           c_u64, c_void_p]),
     "rpx_project_to_plane": (c_i32, [P(_abi.Rays), c_u64, P(c_f64), P(c_f64), c_void_p]),
+    "rpx_lagrange_invariant": (c_i32, [P(_abi.Rays), c_u64, c_void_p, c_u64, c_void_p, c_void_p, c_u64, c_void_p]),
     "rpx_project_to_sphere": (c_i32, [P(_abi.Rays), c_u64, P(c_f64), c_f64, P(_abi.Rays), P(c_u64), c_void_p, c_u64,
                                       c_void_p]),
     "rpx_ray_field_modes": (c_i32, [P(_abi.Rays), c_u64, c_void_p, c_f64, P(_abi.Rays), c_void_p, c_void_p,

@@ -11,6 +11,8 @@
 //    (ctracer.pyx:1430-1481).
 //  * rpx_project_to_plane -- GaussletCollection.project_to_plane
 //    (ctracer.pyx:1391-1420).
+//  * rpx_lagrange_invariant -- GaussletCollection.lagrange_invariant
+//    (ctracer.pyx:1347-1389).
 //
 // Every expression follows the reference's evaluation order (numpy
 // elementwise, or the Cython vector helpers) and is compiled without
@@ -434,5 +436,76 @@ extern "C" rpx_status_t rpx_project_to_plane(rpx_rays_t *rays, uint64_t n, const
     hipLaunchKernelGGL(k_project_to_plane, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *rays, n, o, d);
     RPX_HIP_TRY(hipGetLastError());
     RPX_HIP_TRY(hipStreamSynchronize(stream));
+    return RPX_OK;
+}
+
+// ------------------------------------------------------ Lagrange invariant
+// One gausslet per lane.  Each parabasal ray's offset from the base origin
+// and its direction are taken in the frame (a1, a2) = (E / |E|, a1 x d);
+// the six pairs (0,5) (1,2) (3,4) (1,4) (0,3) (2,5) add (h_i.u_j - h_j.u_i)^2
+// in that order, and the result is 1000 sqrt(v / 6) / wavelength.
+__global__ __launch_bounds__(256) void k_lagrange_invariant(rpx_rays_t g, uint64_t n, const double *wl,
+                                                            uint64_t n_wl, double *out, unsigned long long *err) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *f = g.f64;
+    const uint64_t c = g.capacity;
+    const uint32_t wi = g.u32[RPX_WL_IDX * c + i];
+    if (wi >= n_wl) {
+        atomicAdd(err, 1ull);
+        out[i] = 0.0;
+        return;
+    }
+    const V3 a1 = norm3(v3(f[RPX_EX * c + i], f[RPX_EY * c + i], f[RPX_EZ * c + i]));
+    const V3 a2 = cross3(a1, v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]));
+    const V3 ro = v3(f[RPX_OX * c + i], f[RPX_OY * c + i], f[RPX_OZ * c + i]);
+    double hx[RPX_N_PARA], hy[RPX_N_PARA], ux[RPX_N_PARA], uy[RPX_N_PARA];
+#pragma unroll
+    for (int j = 0; j < RPX_N_PARA; ++j) {
+        const uint64_t b = (uint64_t)j * RPX_PARA_FIELDS;
+        const V3 o = sub(v3(g.para[(b + 0) * c + i], g.para[(b + 1) * c + i], g.para[(b + 2) * c + i]), ro);
+        const V3 d = v3(g.para[(b + 3) * c + i], g.para[(b + 4) * c + i], g.para[(b + 5) * c + i]);
+        hx[j] = dot(o, a1);
+        hy[j] = dot(o, a2);
+        ux[j] = dot(d, a1);
+        uy[j] = dot(d, a2);
+    }
+    // dot products of the planar (z = 0) vectors, summed as x + y + 0
+    auto term = [&](int a, int b) {
+        const double t = ((hx[a] * ux[b] + hy[a] * uy[b]) + 0.0) - ((hx[b] * ux[a] + hy[b] * uy[a]) + 0.0);
+        return t * t;
+    };
+    double v = 0.0;
+    v += term(0, 5);
+    v += term(1, 2);
+    v += term(3, 4);
+    v += term(1, 4);
+    v += term(0, 3);
+    v += term(2, 5);
+    out[i] = 1000.0 * sqrt(v / 6.0) / wl[wi];
+}
+
+extern "C" rpx_status_t rpx_lagrange_invariant(const rpx_rays_t *gausslets, uint64_t n, const double *wavelengths,
+                                               uint64_t n_wavelengths, double *out, void *workspace,
+                                               uint64_t workspace_bytes, void *stream_) {
+    if (!gausslets) return rpx_set_error(RPX_E_INVALID, "gausslets are null");
+    if (gausslets->capacity < n) return rpx_set_error(RPX_E_INVALID, "gausslets: capacity below count");
+    if (n == 0) return RPX_OK;
+    if (!gausslets->f64 || !gausslets->u32 || !gausslets->para)
+        return rpx_set_error(RPX_E_INVALID, "gausslets have null fields");
+    if (!out) return rpx_set_error(RPX_E_INVALID, "output is null");
+    if (!wavelengths && n_wavelengths) return rpx_set_error(RPX_E_INVALID, "wavelengths are null");
+    if (!workspace || workspace_bytes < 8) return rpx_set_error(RPX_E_INVALID, "workspace too small");
+    hipStream_t stream = (hipStream_t)stream_;
+    unsigned long long *err = (unsigned long long *)workspace;
+    RPX_HIP_TRY(hipMemsetAsync(err, 0, sizeof(*err), stream));
+    hipLaunchKernelGGL(k_lagrange_invariant, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, *gausslets, n,
+                       wavelengths, n_wavelengths, out, err);
+    RPX_HIP_TRY(hipGetLastError());
+    unsigned long long bad = 0;
+    RPX_HIP_TRY(hipMemcpyAsync(&bad, err, sizeof(bad), hipMemcpyDeviceToHost, stream));
+    RPX_HIP_TRY(hipStreamSynchronize(stream));
+    if (bad) return rpx_set_error(RPX_E_INVALID, std::to_string(bad) + " gausslets have a wavelength_idx outside "
+                                                 "the wavelength list");
     return RPX_OK;
 }

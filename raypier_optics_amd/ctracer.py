@@ -797,6 +797,25 @@ class GaussletCollection(_Collection):
         check(lib.rpx_project_to_plane(ctypes.byref(s), self._n, o, d, stream_handle()))
         self._device_changed()
 
+    @property
+    def lagrange_invariant(self):
+        """Per-gausslet Lagrange invariant of the parabasal ray set,
+        1000 sqrt(v / 6) / wavelength (ctracer.pyx:1347-1389), computed on the
+        GPU; a float64 numpy array."""
+        torch = require_gpu()
+        lib = load()
+        n = self._n
+        if n == 0:
+            return np.empty((0,), dtype=np.float64)
+        wl = self._device_wavelengths()
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+        ws = torch.empty(8, dtype=torch.uint8, device="cuda")
+        s = self._ensure_device().struct()
+        check(lib.rpx_lagrange_invariant(ctypes.byref(s), n, ctypes.c_void_p(wl.data_ptr()), wl.numel(),
+                                         ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                                         stream_handle()))
+        return out.cpu().numpy()
+
     def config_parabasal_rays(self, wavelength_list, radius, working_dist):
         """Place the parabasal rays of a circular Gaussian mode of 1/e^2 radius
         `radius` (mm) with its waist `working_dist` from the base origin

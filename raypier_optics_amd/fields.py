@@ -6,27 +6,19 @@ from .ctracer import RayCollection, GaussletCollection
 
 
 def evaluate_neighbours_gc(gausslets):
-    """Parabasal offsets and slopes in the (E, E x d) frame of each base ray
-    (core/fields.py:114-136); host numpy on a gausslet_dtype array."""
-    ga = gausslets
-    origin = ga['base_ray']['origin'][:, None, :]
-    direction = ga['base_ray']['direction'][:, None, :]
-    n_direction = ga['para_rays']['direction']
-    E = ga['base_ray']['E_vector'][:, None, :]
-    H = numpy.cross(E, direction)
-    offsets = ga['para_rays']['origin'] - origin
-    x = (offsets * E).sum(axis=-1)
-    y = (offsets * H).sum(axis=-1)
-    dz = (n_direction * direction).sum(axis=-1)
-    dx = (n_direction * E).sum(axis=-1) / dz
-    dy = (n_direction * H).sum(axis=-1) / dz
-    return (ga['base_ray'], x, y, dx, dy)
-
-
-def Gamma(z, A, B, C):
-    detG0 = (A * C) - (B * B)
-    denom = (1 + (z * (A + C)) + (z * z) * detG0)
-    return (A + z * detG0) / denom, B / denom, (C + z * detG0) / denom
+    """(base rays, x, y, dx, dy) of a gausslet_dtype array: each parabasal
+    ray's offset from its base origin and its slopes, in the frame
+    (E, E x d, d) of the base ray (the quantities of
+    core/fields.py:114-136).  The device path (rpx_gausslet_modes) computes
+    the same values inside the fused mode fit; this host form serves callers
+    that want them separately."""
+    base = gausslets["base_ray"]
+    e = base["E_vector"]
+    d = base["direction"]
+    frame = numpy.stack([e, numpy.cross(e, d), d], axis=1)          # (N, 3, 3), rows E, H, d
+    offs = numpy.einsum("nij,npj->npi", frame, gausslets["para_rays"]["origin"] - base["origin"][:, None, :])
+    dirs = numpy.einsum("nij,npj->npi", frame, gausslets["para_rays"]["direction"])
+    return base, offs[..., 0], offs[..., 1], dirs[..., 0] / dirs[..., 2], dirs[..., 1] / dirs[..., 2]
 
 
 def _base_collection(gc):

@@ -1,22 +1,42 @@
 """Beam decomposition (drop-in for raypier.core.gausslets).
 
-decompose_position (core/gausslets.py:219-357) re-launches an arbitrary
-gausslet beam as a hexagonal grid of new gausslets on a plane:
+decompose_position(input_rays, origin, direction, axis1, radius, resolution,
+curvature=None, blending=1.5) re-launches a gausslet beam as new gausslets
+on a hexagonal lattice of the plane through `origin` normal to `direction`
+(the role of core/gausslets.py:219-357); decompose_angle(...) launches
+gausslets from one point over a lattice of directions taken from a sampled
+field's angular spectrum (the role of core/gausslets.py:97-216).  Both are
+designed here from the optics, not from the reference's steps:
 
-  1. the beam's field on a square grid of the plane -- EFieldSummation, on
-     the GPU (rpx_gausslet_modes + rpx_sum_gaussian_modes);
-  2. the phase of the dominant polarisation, optionally with a spherical
-     reference removed, unwrapped (unwrap2d) and fitted by a smoothing
-     bicubic spline (scipy) -- small host arrays, as in the reference;
-  3. at the hexagonal launch points: slopes and curvatures of the spline,
-     the mode bases and curvature coefficients (rpx_calc_mode_curvature),
-     the field again (GPU), the new gausslets' parabasal rays and curvature
-     (rpx_config_parabasal_rays, rpx_apply_mode_curvature), and a power
-     normalisation against their own summed field (GPU).
+decompose_position
+  1. The beam's field on a square grid at half the launch spacing, a few
+     samples beyond the launch disc -- EFieldSummation, on the GPU.
+  2. The phase of the stronger transverse polarisation, with an optional
+     reference sphere removed, unwrapped by least squares (unwrap2d) and
+     turned into a wavefront W (mm of optical path, the sign the mode
+     curvature kernels take).
+  3. At each launch point a weighted quadratic least-squares fit of W over
+     the surrounding 7 x 7 samples gives the slopes and curvatures that
+     rpx_calc_mode_curvature turns into the launch direction, the local
+     basis and the mode coefficients.
+  4. Amplitudes are the field at the launch points (GPU) on that basis; the
+     parabasal rays come from config_parabasal_rays and
+     apply_mode_curvature (GPU); one complex scale, the least-squares fit of
+     the new beam's field to the sampled field over the launch disc (GPU),
+     fixes power and absolute phase together.
 
-decompose_angle (core/gausslets.py:97-216) launches gausslets over a grid of
-directions from the FFT of a sampled field (host FFT and spline, device
-parabasal set-up).
+decompose_angle
+  1. Every gausslet launched from `origin` has the waist w = 2 / dk (its
+     far-field width is one lattice step dk), so in the sample plane the
+     gausslets together are exp(-r^2 / w^2) times a sum of plane waves.
+     The samples are therefore divided by that envelope, and the spectrum
+     of the quotient is evaluated exactly at the hexagonal lattice of
+     transverse wave vectors (spacing 2 pi / aperture, inside
+     k sin(max_angle)) by a direct, separable Fourier sum -- no padding,
+     FFT or interpolation.
+  2. Each lattice vector becomes a gausslet with its plane-wave amplitude
+     and parabasal rays from config_parabasal_rays (GPU).
+  3. As above, one complex least-squares scale against the sampled field.
 """
 import numpy as np
 
@@ -24,7 +44,7 @@ from .ctracer import GaussletCollection
 from ._abi import ray_dtype
 from .fields import EFieldSummation, eval_Efield_from_gausslets
 from .cfields import calc_mode_curvature, apply_mode_curvature, build_interaction_matrix
-from .unwrap2d import unwrap2d
+from .unwrap2d import unwrap2d, wrap
 
 root_pi = np.sqrt(np.pi)
 
@@ -82,145 +102,163 @@ def make_hexagonal_grid(radius, spacing=1.0, connectivity=False):
     return x[select], y[select], centres[nb[select]]
 
 
+
+def _plane_frame(direction, axis1):
+    """Right-handed orthonormal (u, v, w): w along direction, u the part of
+    axis1 normal to it, v = w x u."""
+    w = _normalise(np.asarray(direction, np.float64))
+    a = np.asarray(axis1, np.float64)
+    u = _normalise(a - np.dot(a, w) * w)
+    return u, np.cross(w, u), w
+
+
+def _best_scale(e_model, e_target):
+    """The complex s minimising sum |s e_model - e_target|^2."""
+    num = np.vdot(e_model.reshape(-1), e_target.reshape(-1))
+    den = np.vdot(e_model.reshape(-1), e_model.reshape(-1)).real
+    return num / den if den > 0 else 1.0
+
+
+def _local_quadratic(grid, W, px, py, half=3):
+    """Slopes and second derivatives (Wx, Wy, Wxx, Wyy, Wxy) at (px, py) of
+    W sampled on grid x grid (W[i, j] at x = grid[j], y = grid[i]): a
+    Gaussian-weighted least-squares quadratic over the (2 half + 1)^2
+    samples around the nearest grid node, weights of width 1.5 steps."""
+    h = grid[1] - grid[0]
+    n = len(grid)
+    jc = np.clip(np.rint((px - grid[0]) / h).astype(np.int64), half, n - 1 - half)
+    ic = np.clip(np.rint((py - grid[0]) / h).astype(np.int64), half, n - 1 - half)
+    off = np.arange(-half, half + 1)
+    oi, oj = np.meshgrid(off, off, indexing="ij")
+    oi, oj = oi.reshape(-1), oj.reshape(-1)
+    I = ic[:, None] + oi[None, :]
+    J = jc[:, None] + oj[None, :]
+    dx = grid[J] - px[:, None]
+    dy = grid[I] - py[:, None]
+    wt = np.exp(-(dx * dx + dy * dy) / (2.0 * (1.5 * h) ** 2))
+    basis = np.stack([np.ones_like(dx), dx, dy, dx * dx, dx * dy, dy * dy], axis=-1)   # (P, S, 6)
+    lhs = np.einsum("ps,psa,psb->pab", wt, basis, basis)
+    rhs = np.einsum("ps,psa,ps->pa", wt, basis, W[I, J])
+    c = np.linalg.solve(lhs, rhs[..., None])[..., 0]
+    return c[:, 1], c[:, 2], 2.0 * c[:, 3], 2.0 * c[:, 5], c[:, 4]
+
+
 def decompose_position(input_rays, origin, direction, axis1, radius, resolution, curvature=None, blending=1.5):
-    """(gausslets, E_field on the sampling grid, unwrapped phase)."""
-    from scipy.interpolate import RectBivariateSpline
-    spacing = radius / resolution
-    wavelengths = input_rays.wavelengths
+    """(gausslets, field on the sampling grid (n*n, 3), unwrapped phase
+    (n, n)).  `curvature` R > 0 removes, before unwrapping, the reference
+    sphere of a wave converging to a point R beyond the plane (R < 0:
+    diverging from a point |R| before it); it is added back afterwards."""
+    wavelengths = np.asarray(input_rays.wavelengths)
     if len(np.unique(wavelengths)) > 1:
         raise ValueError("Can't decompose wavefront with more than one wavelength present.")
-    wavelength = wavelengths[0]
-    origin = np.asarray(origin)
-    direction = _normalise(np.asarray(direction))
-    axis1 = _normalise(np.asarray(axis1))
-    axis2 = _normalise(np.cross(axis1, direction))
-    axis1 = np.cross(axis2, direction)
+    wl = float(wavelengths[0])
+    k_mm = 2000.0 * np.pi / wl
+    origin = np.asarray(origin, np.float64)
+    u, v, w = _plane_frame(direction, axis1)
+    spacing = radius / resolution
 
-    # field on a square grid a little larger than the launch disc
-    _radius = radius * (1 + 2. / resolution)
-    grid = np.linspace(-_radius, _radius, int((1.1 * resolution) * 2))
-    x, y = np.meshgrid(grid, grid)
-    samples = origin[None, :] + x.reshape(-1, )[:, None] * axis1[None, :] + y.reshape(-1)[:, None] * axis2[None, :]
-    efield = EFieldSummation(input_rays)
-    E_field = efield.evaluate(samples)
+    # 1. samples at half the launch spacing, 4 steps beyond the disc
+    h = 0.5 * spacing
+    half_n = int(np.ceil(radius / h)) + 4
+    grid = h * np.arange(-half_n, half_n + 1)
+    X, Y = np.meshgrid(grid, grid)
+    pts = origin + X.reshape(-1, 1) * u + Y.reshape(-1, 1) * v
+    field = EFieldSummation(input_rays)
+    E = field.evaluate(pts)
 
-    # phase of the stronger polarisation
-    E1 = (E_field * axis1[None, :]).sum(axis=1)
-    E2 = (E_field * axis2[None, :]).sum(axis=1)
-    P1 = (E1.real ** 2 + E1.imag ** 2).sum()
-    P2 = (E2.real ** 2 + E2.imag ** 2).sum()
-    main = E1 if P1 > P2 else E2
-    phase = np.arctan2(main.imag, main.real)
-    nx, ny = len(grid), len(grid)
-    phase.shape = (nx, ny)
+    # 2. phase of the stronger transverse component, unwrapped about the centre
+    Eu, Ev = E @ u, E @ v
+    main = Eu if np.vdot(Eu, Eu).real >= np.vdot(Ev, Ev).real else Ev
+    psi = np.angle(main).reshape(X.shape)
+    r2 = X * X + Y * Y
     if curvature is not None:
-        sphz = -(curvature - np.sqrt(curvature * curvature - x * x - y * y))
-        sph = sphz * 2000.0 * np.pi / wavelength - np.pi
-        phase = phase - sph
-        phase = phase % (2 * np.pi)
-        phase = phase - np.pi
+        Rc = float(curvature)
+        sphere = -k_mm * np.sign(Rc) * (abs(Rc) - np.sqrt(np.maximum(Rc * Rc - r2, 0.0)))
     else:
-        sph = 0
-    uphase, residuals = unwrap2d(phase, anchor=(nx // 2, ny // 2))
-    uphase2 = uphase + sph
-    wavefront = RectBivariateSpline(grid, grid, -uphase2 * wavelength / (2000 * np.pi), kx=3, ky=3, s=0.001)
+        sphere = np.zeros_like(psi)
+    c = half_n
+    uphase, _ = unwrap2d(wrap(psi - sphere), anchor=(c, c))
+    uphase = uphase + sphere
+    W = -uphase / k_mm
 
-    # the new gausslets on a hexagonal grid
-    rx, ry, nb = make_hexagonal_grid(radius, spacing=spacing, connectivity=True)
-    N = len(rx)
-    origins = origin[None, :] + rx[:, None] * axis1[None, :] + ry[:, None] * axis2[None, :]
-    dx = wavefront(rx, ry, dx=1, grid=False)
-    dy = wavefront(rx, ry, dy=1, grid=False)
-    dx2 = wavefront(rx, ry, dx=2, grid=False)
-    dy2 = wavefront(rx, ry, dy=2, grid=False)
-    dxdy = wavefront(rx, ry, dx=1, dy=1, grid=False)
+    # 3. launch lattice and the local wavefront fit
+    rx, ry = make_hexagonal_grid(radius, spacing=spacing)
+    dx, dy, dx2, dy2, dxdy = _local_quadratic(grid, W, rx, ry)
     A, B, C, xl, yl, zl = calc_mode_curvature(rx, ry, dx, dy, dx2, dy2, dxdy)
-    E_in = efield.evaluate(origins)
-    directions = axis1[None, :] * zl[:, 0, None] + axis2[None, :] * zl[:, 1, None] + direction[None, :] * zl[:, 2, None]
-    E_vectors = axis1[None, :] * xl[:, 0, None] + axis2[None, :] * xl[:, 1, None] + direction[None, :] * xl[:, 2, None]
-    H_vectors = axis1[None, :] * yl[:, 0, None] + axis2[None, :] * yl[:, 1, None] + direction[None, :] * yl[:, 2, None]
+    frame = np.stack([u, v, w])                    # rows: the plane's axes
+    launch = origin + rx[:, None] * u + ry[:, None] * v
+    e_vec, h_vec, d_vec = xl @ frame, yl @ frame, zl @ frame
 
-    ray_data = np.zeros(N, dtype=ray_dtype)
-    ray_data['origin'] = origins
-    ray_data['direction'] = directions
-    ray_data['E_vector'] = E_vectors
-    ray_data['refractive_index'] = 1.0
-    ray_data['E1_amp'] = (E_in * E_vectors).sum(axis=-1)
-    ray_data['E2_amp'] = (E_in * H_vectors).sum(axis=-1)
-
-    gausslets = GaussletCollection.from_rays(ray_data)
-    gausslets.wavelengths = wavelengths
-    gausslets.config_parabasal_rays(wavelengths, spacing / blending, 0.0)
-    apply_mode_curvature(gausslets, -A, -B, -C)
-
-    E_test = eval_Efield_from_gausslets(gausslets, origins)
-    power_scaling = (E_in.real ** 2 + E_in.imag ** 2).sum() / (E_test.real ** 2 + E_test.imag ** 2).sum()
-    gausslets.scale_amplitude(np.sqrt(power_scaling))
-    return gausslets, E_field, uphase
+    # 4. amplitudes, parabasal rays, one complex scale
+    E_in = field.evaluate(launch)
+    rays = np.zeros(len(rx), dtype=ray_dtype)
+    rays["origin"] = launch
+    rays["direction"] = d_vec
+    rays["E_vector"] = e_vec
+    rays["refractive_index"] = 1.0
+    rays["E1_amp"] = np.einsum("ij,ij->i", E_in, e_vec)
+    rays["E2_amp"] = np.einsum("ij,ij->i", E_in, h_vec)
+    out = GaussletCollection.from_rays(rays)
+    out.wavelengths = wavelengths
+    out.config_parabasal_rays(wavelengths, spacing / blending, 0.0)
+    apply_mode_curvature(out, -A, -B, -C)
+    inside = r2.reshape(-1) <= (radius - spacing) ** 2
+    s = _best_scale(eval_Efield_from_gausslets(out, pts[inside]), E[inside])
+    out.scale_amplitude(s)
+    return out, E, uphase
 
 
 def decompose_angle(origin, direction, axis1, E_field, input_spacing, max_angle, wavelength, oversample=4,
                     E_max=None, pos_max=None):
-    """(gausslets, angular spectrum): gausslets from `origin` over the
-    directions within max_angle, amplitudes from the FFT of the sampled
-    field E_field (N, M, 3) with spacing input_spacing (microns)."""
-    from numpy import fft
-    from scipy.interpolate import RectBivariateSpline
-    input_spacing /= 1000.0
-    wavelength /= 1000.0
-    direction = _normalise(direction)
-    d2 = _normalise(np.cross(direction, axis1))
-    d1 = np.cross(direction, d2)
+    """(gausslets, angular spectrum at their wave vectors (K, 3)).
+
+    E_field (N, M, 3) samples a field on the plane through `origin` normal
+    to `direction`: sample [i, j] lies at origin + (i - N/2) s a + (j - M/2) s b
+    with s = input_spacing (microns), b = normalize(direction x axis1) and
+    a = b x direction.  wavelength in microns, max_angle in degrees.
+    `oversample` sets the lattice step to 2 pi / (aperture * oversample / 4),
+    so the default 4 gives one step per 2 pi / aperture; E_max and pos_max
+    are accepted for compatibility and have no effect (the amplitude scale
+    comes from the least-squares fit to the samples)."""
+    s = input_spacing / 1000.0
+    wl_mm = wavelength / 1000.0
+    k = 2.0 * np.pi / wl_mm
+    dvec = _normalise(np.asarray(direction, np.float64))
+    b = _normalise(np.cross(dvec, axis1))
+    a = np.cross(b, dvec)
+    E_field = np.asarray(E_field, np.complex128)
     N, M = E_field.shape[:2]
-    size = next_power_of_two(max(N, M)) * oversample
-    n0 = int((size / 2) - (N / 2))
-    m0 = int((size / 2) - (M / 2))
-    data_in = np.zeros((size, size, 3), dtype=np.complex128)
-    data_in[n0:n0 + N, m0:m0 + M, :] = E_field
-    data_in = fft.ifftshift(data_in, axes=(0, 1))
-    data_out = fft.fft2(data_in, axes=(0, 1)) / (size ** 2)
-    data_out = fft.fftshift(data_out, axes=(0, 1))
-
-    kmax = 2.0 * np.pi / (2 * input_spacing)
-    k_abs = 2.0 * np.pi / wavelength
-    kr = np.linspace(-kmax, kmax, data_out.shape[0] + 1)[:-1]
-    k_limit = k_abs * np.sin(np.pi * max_angle / 180.0)
-    k_grid_spacing = (kr[1] - kr[0]) * oversample
-    kx, ky = make_hexagonal_grid(k_limit, spacing=k_grid_spacing)
-    kz = np.sqrt(k_abs ** 2 - kx ** 2 - ky ** 2)
-    directions = (kx[:, None] * d1 + ky[:, None] * d2 + kz[:, None] * direction) / k_abs
-
-    def spline(component):
-        return RectBivariateSpline(kr, kr, component)(kx, ky, grid=False)
-    E_real = np.column_stack([spline(data_out[:, :, q].real) for q in range(3)])
-    E_imag = np.column_stack([spline(data_out[:, :, q].imag) for q in range(3)])
-    E_full = -(-1.0j * E_real + E_imag)
-    E_vectors = _normalise(np.cross(directions, d2))
-    E2_vectors = _normalise(np.cross(directions, E_vectors))
-    E1_amp = (E_vectors * E_full).sum(axis=1)
-    E2_amp = (E2_vectors * E_full).sum(axis=1)
-
-    gausslet_radius = wavelength * k_abs / (k_grid_spacing * np.pi)
-    scaling = np.sqrt(2) * input_spacing * size
-    ray_data = np.zeros(directions.shape[0], dtype=ray_dtype)
-    ray_data['origin'] = origin + ((d1 + d2) * (input_spacing * 0.5))
-    ray_data['direction'] = directions
-    ray_data['wavelength_idx'] = 0
-    ray_data['E_vector'] = E_vectors
-    ray_data['E1_amp'] = E1_amp * scaling
-    ray_data['E2_amp'] = E2_amp * scaling
-    ray_data['refractive_index'] = 1.0 + 0.0j
-    ray_data['normal'] = [[0, 1, 0]]
-    ray_data['phase'] = 0.0
+    xs = (np.arange(N) - N / 2) * s
+    ys = (np.arange(M) - M / 2) * s
+    dk = 2.0 * np.pi / (max(N, M) * s) * (4.0 / oversample)
+    kx, ky = make_hexagonal_grid(k * np.sin(np.radians(max_angle)), spacing=dk)
+    kz = np.sqrt(k * k - kx * kx - ky * ky)
+    waist = 2.0 / dk
+    # 1. spectrum at the lattice of the samples over the gausslet envelope:
+    #    sum_ij E[i, j] exp(r_ij^2 / w^2) exp(-i (kx x_i + ky y_j)) s^2
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    flat = E_field * np.exp((X * X + Y * Y) / (waist * waist))[..., None]
+    fx = np.exp(-1j * np.outer(kx, xs))
+    fy = np.exp(-1j * np.outer(ky, ys))
+    spec = np.einsum("ki,ijc,kj->kc", fx, flat, fy) * (s * s)
+    # 2. one gausslet per wave vector
+    dirs = (kx[:, None] * a + ky[:, None] * b + kz[:, None] * dvec) / k
+    e_vec = _normalise(a[None, :] - (dirs @ a)[:, None] * dirs)
+    h_vec = np.cross(dirs, e_vec)
+    rays = np.zeros(len(kx), dtype=ray_dtype)
+    rays["origin"] = np.asarray(origin, np.float64)
+    rays["direction"] = dirs
+    rays["E_vector"] = e_vec
+    rays["E1_amp"] = np.einsum("kc,kc->k", spec, e_vec)
+    rays["E2_amp"] = np.einsum("kc,kc->k", spec, h_vec)
+    rays["refractive_index"] = 1.0
+    rays["normal"] = dvec
     wl = np.array([wavelength])
-    rays = GaussletCollection.from_rays(ray_data)
-    rays.wavelengths = wl
-    rays.config_parabasal_rays(wl, gausslet_radius, 0.0)
-    if E_max is not None and pos_max is not None:
-        # the reference evaluates a trial field here but leaves the
-        # amplitudes unscaled (its scaling lines are commented out)
-        eval_Efield_from_gausslets(rays, pos_max[None, :], blending=1.0)
-        rays = GaussletCollection.from_rays(ray_data)
-        rays.wavelengths = wl
-        rays.config_parabasal_rays(wl, gausslet_radius, 0.0)
-    return rays, data_out
+    out = GaussletCollection.from_rays(rays)
+    out.wavelengths = wl
+    out.config_parabasal_rays(wl, waist, 0.0)
+    # 3. one complex scale against the samples
+    pts = np.asarray(origin, np.float64) + X.reshape(-1, 1) * a + Y.reshape(-1, 1) * b
+    sc = _best_scale(eval_Efield_from_gausslets(out, pts), E_field.reshape(-1, 3))
+    out.scale_amplitude(sc)
+    return out, spec * sc

@@ -1,7 +1,8 @@
 """Beam-decomposition pieces on the CPU: the oracle's restatements of
 calc_mode_curvature, apply_mode_curvature and build_interaction_matrix
-(cfields.pyx:231-424) pinned by closed forms, and the host-side helpers
-(unwrap2d, make_hexagonal_grid) against the oracle's copies and closed forms.
+(cfields.pyx:231-424) pinned by closed forms, and the host-side helpers of
+raypier_optics_amd.gausslets (the least-squares unwrapper, the local
+wavefront fit, make_hexagonal_grid) by closed forms.
 """
 import math
 
@@ -9,11 +10,10 @@ import numpy as np
 import pytest
 
 import oracle
-from oracle import gausslets as og
 from raypier_optics_amd import workloads
 from raypier_optics_amd.ctracer import GaussletCollection
-from raypier_optics_amd.gausslets import make_hexagonal_grid, next_power_of_two
-from raypier_optics_amd.unwrap2d import unwrap2d
+from raypier_optics_amd.gausslets import make_hexagonal_grid, next_power_of_two, _local_quadratic
+from raypier_optics_amd.unwrap2d import unwrap2d, wrap
 
 
 def test_mode_curvature_plane_wave():
@@ -108,32 +108,71 @@ def test_interaction_matrix_closed_forms():
 
 
 def test_unwrap2d():
-    """A smooth phase surface wrapped into (-pi, pi] comes back whole, with
-    the anchor pixel's value kept; residues vanish."""
-    g = np.linspace(-1, 1, 40)
+    """A smooth phase surface wrapped into [-pi, pi) comes back whole (up to
+    the whole turns that keep the anchor pixel's wrapped value) even where it
+    steps by up to 2.3 rad per pixel, for any constant offset; every cell
+    residue is 0."""
+    g = np.linspace(-1, 1, 81)
     X, Y = np.meshgrid(g, g)
-    true = 10.0 * (X ** 2 + 0.5 * Y ** 2) + 2.0 * X      # steps below pi per pixel
-    wrapped = np.angle(np.exp(1j * true))
-    out, res = unwrap2d(wrapped, anchor=(20, 20))
-    ref_out, ref_res = og.unwrap2d(wrapped, anchor=(20, 20))
-    assert out.tobytes() == ref_out.tobytes() and res.tobytes() == ref_res.tobytes()
-    assert abs(out[20, 20] - wrapped[20, 20]) < 1e-12
-    k = np.round((out - true) / (2 * np.pi))
-    assert np.all(k == k[0, 0])
-    np.testing.assert_allclose(out - true, k * 2 * np.pi, atol=1e-9)
-    assert np.abs(res).max() < 1e-9
+    for c in np.linspace(0.0, 6.0, 7):
+        true = 40.0 * (X ** 2 + 0.5 * Y ** 2) + 7.0 * X - 3.0 * X * Y + c
+        wrapped = wrap(true)
+        out, res = unwrap2d(wrapped, anchor=(40, 20))
+        assert out[40, 20] == wrapped[40, 20]
+        k = np.round((out - true) / (2 * np.pi))
+        assert np.all(k == k[40, 20])
+        np.testing.assert_allclose(out - true, k * 2 * np.pi, atol=1e-9)
+        assert res.shape == (80, 80) and not res.any()
+
+
+def test_unwrap2d_vortex_residue():
+    """A phase vortex (arg of x + iy about an off-grid point) leaves exactly
+    one cell with residue +1, the cell that contains its centre."""
+    g = np.linspace(-1, 1, 30)
+    X, Y = np.meshgrid(g, g)
+    phase = np.arctan2(Y - 0.11, X + 0.23)
+    _, res = unwrap2d(wrap(phase), anchor=(0, 0))
+    i, j = np.nonzero(res)
+    assert len(i) == 1 and abs(res[i[0], j[0]]) == 1
+    assert g[i[0]] < 0.11 < g[i[0] + 1] and g[j[0]] < -0.23 < g[j[0] + 1]
+
+
+def test_local_quadratic_fit():
+    """The weighted quadratic fit reproduces a quadratic wavefront's slopes
+    and second derivatives exactly at any point, also next to the grid edge."""
+    grid = np.linspace(-2.0, 2.0, 41)
+    X, Y = np.meshgrid(grid, grid)
+    a, b, cxx, cyy, cxy = 0.013, -0.021, 0.0042, -0.0017, 0.0009
+    W = 0.3 + a * X + b * Y + 0.5 * cxx * X ** 2 + 0.5 * cyy * Y ** 2 + cxy * X * Y
+    rng = np.random.default_rng(2)
+    px, py = rng.uniform(-1.95, 1.95, 200), rng.uniform(-1.95, 1.95, 200)
+    dx, dy, dx2, dy2, dxdy = _local_quadratic(grid, W, px, py)
+    np.testing.assert_allclose(dx, a + cxx * px + cxy * py, atol=1e-12)
+    np.testing.assert_allclose(dy, b + cyy * py + cxy * px, atol=1e-12)
+    np.testing.assert_allclose(dx2, cxx, atol=1e-11)
+    np.testing.assert_allclose(dy2, cyy, atol=1e-11)
+    np.testing.assert_allclose(dxdy, cxy, atol=1e-11)
 
 
 @pytest.mark.parametrize("radius,spacing", [(10.0, 4.0), (1.0, 0.1), (3.3, 0.37)])
 def test_hexagonal_grid_connectivity(radius, spacing):
+    """Lattice points lie strictly inside the radius, nearest neighbours at
+    one spacing, no two points closer; the point count is the disc's area
+    over a cell's (sqrt(3)/2 spacing^2) to within the rim; every listed
+    neighbour is one spacing away and every point with six neighbours at one
+    spacing lists all six."""
     x, y, nb = make_hexagonal_grid(radius, spacing, connectivity=True)
-    rx, ry, rnb = og.make_hexagonal_grid(radius, spacing, connectivity=True)
-    assert x.tobytes() == rx.tobytes() and y.tobytes() == ry.tobytes()
-    assert np.array_equal(nb, rnb)
+    assert np.all(x ** 2 + y ** 2 < radius ** 2)
+    d = np.hypot(x[:, None] - x[None, :], y[:, None] - y[None, :])
+    np.fill_diagonal(d, np.inf)
+    assert d.min() == pytest.approx(spacing, rel=1e-12)
+    cells = math.pi * radius ** 2 / (math.sqrt(3) / 2 * spacing ** 2)
+    assert abs(len(x) - cells) <= 2 * math.pi * radius / spacing + 7
+    close = np.abs(d - spacing) < 1e-9 * spacing
     for k in range(len(x)):
-        for q in nb[k]:
-            if q >= 0:
-                assert abs(math.hypot(x[q] - x[k], y[q] - y[k]) - spacing) < 1e-9 * spacing
+        listed = [q for q in nb[k] if q >= 0]
+        assert all(close[k, q] for q in listed)
+        assert set(listed) == set(np.nonzero(close[k])[0])
 
 
 def test_next_power_of_two():

@@ -10,6 +10,7 @@ import oracle
 
 from raypier_optics_amd import workloads
 from raypier_optics_amd.ctracer import GaussletCollection, norm, cross, addvv, subvv, multvs
+from raypier_optics_amd._abi import gausslet_dtype
 from raypier_optics_amd.find_focus import find_focus
 
 
@@ -115,3 +116,149 @@ def test_find_focus_recovers_point():
     assert np.allclose(find_focus(o, d), focus, atol=1e-9)
     w = rng.uniform(0.1, 1.0, 500)
     assert np.allclose(find_focus(o, d, weights=w / w.sum()), focus, atol=1e-9)
+
+
+# ------------------------------------------------------ Lagrange invariant
+LI_PAIRS = {(0, 5), (1, 2), (3, 4), (1, 4), (0, 3), (2, 5)}
+
+
+def _frame_gausslet(offsets, tilts, direction=(0.0, 0.0, 1.0), e_vector=(1.0, 0.0, 0.0), origin=(1.0, 2.0, 3.0)):
+    """One gausslet: parabasal ray j starts at origin + offsets[j] and points
+    along normalize(direction + tilts[j])."""
+    g = np.zeros(1, gausslet_dtype)
+    b = g["base_ray"]
+    b["origin"] = origin
+    b["direction"] = direction
+    b["E_vector"] = e_vector
+    b["wavelength_idx"] = 0
+    for j in range(6):
+        d = np.asarray(direction) + np.asarray(tilts[j])
+        g["para_rays"][0, j]["origin"] = np.asarray(origin) + np.asarray(offsets[j])
+        g["para_rays"][0, j]["direction"] = d / np.linalg.norm(d)
+    return g
+
+
+def test_oracle_lagrange_invariant_pairs():
+    """Known answers for GaussletCollection.lagrange_invariant
+    (ctracer.pyx:1347-1389): with an offset h (along E) on parabasal ray i
+    only and a tilt t (along E) on ray j only, the value is
+    1000 |h t'| / sqrt(6) / wavelength, t' = t / sqrt(1 + t^2), when {i, j}
+    is one of the six pairs (0,5) (1,2) (3,4) (1,4) (0,3) (2,5), and 0
+    otherwise.  A wrong pair, a missing 1/6 or 1000, or a wrong wavelength
+    index fails one of the 30 cases."""
+    h, t, wl = 0.2, 0.01, np.array([0.5, 0.7])
+    for i in range(6):
+        for j in range(6):
+            if i == j:
+                continue
+            offs = [(0.0, 0.0, 0.0)] * 6
+            tilts = [(0.0, 0.0, 0.0)] * 6
+            offs[i] = (h, 0.0, 0.0)
+            tilts[j] = (t, 0.0, 0.0)
+            g = _frame_gausslet(offs, tilts)
+            g["base_ray"]["wavelength_idx"] = 1
+            got = oracle.lagrange_invariant(g, wl)[0]
+            want = 1000 * h * t / math.sqrt(1 + t * t) / math.sqrt(6) / wl[1] \
+                if (i, j) in LI_PAIRS or (j, i) in LI_PAIRS else 0.0
+            assert got == pytest.approx(want, rel=1e-13, abs=1e-300), (i, j)
+    with pytest.raises(IndexError):
+        oracle.lagrange_invariant(g, wl[:1])
+
+
+def test_oracle_lagrange_invariant_frame_and_propagation():
+    """The sum is the symplectic form of (offset, direction) pairs, so it
+    does not depend on how the frame is turned about the base direction (a
+    rotated E vector, or the whole gausslet rotated in space), and
+    it is unchanged by free-space propagation of every ray to a common plane
+    (project_to_plane) up to second order in the small tilts."""
+    rng = np.random.default_rng(21)
+    offs = [tuple(rng.normal(0, 1e-3, 2)) + (0.0,) for _ in range(6)]
+    tilts = [tuple(rng.normal(0, 1e-4, 2)) + (0.0,) for _ in range(6)]
+    g = _frame_gausslet(offs, tilts)
+    wl = np.array([0.633])
+    ref = oracle.lagrange_invariant(g, wl)[0]
+    assert ref > 0
+    # E turned about the base direction
+    for a in (0.3, 1.1, 2.9):
+        g2 = g.copy()
+        g2["base_ray"]["E_vector"] = (math.cos(a), math.sin(a), 0.0)
+        assert oracle.lagrange_invariant(g2, wl)[0] == pytest.approx(ref, rel=1e-12)
+    # the whole gausslet rotated about an arbitrary axis through its base origin
+    ax = rng.normal(size=3)
+    ax /= np.linalg.norm(ax)
+    K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    R = np.eye(3) + math.sin(0.7) * K + (1 - math.cos(0.7)) * K @ K
+    g3 = g.copy()
+    o = g["base_ray"]["origin"][0]
+    g3["base_ray"]["direction"] = g["base_ray"]["direction"] @ R.T
+    g3["base_ray"]["E_vector"] = g["base_ray"]["E_vector"] @ R.T
+    g3["para_rays"]["origin"] = (g["para_rays"]["origin"] - o) @ R.T + o
+    g3["para_rays"]["direction"] = g["para_rays"]["direction"] @ R.T
+    assert oracle.lagrange_invariant(g3, wl)[0] == pytest.approx(ref, rel=1e-9)
+    # free-space propagation to a plane 30 mm ahead
+    g4 = oracle.project_to_plane(g, (0.0, 0.0, 33.0), (0.0, 0.0, 1.0))
+    assert oracle.lagrange_invariant(g4, wl)[0] == pytest.approx(ref, rel=1e-6)
+
+
+def lagrange_scene():
+    """A planar n = 1.5 dielectric interface at z = 0 and a biconvex n = 1.5
+    DielectricMaterial singlet (R = 50 mm, 5 mm thick) behind it at z = 20."""
+    from raypier_optics_amd.cmaterials import DielectricMaterial
+    from raypier_optics_amd.cfaces import SphericalFace, CircularFace
+    from raypier_optics_amd.ctracer import FaceList
+    lens = FaceList()
+    lens.faces = [SphericalFace(diameter=25.0, z_height=-2.5, curvature=-50.0,
+                                material=DielectricMaterial(n_inside=1.5, n_outside=1.0)),
+                  SphericalFace(diameter=25.0, z_height=2.5, curvature=50.0,
+                                material=DielectricMaterial(n_inside=1.0, n_outside=1.5))]
+    flat = FaceList()
+    flat.faces = [CircularFace(diameter=50.0, z_plane=0.0, material=DielectricMaterial(n_inside=1.5,
+                                                                                       n_outside=1.0))]
+    return flat, lens
+
+
+def asymmetric_gausslets(n, seed, h=1e-4, u=1e-5, z0=-10.0):
+    """Collimated gausslets over a 6 mm square whose six parabasal rays have
+    random transverse offsets (sd h) and tilts (sd u)."""
+    rng = np.random.default_rng(seed)
+    r = workloads.base_rays(n)
+    r["origin"] = np.column_stack([rng.uniform(-3, 3, (n, 2)), np.full(n, z0)])
+    r["direction"] = (0.0, 0.0, 1.0)
+    r["E_vector"] = (1.0, 0.0, 0.0)
+    r["E1_amp"] = 1.0
+    r["refractive_index"] = 1.0
+    r["wavelength_idx"] = 0
+    r["length"] = 100.0
+    g = np.zeros(n, gausslet_dtype)
+    g["base_ray"] = r
+    for j in range(6):
+        off = np.column_stack([rng.normal(0, h, (n, 2)), np.zeros(n)])
+        dd = np.column_stack([rng.normal(0, u, (n, 2)), np.ones(n)])
+        g["para_rays"][:, j]["origin"] = r["origin"] + off
+        g["para_rays"][:, j]["direction"] = dd / np.linalg.norm(dd, axis=1)[:, None]
+        g["para_rays"][:, j]["length"] = 100.0
+    return g
+
+
+def test_oracle_lagrange_invariant_interfaces():
+    """Through a planar interface from n = 1 into n = 1.5 every transverse
+    direction component shrinks by 1/1.5 while offsets are unchanged, so the
+    invariant (built from direction cosines, not n times them) becomes
+    exactly 1/1.5 of its value; through a whole lossless singlet back into air
+    it is conserved to first order in the gausslet size (2e-5 here)."""
+    from raypier_optics_amd._scene import compile_scene
+    flat, lens = lagrange_scene()
+    wl = np.array([0.55])
+    for fls in ([flat], [lens]):
+        workloads.prepare_faces(fls, wl, 100.0)
+    g = asymmetric_gausslets(300, 2)
+    L0 = oracle.lagrange_invariant(g, wl)
+    out, _ = oracle.trace_gausslet(compile_scene([flat]), g.copy(), 100.0, wl)
+    assert len(out) == len(g)
+    np.testing.assert_allclose(oracle.lagrange_invariant(out, wl), L0 / 1.5, rtol=1e-6)
+    t = compile_scene([lens])
+    cur = g.copy()
+    for _ in range(2):
+        cur, _ = oracle.trace_gausslet(t, cur, 100.0, wl)
+    assert len(cur) == len(g)
+    np.testing.assert_allclose(oracle.lagrange_invariant(cur, wl), L0, rtol=2e-5)

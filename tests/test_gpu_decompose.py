@@ -139,7 +139,7 @@ def test_empty_decomposition_and_two_faces(gpu):
 # ------------------------------------------------ decomposition mode maths
 from raypier_optics_amd import cfields, fields
 from raypier_optics_amd import gausslets as rg
-from oracle import gausslets as og
+from raypier_optics_amd.sources import GaussianPointSource
 
 
 def _slopes(n, seed):
@@ -189,47 +189,92 @@ def _beam():
     return gc
 
 
-@pytest.mark.parametrize("curvature", [None, 1e5])
-def test_decompose_position(gpu, curvature):
+def _plane(z, half, n):
+    g = np.linspace(-half, half, n)
+    X, Y = np.meshgrid(g, g)
+    return np.column_stack([X.ravel(), Y.ravel(), np.full(X.size, z)])
+
+
+def _rel_err(e_out, e_in):
+    return np.abs(e_out - e_in).max() / np.abs(e_in).max()
+
+
+def test_decompose_position_plane_wave(gpu):
+    """A collimated Gaussian beam (Rayleigh range ~12 m) re-launched on a
+    plane 1 mm from its source: every new gausslet points along the axis
+    (the wavefront is flat), the re-launched beam carries the input's
+    complex field 1.5 mm further on (amplitude and phase, within 5% of the
+    peak) and its power through that plane (within 2%)."""
     beam = _beam()
-    plane_z = -79.0
-    kw = dict(origin=(0.0, 0.0, plane_z), direction=(0.0, 0.0, 1.0), axis1=(1.0, 0.0, 0.0), radius=3.0,
-              resolution=12.0, curvature=curvature, blending=1.5)
-    got, E_field, uphase = rg.decompose_position(beam, **kw)
-    ref, rE, ruphase = og.decompose_position(beam.copy_as_array(), np.asarray(beam.wavelengths), **kw)
+    got, E_field, uphase = rg.decompose_position(beam, origin=(0.0, 0.0, -79.0), direction=(0.0, 0.0, 1.0),
+                                                 axis1=(1.0, 0.0, 0.0), radius=3.0, resolution=12.0)
     g = got.copy_as_array()
-    assert len(g) == len(ref) > 150
-    scale = np.abs(rE).max()
-    assert np.abs(E_field - rE).max() <= 1e-9 * scale
-    assert np.abs(uphase - ruphase).max() <= 1e-6
-    b, rb = g["base_ray"], ref["base_ray"]
-    assert b["origin"].tobytes() == rb["origin"].tobytes()
-    assert np.abs(b["direction"] - rb["direction"]).max() <= 1e-9
-    amp = np.abs(rb["E1_amp"]).max()
-    assert np.abs(b["E1_amp"] - rb["E1_amp"]).max() <= 1e-6 * amp
-    assert np.abs(g["para_rays"]["direction"] - ref["para_rays"]["direction"]).max() <= 1e-9
-    # the decomposed beam carries the input's field across the launch disc
-    pts = np.stack([np.linspace(-2.0, 2.0, 21), np.zeros(21), np.full(21, plane_z + 0.5)], axis=1)
+    assert len(g) > 400
+    d = g["base_ray"]["direction"]
+    assert np.abs(d[:, :2]).max() < 1e-4
+    assert np.all(np.abs(g["base_ray"]["origin"][:, 2] + 79.0) < 1e-12)
+    pts = _plane(-77.5, 2.0, 41)
     e_in = fields.eval_Efield_from_gausslets(beam, pts)
     e_out = fields.eval_Efield_from_gausslets(got, pts)
-    err = np.abs(np.abs(e_out[:, 0]) - np.abs(e_in[:, 0])).max() / np.abs(e_in[:, 0]).max()
-    print("decomposed field vs input: %.3g of max" % err)
+    err = _rel_err(e_out, e_in)
+    p_in, p_out = (np.abs(e_in) ** 2).sum(), (np.abs(e_out) ** 2).sum()
+    print("plane wave: field error %.3g of max, power ratio %.5f" % (err, p_out / p_in))
+    assert err <= 0.05
+    assert abs(p_out / p_in - 1) <= 0.02
+
+
+def test_decompose_position_spherical_wave(gpu):
+    """A Gaussian point source focusing 40 mm ahead, decomposed 30 mm before
+    its focus (reference sphere R = 30 removed for unwrapping): the new
+    gausslets' directions point at the focus -- the transverse slope at
+    radius r is -r / sqrt(r^2 + R^2), a wavefront curvature of 1 / R -- to
+    2e-3, and the re-launched beam reproduces the converging field 2 mm
+    further on within 5% of the peak."""
+    src = GaussianPointSource(origin=(0.0, 0.0, 0.0), direction=(0.0, 0.0, 1.0), working_dist=40.0,
+                              numerical_aperture=0.04, resolution=10.0, beam_waist=5.0, wavelength=1.064)
+    beam = src.input_rays
+    beam.project_to_plane((0.0, 0.0, 9.0), (0.0, 0.0, 1.0))
+    got, E_field, uphase = rg.decompose_position(beam, origin=(0.0, 0.0, 10.0), direction=(0.0, 0.0, 1.0),
+                                                 axis1=(1.0, 0.0, 0.0), radius=1.0, resolution=10.0,
+                                                 curvature=30.0)
+    b = got.copy_as_array()["base_ray"]
+    assert len(b) > 250
+    o, d = b["origin"], b["direction"]
+    want = np.column_stack([-o[:, 0], -o[:, 1], np.full(len(o), 30.0)])
+    want /= np.linalg.norm(want, axis=1)[:, None]
+    dev = np.abs(d - want).max()
+    print("spherical wave: direction error %.3g" % dev)
+    assert dev <= 2e-3
+    pts = _plane(12.0, 0.6, 31)
+    err = _rel_err(fields.eval_Efield_from_gausslets(got, pts), fields.eval_Efield_from_gausslets(beam, pts))
+    print("spherical wave: field error %.3g of max" % err)
     assert err <= 0.05
 
 
 def test_decompose_angle(gpu):
-    """A Gaussian spot sampled on a grid, decomposed into directions: the
-    host steps are the reference's numpy/scipy calls, the parabasal set-up
-    runs on the device and agrees bit for bit."""
-    n = 32
-    s = 2.0                                             # sample spacing, microns
+    """A tilted Gaussian spot (2 degrees about y, 12 um radius, lambda 1 um)
+    sampled on a 32 x 32 grid at 2 um: the angular decomposition's brightest
+    gausslet lies within one lattice step of the tilt direction, and the
+    gausslets together reproduce the sampled field (complex) within 2% of
+    its peak."""
+    n, s, wl = 32, 2.0, 1.0
     g = (np.arange(n) - n / 2) * s
-    X, Y = np.meshgrid(g, g)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    k = 2 * np.pi / wl
+    tilt = np.radians(2.0)
     E = np.zeros((n, n, 3), np.complex128)
-    E[:, :, 0] = np.exp(-(X ** 2 + Y ** 2) / (12.0 ** 2))
-    args = ((0.0, 0.0, 0.0), np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0]), E, s, 10.0, 1.0)
-    got, spec = rg.decompose_angle(*args)
-    ref, rspec = og.decompose_angle(*args)
-    assert spec.tobytes() == rspec.tobytes()
-    assert got.n_rays == len(ref) > 20
-    assert got.copy_as_array().tobytes() == ref.tobytes()
+    E[:, :, 1] = np.exp(-(X ** 2 + Y ** 2) / (12.0 ** 2)) * np.exp(1j * k * np.sin(tilt) * X)
+    axis1 = np.array([1.0, 0.0, 0.0])
+    got, spec = rg.decompose_angle((0.0, 0.0, 0.0), np.array([0.0, 0.0, 1.0]), axis1, E, s, 10.0, wl)
+    arr = got.copy_as_array()["base_ray"]
+    assert len(arr) > 20
+    # sample axes: b = normalize(d x axis1) = +y, a = b x d = +x
+    power = np.abs(arr["E1_amp"]) ** 2 + np.abs(arr["E2_amp"]) ** 2
+    best = arr["direction"][np.argmax(power)]
+    step = 2 * np.pi / (n * s * 1e-3) / (2 * np.pi / (wl * 1e-3))
+    assert abs(best[0] - np.sin(tilt)) <= step and abs(best[1]) <= step
+    pts = np.column_stack([X.ravel(), Y.ravel(), np.zeros(X.size)]) * 1e-3
+    e_out = fields.eval_Efield_from_gausslets(got, pts)
+    err = _rel_err(e_out, E.reshape(-1, 3))
+    print("angle decomposition: field error %.3g of max" % err)
+    assert err <= 0.02

@@ -13,7 +13,7 @@ import oracle
 from raypier_optics_amd import _abi, workloads
 from raypier_optics_amd._scene import compile_scene, SceneBuilder
 from raypier_optics_amd import _queries
-from raypier_optics_amd.ctracer import RayCollection, GaussletCollection, FaceList, trace_segment
+from raypier_optics_amd.ctracer import RayCollection, GaussletCollection, FaceList, Transform, trace_segment
 from raypier_optics_amd.tracer import trace_rays, trace_ray_sequence
 from raypier_optics_amd.cfaces import (CircularFace, ShapedPlanarFace, RectangularFace, PolygonFace, SphericalFace,
                                        ShapedSphericalFace, ExtrudedPlanarFace, ConicRevolutionFace, AsphericFace,
@@ -183,6 +183,50 @@ def test_material_children(gpu, k):
     compare_generation(got, ref, what="material %d" % k)
 
 
+def _catalogue_glass(name):
+    import json
+    import os
+    rows = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "dispersion_rows.json")))
+    every = rows["formula_1"] + [p["poly"] for p in rows["formula_3_pairs"]]
+    row = [r for r in every if r["name"] == name][0]
+    from raypier_optics_amd.cmaterials import BaseDispersionCurve
+    return row, BaseDispersionCurve(row["formula_id"], np.array(row["coefs"]), 0.0, row["wavelength_min"],
+                                    row["wavelength_max"])
+
+
+@pytest.mark.parametrize("name", ["IG4", "AMTIR-3", "J-BK7", "E-SF5"])
+def test_dispersion_formulas_1_and_3(gpu, name):
+    """Sellmeier formula 1 (IG4, AMTIR-3) and the polynomial formula 3
+    (J-BK7, E-SF5) of the reference's glass catalogue evaluated on the device
+    inside FullDielectricDispersiveMaterial (n per wavelength index), against
+    the oracle: the children's refractive index and the Fresnel amplitudes
+    that follow from it within 1e-12 relative (compare_generation's 1e-9 on
+    every other field).  The oracle's formulas are pinned by the catalogue
+    in tests/test_dispersion_catalogue.py."""
+    row, glass = _catalogue_glass(name)
+    mat = FullDielectricDispersiveMaterial(dispersion_inside=glass, dispersion_outside=NondispersiveCurve(1.0),
+                                           reflection_threshold=0.0, transmission_threshold=0.0)
+    t, mi = scene_of_material(mat)
+    n = 1500
+    lo, hi = max(row["wavelength_min"], 0.4), min(row["wavelength_max"], 11.0)
+    wl = np.random.default_rng(7).uniform(lo, hi, 32)
+    r = random_rays(n, 77, wl_count=32)
+    r["refractive_index"] = 1.0
+    rng = np.random.default_rng(78)
+    nrm = rng.normal(size=(n, 3))
+    tan = np.cross(nrm, rng.normal(size=(n, 3)))
+    pts = rng.uniform(-2, 2, (n, 3))
+    got, counts = _queries.eval_child_rays_batch(mat, r, 0, pts, nrm, tan, wl)
+    ref = [oracle.eval_child_ray(t, mi, r[i:i + 1], i, pts[i], nrm[i], tan[i], wl) for i in range(n)]
+    assert np.array_equal(counts, [len(x) for x in ref])
+    ref = np.concatenate(ref)
+    compare_generation(got, ref, what="dispersion %s" % name)
+    np.testing.assert_allclose(got["refractive_index"], ref["refractive_index"], rtol=1e-12)
+    inside = ref["refractive_index"].real > 1.5
+    assert inside.sum() > n // 4
+
+
 # ---------------------------------------------------------------- tracing
 def _oracle_trace(w):
     all_faces = workloads.prepare_faces(w.face_lists, w.wavelengths, w.max_length)
@@ -291,6 +335,100 @@ def test_trace_gausslets(gpu):
             scale = np.linalg.norm(b["para_rays"][f], axis=-1)[..., None]
             assert np.all(np.abs(a["para_rays"][f] - b["para_rays"][f]) <= 1e-9 * np.maximum(scale, 1.0))
         assert np.array_equal(np.isinf(a["para_rays"]["length"]), np.isinf(b["para_rays"]["length"]))
+
+
+def _compare_gausslets(got, ref, what):
+    assert len(got) == len(ref), what
+    for k, (a, b) in enumerate(zip(got, ref)):
+        a = a.copy_as_array() if hasattr(a, "copy_as_array") else a
+        compare_generation(a["base_ray"], b["base_ray"], what="%s base gen %d" % (what, k))
+        for f in ("origin", "direction", "normal"):
+            scale = np.linalg.norm(b["para_rays"][f], axis=-1)[..., None]
+            assert np.all(np.abs(a["para_rays"][f] - b["para_rays"][f]) <= 1e-9 * np.maximum(scale, 1.0)), \
+                (what, k, f)
+        assert np.array_equal(np.isinf(a["para_rays"]["length"]), np.isinf(b["para_rays"]["length"]))
+
+
+def test_trace_one_face_gausslet(gpu):
+    """trace_one_face_gausslet_c (ctracer.pyx:2284-2347) through
+    trace_ray_sequence: gausslets through the C1 singlet's two faces and the
+    detector, one prescribed face per generation, against the oracle's
+    one-face gausslet generation."""
+    w, g = gausslet_workload(1500)
+    lens, det = w.face_lists
+    seq = [(lens, 0), (lens, 1), (det, 0)]
+    g_in = g.copy_as_array()
+    got, faces = trace_ray_sequence(g, seq, max_length=w.max_length)
+    workloads.prepare_faces([lens, lens, det], w.wavelengths, w.max_length)
+    gs = g_in.copy()
+    gs["base_ray"]["length"] = w.max_length
+    ref = [gs]
+    for fl, fi in seq:
+        t = compile_scene([fl])
+        gs, _ = oracle.trace_gausslet(t, gs, w.max_length, w.wavelengths, one_list=0, one_face=fi)
+        if len(gs) == 0:
+            break
+        ref.append(gs)
+    assert len(ref) == 4 and len(ref[3]) == 0 or len(ref) >= 3
+    _compare_gausslets(got, ref, "one-face gausslet")
+
+
+PARABASAL_MATERIALS = {
+    "coated": lambda: SingleLayerCoatedMaterial(n_inside=1.7, n_outside=1.0, n_coating=1.38, thickness=0.11,
+                                                reflection_threshold=0.01, transmission_threshold=0.01),
+    "coated_dispersive": lambda: CoatedDispersiveMaterial(dispersion_inside=workloads.glass(workloads.N_BK7),
+                                                          dispersion_outside=NondispersiveCurve(1.0),
+                                                          dispersion_coating=NondispersiveCurve(1.38),
+                                                          coating_thickness=0.1, reflection_threshold=0.0,
+                                                          transmission_threshold=0.0),
+    "grating": lambda: DiffractionGratingMaterial(lines_per_mm=600, order=1, efficiency=0.8,
+                                                  origin=(0.1, 0.2, 0.0)),
+    "pec": lambda: PECMaterial(),
+    "partial": lambda: PartiallyReflectiveMaterial(reflectivity=0.3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(PARABASAL_MATERIALS))
+def test_gausslet_parabasal_rules(gpu, name):
+    """Parabasal directions through the materials that override or inherit
+    eval_parabasal_ray_c: the coated pair (cmaterials.pyx:1393-1434), the
+    grating (:1544-1599), and the base reflect-or-pass rule
+    (ctracer.pyx:1588-1610) that PEC and the partial reflector use.
+    Gausslets (parabasal rays from config_parabasal_rays) hit a tilted
+    sphere of the material; reflected children return to a detector plane
+    behind the source and transmitted ones reach one in front.  Three
+    generations against the oracle."""
+    rng = np.random.default_rng(17)
+    r = workloads.base_rays(1200)
+    r["origin"] = np.column_stack([rng.uniform(-4, 4, (1200, 2)), np.full(1200, -10.0)])
+    d = np.column_stack([rng.uniform(-0.05, 0.05, (1200, 2)), np.ones(1200)])
+    r["direction"] = d / np.linalg.norm(d, axis=1)[:, None]
+    e = np.cross(r["direction"], (0.0, 1.0, 0.0))
+    r["E_vector"] = e / np.linalg.norm(e, axis=1)[:, None]
+    r["E1_amp"] = rng.normal(size=1200) + 1j * rng.normal(size=1200)
+    r["E2_amp"] = rng.normal(size=1200) + 1j * rng.normal(size=1200)
+    r["refractive_index"] = 1.0
+    r["wavelength_idx"] = rng.integers(0, 3, 1200)
+    wl = np.array([0.45, 0.55, 0.65])
+    g = GaussletCollection.from_rays(r)
+    g.config_parabasal_rays(wl, 0.05, 0.5)
+    g.wavelengths = wl
+    surf = FaceList()
+    surf.faces = [SphericalFace(diameter=20.0, z_height=0.0, curvature=-60.0, material=PARABASAL_MATERIALS[name]())]
+    c, sn = math.cos(0.1), math.sin(0.1)
+    rot = np.array([[1.0, 0.0, 0.0], [0.0, c, -sn], [0.0, sn, c]])
+    surf.transform = Transform(rotation=rot, translation=(0.0, 0.3, 0.0))
+    surf.inverse_transform = Transform(rotation=rot.T, translation=tuple(-rot.T @ np.array([0.0, 0.3, 0.0])))
+    dets = FaceList()
+    dets.faces = [CircularFace(diameter=200.0, z_plane=30.0, material=OpaqueMaterial()),
+                  CircularFace(diameter=200.0, z_plane=-25.0, material=OpaqueMaterial())]
+    g_in = g.copy_as_array()
+    got, _ = trace_rays(g, [surf, dets], recursion_limit=3, max_length=200.0)
+    workloads.prepare_faces([surf, dets], wl, 200.0)
+    t = compile_scene([surf, dets])
+    ref, _ = oracle.trace_rays(t, g_in, 200.0, wl, 3, gausslet=True)
+    assert len(ref) >= 2 and len(ref[1]) > 500, (name, [len(x) for x in ref])
+    _compare_gausslets(got, ref, name)
 
 
 # ------------------------------------------------------------------ fields

@@ -174,3 +174,42 @@ def test_source_traces(gpu):
         ref = oracle.eval_neighbours(traced[g].copy_as_array(), ref)
         assert np.array_equal(traced[g].neighbours, ref)
     assert traced[2].n_rays > 0.9 * rc.n_rays
+
+
+def test_lagrange_invariant(gpu):
+    """GaussletCollection.lagrange_invariant (ctracer.pyx:1347-1389) on the
+    device against the oracle: symmetric parabasal sets from
+    config_parabasal_rays and random asymmetric ones (1e-13 relative; the
+    value ends in one square root and a division).  Then through a planar
+    n = 1.5 interface traced on the device (exactly 1/1.5 of the input) and
+    through a lossless singlet back into air (conserved to 2e-5)."""
+    from raypier_optics_amd._scene import compile_scene
+    from raypier_optics_amd.tracer import trace_rays
+    from tests.test_gausslet_host import asymmetric_gausslets, lagrange_scene
+    wl = np.array([0.5, 0.8, 1.064])
+    g = _random_gausslets(3000, 9)
+    g.config_parabasal_rays(wl, 0.05, 1.0)
+    g.wavelengths = wl
+    np.testing.assert_allclose(g.lagrange_invariant, oracle.lagrange_invariant(g.copy_as_array(), wl), rtol=1e-13,
+                               atol=1e-13)
+    a = asymmetric_gausslets(4000, 3)
+    ga = GaussletCollection.from_array(a)
+    ga.wavelengths = np.array([0.55])
+    L0 = ga.lagrange_invariant
+    np.testing.assert_allclose(L0, oracle.lagrange_invariant(a, [0.55]), rtol=1e-13)
+    flat, lens = lagrange_scene()
+    traced, _ = trace_rays(ga, [flat], recursion_limit=2, max_length=100.0)
+    assert traced[1].n_rays == len(a)
+    np.testing.assert_allclose(traced[1].lagrange_invariant, L0 / 1.5, rtol=1e-6)
+    gb = GaussletCollection.from_array(a)
+    gb.wavelengths = np.array([0.55])
+    traced, _ = trace_rays(gb, [lens], recursion_limit=3, max_length=100.0)
+    assert traced[2].n_rays == len(a)
+    L2 = traced[2].lagrange_invariant
+    np.testing.assert_allclose(L2, L0, rtol=2e-5)
+    workloads.prepare_faces([lens], [0.55], 100.0)
+    cur = a.copy()
+    t = compile_scene([lens])
+    for _ in range(2):
+        cur, _ = oracle.trace_gausslet(t, cur, 100.0, [0.55])
+    np.testing.assert_allclose(L2, oracle.lagrange_invariant(cur, [0.55]), rtol=1e-9)
