@@ -307,6 +307,40 @@ rpx_status_t rpx_trace_segment(const rpx_scene_t *scene,
                                void *workspace, uint64_t workspace_bytes,
                                void *stream);
 
+/* The generation loop of core/tracer.py:9-47 trace_rays for a ray
+ * collection (non-sequential, ctracer.pyx:2062-2118 per generation) with no
+ * host round trip between generations.  Generation 0 is `in` (n_in rays,
+ * lengths and end faces updated in place, as by rpx_trace_segment); the
+ * children of generation g (g >= 0) are written to `pool` (caller-owned
+ * SoA, field stride pool->capacity) at element offsets[g+1], generation
+ * after generation, and traced in turn, until a generation is empty or
+ * max_generations generations are traced.  The children of the last of
+ * max_generations traced generations are counted, not stored (trace_rays
+ * discards them).
+ *   counts, offsets  host arrays of max_generations + 1: counts[g] rays in
+ *                    generation g (counts[0] = n_in), offsets[g] its first
+ *                    ray's element in the pool (g >= 1)
+ *   n_traced         generations traced: 0..n_traced-1 are trace_rays' list
+ *   face_hits        HOST uint32[n_global_faces], incremented by the hits of
+ *                    every traced generation (Face.count)
+ *   workspace        rpx_chain_workspace_bytes(max(n_in, pool->capacity),
+ *                    max_generations) bytes
+ * RPX_E_CAPACITY when the pool could not be sure to hold the next
+ * generation's children: generations 0..n_traced-1 are traced and generation
+ * n_traced (counts[n_traced] rays at offsets[n_traced]) is stored but not
+ * traced -- continue from it with a fresh pool.  The call returns after the
+ * stream has completed. */
+uint64_t rpx_chain_workspace_bytes(uint64_t max_rays_per_generation, int32_t max_generations);
+rpx_status_t rpx_trace_rays(const rpx_scene_t *scene,
+                            const rpx_rays_t *in, uint64_t n_in,
+                            float max_length,
+                            const double *wavelengths, uint64_t n_wavelengths,
+                            int32_t max_generations, rpx_rays_t *pool,
+                            uint64_t *counts, uint64_t *offsets, int32_t *n_traced,
+                            uint32_t *face_hits,
+                            void *workspace, uint64_t workspace_bytes,
+                            void *stream);
+
 /* Sequential variant, one prescribed face: ctracer.pyx:2121-2170
  * trace_one_face_segment_c.  face_list / face_idx select FaceList and the
  * index of the face inside it. */
@@ -568,9 +602,14 @@ rpx_status_t rpx_trace_segment_split(const rpx_scene_t *scene,
  * finished.  A job's generations follow rpx_trace_segment (float64) exactly;
  * what it returns is a summary: interactions, generations and the moments of
  * its detector hits.  Buffers are the server's own (stream-ordered
- * allocations); the caller's rays are copied at admission and may be freed
- * after rpx_server_submit returns; the wavelength table must stay valid until
- * the job finishes. */
+ * allocations from a pool of its own); the caller's rays are copied at
+ * submission and may be freed after rpx_server_submit returns; the wavelength
+ * table must stay valid until the job finishes.  The budget covers every
+ * buffer the server holds: a waiting job's input copy from submission (a
+ * submit that would exceed it returns RPX_E_CAPACITY), an active job's input
+ * and children buffers.  A job whose buffer cannot be allocated, or whose
+ * launch reports an error, finishes with that status; the server stays
+ * usable. */
 typedef struct rpx_server rpx_server_t;
 
 typedef struct {
@@ -610,6 +649,10 @@ rpx_status_t rpx_server_step(rpx_server_t *server, rpx_job_result_t *finished, u
 rpx_status_t rpx_server_counts(const rpx_server_t *server, uint32_t *waiting, uint32_t *active);
 /* Seconds on the server's clock (the clock of t_submit / t_admit / t_finish). */
 double rpx_server_clock(void);
+/* Test hook (no counterpart in the reference): after n_allocations further
+ * job-buffer allocations succeed, the next one fails once, as if the device
+ * were out of memory; n_allocations < 0 cancels a pending failure. */
+rpx_status_t rpx_server_fail_allocations_after(rpx_server_t *server, int64_t n_allocations);
 
 /* ------------------------------------------------- sequential adjoint
  * Spot RMS of a coaxial sequential lens and its gradient by a hand-written

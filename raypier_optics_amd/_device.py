@@ -75,6 +75,34 @@ class DeviceRays(object):
         return host.view(dt).copy()
 
 
+class DeviceRaysView(DeviceRays):
+    """Rays [offset, offset + n) of a pool DeviceRays: the same buffer and
+    field stride, so a generation written into a pool is a collection of its
+    own without a copy.  Holds a reference to the pool."""
+
+    def __init__(self, pool, offset, n):
+        self.pool = pool
+        self.offset = int(offset)
+        self.capacity = int(n)
+        self.gausslet = pool.gausslet
+        self.buf = pool.buf
+        self.para = pool.para
+
+    def f64(self):
+        return self.pool.f64()[:, self.offset: self.offset + self.capacity]
+
+    def u32(self):
+        return self.pool.u32()[:, self.offset: self.offset + self.capacity]
+
+    def para_view(self):
+        return self.pool.para_view()[:, :, self.offset: self.offset + self.capacity]
+
+    def struct(self):
+        s = self.pool.struct()
+        o = self.offset
+        return _abi.Rays(s.f64 + 8 * o, s.u32 + 4 * o, (s.para + 8 * o) if s.para else 0, self.pool.capacity)
+
+
 def _torch():
     import torch
     return torch

@@ -31,6 +31,9 @@ _SIGNATURES = {
     "rpx_trace_workspace_bytes": (c_u64, [c_u64]),
     "rpx_trace_segment": (c_i32, [c_void_p, P(_abi.Rays), c_u64, c_f32, c_void_p, c_u64, P(_abi.Rays),
                                   P(c_u64), c_void_p, c_void_p, c_u64, c_void_p]),
+    "rpx_chain_workspace_bytes": (c_u64, [c_u64, c_i32]),
+    "rpx_trace_rays": (c_i32, [c_void_p, P(_abi.Rays), c_u64, c_f32, c_void_p, c_u64, c_i32, P(_abi.Rays),
+                               c_void_p, c_void_p, P(c_i32), c_void_p, c_void_p, c_u64, c_void_p]),
     "rpx_trace_one_face_segment": (c_i32, [c_void_p, c_i32, c_i32, P(_abi.Rays), c_u64, c_f32, c_void_p, c_u64,
                                            P(_abi.Rays), P(c_u64), c_void_p, c_void_p, c_u64, c_void_p]),
     "rpx_trace_gausslet": (c_i32, [c_void_p, c_i32, c_i32, P(_abi.Rays), c_u64, c_f64, c_void_p, c_u64,
@@ -47,6 +50,7 @@ _SIGNATURES = {
     "rpx_server_step": (c_i32, [c_void_p, c_void_p, c_u32, P(c_u32)]),
     "rpx_server_counts": (c_i32, [c_void_p, P(c_u32), P(c_u32)]),
     "rpx_server_clock": (c_f64, []),
+    "rpx_server_fail_allocations_after": (c_i32, [c_void_p, ctypes.c_int64]),
     "rpx_lens_workspace_bytes": (c_u64, [c_u64, c_i32]),
     "rpx_lens_spot": (c_i32, [P(_abi.Lens), P(_abi.Rays), c_u64, c_void_p, c_u64, c_void_p, c_void_p, c_u64,
                               c_void_p]),

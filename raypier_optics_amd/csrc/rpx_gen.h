@@ -26,8 +26,10 @@ struct GenControl {
     uint64_t hits;           // rays whose nearest-face search found a face (gausslet passes)
     uint64_t error;          // nonzero: a look-back wait ran out of polls
     uint32_t tile_counter;   // next tile id handed to a starting workgroup
-    uint32_t pad[7];
+    uint32_t wg_done;        // device-resident chain: workgroups that have finished their tile
+    uint32_t pad[6];
     uint32_t hit_shards[RPX_SHARDS];                  // single pass: rays that hit, per shard
+    uint32_t face_totals[RPX_LDS_FACES];              // device-resident chain: hits per face record
     uint32_t face_shards[RPX_SHARDS][RPX_LDS_FACES];  // single pass: hits per face record, per shard
 };
 // bytes of GenControl the host reads back after a single-pass generation
@@ -43,6 +45,21 @@ struct Workspace {
 };
 
 static uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+
+// One generation of a device-resident chain: its ray count, its first ray's
+// position in the pool, and where its children go (offset + count for a
+// generation in the pool; 0 for the chain's first, caller-owned generation).
+struct GenLink {
+    uint64_t count, offset, out_offset, pad;
+};
+
+// The rays from element `off` of a SoA buffer on (same field stride).
+RPX_DEV rpx_rays_t advance_rays(rpx_rays_t r, uint64_t off) {
+    r.f64 += off;
+    r.u32 += off;
+    if (r.para) r.para += off;
+    return r;
+}
 
 static Workspace carve(void *ws, uint64_t n_in) {
     uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
@@ -133,21 +150,29 @@ static __device__ __forceinline__ uint32_t block_scan_small(uint32_t c, uint32_t
 // the wait is also bounded by a poll limit (error flag instead of a hang).
 
 #define RPX_LOOKBACK_POLLS (1u << 22)
+// A status word: flag (bits 62-63) | tag (bits 48-61) | value (bits 0-47).
+// A word whose tag is not the scan's own reads as unpublished, so the
+// generations of a device-resident chain share one status array without
+// clearing it between generations (each has its own tag).
 static constexpr uint64_t ST_AGG = 1ull << 62;     // tile aggregate published
 static constexpr uint64_t ST_PREFIX = 2ull << 62;  // inclusive prefix published
-static constexpr uint64_t ST_VALUE = (1ull << 62) - 1;
+static constexpr uint64_t ST_VALUE = (1ull << 48) - 1;
+static constexpr int ST_TAG_SHIFT = 48;
+static constexpr uint64_t ST_TAG_MASK = (1ull << 14) - 1;
 
 static __global__ __launch_bounds__(256) void k_clear(uint64_t *tile_status, uint64_t nb, GenControl *ctl) {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nb) tile_status[k] = 0;
     if (k < RPX_SHARDS * RPX_LDS_FACES) (&ctl->face_shards[0][0])[k] = 0;
     if (k < RPX_SHARDS) ctl->hit_shards[k] = 0;
+    if (k < RPX_LDS_FACES) ctl->face_totals[k] = 0;
     if (k == 0) {
         ctl->children = 0;
         ctl->parents = 0;
         ctl->hits = 0;
         ctl->error = 0;
         ctl->tile_counter = 0;
+        ctl->wg_done = 0;
     }
 }
 
@@ -185,22 +210,25 @@ static __device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
 // Run by one whole wavefront.  Publishes the tile's aggregate, sums the
 // predecessors' aggregates 64 tiles at a time until an inclusive prefix is
 // found, publishes this tile's inclusive prefix and returns the exclusive one.
-// The scan starts at tile `first` (tiles before it belong to another scan);
-// a wait that runs out of polls adds 1 to *error.
-static __device__ uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t first, uint64_t agg,
-                                    uint64_t *error) {
+// The scan starts at tile `first` (tiles before it belong to another scan)
+// and reads only words carrying its `tag`; a wait that runs out of polls adds
+// 1 to *error.
+static __device__ uint64_t lookback_tagged(uint64_t *status, uint64_t tile, uint64_t first, uint64_t agg,
+                                           uint64_t *error, uint64_t tag) {
     const int lane = threadIdx.x & 63;
+    const uint64_t tagbits = (tag & ST_TAG_MASK) << ST_TAG_SHIFT;
     if (tile == first) {
-        if (lane == 0) status_store(&status[tile], ST_PREFIX | agg);
+        if (lane == 0) status_store(&status[tile], ST_PREFIX | tagbits | agg);
         return 0;
     }
-    if (lane == 0) status_store(&status[tile], ST_AGG | agg);
+    if (lane == 0) status_store(&status[tile], ST_AGG | tagbits | agg);
     uint64_t excl = 0;
     int64_t end = (int64_t)tile - 1;
     uint32_t polls = 0;
     for (;;) {
         const int64_t idx = end - lane;
-        const uint64_t st = (idx >= (int64_t)first) ? status_load(&status[idx]) : ST_PREFIX;
+        uint64_t st = (idx >= (int64_t)first) ? status_load(&status[idx]) : (ST_PREFIX | tagbits);
+        if (((st >> ST_TAG_SHIFT) & ST_TAG_MASK) != (tag & ST_TAG_MASK)) st = 0;   // an older scan's word
         const uint64_t flag = st >> 62;
         const uint64_t pm = __ballot(flag == 2);
         const int first_prefix = pm ? (__ffsll((unsigned long long)pm) - 1) : 64;
@@ -220,8 +248,14 @@ static __device__ uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t fi
         if (first_prefix < 64) break;
         end -= 64;
     }
-    if (lane == 0) status_store(&status[tile], ST_PREFIX | (excl + agg));
+    if (lane == 0) status_store(&status[tile], ST_PREFIX | tagbits | (excl + agg));
     return excl;
+}
+
+// The untagged scan of a status array cleared before each use.
+static __device__ uint64_t lookback(uint64_t *status, uint64_t tile, uint64_t first, uint64_t agg,
+                                    uint64_t *error) {
+    return lookback_tagged(status, tile, first, agg, error, 0);
 }
 
 // A pinned per-thread block, mapped for every device, that control heads are

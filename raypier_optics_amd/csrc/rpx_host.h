@@ -12,6 +12,7 @@ struct rpx_scene {
     uint32_t features;        // face / material families present (F_*, M_*)
     bool negative_face_idx;   // a face has global_idx < 0: usable for selection, not for tracing
     std::vector<rpx_shape_op_t> host_shape_ops;   // for validating query roots
+    std::vector<int32_t> face_global_idx;         // Face.idx of each face record (host copy)
 };
 
 // Checks that ops[begin..root] is a well-formed postfix program.

@@ -282,11 +282,22 @@ struct rpx_server {
     std::vector<int> free_slots;
     std::deque<rpx_job_result_t> done;
     uint64_t next_id = 1;
+    int64_t allocs_until_failure = -1;   // test hook: the allocation after this many fails; < 0 never
 };
 
 static rpx_status_t alloc_buffer(rpx_server_t *s, Buffer &b, uint64_t cap) {
+    if (s->allocs_until_failure == 0) {
+        s->allocs_until_failure = -1;
+        b = Buffer();
+        return rpx_set_error(RPX_E_HIP, "job buffer allocation failed (injected)");
+    }
+    if (s->allocs_until_failure > 0) --s->allocs_until_failure;
     b.cap = std::max<uint64_t>(cap, 1);
-    RPX_HIP_TRY(hipMallocFromPoolAsync(&b.p, b.bytes(), s->pool, s->stream));
+    hipError_t e = hipMallocFromPoolAsync(&b.p, b.bytes(), s->pool, s->stream);
+    if (e != hipSuccess) {
+        b = Buffer();
+        return rpx_set_error(RPX_E_HIP, std::string("job buffer allocation: ") + hipGetErrorString(e));
+    }
     return RPX_OK;
 }
 
@@ -455,6 +466,10 @@ extern "C" rpx_status_t rpx_server_submit(rpx_server_t *s, const rpx_job_desc_t 
     if ((d->n_rays + RPX_BLOCK - 1) / RPX_BLOCK + 1 > s->max_tiles)
         return rpx_set_error(RPX_E_CAPACITY, "job larger than the server's rays in flight");
     if (d->n_wavelengths && !d->wavelengths) return rpx_set_error(RPX_E_INVALID, "wavelength table is null");
+    // a waiting job's input copy counts against the budget from submission
+    const uint64_t in_bytes = std::max<uint64_t>(d->n_rays, 1) * RAY_BYTES;
+    if (s->used + in_bytes > s->budget)
+        return rpx_set_error(RPX_E_CAPACITY, "the job's input would exceed the server's HBM budget");
     Job *j = new Job();
     j->id = s->next_id++;
     j->desc = *d;
@@ -483,6 +498,7 @@ extern "C" rpx_status_t rpx_server_submit(rpx_server_t *s, const rpx_job_desc_t 
             return rpx_set_error(RPX_E_HIP, std::string("job input copy: ") + hipGetErrorString(e));
         }
     }
+    s->used += j->cur.bytes();
     s->waiting.push_back(j);
     *job_id = j->id;
     return RPX_OK;
@@ -517,7 +533,8 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
     for (Job *j : s->active) tiles_active += (j->n + RPX_BLOCK - 1) / RPX_BLOCK;
     while (!s->waiting.empty() && !s->free_slots.empty()) {
         Job *j = s->waiting.front();
-        const uint64_t need = j->cur.bytes() + (uint64_t)j->max_children * j->n * RAY_BYTES;
+        // its input is already counted; the children buffer is what it adds
+        const uint64_t need = std::max<uint64_t>((uint64_t)j->max_children * j->n, 1) * RAY_BYTES;
         const uint64_t tiles = (j->n + RPX_BLOCK - 1) / RPX_BLOCK;
         if (!s->active.empty() && (s->used + need > s->budget || tiles_active + tiles > s->max_tiles)) break;
         s->waiting.pop_front();
@@ -525,9 +542,16 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
         s->free_slots.pop_back();
         j->t_admit = now_s();
         rpx_status_t st = alloc_buffer(s, j->next, (uint64_t)j->max_children * j->n);
-        if (st) return st;
-        RPX_HIP_TRY(hipMemsetAsync(s->d_moments + j->slot * 6, 0, sizeof(double) * 6, s->stream));
-        s->used += need;
+        if (st == RPX_OK) {
+            hipError_t e = hipMemsetAsync(s->d_moments + j->slot * 6, 0, sizeof(double) * 6, s->stream);
+            if (e != hipSuccess) st = rpx_set_error(RPX_E_HIP, std::string("moments clear: ") + hipGetErrorString(e));
+        }
+        if (st != RPX_OK) {            // the job ends here with the error; its slot and buffers go back
+            j->status = st;
+            finish(s, j, now_s());
+            continue;
+        }
+        s->used += j->next.bytes();
         tiles_active += tiles;
         s->active.push_back(j);
     }
@@ -595,8 +619,24 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
         if (st) return st;
         RPX_HIP_TRY(hipGraphLaunch(G.exec, s->stream));
         RPX_HIP_TRY(hipStreamSynchronize(s->stream));
-        if (s->h_ctl_back->error)
-            return rpx_set_error(RPX_E_HIP, "batched generation: a look-back wait exceeded its poll limit");
+        if (s->h_ctl_back->error) {
+            // the launch's counts and moments cannot be trusted: every job in
+            // it ends with the error rather than being retried on moments
+            // that already hold this step's partial sums
+            const double t = now_s();
+            std::vector<Job *> still;
+            for (Job *j : s->active) {
+                if (std::find(batch.begin(), batch.end(), j) == batch.end()) {
+                    still.push_back(j);
+                } else {
+                    j->status = RPX_E_HIP;
+                    finish(s, j, t);
+                }
+            }
+            s->active.swap(still);
+            return rpx_set_error(RPX_E_HIP, "batched generation: a look-back wait exceeded its poll limit; the "
+                                            "jobs of that launch finished with RPX_E_HIP");
+        }
         const double t = now_s();
         std::vector<Job *> still;
         size_t b = 0;
@@ -625,7 +665,11 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
                 continue;
             }
             rpx_status_t st = alloc_buffer(s, j->next, (uint64_t)j->max_children * j->n);
-            if (st) return st;
+            if (st != RPX_OK) {            // the job ends with the error; the others carry on
+                j->status = st;
+                finish(s, j, t);
+                continue;
+            }
             s->used += j->next.bytes();
             still.push_back(j);
         }
@@ -635,6 +679,12 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
         finished[(*n_finished)++] = s->done.front();
         s->done.pop_front();
     }
+    return RPX_OK;
+}
+
+extern "C" rpx_status_t rpx_server_fail_allocations_after(rpx_server_t *s, int64_t n_allocations) {
+    if (!s) return rpx_set_error(RPX_E_INVALID, "server is null");
+    s->allocs_until_failure = n_allocations;
     return RPX_OK;
 }
 

@@ -266,6 +266,8 @@ extern "C" rpx_status_t rpx_scene_create(const rpx_scene_desc_t *d, rpx_scene_t 
     for (int i = 0; i < nlist; ++i)
         if (d->faces[i].global_idx < 0) s->negative_face_idx = true;
     s->host_shape_ops.assign(d->shape_ops, d->shape_ops + d->n_shape_ops);
+    s->face_global_idx.resize(d->n_faces);
+    for (int i = 0; i < d->n_faces; ++i) s->face_global_idx[i] = d->faces[i].global_idx;
     for (int l = 0; l < d->n_face_lists; ++l) s->list_sizes[l] = d->face_lists[l].n_faces;
     *out = s;
     return RPX_OK;
@@ -297,6 +299,16 @@ struct TraceArgs {
     uint32_t *face_hits;
     Workspace ws;
     int32_t one_list, one_face;  // one-face mode: list and face index inside it (-1 = off)
+    // device-resident generation chain (k_generation only; link = nullptr
+    // otherwise): this generation's count and offsets are read from link[0]
+    // and its children's written to link[1]; `in` (when in_pool) and `out`
+    // are the pool with its field stride, the children bounded by
+    // pool_rays - link[0].out_offset, or not stored when !store_children
+    GenLink *link;
+    uint64_t pool_rays;
+    int32_t in_pool, store_children;
+    uint64_t status_tag;     // look-back tag of this generation (0 outside a chain)
+    GenControl *head_out;    // chain: where the last workgroup to finish writes the control head
 };
 
 struct CountEmit {
@@ -543,6 +555,23 @@ void k_generation(TraceArgs A) {
     __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
     __shared__ uint32_t lds_tile, lds_hits;
     __shared__ uint64_t lds_excl;
+    // A generation of a device-resident chain learns its size and place in
+    // the pool from the link its parent generation wrote.  Its grid is sized
+    // on the host from a guess made before that size is known: workgroups
+    // past the generation's tiles leave before taking a tile number, and
+    // tiles past the grid are taken by a second launch of the same
+    // generation (the tile counter and look-back carry on across it).
+    uint64_t n_in = A.n_in, out_limit = A.out.capacity;
+    rpx_rays_t in = A.in, out = A.out;
+    if (A.link) {
+        const GenLink L = A.link[0];
+        n_in = L.count;
+        if (A.in_pool) in = advance_rays(A.in, L.offset);
+        out = advance_rays(A.out, L.out_offset);
+        out_limit = A.store_children ? (A.pool_rays > L.out_offset ? A.pool_rays - L.out_offset : 0) : 0;
+    }
+    const uint32_t n_tiles = (uint32_t)((n_in + RPX_BLOCK - 1) / RPX_BLOCK);
+    if (blockIdx.x >= n_tiles) return;
     // the tile number is fetched first, so that its round trip overlaps the
     // scene staging
     uint32_t my_tile = 0;
@@ -558,9 +587,9 @@ void k_generation(TraceArgs A) {
     const uint64_t i = (uint64_t)tile * RPX_BLOCK + threadIdx.x;
     PairEmit ce;
     bool hit = false;
-    if (i < A.n_in) {
-        const double *f = A.in.f64;
-        const uint64_t c = A.in.capacity;
+    if (i < n_in) {
+        const double *f = in.f64;
+        const uint64_t c = in.capacity;
         V3 origin = v3(f[RPX_OX * c + i], f[RPX_OY * c + i], f[RPX_OZ * c + i]);
         V3 dir = v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]);
         double length = A.max_length;
@@ -599,15 +628,15 @@ void k_generation(TraceArgs A) {
                 hit_list = A.one_list;
             }
         }
-        A.in.f64[RPX_LENGTH * c + i] = length;
-        A.in.u32[RPX_END_FACE_IDX * c + i] = end_face;
+        in.f64[RPX_LENGTH * c + i] = length;
+        in.u32[RPX_END_FACE_IDX * c + i] = end_face;
         if (hit_rec >= 0) {
             hit = true;
             atomicAdd(&lds_hist[hit_rec], 1u);
             const rpx_face_t &fc = S.faces[hit_rec];
             if (fc.material >= 0) {
                 const rpx_face_list_t &fl = S.face_lists[hit_list];
-                Ray r = load_ray(A.in, i);
+                Ray r = load_ray(in, i);
                 r.length = length;
                 r.end_face_idx = end_face;
                 V3 hp = add(r.origin, mul(r.direction, length));
@@ -624,18 +653,28 @@ void k_generation(TraceArgs A) {
         if ((threadIdx.x & 63) == 0) atomicAdd(&lds_hits, (uint32_t)__popcll(mh));
     }
     if (threadIdx.x < 64) {
-        const uint64_t excl = lookback(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error);
+        const uint64_t excl = lookback_tagged(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error, A.status_tag);
         if (threadIdx.x == 0) lds_excl = excl;
     }
     __syncthreads();
     const uint64_t pos = lds_excl + local;
-    if (count > 0 && pos < A.out.capacity) store_ray(A.out, pos, ce.first());
-    if (count > 1 && pos + 1 < A.out.capacity) store_ray(A.out, pos + 1, ce.r0);
+    if (count > 0 && pos < out_limit) store_ray(out, pos, ce.first());
+    if (count > 1 && pos + 1 < out_limit) store_ray(out, pos + 1, ce.r0);
     if (threadIdx.x == 0) {
         add_hits(A.ws.ctl, tile, lds_hits);
-        if ((uint64_t)tile == (A.n_in - 1) / RPX_BLOCK) A.ws.ctl->children = lds_excl + block_total;
+        if ((uint64_t)tile == (n_in - 1) / RPX_BLOCK) {
+            const uint64_t kids = lds_excl + block_total;
+            A.ws.ctl->children = kids;
+            if (A.link) {
+                const uint64_t at = A.link[0].out_offset;
+                A.link[1].count = kids;
+                A.link[1].offset = at;
+                A.link[1].out_offset = at + kids;
+                A.head_out->children = kids;   // host-mapped: read once the generation has ended
+            }
+        }
     }
-    if (A.face_hits) {
+    if (A.face_hits || A.link) {
         uint32_t *shard = A.ws.ctl->face_shards[tile % RPX_SHARDS];
         for (int k = threadIdx.x; k < S.n_list_faces; k += blockDim.x) {
             uint32_t h = lds_hist[k];
@@ -769,6 +808,12 @@ static rpx_status_t run_generation(const rpx_scene_t *scene, int one_list, int o
     A.ws = carve(workspace, n_in);
     A.one_list = one_list;
     A.one_face = one_face;
+    A.link = nullptr;
+    A.pool_rays = 0;
+    A.in_pool = 0;
+    A.store_children = 1;
+    A.status_tag = 0;
+    A.head_out = nullptr;
     const uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
     bool timing;
     {
@@ -903,6 +948,324 @@ extern "C" rpx_status_t rpx_trace_gausslet(const rpx_scene_t *scene, int32_t fac
         return set_error(RPX_E_INVALID, "face list / face index out of range");
     return run_generation(scene, face_list, face_list >= 0 ? face_idx : -1, true, in, n_in, max_length,
                           wavelengths, n_wavelengths, out, n_out, face_hits, workspace, workspace_bytes, stream);
+}
+
+// ------------------------------------------------ device-resident chain
+// rpx_trace_rays: every generation of core/tracer.py:9-47 launched back to
+// back on the stream.  Generation g reads its size and pool position from
+// link[g] (written by generation g-1's last tile) and writes its children's
+// to link[g+1]; its grid is sized by a host-side bound, and workgroups past
+// the real tiles leave at once.  The host learns generation g's size only
+// after generation g-1 has finished, which it waits for while generation g
+// runs (so the GPU is never idle on the host), and it launches generation g
+// only when the pool is sure to hold its children under the bound it has;
+// if not, it waits for the exact size, and if the pool still cannot take
+// the worst case it stops and returns RPX_E_CAPACITY with generation g
+// stored and untraced, for the caller to continue with a fresh pool.
+
+// Chain workspace: the tile status array (shared by every generation, words
+// tagged with the generation), one control block per generation and the
+// generation links.
+struct ChainWs {
+    uint64_t *tile_status;
+    GenControl *ctl;
+    GenLink *link;
+};
+static uint64_t chain_tiles(uint64_t max_rays) { return (max_rays + RPX_BLOCK - 1) / RPX_BLOCK; }
+static ChainWs carve_chain(void *ws, uint64_t max_rays, int32_t max_generations) {
+    char *p = (char *)ws;
+    ChainWs w;
+    w.tile_status = (uint64_t *)p;
+    p += align256(8 * std::max<uint64_t>(chain_tiles(max_rays), 1));
+    w.ctl = (GenControl *)p;
+    p += align256(sizeof(GenControl) * (uint64_t)(max_generations + 1));
+    w.link = (GenLink *)p;
+    return w;
+}
+
+// Clears a chain's workspace and host heads; link[0] describes the input.
+static __global__ void k_chain_init(ChainWs w, uint64_t n_tiles, int32_t n_gen, uint64_t n_in, GenControl *heads) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_tiles) w.tile_status[k] = 0;
+    const uint64_t ctl_words = sizeof(GenControl) / 4 * (uint64_t)(n_gen + 1);
+    if (k < ctl_words) ((uint32_t *)w.ctl)[k] = 0;
+    const uint64_t head_words = sizeof(GenControl) / 4 * (uint64_t)(n_gen + 1);
+    if (heads && k < head_words) ((uint32_t *)heads)[k] = 0;
+    if (k <= (uint64_t)n_gen) {
+        w.link[k].count = k == 0 ? n_in : 0;
+        w.link[k].offset = 0;
+        w.link[k].out_offset = 0;
+        w.link[k].pad = 0;
+    }
+}
+
+// Per-face and ray hit totals and the look-back error count of every traced
+// generation of a chain into its host head (the generations themselves write
+// only their child count there).  One workgroup per generation.
+static __global__ __launch_bounds__(64) void k_chain_fold(const GenControl *ctl, GenControl *heads,
+                                                           int n_list_faces) {
+    const GenControl &c = ctl[blockIdx.x];
+    GenControl &h = heads[blockIdx.x];
+    for (int k = threadIdx.x; k < n_list_faces; k += 64) {
+        uint32_t t = 0;
+        for (int q = 0; q < RPX_SHARDS; ++q) t += c.face_shards[q][k];
+        h.face_totals[k] = t;
+    }
+    if (threadIdx.x < RPX_SHARDS) h.hit_shards[threadIdx.x] = c.hit_shards[threadIdx.x];
+    if (threadIdx.x == 0) h.error = c.error;
+}
+
+// Pinned, device-mapped control heads, one per generation of a chain, per
+// host thread; grown on demand and held until the thread ends.
+struct PinnedHeads {
+    GenControl *host = nullptr, *dev = nullptr;
+    int n = 0;
+    ~PinnedHeads() {
+        if (host) (void)hipHostFree(host);
+    }
+};
+static GenControl *pinned_heads(int n, GenControl **dev) {
+    thread_local PinnedHeads h;
+    if (h.n < n) {
+        if (h.host) (void)hipHostFree(h.host);
+        h.host = h.dev = nullptr;
+        h.n = 0;
+        GenControl *p = nullptr;
+        void *d = nullptr;
+        const int want = std::max(n, 64);
+        if (hipHostMalloc((void **)&p, sizeof(GenControl) * want, hipHostMallocMapped | hipHostMallocPortable) !=
+            hipSuccess)
+            return nullptr;
+        if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
+            (void)hipHostFree(p);
+            return nullptr;
+        }
+        h.host = p;
+        h.dev = (GenControl *)d;
+        h.n = want;
+    }
+    *dev = h.dev;
+    return h.host;
+}
+
+extern "C" uint64_t rpx_chain_workspace_bytes(uint64_t max_rays_per_generation, int32_t max_generations) {
+    if (max_generations < 1) max_generations = 1;
+    return align256(8 * std::max<uint64_t>(chain_tiles(max_rays_per_generation), 1)) +
+           align256(sizeof(GenControl) * (uint64_t)(max_generations + 1)) +
+           align256(sizeof(GenLink) * (uint64_t)(max_generations + 1));
+}
+
+extern "C" rpx_status_t rpx_trace_rays(const rpx_scene_t *scene, const rpx_rays_t *in, uint64_t n_in,
+                                       float max_length, const double *wavelengths, uint64_t n_wavelengths,
+                                       int32_t max_generations, rpx_rays_t *pool, uint64_t *counts,
+                                       uint64_t *offsets, int32_t *n_traced, uint32_t *face_hits,
+                                       void *workspace, uint64_t workspace_bytes, void *stream_) {
+    if (!n_traced || !counts || !offsets) return set_error(RPX_E_INVALID, "counts / offsets / n_traced are null");
+    *n_traced = 0;
+    counts[0] = n_in;
+    offsets[0] = 0;
+    if (rpx_status_t st = rpx_check_trace_scene(scene)) return st;
+    if (rpx_status_t st = check_rays(in, n_in, false, "input rays")) return st;
+    if (!pool || (pool->capacity > 0 && (!pool->f64 || !pool->u32))) return set_error(RPX_E_INVALID, "pool");
+    if (max_generations < 0) return set_error(RPX_E_INVALID, "max_generations is negative");
+    if (n_in == 0 || max_generations == 0) return RPX_OK;
+    if (!wavelengths && n_wavelengths) return set_error(RPX_E_INVALID, "wavelength table is null");
+    const uint64_t mc = (uint64_t)std::max(scene->view.max_children, 0);
+    // the per-generation workspace must cover the largest generation the
+    // pool can hold (or the input); a generation's grid is at most that
+    const uint64_t max_gen = std::max<uint64_t>(n_in, pool->capacity);
+    if (max_gen > 0xFFFFFFFFull) return set_error(RPX_E_UNSUPPORTED, "more than 2^32-1 rays in one generation");
+    if (!workspace || workspace_bytes < rpx_chain_workspace_bytes(max_gen, max_generations))
+        return set_error(RPX_E_INVALID, "workspace too small");
+    hipStream_t stream = (hipStream_t)stream_;
+    GenControl *heads_dev = nullptr;
+    GenControl *heads = pinned_heads(max_generations + 1, &heads_dev);
+    if (!heads) return set_error(RPX_E_HIP, "pinned control heads could not be allocated");
+    const ChainWs cw = carve_chain(workspace, max_gen, max_generations);
+    if (!face_hits && scene->view.n_list_faces > 0) return set_error(RPX_E_INVALID, "face_hits is null");
+    bool timing;
+    {
+        std::lock_guard<std::mutex> g(g_timing_mutex);
+        timing = g_timing_on;
+    }
+    static thread_local std::vector<hipEvent_t> evs;
+    auto event = [&](size_t k) -> hipEvent_t {
+        while (evs.size() <= k) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            evs.push_back(e);
+        }
+        return evs[k];
+    };
+    // event 2g: generation g's kernel starts, 2g+1: it has ended (its head is written)
+    if (!event(2 * (size_t)max_generations + 1)) return set_error(RPX_E_HIP, "event creation failed");
+    {
+        const uint64_t tiles = chain_tiles(max_gen);
+        const uint64_t words = std::max<uint64_t>(tiles, sizeof(GenControl) / 4 * (uint64_t)(max_generations + 1));
+        hipLaunchKernelGGL(k_chain_init, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, stream, cw, tiles,
+                           max_generations, n_in, heads_dev);
+        HIP_TRY(hipGetLastError());
+    }
+    const uint32_t variant = scene_variant(scene);
+    TraceArgs A;
+    A.S = scene->view;
+    A.in = *in;
+    A.out = *pool;
+    A.n_in = 0;
+    A.max_length = (double)max_length;
+    A.wavelengths.wl = wavelengths;
+    A.wavelengths.n = wavelengths ? n_wavelengths : 0;
+    A.face_hits = nullptr;          // per-face totals come back in the heads
+    A.ws.tags = nullptr;
+    A.ws.block_totals = nullptr;
+    A.ws.block_parents = nullptr;
+    A.ws.block_offsets = nullptr;
+    A.ws.tile_status = cw.tile_status;
+    A.one_list = -1;
+    A.one_face = -1;
+    A.pool_rays = pool->capacity;
+    // counts[g] is exact once known[g]; offsets[g] with it (generation g's
+    // place in the pool, g >= 1).  grid[g]: tiles of generation g's first
+    // launch, a guess when counts[g] was not known yet; a generation is
+    // closed (a second launch for tiles past the guess, then its done event)
+    // once its size is known, before the next generation is launched.
+    // Heads are read in generation order.
+    std::vector<char> known(max_generations + 2, 0), closed(max_generations + 2, 0);
+    std::vector<uint64_t> grid(max_generations + 2, 0);
+    known[0] = 1;
+    int launched = 0, synced = 0;   // generations launched; heads read (generations 0..synced-1)
+    rpx_status_t result = RPX_OK;
+    auto tiles_of = [](uint64_t n) { return (n + RPX_BLOCK - 1) / RPX_BLOCK; };
+    auto set_gen = [&](int g) {
+        A.link = cw.link + g;
+        A.ws.ctl = cw.ctl + g;
+        A.status_tag = (uint64_t)g + 1;
+        A.head_out = heads_dev + g;
+        A.in_pool = g > 0;
+        A.in = g > 0 ? *pool : *in;
+        A.store_children = g + 1 < max_generations ? 1 : 0;
+    };
+    auto close_gen = [&](int g) -> rpx_status_t {
+        if (closed[g]) return RPX_OK;
+        const uint64_t need = tiles_of(counts[g]);
+        if (need > grid[g]) {
+            set_gen(g);
+            launch_generation(variant, dim3((unsigned)(need - grid[g])), stream, A);
+            HIP_TRY(hipGetLastError());
+        }
+        HIP_TRY(hipEventRecord(evs[2 * g + 1], stream));
+        closed[g] = 1;
+        return RPX_OK;
+    };
+    auto read_heads_through = [&](int last) -> rpx_status_t {
+        for (int g = synced; g <= last; ++g) {
+            if (rpx_status_t st = close_gen(g)) return st;
+            HIP_TRY(hipEventSynchronize(evs[2 * g + 1]));
+            const GenControl &h = heads[g];
+            if (h.error) return set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
+            counts[g + 1] = h.children;
+            offsets[g + 1] = g == 0 ? 0 : offsets[g] + counts[g];
+            known[g + 1] = 1;
+            synced = g + 1;
+        }
+        return RPX_OK;
+    };
+    for (int g = 0; g < max_generations; ++g) {
+        // the size of generation g-1 bounds generation g: learn it from
+        // generation g-2's head (which has finished or is about to while
+        // generation g-1 keeps the GPU busy), and close generation g-1
+        if (g >= 2 && !known[g - 1])
+            if (rpx_status_t st = read_heads_through(g - 2)) return st;
+        if (g >= 1 && known[g - 1] && !closed[g - 1])
+            if (rpx_status_t st = close_gen(g - 1)) return st;
+        if (known[g] && counts[g] == 0) break;
+        if (g >= 1 && counts[g - 1] == 0) break;
+        uint64_t bound = known[g] ? counts[g] : mc * counts[g - 1];
+        if (bound > max_gen) {   // beyond the workspace's tiles: take the exact size
+            if (rpx_status_t st = read_heads_through(g - 1)) return st;
+            if (counts[g] == 0) break;
+            bound = counts[g];
+        }
+        if (g + 1 < max_generations) {
+            // where the children could end: generation g's own place (from
+            // offsets[g], exact once counts[g-1] is) plus its worst case
+            auto need_for = [&](uint64_t b) {
+                const uint64_t start = g == 0 ? 0 : (g == 1 ? 0 : offsets[g - 1] + counts[g - 1]) + b;
+                return start + mc * b;
+            };
+            uint64_t need = need_for(bound);
+            if (need > pool->capacity && !known[g]) {
+                if (rpx_status_t st = read_heads_through(g - 1)) return st;
+                if (counts[g] == 0) break;
+                bound = counts[g];
+                need = need_for(bound);
+            }
+            if (need > pool->capacity) {
+                result = set_error(RPX_E_CAPACITY, "pool of " + std::to_string(pool->capacity) + " rays cannot "
+                                   "take generation " + std::to_string(g) + "'s children (" + std::to_string(need) +
+                                   " at worst): it is stored and not traced");
+                break;
+            }
+        }
+        // the grid: exact when the size is known, else the last observed
+        // children-per-parent ratio applied to the parent generation
+        uint64_t guess;
+        if (known[g]) {
+            guess = counts[g];
+        } else {
+            const double ratio = (g >= 2 && counts[g - 2] > 0) ? (double)counts[g - 1] / (double)counts[g - 2] : 1.0;
+            guess = std::min<uint64_t>(bound, (uint64_t)std::ceil(ratio * (double)counts[g - 1]));
+        }
+        grid[g] = std::max<uint64_t>(tiles_of(guess), 1);
+        set_gen(g);
+        if (timing) HIP_TRY(hipEventRecord(evs[2 * g], stream));
+        launch_generation(variant, dim3((unsigned)grid[g]), stream, A);
+        HIP_TRY(hipGetLastError());
+        launched = g + 1;
+        if (known[g])
+            if (rpx_status_t st = close_gen(g)) return st;
+    }
+    if (launched > 0) {
+        // the size of the last launched generation (from its parent's head)
+        // closes it; then every generation's hit totals, then every head
+        if (launched >= 2 && synced < launched - 1)
+            if (rpx_status_t st = read_heads_through(launched - 2)) return st;
+        if (rpx_status_t st = close_gen(launched - 1)) return st;
+        hipLaunchKernelGGL(k_chain_fold, dim3((unsigned)launched), dim3(64), 0, stream, (const GenControl *)cw.ctl,
+                           heads_dev, scene->view.n_list_faces);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(evs[2 * max_generations], stream));
+        if (launched > synced)
+            if (rpx_status_t st = read_heads_through(launched - 1)) return st;
+        HIP_TRY(hipEventSynchronize(evs[2 * max_generations]));
+        for (int g = 0; g < launched; ++g)
+            if (heads[g].error)
+                return set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
+    }
+    int traced = 0;
+    for (int g = 0; g < launched; ++g) {
+        if (counts[g] == 0) break;
+        traced = g + 1;
+    }
+    *n_traced = traced;
+    for (int g = 0; g < traced; ++g)
+        for (int k = 0; k < scene->view.n_list_faces; ++k)
+            face_hits[scene->face_global_idx[k]] += heads[g].face_totals[k];
+    if (timing) {
+        for (int g = 0; g < traced; ++g) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, evs[2 * g], evs[2 * g + 1]));
+            uint64_t hits = 0;
+            for (int s2 = 0; s2 < RPX_SHARDS; ++s2) hits += heads[g].hit_shards[s2];
+            const bool stored = g + 1 < max_generations;
+            std::lock_guard<std::mutex> lk(g_timing_mutex);
+            g_timing_ms[RPX_TIMING_GENERATION] += ms;
+            g_timing_launches[RPX_TIMING_GENERATION] += 1;
+            g_timing_bytes[RPX_TIMING_GENERATION] += (double)counts[g] * (48.0 + 12.0) + (double)hits * 100.0 +
+                                                     (stored ? (double)counts[g + 1] * 188.0 : 0.0);
+        }
+    }
+    return result;
 }
 
 // ------------------------------------------------------ AoS <-> SoA

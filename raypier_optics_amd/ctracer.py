@@ -1013,9 +1013,16 @@ class _TraceContext(object):
         self.tables = compile_scene(self.face_sets)
         self.handle = self.tables.handle()
         self.max_children = self.tables.max_children
-        torch = require_gpu()
-        self.hits = torch.zeros(max(self.tables.n_global_faces, 1), dtype=torch.int32, device="cuda")
+        self._hits = None
         self._ws = None
+
+    @property
+    def hits(self):
+        """Device per-face hit counters of the one-call-per-generation path."""
+        if self._hits is None:
+            torch = require_gpu()
+            self._hits = torch.zeros(max(self.tables.n_global_faces, 1), dtype=torch.int32, device="cuda")
+        return self._hits
 
     def workspace(self, n):
         lib = load()
@@ -1025,8 +1032,16 @@ class _TraceContext(object):
             self._ws = torch.empty(need, dtype=torch.uint8, device="cuda")
         return self._ws
 
+    def chain_workspace(self, nbytes):
+        if self._ws is None or self._ws.numel() < nbytes:
+            torch = require_gpu()
+            self._ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        return self._ws
+
     def flush_counts(self):
         """Add the device hit counters to Face.count of all_faces."""
+        if self._hits is None:
+            return
         h = self.hits.cpu().numpy()
         for g in np.nonzero(h)[0]:
             if g < len(self.all_faces):

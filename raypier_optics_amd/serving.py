@@ -117,6 +117,11 @@ class TraceServer(object):
             if n.value < len(self._results):
                 return out
 
+    def fail_allocations_after(self, n):
+        """Test hook: the job-buffer allocation after the next n fails once
+        (n < 0: none)."""
+        check(self.lib.rpx_server_fail_allocations_after(self.h, int(n)))
+
     def counts(self):
         w, a = ctypes.c_uint32(0), ctypes.c_uint32(0)
         check(self.lib.rpx_server_counts(self.h, ctypes.byref(w), ctypes.byref(a)))

@@ -3,11 +3,16 @@ from itertools import chain
 
 import numpy
 
-from .ctracer import RayCollection, GaussletCollection, _TraceContext, _generation
+import ctypes
+
+from . import _abi
+from ._lib import load, check, require_gpu, stream_handle
+from .ctracer import RayCollection, GaussletCollection, _TraceContext, _generation, _wavelength_table
 
 
-def _prepare(input_rays, face_lists, max_length):
-    input_rays.reset_length(max_length)
+def _prepare(input_rays, face_lists, max_length, reset=True):
+    if reset:
+        input_rays.reset_length(max_length)
     wavelengths = numpy.asarray(input_rays.wavelengths)
     all_faces = list(chain(*(fs.faces for fs in face_lists)))
     for i, f in enumerate(all_faces):
@@ -19,6 +24,82 @@ def _prepare(input_rays, face_lists, max_length):
     return all_faces
 
 
+# the device-resident chain is sized for at most this many generations ahead
+_CHAIN_GENERATIONS = 16
+
+
+def _pool_rays(n, max_children, remaining):
+    """Pool size for a device-resident chain from a generation of n rays:
+    room for the generations still allowed (at most _CHAIN_GENERATIONS) at
+    about n rays each, plus the worst case the chain has to be sure of before
+    it launches a generation without waiting (its own size bound and its
+    children's), rounded up to a power of two and capped at 40 % of the
+    device memory free now.  (A field stride of 2^k rays puts every SoA plane
+    on the same boundary; on C2 the generation kernel ran 3-4 % faster with
+    such strides than with 15 x 2^22 or odd ones.)"""
+    torch = require_gpu()
+    mc = max(int(max_children), 1)
+    want = n * (min(remaining, _CHAIN_GENERATIONS) + 1 + 2 * mc)
+    want = 1 << max(int(want - 1).bit_length(), 0)
+    free, _ = torch.cuda.mem_get_info()
+    cap = int(0.4 * free) // (_abi.N_F64 * 8 + _abi.N_U32 * 4)
+    return max(min(want, cap), mc * n, 1)
+
+
+def _trace_chain(ctx, rays, max_length, limit):
+    """trace_rays' generation loop for a RayCollection on the device
+    (rpx_trace_rays): generations are written into pools and traced with no
+    host round trip between them; a pool that cannot be sure to hold the
+    next generation ends the call, and the loop continues from that
+    generation with a fresh pool.  Returns the traced collections, the input
+    first."""
+    from ._device import DeviceRays, DeviceRaysView
+    torch = require_gpu()
+    lib = load()
+    traced = [rays]
+    current = rays
+    remaining = limit
+    hits = (ctypes.c_uint32 * max(ctx.tables.n_global_faces, 1))()
+    while remaining > 0 and current.n_rays > 0:
+        n = current.n_rays
+        dev_in = current._ensure_device()
+        wl = _wavelength_table(current, ctx)
+        pool = DeviceRays(_pool_rays(n, ctx.max_children, remaining))
+        ws_bytes = int(lib.rpx_chain_workspace_bytes(max(n, pool.capacity), remaining))
+        ws = ctx.chain_workspace(ws_bytes)
+        counts = (ctypes.c_uint64 * (remaining + 1))()
+        offsets = (ctypes.c_uint64 * (remaining + 1))()
+        n_traced = ctypes.c_int32(0)
+        s_in, s_pool = dev_in.struct(), pool.struct()
+        code = lib.rpx_trace_rays(ctx.handle, ctypes.byref(s_in), n, float(max_length), ctypes.c_void_p(wl.data_ptr()),
+                                  len(current.wavelengths), remaining, ctypes.byref(s_pool), counts, offsets,
+                                  ctypes.byref(n_traced), hits, ctypes.c_void_p(ws.data_ptr()), ws_bytes,
+                                  stream_handle())
+        if code != _abi.RPX_E_CAPACITY:
+            check(code)
+        current._device_changed()
+        k_traced = n_traced.value
+        # generations 1..k_traced-1 of this call were traced; on a full pool
+        # generation k_traced is stored and continues the loop
+        last = k_traced + 1 if code == _abi.RPX_E_CAPACITY else k_traced
+        parent = current
+        for k in range(1, last):
+            gen = RayCollection.from_device(DeviceRaysView(pool, offsets[k], counts[k]), counts[k])
+            gen.parent = parent
+            traced.append(gen)
+            parent = gen
+        remaining -= k_traced
+        if code != _abi.RPX_E_CAPACITY:
+            break
+        current = traced[-1]
+        if k_traced == 0:
+            raise RuntimeError("device-resident trace made no progress")
+    for g, h in enumerate(hits):
+        if h and g < len(ctx.all_faces):
+            ctx.all_faces[g].count += int(h)
+    return traced
+
+
 def trace_rays(input_rays, face_lists, recursion_limit=100, max_length=100.0):
     """Non-sequential trace of a RayCollection or GaussletCollection
     (core/tracer.py:9-47): returns (traced_rays, all_faces).
@@ -26,12 +107,20 @@ def trace_rays(input_rays, face_lists, recursion_limit=100, max_length=100.0):
     As in the reference, each collection is appended before it is traced, so
     with recursion_limit=L the L-th traced generation is not returned (its
     parents still receive their length and end_face_idx)."""
-    all_faces = _prepare(input_rays, face_lists, max_length)
+    # the device-resident loop writes every input ray's length itself (the
+    # distance to its face, or max_length), so the reset is the kernel's
+    chain = isinstance(input_rays, RayCollection)
+    all_faces = _prepare(input_rays, face_lists, max_length, reset=not chain or input_rays.n_rays == 0
+                         or recursion_limit <= 0)
     face_sets = list(face_lists)
     for fs in face_sets:
         fs.sync_transforms()
     ctx = _TraceContext(face_sets, all_faces)
     decomp_faces = [f for f in all_faces if f.material.is_decomp_material()]
+    if chain:
+        traced_rays = _trace_chain(ctx, input_rays, max_length, recursion_limit) \
+            if input_rays.n_rays > 0 and recursion_limit > 0 else []
+        return traced_rays, all_faces
     traced_rays = []
     rays = input_rays
     count = 0

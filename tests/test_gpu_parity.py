@@ -262,6 +262,49 @@ def test_trace_c2_subset(gpu):
     assert len(got) == 6   # traced: input + 5 children generations (asphere x2, doublet x3); detector ends it
 
 
+@pytest.mark.parametrize("limit", [1, 2, 3, 4, 7])
+def test_trace_chain_pool_resumes(gpu, monkeypatch, limit):
+    """The device-resident generation loop (rpx_trace_rays) with pools of the
+    smallest size it accepts (max_children x the generation), so that it
+    stops on a full pool and the loop resumes from the stored generation
+    many times: every generation, its parent links and the face counts
+    against the oracle, for several recursion limits (the last generation's
+    children are counted and discarded)."""
+    from raypier_optics_amd import tracer
+    monkeypatch.setattr(tracer, "_pool_rays", lambda n, mc, remaining: max(mc * n, 1))
+    w = workloads.singlet_c1(n_rays=3000, depth=limit)
+    ref, hits = _oracle_trace(w)
+    got, faces = _gpu_trace(workloads.singlet_c1(n_rays=3000, depth=limit))
+    _compare_traces(got, ref, hits, faces, "C1 small pools, limit %d" % limit)
+    for g in range(1, len(got)):
+        assert got[g].parent is got[g - 1]
+
+
+def test_trace_chain_matches_generation_path(gpu):
+    """The device-resident loop and the one-call-per-generation path
+    (trace_segment on each generation) give bit-identical generations and
+    face counts on the Michelson (C3) scene at 4,096 rays, depth 12."""
+    w = workloads.michelson_c3(n_rays=4096, order=1)
+    got, faces = trace_rays(w.collection(), w.face_lists, recursion_limit=w.recursion_limit,
+                            max_length=w.max_length)
+    counts_chain = [f.count for f in faces]
+    all_faces = workloads.prepare_faces(w.face_lists, w.wavelengths, w.max_length)
+    rc = w.collection()
+    rc.reset_length(w.max_length)
+    seq = [rc]
+    while len(seq) < w.recursion_limit and seq[-1].n_rays > 0:
+        kids = trace_segment(seq[-1], w.face_lists, all_faces, max_length=w.max_length)
+        if kids.n_rays == 0:
+            break
+        seq.append(kids)
+    if len(seq) == w.recursion_limit:   # the last generation is traced too (its children are discarded)
+        trace_segment(seq[-1], w.face_lists, all_faces, max_length=w.max_length)
+    assert len(got) == len(seq)
+    for g, (a, b) in enumerate(zip(got, seq)):
+        assert a.copy_as_array().tobytes() == b.copy_as_array().tobytes(), g
+    assert counts_chain == [f.count for f in all_faces]
+
+
 def test_trace_one_face_sequence(gpu):
     w = workloads.singlet_c1(n_rays=3000)
     lens, det = w.face_lists

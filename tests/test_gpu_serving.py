@@ -91,14 +91,18 @@ def _last_children(w, traced):
 def test_server_budget_admits_in_order(gpu):
     jobs = _jobs(seeds=range(20, 24), sizes=[50000] * 4)
     alone = [_alone(w) for w in jobs]
-    # room for one job's buffers at a time: 50k rays x 188 B x (1 + 2 children)
-    server = TraceServer(hbm_budget_bytes=30 << 20, max_jobs=8, max_rays_in_flight=1 << 22)
+    # the four input copies (counted from submission, 4 x 50k x 188 B) plus
+    # one job's children buffer (1 x 50k x 188 B): the budget first lets one
+    # job in, then two once the first has released its input
+    server = TraceServer(hbm_budget_bytes=50 << 20, max_jobs=8, max_rays_in_flight=1 << 22)
     ids = [server.submit(w) for w in jobs]
     res = []
+    active = []
     while server.busy():
         res += server.step()
-        assert server.counts()[1] <= 1
+        active.append(server.counts()[1])
     server.close()
+    assert active[0] == 1 and max(active) < len(jobs)
     by_id = {r.job_id: r for r in res}
     admits = [by_id[j].t_admit for j in ids]
     assert admits == sorted(admits)
@@ -139,3 +143,54 @@ def test_two_servers_at_once(gpu):
         a, b = res[0][ids[0][k]], res[1][ids[1][k]]
         _same(a, b, "job %d" % k)
         assert a.detector_hits > 0
+
+
+def test_allocation_failures_end_only_their_job(gpu):
+    """One job-buffer allocation made to fail (rpx_server_fail_allocations_after)
+    at admission or between generations: the job that hit the failure
+    finishes with RPX_E_HIP, every other job completes bit-identical to its
+    run alone, the server keeps stepping, and destroy frees everything."""
+    import ctypes
+    from raypier_optics_amd import _abi
+    jobs = _jobs(seeds=range(4), sizes=[3000, 5000, 2000, 4000])
+    alone = [_alone(w) for w in jobs]
+    # allocation order: 4 input copies at submit, then per step the children
+    # buffers of newly admitted jobs and of every job that continues
+    for after in (5, 7, 9):
+        server = TraceServer(hbm_budget_bytes=8 << 30, max_jobs=16, max_rays_in_flight=1 << 22)
+        ids = [server.submit(w) for w in jobs]
+        server.fail_allocations_after(after - 4)
+        res = {r.job_id: r for r in _drain(server)}
+        w, a = server.counts()
+        assert (w, a) == (0, 0)
+        server.close()
+        failed = [k for k, jid in enumerate(ids) if res[jid].status != 0]
+        assert len(failed) == 1, (after, failed)
+        for k, jid in enumerate(ids):
+            r = res[jid]
+            if r.status != 0:
+                assert r.status == _abi.RPX_E_HIP
+            else:
+                _same(alone[k], r, "job %d, failure after %d allocations" % (k, after))
+        print("failure after %d allocations: jobs %s failed" % (after, failed))
+
+
+def test_submit_respects_budget(gpu):
+    """Waiting jobs' input copies count against the HBM budget: a submit that
+    would exceed it is refused with RPX_E_CAPACITY (and the server still
+    runs the jobs it accepted)."""
+    from raypier_optics_amd._lib import RpxError
+    from raypier_optics_amd import _abi
+    jobs = _jobs(seeds=range(3), sizes=[20000, 20000, 20000])
+    budget = int(2.5 * 20000 * 188)
+    server = TraceServer(hbm_budget_bytes=budget, max_jobs=16, max_rays_in_flight=1 << 22)
+    try:
+        server.submit(jobs[0])
+        server.submit(jobs[1])
+        with pytest.raises(RpxError) as e:
+            server.submit(jobs[2])
+        assert e.value.code == _abi.RPX_E_CAPACITY
+        res = _drain(server)
+        assert len(res) == 2 and all(r.status == 0 for r in res)
+    finally:
+        server.close()
