@@ -592,6 +592,21 @@ rpx_status_t rpx_trace_segment_split(const rpx_scene_t *scene,
                                      void *workspace, uint64_t workspace_bytes,
                                      void *stream);
 
+/* The generation loop of trace_rays in the FP32-split form, device-resident
+ * exactly as rpx_trace_rays (same counts / offsets / n_traced / face_hits /
+ * RPX_E_CAPACITY contract; the pool is an FP32-split SoA buffer and the
+ * workspace rpx_chain_workspace_bytes(max(n_in, pool->capacity),
+ * max_generations) bytes).  No counterpart in the reference. */
+rpx_status_t rpx_trace_rays_split(const rpx_scene_t *scene,
+                                  const rpx_rays_split_t *in, uint64_t n_in,
+                                  float max_length,
+                                  const double *wavelengths, uint64_t n_wavelengths,
+                                  int32_t max_generations, rpx_rays_split_t *pool,
+                                  uint64_t *counts, uint64_t *offsets, int32_t *n_traced,
+                                  uint32_t *face_hits,
+                                  void *workspace, uint64_t workspace_bytes,
+                                  void *stream);
+
 /* ------------------------------------------------------------- serving
  * A continuous-batching trace server (no counterpart in the reference, whose
  * trace loop runs one collection at a time).  Jobs -- a scene, device rays,

@@ -44,6 +44,9 @@ _SIGNATURES = {
     "rpx_rays_from_split": (c_i32, [P(_abi.RaysSplit), c_u64, P(_abi.Rays), c_void_p]),
     "rpx_trace_segment_split": (c_i32, [c_void_p, c_i32, c_i32, P(_abi.RaysSplit), c_u64, c_f32, c_void_p, c_u64,
                                         P(_abi.RaysSplit), P(c_u64), c_void_p, c_void_p, c_u64, c_void_p]),
+    "rpx_trace_rays_split": (c_i32, [c_void_p, P(_abi.RaysSplit), c_u64, c_f32, c_void_p, c_u64, c_i32,
+                                     P(_abi.RaysSplit), c_void_p, c_void_p, P(c_i32), c_void_p, c_void_p, c_u64,
+                                     c_void_p]),
     "rpx_server_create": (c_i32, [c_u64, c_u32, c_u64, c_void_p, P(c_void_p)]),
     "rpx_server_destroy": (c_i32, [c_void_p]),
     "rpx_server_submit": (c_i32, [c_void_p, P(_abi.JobDesc), P(c_u64)]),

@@ -2,7 +2,11 @@
 // block size and LDS limits, the per-generation workspace, scene staging in
 // LDS, ballot scans and the decoupled look-back that places children.
 #pragma once
+#include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <string>
+#include <vector>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include "rpx_internal.h"
@@ -305,6 +309,303 @@ static inline hipError_t finish_generation(GenControl &out, const GenControl *d_
     hipError_t e = hipStreamSynchronize(stream);
     if (e == hipSuccess) memcpy(&out, p.host, RPX_CTL_HEAD);
     return e;
+}
+
+// ------------------------------------------------ device-resident chain
+// trace_rays' generations launched back to back on one stream, for the
+// float64 and the FP32-split generation kernels alike.  Generation g reads
+// its size and pool position from link[g] (written by the last tile of
+// generation g-1) and writes its children's to link[g+1]; look-back status
+// words carry the generation as their tag, so one status array serves the
+// chain without clears; each generation has a control block of its own, and
+// its last tile writes its child count to a host-mapped head.  The host
+// learns generation g's size once generation g-1 has ended, which it waits
+// for while generation g runs (one generation late, so the GPU never idles
+// on the host), sizes generation g's grid from the last observed
+// children-per-parent ratio (workgroups past the tiles leave at once; tiles
+// past the grid get a second launch before the next generation), and only
+// launches a generation whose children the pool is sure to hold -- else it
+// stops with RPX_E_CAPACITY, generation g stored and untraced, for the
+// caller to continue with a fresh pool.  Hit totals are folded once, after
+// the last generation.
+
+struct ChainWs {
+    uint64_t *tile_status;
+    GenControl *ctl;
+    GenLink *link;
+};
+static inline uint64_t chain_tiles(uint64_t max_rays) { return (max_rays + RPX_BLOCK - 1) / RPX_BLOCK; }
+static inline uint64_t chain_workspace_bytes(uint64_t max_rays, int32_t max_generations) {
+    if (max_generations < 1) max_generations = 1;
+    return align256(8 * std::max<uint64_t>(chain_tiles(max_rays), 1)) +
+           align256(sizeof(GenControl) * (uint64_t)(max_generations + 1)) +
+           align256(sizeof(GenLink) * (uint64_t)(max_generations + 1));
+}
+static inline ChainWs carve_chain(void *ws, uint64_t max_rays, int32_t max_generations) {
+    char *p = (char *)ws;
+    ChainWs w;
+    w.tile_status = (uint64_t *)p;
+    p += align256(8 * std::max<uint64_t>(chain_tiles(max_rays), 1));
+    w.ctl = (GenControl *)p;
+    p += align256(sizeof(GenControl) * (uint64_t)(max_generations + 1));
+    w.link = (GenLink *)p;
+    return w;
+}
+
+// Clears a chain's workspace and host heads; link[0] describes the input.
+static __global__ void k_chain_init(ChainWs w, uint64_t n_tiles, int32_t n_gen, uint64_t n_in, GenControl *heads) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_tiles) w.tile_status[k] = 0;
+    const uint64_t words = sizeof(GenControl) / 4 * (uint64_t)(n_gen + 1);
+    if (k < words) {
+        ((uint32_t *)w.ctl)[k] = 0;
+        if (heads) ((uint32_t *)heads)[k] = 0;
+    }
+    if (k <= (uint64_t)n_gen) {
+        w.link[k].count = k == 0 ? n_in : 0;
+        w.link[k].offset = 0;
+        w.link[k].out_offset = 0;
+        w.link[k].pad = 0;
+    }
+}
+
+// Per-face and ray hit totals and the look-back error count of every traced
+// generation of a chain into its host head (the generations themselves write
+// only their child count there).  One workgroup per generation.
+static __global__ __launch_bounds__(64) void k_chain_fold(const GenControl *ctl, GenControl *heads,
+                                                           int n_list_faces) {
+    const GenControl &c = ctl[blockIdx.x];
+    GenControl &h = heads[blockIdx.x];
+    for (int k = threadIdx.x; k < n_list_faces; k += 64) {
+        uint32_t t = 0;
+        for (int q = 0; q < RPX_SHARDS; ++q) t += c.face_shards[q][k];
+        h.face_totals[k] = t;
+    }
+    if (threadIdx.x < RPX_SHARDS) h.hit_shards[threadIdx.x] = c.hit_shards[threadIdx.x];
+    if (threadIdx.x == 0) h.error = c.error;
+}
+
+// Pinned, device-mapped control heads, one per generation of a chain, per
+// host thread; grown on demand and held until the thread ends.
+struct PinnedHeads {
+    GenControl *host = nullptr, *dev = nullptr;
+    int n = 0;
+    ~PinnedHeads() {
+        if (host) (void)hipHostFree(host);
+    }
+};
+static inline GenControl *pinned_heads(int n, GenControl **dev) {
+    thread_local PinnedHeads h;
+    if (h.n < n) {
+        if (h.host) (void)hipHostFree(h.host);
+        h.host = h.dev = nullptr;
+        h.n = 0;
+        GenControl *p = nullptr;
+        void *d = nullptr;
+        const int want = std::max(n, 64);
+        if (hipHostMalloc((void **)&p, sizeof(GenControl) * want, hipHostMallocMapped | hipHostMallocPortable) !=
+            hipSuccess)
+            return nullptr;
+        if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
+            (void)hipHostFree(p);
+            return nullptr;
+        }
+        h.host = p;
+        h.dev = (GenControl *)d;
+        h.n = want;
+    }
+    *dev = h.dev;
+    return h.host;
+}
+
+// What a generation launch of a chain needs besides the kernel's own
+// arguments.
+struct ChainGen {
+    GenLink *link;
+    GenControl *ctl;
+    uint64_t *tile_status;
+    uint64_t tag;
+    GenControl *head_out;
+    int32_t in_pool, store_children;
+};
+
+struct ChainSpec {
+    uint64_t n_in, pool_rays, max_children;
+    int32_t max_generations;
+    int timing_slot;                          // RPX_TIMING_* of the generation kernel
+    double bytes_in, bytes_hit, bytes_child;  // algorithmic bytes per input ray / hit / child
+};
+
+static inline rpx_status_t check_chain(const ChainSpec &c, const rpx_scene_t *scene, void *workspace,
+                                       uint64_t workspace_bytes) {
+    const uint64_t max_gen = std::max<uint64_t>(c.n_in, c.pool_rays);
+    if (max_gen > 0xFFFFFFFFull) return rpx_set_error(RPX_E_UNSUPPORTED, "more than 2^32-1 rays in one generation");
+    if (!workspace || workspace_bytes < chain_workspace_bytes(max_gen, c.max_generations))
+        return rpx_set_error(RPX_E_INVALID, "workspace too small");
+    if ((int)scene->face_global_idx.size() < scene->view.n_list_faces)
+        return rpx_set_error(RPX_E_INVALID, "scene face table");
+    return RPX_OK;
+}
+
+// Runs the chain; `launch(gen, grid, stream)` launches one generation's
+// kernel (grid workgroups) with the chain fields of `gen` and returns the
+// launch error.  counts / offsets / n_traced / face_hits (host) as
+// rpx_trace_rays documents them.
+template <class Launch>
+static rpx_status_t run_chain(const ChainSpec &c, const rpx_scene_t *scene, void *workspace, Launch &&launch,
+                              uint64_t *counts, uint64_t *offsets, int32_t *n_traced, uint32_t *face_hits,
+                              hipStream_t stream) {
+    const int32_t L = c.max_generations;
+    const uint64_t mc = c.max_children, max_gen = std::max<uint64_t>(c.n_in, c.pool_rays);
+    GenControl *heads_dev = nullptr;
+    GenControl *heads = pinned_heads(L + 1, &heads_dev);
+    if (!heads) return rpx_set_error(RPX_E_HIP, "pinned control heads could not be allocated");
+    const ChainWs cw = carve_chain(workspace, max_gen, L);
+    const bool timing = rpx_timing_is_on();
+    static thread_local std::vector<hipEvent_t> evs;
+    // event 2g: generation g starts, 2g+1: it has ended; 2L: the hit fold has ended
+    while (evs.size() <= 2 * (size_t)L) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return rpx_set_error(RPX_E_HIP, "event creation failed");
+        evs.push_back(e);
+    }
+    {
+        const uint64_t tiles = chain_tiles(max_gen);
+        const uint64_t words = std::max<uint64_t>(tiles, sizeof(GenControl) / 4 * (uint64_t)(L + 1));
+        hipLaunchKernelGGL(k_chain_init, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, stream, cw, tiles, L,
+                           c.n_in, heads_dev);
+        RPX_HIP_TRY(hipGetLastError());
+    }
+    // counts[g] is exact once known[g], offsets[g] with it; grid[g]: tiles of
+    // generation g's first launch; a generation is closed (second launch for
+    // tiles past its grid, then its end event) once its size is known and
+    // before the next generation is launched.  Heads are read in order.
+    std::vector<char> known(L + 2, 0), closed(L + 2, 0);
+    std::vector<uint64_t> grid(L + 2, 0);
+    known[0] = 1;
+    int launched = 0, synced = 0;
+    rpx_status_t result = RPX_OK;
+    auto tiles_of = [](uint64_t n) { return (n + RPX_BLOCK - 1) / RPX_BLOCK; };
+    auto gen = [&](int g) {
+        ChainGen cg;
+        cg.link = cw.link + g;
+        cg.ctl = cw.ctl + g;
+        cg.tile_status = cw.tile_status;
+        cg.tag = (uint64_t)g + 1;
+        cg.head_out = heads_dev + g;
+        cg.in_pool = g > 0;
+        cg.store_children = g + 1 < L ? 1 : 0;
+        return cg;
+    };
+    auto close_gen = [&](int g) -> rpx_status_t {
+        if (closed[g]) return RPX_OK;
+        const uint64_t need = tiles_of(counts[g]);
+        if (need > grid[g]) RPX_HIP_TRY(launch(gen(g), need - grid[g], stream));
+        RPX_HIP_TRY(hipEventRecord(evs[2 * g + 1], stream));
+        closed[g] = 1;
+        return RPX_OK;
+    };
+    auto read_heads_through = [&](int last) -> rpx_status_t {
+        for (int g = synced; g <= last; ++g) {
+            if (rpx_status_t st = close_gen(g)) return st;
+            RPX_HIP_TRY(hipEventSynchronize(evs[2 * g + 1]));
+            counts[g + 1] = heads[g].children;
+            offsets[g + 1] = g == 0 ? 0 : offsets[g] + counts[g];
+            known[g + 1] = 1;
+            synced = g + 1;
+        }
+        return RPX_OK;
+    };
+    for (int g = 0; g < L; ++g) {
+        // the size of generation g-1 bounds generation g: learn it from
+        // generation g-2's head and close generation g-1
+        if (g >= 2 && !known[g - 1])
+            if (rpx_status_t st = read_heads_through(g - 2)) return st;
+        if (g >= 1 && known[g - 1] && !closed[g - 1])
+            if (rpx_status_t st = close_gen(g - 1)) return st;
+        if (known[g] && counts[g] == 0) break;
+        if (g >= 1 && counts[g - 1] == 0) break;
+        uint64_t bound = known[g] ? counts[g] : mc * counts[g - 1];
+        if (bound > max_gen) {   // beyond the workspace's tiles: take the exact size
+            if (rpx_status_t st = read_heads_through(g - 1)) return st;
+            if (counts[g] == 0) break;
+            bound = counts[g];
+        }
+        if (g + 1 < L) {
+            // where the children could end: generation g's own place (from
+            // offsets[g], exact once counts[g-1] is) plus its worst case
+            auto need_for = [&](uint64_t b) {
+                const uint64_t start = g == 0 ? 0 : (g == 1 ? 0 : offsets[g - 1] + counts[g - 1]) + b;
+                return start + mc * b;
+            };
+            uint64_t need = need_for(bound);
+            if (need > c.pool_rays && !known[g]) {
+                if (rpx_status_t st = read_heads_through(g - 1)) return st;
+                if (counts[g] == 0) break;
+                bound = counts[g];
+                need = need_for(bound);
+            }
+            if (need > c.pool_rays) {
+                result = rpx_set_error(RPX_E_CAPACITY, "pool of " + std::to_string(c.pool_rays) + " rays cannot "
+                                       "take generation " + std::to_string(g) + "'s children (" +
+                                       std::to_string(need) + " at worst): it is stored and not traced");
+                break;
+            }
+        }
+        // exact when the size is known, else the last observed
+        // children-per-parent ratio applied to the parent generation
+        uint64_t guess;
+        if (known[g]) {
+            guess = counts[g];
+        } else {
+            const double ratio = (g >= 2 && counts[g - 2] > 0) ? (double)counts[g - 1] / (double)counts[g - 2] : 1.0;
+            guess = std::min<uint64_t>(bound, (uint64_t)std::ceil(ratio * (double)counts[g - 1]));
+        }
+        grid[g] = std::max<uint64_t>(tiles_of(guess), 1);
+        if (timing) RPX_HIP_TRY(hipEventRecord(evs[2 * g], stream));
+        RPX_HIP_TRY(launch(gen(g), grid[g], stream));
+        launched = g + 1;
+        if (known[g])
+            if (rpx_status_t st = close_gen(g)) return st;
+    }
+    if (launched > 0) {
+        // the size of the last launched generation (from its parent's head)
+        // closes it; then every generation's hit totals, then every head
+        if (launched >= 2 && synced < launched - 1)
+            if (rpx_status_t st = read_heads_through(launched - 2)) return st;
+        if (rpx_status_t st = close_gen(launched - 1)) return st;
+        hipLaunchKernelGGL(k_chain_fold, dim3((unsigned)launched), dim3(64), 0, stream, (const GenControl *)cw.ctl,
+                           heads_dev, scene->view.n_list_faces);
+        RPX_HIP_TRY(hipGetLastError());
+        RPX_HIP_TRY(hipEventRecord(evs[2 * L], stream));
+        if (launched > synced)
+            if (rpx_status_t st = read_heads_through(launched - 1)) return st;
+        RPX_HIP_TRY(hipEventSynchronize(evs[2 * L]));
+        for (int g = 0; g < launched; ++g)
+            if (heads[g].error)
+                return rpx_set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
+    }
+    int traced = 0;
+    for (int g = 0; g < launched; ++g) {
+        if (counts[g] == 0) break;
+        traced = g + 1;
+    }
+    *n_traced = traced;
+    for (int g = 0; g < traced; ++g)
+        for (int k = 0; k < scene->view.n_list_faces; ++k)
+            face_hits[scene->face_global_idx[k]] += heads[g].face_totals[k];
+    if (timing) {
+        for (int g = 0; g < traced; ++g) {
+            float ms = 0;
+            RPX_HIP_TRY(hipEventElapsedTime(&ms, evs[2 * g], evs[2 * g + 1]));
+            uint64_t hits = 0;
+            for (int s2 = 0; s2 < RPX_SHARDS; ++s2) hits += heads[g].hit_shards[s2];
+            const bool stored = g + 1 < L;
+            rpx_timing_add(c.timing_slot, ms, (double)counts[g] * c.bytes_in + (double)hits * c.bytes_hit +
+                                                  (stored ? (double)counts[g + 1] * c.bytes_child : 0.0));
+        }
+    }
+    return result;
 }
 
 static inline uint32_t scene_variant(const rpx_scene_t *s) {

@@ -24,7 +24,21 @@ struct SplitArgs {
     uint32_t *face_hits;
     Workspace ws;
     int32_t one_list, one_face;
+    // device-resident chain (link = nullptr otherwise), as TraceArgs
+    GenLink *link;
+    uint64_t pool_rays;
+    int32_t in_pool, store_children;
+    uint64_t status_tag;
+    GenControl *head_out;
 };
+
+// The rays from element `off` of an FP32-split SoA buffer on (same stride).
+RPX_DEV rpx_rays_split_t advance_split(rpx_rays_split_t r, uint64_t off) {
+    r.f32 += off;
+    r.f64 += off;
+    r.u32 += off;
+    return r;
+}
 
 // Collects the (at most two) children of one ray as a two-slot shift
 // register (r0 latest), which the compiler keeps in registers.
@@ -47,6 +61,18 @@ void k_generation_split(SplitArgs A) {
     __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
     __shared__ uint32_t lds_tile, lds_hits;
     __shared__ uint64_t lds_excl;
+    // in a device-resident chain, size and pool place come from the link and
+    // workgroups past the generation's tiles leave at once (as k_generation)
+    uint64_t n_in = A.n_in, out_limit = A.out.capacity;
+    rpx_rays_split_t in = A.in, out = A.out;
+    if (A.link) {
+        const GenLink L = A.link[0];
+        n_in = L.count;
+        if ((uint64_t)blockIdx.x * RPX_BLOCK >= n_in) return;
+        if (A.in_pool) in = advance_split(A.in, L.offset);
+        out = advance_split(A.out, L.out_offset);
+        out_limit = A.store_children ? (A.pool_rays > L.out_offset ? A.pool_rays - L.out_offset : 0) : 0;
+    }
     // the tile number is fetched first, so that its round trip overlaps the
     // scene staging
     uint32_t my_tile = 0;
@@ -62,9 +88,9 @@ void k_generation_split(SplitArgs A) {
     const uint64_t i = (uint64_t)tile * RPX_BLOCK + threadIdx.x;
     PairEmitS ce;
     bool hit = false;
-    if (i < A.n_in) {
-        const float *f = A.in.f32;
-        const uint64_t c = A.in.capacity;
+    if (i < n_in) {
+        const float *f = in.f32;
+        const uint64_t c = in.capacity;
         const V3 origin = v3(f[RPX_S_OX * c + i], f[RPX_S_OY * c + i], f[RPX_S_OZ * c + i]);
         const V3 dir = v3(f[RPX_S_DX * c + i], f[RPX_S_DY * c + i], f[RPX_S_DZ * c + i]);
         double length = A.max_length;
@@ -98,15 +124,15 @@ void k_generation_split(SplitArgs A) {
                 else test(face_uniform(A.S.faces, k));
             }
         }
-        A.in.f32[RPX_S_LENGTH * c + i] = (float)length;
-        A.in.u32[RPX_END_FACE_IDX * c + i] = end_face;
+        in.f32[RPX_S_LENGTH * c + i] = (float)length;
+        in.u32[RPX_END_FACE_IDX * c + i] = end_face;
         if (hit_rec >= 0) {
             hit = true;
             atomicAdd(&lds_hist[hit_rec], 1u);
             const rpx_face_t &fc = S.faces[hit_rec];
             if (fc.material >= 0) {
                 const rpx_face_list_t &fl = S.face_lists[hit_list];
-                RayS r = load_ray_s(A.in, i);
+                RayS r = load_ray_s(in, i);
                 r.length = (float)length;
                 r.end_face_idx = end_face;
                 const V3 hp = add(origin, mul(dir, length));
@@ -123,18 +149,28 @@ void k_generation_split(SplitArgs A) {
         if ((threadIdx.x & 63) == 0) atomicAdd(&lds_hits, (uint32_t)__popcll(mh));
     }
     if (threadIdx.x < 64) {
-        const uint64_t excl = lookback(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error);
+        const uint64_t excl = lookback_tagged(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error, A.status_tag);
         if (threadIdx.x == 0) lds_excl = excl;
     }
     __syncthreads();
     const uint64_t pos = lds_excl + local;
-    if (count > 0 && pos < A.out.capacity) store_ray_s(A.out, pos, ce.first());
-    if (count > 1 && pos + 1 < A.out.capacity) store_ray_s(A.out, pos + 1, ce.r0);
+    if (count > 0 && pos < out_limit) store_ray_s(out, pos, ce.first());
+    if (count > 1 && pos + 1 < out_limit) store_ray_s(out, pos + 1, ce.r0);
     if (threadIdx.x == 0) {
         add_hits(A.ws.ctl, tile, lds_hits);
-        if ((uint64_t)tile == (A.n_in - 1) / RPX_BLOCK) A.ws.ctl->children = lds_excl + block_total;
+        if ((uint64_t)tile == (n_in - 1) / RPX_BLOCK) {
+            const uint64_t kids = lds_excl + block_total;
+            A.ws.ctl->children = kids;
+            if (A.link) {
+                const uint64_t at = A.link[0].out_offset;
+                A.link[1].count = kids;
+                A.link[1].offset = at;
+                A.link[1].out_offset = at + kids;
+                A.head_out->children = kids;   // host-mapped: read once the generation has ended
+            }
+        }
     }
-    if (A.face_hits) {
+    if (A.face_hits || A.link) {
         uint32_t *shard = A.ws.ctl->face_shards[tile % RPX_SHARDS];
         for (int k = threadIdx.x; k < S.n_list_faces; k += blockDim.x) {
             uint32_t h = lds_hist[k];
@@ -255,6 +291,12 @@ extern "C" rpx_status_t rpx_trace_segment_split(const rpx_scene_t *scene, int32_
     A.ws = carve(workspace, n_in);
     A.one_list = face_list < 0 ? -1 : face_list;
     A.one_face = face_list < 0 ? -1 : face_idx;
+    A.link = nullptr;
+    A.pool_rays = 0;
+    A.in_pool = 0;
+    A.store_children = 1;
+    A.status_tag = 0;
+    A.head_out = nullptr;
     const uint64_t nb = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
     static thread_local EventSet ev;
     const bool timing = rpx_timing_is_on() && ev.ok;
@@ -289,4 +331,65 @@ extern "C" rpx_status_t rpx_trace_segment_split(const rpx_scene_t *scene, int32_
         return rpx_set_error(RPX_E_CAPACITY, "output capacity " + std::to_string(out->capacity) + " below " +
                                                  std::to_string(ctl.children) + " children");
     return RPX_OK;
+}
+
+// rpx_trace_rays_split: the FP32-split generations of trace_rays as one
+// device-resident chain (run_chain in rpx_gen.h).
+extern "C" rpx_status_t rpx_trace_rays_split(const rpx_scene_t *scene, const rpx_rays_split_t *in, uint64_t n_in,
+                                             float max_length, const double *wavelengths, uint64_t n_wavelengths,
+                                             int32_t max_generations, rpx_rays_split_t *pool, uint64_t *counts,
+                                             uint64_t *offsets, int32_t *n_traced, uint32_t *face_hits,
+                                             void *workspace, uint64_t workspace_bytes, void *stream_) {
+    if (!n_traced || !counts || !offsets)
+        return rpx_set_error(RPX_E_INVALID, "counts / offsets / n_traced are null");
+    *n_traced = 0;
+    counts[0] = n_in;
+    offsets[0] = 0;
+    if (rpx_status_t st = rpx_check_trace_scene(scene)) return st;
+    if (rpx_status_t st = check_split(in, n_in, "input rays")) return st;
+    if (!pool || (pool->capacity > 0 && (!pool->f32 || !pool->f64 || !pool->u32)))
+        return rpx_set_error(RPX_E_INVALID, "pool");
+    if (max_generations < 0) return rpx_set_error(RPX_E_INVALID, "max_generations is negative");
+    if (n_in == 0 || max_generations == 0) return RPX_OK;
+    if (!wavelengths && n_wavelengths) return rpx_set_error(RPX_E_INVALID, "wavelength table is null");
+    if (!face_hits && scene->view.n_list_faces > 0) return rpx_set_error(RPX_E_INVALID, "face_hits is null");
+    ChainSpec spec;
+    spec.n_in = n_in;
+    spec.pool_rays = pool->capacity;
+    spec.max_generations = max_generations;
+    spec.max_children = (uint64_t)std::max(scene->view.max_children, 0);
+    spec.timing_slot = RPX_TIMING_SPLIT;
+    spec.bytes_in = 24.0 + 8.0;    // FP32-split geometry read, length + end face written
+    spec.bytes_hit = 64.0;
+    spec.bytes_child = 112.0;
+    if (rpx_status_t st = check_chain(spec, scene, workspace, workspace_bytes)) return st;
+    const uint32_t variant = split_variant(scene);
+    SplitArgs A;
+    A.S = scene->view;
+    A.out = *pool;
+    A.n_in = 0;
+    A.max_length = (double)max_length;
+    A.wavelengths.wl = wavelengths;
+    A.wavelengths.n = wavelengths ? n_wavelengths : 0;
+    A.face_hits = nullptr;
+    A.ws.tags = nullptr;
+    A.ws.block_totals = nullptr;
+    A.ws.block_parents = nullptr;
+    A.ws.block_offsets = nullptr;
+    A.one_list = -1;
+    A.one_face = -1;
+    A.pool_rays = pool->capacity;
+    auto launch = [&](const ChainGen &cg, uint64_t grid, hipStream_t stream) {
+        A.link = cg.link;
+        A.ws.ctl = cg.ctl;
+        A.ws.tile_status = cg.tile_status;
+        A.status_tag = cg.tag;
+        A.head_out = cg.head_out;
+        A.in_pool = cg.in_pool;
+        A.in = cg.in_pool ? *pool : *in;
+        A.store_children = cg.store_children;
+        launch_split(variant, dim3((unsigned)grid), stream, A);
+        return hipGetLastError();
+    };
+    return run_chain(spec, scene, workspace, launch, counts, offsets, n_traced, face_hits, (hipStream_t)stream_);
 }

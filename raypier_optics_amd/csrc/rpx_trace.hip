@@ -951,110 +951,12 @@ extern "C" rpx_status_t rpx_trace_gausslet(const rpx_scene_t *scene, int32_t fac
 }
 
 // ------------------------------------------------ device-resident chain
-// rpx_trace_rays: every generation of core/tracer.py:9-47 launched back to
-// back on the stream.  Generation g reads its size and pool position from
-// link[g] (written by generation g-1's last tile) and writes its children's
-// to link[g+1]; its grid is sized by a host-side bound, and workgroups past
-// the real tiles leave at once.  The host learns generation g's size only
-// after generation g-1 has finished, which it waits for while generation g
-// runs (so the GPU is never idle on the host), and it launches generation g
-// only when the pool is sure to hold its children under the bound it has;
-// if not, it waits for the exact size, and if the pool still cannot take
-// the worst case it stops and returns RPX_E_CAPACITY with generation g
-// stored and untraced, for the caller to continue with a fresh pool.
-
-// Chain workspace: the tile status array (shared by every generation, words
-// tagged with the generation), one control block per generation and the
-// generation links.
-struct ChainWs {
-    uint64_t *tile_status;
-    GenControl *ctl;
-    GenLink *link;
-};
-static uint64_t chain_tiles(uint64_t max_rays) { return (max_rays + RPX_BLOCK - 1) / RPX_BLOCK; }
-static ChainWs carve_chain(void *ws, uint64_t max_rays, int32_t max_generations) {
-    char *p = (char *)ws;
-    ChainWs w;
-    w.tile_status = (uint64_t *)p;
-    p += align256(8 * std::max<uint64_t>(chain_tiles(max_rays), 1));
-    w.ctl = (GenControl *)p;
-    p += align256(sizeof(GenControl) * (uint64_t)(max_generations + 1));
-    w.link = (GenLink *)p;
-    return w;
-}
-
-// Clears a chain's workspace and host heads; link[0] describes the input.
-static __global__ void k_chain_init(ChainWs w, uint64_t n_tiles, int32_t n_gen, uint64_t n_in, GenControl *heads) {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n_tiles) w.tile_status[k] = 0;
-    const uint64_t ctl_words = sizeof(GenControl) / 4 * (uint64_t)(n_gen + 1);
-    if (k < ctl_words) ((uint32_t *)w.ctl)[k] = 0;
-    const uint64_t head_words = sizeof(GenControl) / 4 * (uint64_t)(n_gen + 1);
-    if (heads && k < head_words) ((uint32_t *)heads)[k] = 0;
-    if (k <= (uint64_t)n_gen) {
-        w.link[k].count = k == 0 ? n_in : 0;
-        w.link[k].offset = 0;
-        w.link[k].out_offset = 0;
-        w.link[k].pad = 0;
-    }
-}
-
-// Per-face and ray hit totals and the look-back error count of every traced
-// generation of a chain into its host head (the generations themselves write
-// only their child count there).  One workgroup per generation.
-static __global__ __launch_bounds__(64) void k_chain_fold(const GenControl *ctl, GenControl *heads,
-                                                           int n_list_faces) {
-    const GenControl &c = ctl[blockIdx.x];
-    GenControl &h = heads[blockIdx.x];
-    for (int k = threadIdx.x; k < n_list_faces; k += 64) {
-        uint32_t t = 0;
-        for (int q = 0; q < RPX_SHARDS; ++q) t += c.face_shards[q][k];
-        h.face_totals[k] = t;
-    }
-    if (threadIdx.x < RPX_SHARDS) h.hit_shards[threadIdx.x] = c.hit_shards[threadIdx.x];
-    if (threadIdx.x == 0) h.error = c.error;
-}
-
-// Pinned, device-mapped control heads, one per generation of a chain, per
-// host thread; grown on demand and held until the thread ends.
-struct PinnedHeads {
-    GenControl *host = nullptr, *dev = nullptr;
-    int n = 0;
-    ~PinnedHeads() {
-        if (host) (void)hipHostFree(host);
-    }
-};
-static GenControl *pinned_heads(int n, GenControl **dev) {
-    thread_local PinnedHeads h;
-    if (h.n < n) {
-        if (h.host) (void)hipHostFree(h.host);
-        h.host = h.dev = nullptr;
-        h.n = 0;
-        GenControl *p = nullptr;
-        void *d = nullptr;
-        const int want = std::max(n, 64);
-        if (hipHostMalloc((void **)&p, sizeof(GenControl) * want, hipHostMallocMapped | hipHostMallocPortable) !=
-            hipSuccess)
-            return nullptr;
-        if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
-            (void)hipHostFree(p);
-            return nullptr;
-        }
-        h.host = p;
-        h.dev = (GenControl *)d;
-        h.n = want;
-    }
-    *dev = h.dev;
-    return h.host;
-}
-
 extern "C" uint64_t rpx_chain_workspace_bytes(uint64_t max_rays_per_generation, int32_t max_generations) {
-    if (max_generations < 1) max_generations = 1;
-    return align256(8 * std::max<uint64_t>(chain_tiles(max_rays_per_generation), 1)) +
-           align256(sizeof(GenControl) * (uint64_t)(max_generations + 1)) +
-           align256(sizeof(GenLink) * (uint64_t)(max_generations + 1));
+    return chain_workspace_bytes(max_rays_per_generation, max_generations);
 }
 
+// rpx_trace_rays: the float64 generations of trace_rays as one device-
+// resident chain (run_chain in rpx_gen.h).
 extern "C" rpx_status_t rpx_trace_rays(const rpx_scene_t *scene, const rpx_rays_t *in, uint64_t n_in,
                                        float max_length, const double *wavelengths, uint64_t n_wavelengths,
                                        int32_t max_generations, rpx_rays_t *pool, uint64_t *counts,
@@ -1070,202 +972,46 @@ extern "C" rpx_status_t rpx_trace_rays(const rpx_scene_t *scene, const rpx_rays_
     if (max_generations < 0) return set_error(RPX_E_INVALID, "max_generations is negative");
     if (n_in == 0 || max_generations == 0) return RPX_OK;
     if (!wavelengths && n_wavelengths) return set_error(RPX_E_INVALID, "wavelength table is null");
-    const uint64_t mc = (uint64_t)std::max(scene->view.max_children, 0);
-    // the per-generation workspace must cover the largest generation the
-    // pool can hold (or the input); a generation's grid is at most that
-    const uint64_t max_gen = std::max<uint64_t>(n_in, pool->capacity);
-    if (max_gen > 0xFFFFFFFFull) return set_error(RPX_E_UNSUPPORTED, "more than 2^32-1 rays in one generation");
-    if (!workspace || workspace_bytes < rpx_chain_workspace_bytes(max_gen, max_generations))
-        return set_error(RPX_E_INVALID, "workspace too small");
-    hipStream_t stream = (hipStream_t)stream_;
-    GenControl *heads_dev = nullptr;
-    GenControl *heads = pinned_heads(max_generations + 1, &heads_dev);
-    if (!heads) return set_error(RPX_E_HIP, "pinned control heads could not be allocated");
-    const ChainWs cw = carve_chain(workspace, max_gen, max_generations);
     if (!face_hits && scene->view.n_list_faces > 0) return set_error(RPX_E_INVALID, "face_hits is null");
-    bool timing;
-    {
-        std::lock_guard<std::mutex> g(g_timing_mutex);
-        timing = g_timing_on;
-    }
-    static thread_local std::vector<hipEvent_t> evs;
-    auto event = [&](size_t k) -> hipEvent_t {
-        while (evs.size() <= k) {
-            hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return nullptr;
-            evs.push_back(e);
-        }
-        return evs[k];
-    };
-    // event 2g: generation g's kernel starts, 2g+1: it has ended (its head is written)
-    if (!event(2 * (size_t)max_generations + 1)) return set_error(RPX_E_HIP, "event creation failed");
-    {
-        const uint64_t tiles = chain_tiles(max_gen);
-        const uint64_t words = std::max<uint64_t>(tiles, sizeof(GenControl) / 4 * (uint64_t)(max_generations + 1));
-        hipLaunchKernelGGL(k_chain_init, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, stream, cw, tiles,
-                           max_generations, n_in, heads_dev);
-        HIP_TRY(hipGetLastError());
-    }
+    ChainSpec spec;
+    spec.n_in = n_in;
+    spec.pool_rays = pool->capacity;
+    spec.max_generations = max_generations;
+    spec.max_children = (uint64_t)std::max(scene->view.max_children, 0);
+    spec.timing_slot = RPX_TIMING_GENERATION;
+    spec.bytes_in = 48.0 + 12.0;    // geometry read, length + end face written
+    spec.bytes_hit = 100.0;         // the rest of a ray that hit
+    spec.bytes_child = 188.0;       // a child record
+    if (rpx_status_t st = check_chain(spec, scene, workspace, workspace_bytes)) return st;
     const uint32_t variant = scene_variant(scene);
     TraceArgs A;
     A.S = scene->view;
-    A.in = *in;
     A.out = *pool;
     A.n_in = 0;
     A.max_length = (double)max_length;
     A.wavelengths.wl = wavelengths;
     A.wavelengths.n = wavelengths ? n_wavelengths : 0;
-    A.face_hits = nullptr;          // per-face totals come back in the heads
+    A.face_hits = nullptr;           // per-face totals come back in the heads
     A.ws.tags = nullptr;
     A.ws.block_totals = nullptr;
     A.ws.block_parents = nullptr;
     A.ws.block_offsets = nullptr;
-    A.ws.tile_status = cw.tile_status;
     A.one_list = -1;
     A.one_face = -1;
     A.pool_rays = pool->capacity;
-    // counts[g] is exact once known[g]; offsets[g] with it (generation g's
-    // place in the pool, g >= 1).  grid[g]: tiles of generation g's first
-    // launch, a guess when counts[g] was not known yet; a generation is
-    // closed (a second launch for tiles past the guess, then its done event)
-    // once its size is known, before the next generation is launched.
-    // Heads are read in generation order.
-    std::vector<char> known(max_generations + 2, 0), closed(max_generations + 2, 0);
-    std::vector<uint64_t> grid(max_generations + 2, 0);
-    known[0] = 1;
-    int launched = 0, synced = 0;   // generations launched; heads read (generations 0..synced-1)
-    rpx_status_t result = RPX_OK;
-    auto tiles_of = [](uint64_t n) { return (n + RPX_BLOCK - 1) / RPX_BLOCK; };
-    auto set_gen = [&](int g) {
-        A.link = cw.link + g;
-        A.ws.ctl = cw.ctl + g;
-        A.status_tag = (uint64_t)g + 1;
-        A.head_out = heads_dev + g;
-        A.in_pool = g > 0;
-        A.in = g > 0 ? *pool : *in;
-        A.store_children = g + 1 < max_generations ? 1 : 0;
+    auto launch = [&](const ChainGen &cg, uint64_t grid, hipStream_t stream) {
+        A.link = cg.link;
+        A.ws.ctl = cg.ctl;
+        A.ws.tile_status = cg.tile_status;
+        A.status_tag = cg.tag;
+        A.head_out = cg.head_out;
+        A.in_pool = cg.in_pool;
+        A.in = cg.in_pool ? *pool : *in;
+        A.store_children = cg.store_children;
+        launch_generation(variant, dim3((unsigned)grid), stream, A);
+        return hipGetLastError();
     };
-    auto close_gen = [&](int g) -> rpx_status_t {
-        if (closed[g]) return RPX_OK;
-        const uint64_t need = tiles_of(counts[g]);
-        if (need > grid[g]) {
-            set_gen(g);
-            launch_generation(variant, dim3((unsigned)(need - grid[g])), stream, A);
-            HIP_TRY(hipGetLastError());
-        }
-        HIP_TRY(hipEventRecord(evs[2 * g + 1], stream));
-        closed[g] = 1;
-        return RPX_OK;
-    };
-    auto read_heads_through = [&](int last) -> rpx_status_t {
-        for (int g = synced; g <= last; ++g) {
-            if (rpx_status_t st = close_gen(g)) return st;
-            HIP_TRY(hipEventSynchronize(evs[2 * g + 1]));
-            const GenControl &h = heads[g];
-            if (h.error) return set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
-            counts[g + 1] = h.children;
-            offsets[g + 1] = g == 0 ? 0 : offsets[g] + counts[g];
-            known[g + 1] = 1;
-            synced = g + 1;
-        }
-        return RPX_OK;
-    };
-    for (int g = 0; g < max_generations; ++g) {
-        // the size of generation g-1 bounds generation g: learn it from
-        // generation g-2's head (which has finished or is about to while
-        // generation g-1 keeps the GPU busy), and close generation g-1
-        if (g >= 2 && !known[g - 1])
-            if (rpx_status_t st = read_heads_through(g - 2)) return st;
-        if (g >= 1 && known[g - 1] && !closed[g - 1])
-            if (rpx_status_t st = close_gen(g - 1)) return st;
-        if (known[g] && counts[g] == 0) break;
-        if (g >= 1 && counts[g - 1] == 0) break;
-        uint64_t bound = known[g] ? counts[g] : mc * counts[g - 1];
-        if (bound > max_gen) {   // beyond the workspace's tiles: take the exact size
-            if (rpx_status_t st = read_heads_through(g - 1)) return st;
-            if (counts[g] == 0) break;
-            bound = counts[g];
-        }
-        if (g + 1 < max_generations) {
-            // where the children could end: generation g's own place (from
-            // offsets[g], exact once counts[g-1] is) plus its worst case
-            auto need_for = [&](uint64_t b) {
-                const uint64_t start = g == 0 ? 0 : (g == 1 ? 0 : offsets[g - 1] + counts[g - 1]) + b;
-                return start + mc * b;
-            };
-            uint64_t need = need_for(bound);
-            if (need > pool->capacity && !known[g]) {
-                if (rpx_status_t st = read_heads_through(g - 1)) return st;
-                if (counts[g] == 0) break;
-                bound = counts[g];
-                need = need_for(bound);
-            }
-            if (need > pool->capacity) {
-                result = set_error(RPX_E_CAPACITY, "pool of " + std::to_string(pool->capacity) + " rays cannot "
-                                   "take generation " + std::to_string(g) + "'s children (" + std::to_string(need) +
-                                   " at worst): it is stored and not traced");
-                break;
-            }
-        }
-        // the grid: exact when the size is known, else the last observed
-        // children-per-parent ratio applied to the parent generation
-        uint64_t guess;
-        if (known[g]) {
-            guess = counts[g];
-        } else {
-            const double ratio = (g >= 2 && counts[g - 2] > 0) ? (double)counts[g - 1] / (double)counts[g - 2] : 1.0;
-            guess = std::min<uint64_t>(bound, (uint64_t)std::ceil(ratio * (double)counts[g - 1]));
-        }
-        grid[g] = std::max<uint64_t>(tiles_of(guess), 1);
-        set_gen(g);
-        if (timing) HIP_TRY(hipEventRecord(evs[2 * g], stream));
-        launch_generation(variant, dim3((unsigned)grid[g]), stream, A);
-        HIP_TRY(hipGetLastError());
-        launched = g + 1;
-        if (known[g])
-            if (rpx_status_t st = close_gen(g)) return st;
-    }
-    if (launched > 0) {
-        // the size of the last launched generation (from its parent's head)
-        // closes it; then every generation's hit totals, then every head
-        if (launched >= 2 && synced < launched - 1)
-            if (rpx_status_t st = read_heads_through(launched - 2)) return st;
-        if (rpx_status_t st = close_gen(launched - 1)) return st;
-        hipLaunchKernelGGL(k_chain_fold, dim3((unsigned)launched), dim3(64), 0, stream, (const GenControl *)cw.ctl,
-                           heads_dev, scene->view.n_list_faces);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(evs[2 * max_generations], stream));
-        if (launched > synced)
-            if (rpx_status_t st = read_heads_through(launched - 1)) return st;
-        HIP_TRY(hipEventSynchronize(evs[2 * max_generations]));
-        for (int g = 0; g < launched; ++g)
-            if (heads[g].error)
-                return set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
-    }
-    int traced = 0;
-    for (int g = 0; g < launched; ++g) {
-        if (counts[g] == 0) break;
-        traced = g + 1;
-    }
-    *n_traced = traced;
-    for (int g = 0; g < traced; ++g)
-        for (int k = 0; k < scene->view.n_list_faces; ++k)
-            face_hits[scene->face_global_idx[k]] += heads[g].face_totals[k];
-    if (timing) {
-        for (int g = 0; g < traced; ++g) {
-            float ms = 0;
-            HIP_TRY(hipEventElapsedTime(&ms, evs[2 * g], evs[2 * g + 1]));
-            uint64_t hits = 0;
-            for (int s2 = 0; s2 < RPX_SHARDS; ++s2) hits += heads[g].hit_shards[s2];
-            const bool stored = g + 1 < max_generations;
-            std::lock_guard<std::mutex> lk(g_timing_mutex);
-            g_timing_ms[RPX_TIMING_GENERATION] += ms;
-            g_timing_launches[RPX_TIMING_GENERATION] += 1;
-            g_timing_bytes[RPX_TIMING_GENERATION] += (double)counts[g] * (48.0 + 12.0) + (double)hits * 100.0 +
-                                                     (stored ? (double)counts[g + 1] * 188.0 : 0.0);
-        }
-    }
-    return result;
+    return run_chain(spec, scene, workspace, launch, counts, offsets, n_traced, face_hits, (hipStream_t)stream_);
 }
 
 // ------------------------------------------------------ AoS <-> SoA

@@ -61,6 +61,32 @@ class SplitRays(object):
         return _abi.RaysSplit(base + o32, base + o64, base + ou, self.capacity)
 
 
+class SplitRaysView(SplitRays):
+    """Rays [offset, offset + n) of a pool SplitRays (same buffer and field
+    stride): a generation written into a pool, without a copy.  Holds a
+    reference to the pool."""
+
+    def __init__(self, pool, offset, n):
+        self.pool = pool
+        self.offset = int(offset)
+        self.capacity = int(n)
+        self.buf = pool.buf
+
+    def f64(self):
+        return self.pool.f64()[:, self.offset: self.offset + self.capacity]
+
+    def f32(self):
+        return self.pool.f32()[:, self.offset: self.offset + self.capacity]
+
+    def u32(self):
+        return self.pool.u32()[:, self.offset: self.offset + self.capacity]
+
+    def struct(self):
+        s = self.pool.struct()
+        o = self.offset
+        return _abi.RaysSplit(s.f32 + 4 * o, s.f64 + 8 * o, s.u32 + 4 * o, self.pool.capacity)
+
+
 _F = {name: k for k, name in enumerate(_abi.F64_FIELDS[: _abi.SPLIT_F32])}
 _U = {name: k for k, name in enumerate(_abi.U32_FIELDS)}
 _VEC = {"origin": "ox", "direction": "dx", "normal": "nx", "E_vector": "ex"}
@@ -147,24 +173,66 @@ def _split_generation(ctx, rays, max_length, one_list=-1, one_face=-1):
     return child
 
 
+def _split_chain(ctx, rays, max_length, limit):
+    """The FP32-split generations on the device (rpx_trace_rays_split), with
+    the pool and resume bookkeeping of tracer._trace_chain."""
+    from .tracer import _pool_rays
+    lib = load()
+    traced = [rays]
+    current = rays
+    remaining = limit
+    hits = (ctypes.c_uint32 * max(ctx.tables.n_global_faces, 1))()
+    while remaining > 0 and current.n_rays > 0:
+        n = current.n_rays
+        pool = SplitRays(_pool_rays(n, ctx.max_children, remaining))
+        ws_bytes = int(lib.rpx_chain_workspace_bytes(max(n, pool.capacity), remaining))
+        ws = ctx.chain_workspace(ws_bytes)
+        counts = (ctypes.c_uint64 * (remaining + 1))()
+        offsets = (ctypes.c_uint64 * (remaining + 1))()
+        n_traced = ctypes.c_int32(0)
+        s_in, s_pool = current.dev.struct(), pool.struct()
+        code = lib.rpx_trace_rays_split(ctx.handle, ctypes.byref(s_in), n, float(max_length),
+                                        ctypes.c_void_p(current._wl_dev.data_ptr()), len(current._wavelengths),
+                                        remaining, ctypes.byref(s_pool), counts, offsets, ctypes.byref(n_traced),
+                                        hits, ctypes.c_void_p(ws.data_ptr()), ws_bytes, stream_handle())
+        if code != _abi.RPX_E_CAPACITY:
+            check(code)
+        k_traced = n_traced.value
+        last = k_traced + 1 if code == _abi.RPX_E_CAPACITY else k_traced
+        parent = current
+        for k in range(1, last):
+            gen = SplitRayCollection(SplitRaysView(pool, offsets[k], counts[k]), counts[k], current._wavelengths,
+                                     current._wl_dev)
+            gen.parent = parent
+            traced.append(gen)
+            parent = gen
+        remaining -= k_traced
+        if code != _abi.RPX_E_CAPACITY:
+            break
+        current = traced[-1]
+        if k_traced == 0:
+            raise RuntimeError("device-resident trace made no progress")
+    for g, h in enumerate(hits):
+        if h and g < len(ctx.all_faces):
+            ctx.all_faces[g].count += int(h)
+    return traced
+
+
 def trace_rays_split(input_rays, face_lists, recursion_limit=100, max_length=100.0):
     """FP32-split non-sequential trace of a RayCollection; generation
-    bookkeeping as tracer.trace_rays (core/tracer.py:9-47)."""
-    all_faces = _prepare(input_rays, face_lists, max_length)
+    bookkeeping as tracer.trace_rays (core/tracer.py:9-47).  The generations
+    run device-resident (rpx_trace_rays_split), which writes every input
+    ray's length itself."""
+    all_faces = _prepare(input_rays, face_lists, max_length, reset=False)
     face_sets = list(face_lists)
     for fs in face_sets:
         fs.sync_transforms()
     ctx = _TraceContext(face_sets, all_faces)
     _wavelength_table(input_rays, ctx)
     rays = input_rays if isinstance(input_rays, SplitRayCollection) else SplitRayCollection.from_collection(input_rays)
-    traced = []
-    count = 0
-    while rays.n_rays > 0 and count < recursion_limit:
-        traced.append(rays)
-        rays = _split_generation(ctx, rays, max_length)
-        count += 1
-    ctx.flush_counts()
-    return traced, all_faces
+    if rays.n_rays == 0 or recursion_limit <= 0:
+        return [], all_faces
+    return _split_chain(ctx, rays, max_length, recursion_limit), all_faces
 
 
 def trace_ray_sequence_split(input_rays, face_sequence, recursion_limit=100, max_length=100.0):
