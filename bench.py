@@ -9,8 +9,9 @@ face (end_face_idx != 0xFFFFFFFF).
 Prints one JSON line.  The roofline figure is for the kernel with the largest
 share of the timed kernel time, from HIP events recorded on the library's own
 stream; its algorithmic bytes are counted from the ray record sizes.  The CPU
-baseline is the oracle (single thread, C restatement of the reference) on a
-bounded sample of the same rays.
+baseline is the oracle (C restatement of the reference) in its OpenMP form
+on the host threads this process has, over the whole input, with its
+single-thread figure on a bounded sample beside it.
 """
 import argparse
 import ctypes
@@ -55,21 +56,47 @@ def interactions(traced):
     return total
 
 
-def cpu_baseline(w, sample):
+def host_threads():
+    """Threads for the all-core CPU baseline: the cores this process may run
+    on, at most OMP_NUM_THREADS when that is set (the host's share per GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(n, 1)
+
+
+def cpu_baseline(w, sample, all_core_sample):
+    """The oracle (C restatement of the reference trace) on the host: one
+    thread on the first `sample` rays, and its OpenMP form on all the host
+    threads this process has on the first `all_core_sample` rays; all
+    generations of each."""
     import oracle
     from raypier_optics_amd import workloads
     from raypier_optics_amd._scene import compile_scene
-    sub = w.subset(np.arange(sample))
-    workloads.prepare_faces(sub.face_lists, sub.wavelengths, sub.max_length)
-    tables = compile_scene(sub.face_lists)
-    oracle.trace_rays(tables, sub.rays[:1000], sub.max_length, sub.wavelengths, sub.recursion_limit)   # warm
-    t0 = time.perf_counter()
-    traced, _ = oracle.trace_rays(tables, sub.rays, sub.max_length, sub.wavelengths, sub.recursion_limit)
-    dt = time.perf_counter() - t0
-    n_int = int(sum(np.count_nonzero(g["end_face_idx"] != 0xFFFFFFFF) for g in traced))
-    return {"value": n_int / dt, "unit": "ray-surface interactions/s", "cores": 1, "kind": "port",
-            "sample": "first %d rays of the same workload, all generations (%d interactions, %.1f s)" %
-                      (sample, n_int, dt)}
+
+    def run(n, threads):
+        sub = w.subset(np.arange(n))
+        workloads.prepare_faces(sub.face_lists, sub.wavelengths, sub.max_length)
+        tables = compile_scene(sub.face_lists)
+        oracle.trace_rays(tables, sub.rays[:1000], sub.max_length, sub.wavelengths, sub.recursion_limit,
+                          threads=threads)   # warm
+        t0 = time.perf_counter()
+        traced, _ = oracle.trace_rays(tables, sub.rays, sub.max_length, sub.wavelengths, sub.recursion_limit,
+                                      threads=threads)
+        dt = time.perf_counter() - t0
+        n_int = int(sum(np.count_nonzero(g["end_face_idx"] != 0xFFFFFFFF) for g in traced))
+        return n_int, dt
+
+    threads = host_threads()
+    n1, dt1 = run(sample, 1)
+    nk, dtk = run(all_core_sample, threads)
+    return {"value": nk / dtk, "unit": "ray-surface interactions/s", "cores": threads, "kind": "port",
+            "sample": "first %d rays of the same workload, all generations, oracle OpenMP form on %d threads "
+                      "(%d interactions, %.2f s)" % (all_core_sample, threads, nk, dtk),
+            "nproc": os.cpu_count(), "threads": threads,
+            "one_core": {"value": n1 / dt1, "cores": 1,
+                         "sample": "first %d rays, all generations (%d interactions, %.1f s)" % (sample, n1, dt1)}}
 
 
 def copy_calibration(torch, mib):
@@ -201,9 +228,11 @@ def michelson_c3_leg(n_rays=1 << 20, order=1):
     the first allocation of that much device memory in a process costs up
     to a second when an earlier process on the device has just freed it."""
     import torch
-    from raypier_optics_amd import workloads
+    from raypier_optics_amd import workloads, _abi, _lib
     from raypier_optics_amd.tracer import trace_rays
     from raypier_optics_amd.split import trace_rays_split
+    lib = _lib.load()
+    nk = len(_abi.TIMING_KERNELS)
     out = {"workload": "michelson_c3: N-BK7 beamsplitter cube, flat + Zernike (Noll 4-15) PEC mirrors, "
                        "600 l/mm grating order %+d, relative thresholds 1e-4, depth 12" % order, "rays": n_rays}
     for name, tracer in (("fp64", trace_rays), ("fp32split", trace_rays_split)):
@@ -213,22 +242,55 @@ def michelson_c3_leg(n_rays=1 << 20, order=1):
             rc._ensure_device()
             rc._device_wavelengths()
             torch.cuda.synchronize()
+            if timed_run:
+                lib.rpx_timing_enable(1)
+                lib.rpx_timing_read(None, None, None)
             t0 = time.perf_counter()
             traced, _ = tracer(rc, w.face_lists, recursion_limit=w.recursion_limit, max_length=w.max_length)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             if not timed_run:
                 del traced, rc
+        ms = (ctypes.c_double * nk)()
+        launches = (ctypes.c_uint64 * nk)()
+        nbytes = (ctypes.c_double * nk)()
+        lib.rpx_timing_read(ms, launches, nbytes)
+        lib.rpx_timing_enable(0)
         n_int = interactions(traced)
         out[name] = {"seconds": dt, "interactions": n_int, "interactions_per_s": n_int / dt,
-                     "rays_all_generations": int(sum(g.n_rays for g in traced)), "generations": len(traced)}
+                     "rays_all_generations": int(sum(g.n_rays for g in traced)), "generations": len(traced),
+                     "roofline": roofline({"ms": list(ms), "launches": list(launches), "bytes": list(nbytes)},
+                                          None)}
         del traced, rc
         torch.cuda.empty_cache()
     out["fp32split_speedup"] = out["fp64"]["seconds"] / out["fp32split"]["seconds"]
     return out
 
 
-FIELD_FLOPS_PER_PAIR = 150   # real FP ops of one gausslet-point term with per-gausslet terms hoisted
+def strehl_c2(n_rays, det, wavelength=0.55):
+    """Strehl ratio of the C2 lens for its on-axis bundle at 0.55 um
+    (n_rays on the same 6 mm disk, no field angle), float64 and FP32-split,
+    at the float64 trace's best focus near the detector."""
+    from raypier_optics_amd import workloads, metrics
+    from raypier_optics_amd.tracer import trace_rays
+    from raypier_optics_amd.split import SplitRayCollection, trace_rays_split
+    w = workloads.asphere_doublet_c2(n_rays=n_rays, max_angle_deg=0.0, wl_range=(wavelength, wavelength))
+    rc = w.collection()
+    rc._ensure_device()
+    split_in = SplitRayCollection.from_collection(rc)
+    t64 = trace_rays(rc, w.face_lists, recursion_limit=w.recursion_limit, max_length=w.max_length)[0]
+    t32 = trace_rays_split(split_in, w.face_lists, recursion_limit=w.recursion_limit, max_length=w.max_length)[0]
+    hits, _ = metrics.detector_hits(t64, det)
+    center = [float(x) for x in hits.mean(dim=0)]
+    paths64, paths32 = metrics.detector_paths(t64, det), metrics.detector_paths(t32, det)
+    point, s64 = metrics.best_focus_strehl(paths64, wavelength, center)
+    s32 = metrics.strehl_ratio(paths32, wavelength, point)
+    return {"wavelength_um": wavelength, "rays": n_rays, "detector_rays": int(paths64[0].shape[0]),
+            "best_focus_mm": point, "strehl_fp64": s64, "strehl_fp32split": s32,
+            "strehl_rel_err": abs(s32 - s64) / s64 if s64 > 0 else None}
+
+
+FIELD_FLOPS_PER_PAIR = 172   # real FP ops of one gausslet-point term, per-gausslet terms hoisted (symbolic count in tests/test_flop_count.py), besides exp, sincos and 2 sqrt
 
 
 def field_c4(n_gausslets=262144, grid=512):
@@ -253,13 +315,25 @@ def field_c4(n_gausslets=262144, grid=512):
     wl = out.wavelengths
     cfields.sum_gaussian_modes(out, modes, wl, pts[:256], precision=0)      # warm
     cfields.sum_gaussian_modes(out, modes, wl, pts[:256], precision=1)
-    res = {}
+    from raypier_optics_amd import _abi, _lib
+    lib = _lib.load()
+    nk = len(_abi.TIMING_KERNELS)
+    res, kern = {}, {}
     for prec in (0, 1):
         torch.cuda.synchronize()
+        lib.rpx_timing_enable(1)
+        lib.rpx_timing_read(None, None, None)
         t0 = time.perf_counter()
         e = cfields.sum_gaussian_modes(out, modes, wl, pts, precision=prec, as_tensor=True)
         torch.cuda.synchronize()
         res[prec] = (time.perf_counter() - t0, e.cpu().numpy())
+        ms = (ctypes.c_double * nk)()
+        launches = (ctypes.c_uint64 * nk)()
+        units = (ctypes.c_double * nk)()
+        lib.rpx_timing_read(ms, launches, units)
+        lib.rpx_timing_enable(0)
+        slot = _abi.TIMING_KERNELS.index("k_field_sum" if prec == 0 else "k_field_sum_split")
+        kern[prec] = (ms[slot] / max(launches[slot], 1), units[slot] / max(launches[slot], 1))
     pairs = out.n_rays * len(pts)
     e64, e32 = res[0][1], res[1][1]
     amp = np.abs(e64).max()
@@ -273,9 +347,18 @@ def field_c4(n_gausslets=262144, grid=512):
             "max_abs_dE_over_max_E": float(np.abs(e32 - e64).max() / amp),
             "phase_err_rad_where_E_ge_0p1max": float(np.abs(np.angle(e32[mask] / e64[mask])).max()),
             "focus_mm": [float(x) for x in focus],
-            "roofline": {"bound": "valu", "kernel": "k_field_sum", "achieved": p64 * FIELD_FLOPS_PER_PAIR / 1e12,
-                         "peak": 78.6, "unit": "TFLOP/s", "frac": p64 * FIELD_FLOPS_PER_PAIR / 1e12 / 78.6,
-                         "flops_per_pair": FIELD_FLOPS_PER_PAIR}}
+            "roofline": field_roofline("k_field_sum", kern[0], 78.6),
+            "roofline_split": field_roofline("k_field_sum_split", kern[1], 157.3)}
+
+
+def field_roofline(name, kern, peak):
+    """Compute roofline of a field-sum pair kernel: its symbolic flops per
+    launch (FIELD_FLOPS_PER_PAIR x pairs) over its event-timed launch."""
+    avg_ms, pairs = kern
+    achieved = pairs * FIELD_FLOPS_PER_PAIR / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    return {"bound": "valu", "kernel": name, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "flops_per_pair": FIELD_FLOPS_PER_PAIR, "avg_launch_ms": avg_ms,
+            "pairs_per_launch": pairs}
 
 
 def main():
@@ -336,6 +419,7 @@ def main():
     hit_err = float((p64 - p32).abs().max()) if p64.shape == p32.shape else None
     rel = np.abs(s32 - s64) / s64
     del t64, t32, p64, p32
+    strehl = strehl_c2(args.rays, det)
 
     value = r64["interactions"] * args.steps / r64["seconds"]
     ms_per_step = 1000.0 * r64["seconds"] / args.steps
@@ -349,6 +433,7 @@ def main():
         "wavelength_bin_edges_um": [float(x) for x in edges],
         "detector_hits_per_bin": [int(x) for x in n64],
         "detector_hit_max_abs_err_mm": hit_err,
+        "strehl": strehl,
         "record_bytes": 112, "arithmetic": "f32 state and amplitudes; f64 root solves, normals, directions, "
                                           "optical path and phase",
         "roofline": roofline(r32, measured=args.rays),
@@ -379,7 +464,7 @@ def main():
         "michelson": michelson_c3_leg() if args.michelson else None,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, min(args.cpu_sample, args.rays))
+        out["cpu_baseline"] = cpu_baseline(w, min(args.cpu_sample, args.rays), args.rays)
     print(json.dumps(out))
 
 

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RPX_ABI_VERSION 8
+#define RPX_ABI_VERSION 9
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t rpx_status_t;
@@ -718,7 +718,10 @@ rpx_status_t rpx_lens_spot_grad(const rpx_lens_t *lens, const rpx_rays_t *rays, 
  * kernel's algorithm reads + writes (per the ray record sizes); then resets
  * the totals.  Ray generations run k_clear + k_generation (FP32-split:
  * k_clear + k_generation_split); gausslet generations run k_intersect +
- * k_scan + k_emit. */
+ * k_scan + k_emit.  The two field-sum slots time the pair kernel of
+ * rpx_sum_gaussian_modes (k_field_sum, k_field_sum_split) and count
+ * gausslet x point pairs in `bytes`; while timing is on, that call returns
+ * after its pair kernel has completed. */
 enum {
     RPX_TIMING_INTERSECT = 0,
     RPX_TIMING_SCAN = 1,
@@ -726,7 +729,9 @@ enum {
     RPX_TIMING_GENERATION = 3,
     RPX_TIMING_CLEAR = 4,
     RPX_TIMING_SPLIT = 5,
-    RPX_TIMING_SLOTS = 6
+    RPX_TIMING_FIELD_SUM = 6,
+    RPX_TIMING_FIELD_SUM_SPLIT = 7,
+    RPX_TIMING_SLOTS = 8
 };
 rpx_status_t rpx_timing_enable(int32_t on);
 rpx_status_t rpx_timing_read(double *ms, uint64_t *launches, double *bytes);

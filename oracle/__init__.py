@@ -38,6 +38,9 @@ def load():
         vp, u64, i32, f64 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double
         lib.oracle_trace_segment.restype = u64
         lib.oracle_trace_segment.argtypes = [vp, i32, i32, vp, u64, ctypes.c_float, vp, u64, vp, u64, vp]
+        lib.oracle_trace_segment_mt.restype = u64
+        lib.oracle_trace_segment_mt.argtypes = [vp, i32, i32, vp, u64, ctypes.c_float, vp, u64, vp, u64, vp, u64,
+                                                i32, i32]
         lib.oracle_trace_gausslet.restype = u64
         lib.oracle_trace_gausslet.argtypes = [vp, i32, i32, vp, u64, f64, vp, u64, vp, u64, vp]
         lib.oracle_face_intersect.restype = f64
@@ -49,6 +52,8 @@ def load():
         lib.oracle_eval_child_ray.restype = u64
         lib.oracle_eval_child_ray.argtypes = [vp, i32, vp, ctypes.c_uint32, vp, vp, vp, vp, u64, vp, u64]
         lib.oracle_dispersion.argtypes = [vp, vp, u64, vp]
+        lib.oracle_set_f32_geometry.restype = None
+        lib.oracle_set_f32_geometry.argtypes = [i32]
         lib.oracle_evaluate_modes.argtypes = [vp, vp, vp, vp, u64, i32, f64, vp]
         lib.oracle_neighbours_gc.argtypes = [vp, u64, vp, vp, vp, vp]
         lib.oracle_sum_gaussian_modes.argtypes = [vp, u64, vp, vp, u64, vp, u64, f64, vp]
@@ -87,17 +92,22 @@ class _Desc(object):
         return ctypes.byref(self.desc)
 
 
-def trace_segment(tables, rays, max_length, wavelengths, one_list=-1, one_face=-1, max_children=3):
+def trace_segment(tables, rays, max_length, wavelengths, one_list=-1, one_face=-1, max_children=3, threads=1):
     """One generation; `rays` (ray_dtype) is updated in place.  Returns
-    (children, face_hits)."""
+    (children, face_hits).  threads > 1 runs the OpenMP form (same
+    output)."""
     lib = load()
     d = _Desc(tables)
     wl = np.ascontiguousarray(wavelengths, dtype=np.float64)
     cap = max_children * len(rays) + 1
-    out = np.zeros(cap, rays.dtype)
+    out = np.empty(cap, rays.dtype) if threads > 1 else np.zeros(cap, rays.dtype)
     hits = np.zeros(max(tables.n_global_faces, 1), np.uint32)
-    n = lib.oracle_trace_segment(d.ref, one_list, one_face, _p(rays), len(rays), max_length, _p(wl), len(wl),
-                                 _p(out), cap, _p(hits))
+    if threads > 1:
+        n = lib.oracle_trace_segment_mt(d.ref, one_list, one_face, _p(rays), len(rays), max_length, _p(wl), len(wl),
+                                        _p(out), cap, _p(hits), len(hits), max_children, threads)
+    else:
+        n = lib.oracle_trace_segment(d.ref, one_list, one_face, _p(rays), len(rays), max_length, _p(wl), len(wl),
+                                     _p(out), cap, _p(hits))
     assert n <= cap
     return out[:n].copy(), hits
 
@@ -233,8 +243,9 @@ def trace_gausslet(tables, gs, max_length, wavelengths, one_list=-1, one_face=-1
     return out[:n].copy(), hits
 
 
-def trace_rays(tables, rays, max_length, wavelengths, recursion_limit, gausslet=False):
-    """Generation loop of core/tracer.py:9-47 over an already compiled scene."""
+def trace_rays(tables, rays, max_length, wavelengths, recursion_limit, gausslet=False, threads=1):
+    """Generation loop of core/tracer.py:9-47 over an already compiled scene
+    (ray generations on `threads` OpenMP threads when threads > 1)."""
     rays = rays.copy()
     if gausslet:
         rays["base_ray"]["length"] = max_length
@@ -249,7 +260,7 @@ def trace_rays(tables, rays, max_length, wavelengths, recursion_limit, gausslet=
         if gausslet:
             rays, h = trace_gausslet(tables, rays, max_length, wavelengths)
         else:
-            rays, h = trace_segment(tables, rays, max_length, wavelengths)
+            rays, h = trace_segment(tables, rays, max_length, wavelengths, threads=threads)
         hits += h
         count += 1
     return traced, hits
@@ -339,3 +350,62 @@ def sum_gaussian_modes(rays, modes, wavelengths, points, time_ps=0.0):
     out = np.zeros((len(pts), 3), np.complex128)
     lib.oracle_sum_gaussian_modes(_p(r), len(r), _p(m), _p(wl), len(wl), _p(pts), len(pts), time_ps, _p(out))
     return out
+
+
+def _round_f32(a):
+    """Copy of a record array with every finite float64 field (nested ones
+    too) within float32's range rounded to float32."""
+    a = a.copy()
+
+    def walk(arr):
+        for name in arr.dtype.names or ():
+            sub = arr[name]
+            if sub.dtype.names:
+                walk(sub)
+            elif sub.dtype.base == np.float64:
+                v = np.asarray(sub)
+                ok = np.isfinite(v) & (np.abs(v) < 3e38)
+                v2 = np.where(ok, v.astype(np.float32).astype(np.float64), v)
+                arr[name] = v2
+    walk(a)
+    return a
+
+
+def trace_rays_f32_geometry(tables, rays, max_length, wavelengths, recursion_limit, threads=1):
+    """trace_rays with the oracle's float32-geometry emulation: a
+    measurement of the precision form that holds ray geometry, face
+    parameters and refractive indices in float32 and keeps float64 for the
+    root solves (run here in float64 on the float32-rounded operands),
+    optical path and phase.  Face lists, faces and the materials' fixed
+    indices are rounded to float32; so are the input rays' fields other than
+    phase and optical path.  The emulation rounds each result once where the
+    form stores or computes it in float32, so it understates that form's
+    per-ray rounding (several float32 operations per result), never its
+    systematic part (float32 parameters and indices)."""
+    import copy
+    t = copy.copy(tables)
+    t.face_lists = _round_f32(tables.face_lists)
+    t.faces = _round_f32(tables.faces)
+    mats = tables.materials.copy()
+    for name in ("n_inside", "n_outside"):
+        if name in (mats.dtype.names or ()):
+            mats[name] = mats[name].astype(np.float32).astype(np.float64)
+    t.materials = mats
+    r = rays.copy()
+    keep = {"phase", "accumulated_path"}
+    for name in r.dtype.names:
+        base = r.dtype[name].base
+        if name in keep:
+            continue
+        if base == np.float64:
+            v = r[name]
+            r[name] = np.where(np.isfinite(v), v.astype(np.float32).astype(np.float64), v)
+        elif base == np.complex128:
+            v = r[name]
+            r[name] = v.real.astype(np.float32).astype(np.float64) + 1j * v.imag.astype(np.float32).astype(np.float64)
+    lib = load()
+    lib.oracle_set_f32_geometry(1)
+    try:
+        return trace_rays(t, r, max_length, wavelengths, recursion_limit, threads=threads)
+    finally:
+        lib.oracle_set_f32_geometry(0)

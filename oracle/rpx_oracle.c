@@ -735,9 +735,20 @@ static double disp_curve(const rpx_dispersion_t *d, double wavelen) {  /* cmater
     }
 }
 
+/* Float32-geometry emulation (measurement of a precision form, not part of
+ * the reference): while on, refractive indices, hit points, face
+ * orientations and every stored child field except phase and optical path
+ * are rounded to float32 where that form would hold them in float32 (the
+ * caller rounds the scene's face parameters).  Off by default; set only
+ * between traces. */
+static int g_f32 = 0;
+void oracle_set_f32_geometry(int on) { g_f32 = on; }
+static double r32(double x) { return g_f32 ? (double)(float)x : x; }
+static vec_t r32v(vec_t v) { return g_f32 ? V((float)v.x, (float)v.y, (float)v.z) : v; }
+
 static cplx disp_eval(const rpx_dispersion_t *d, double wvl) {  /* c_evaluate_n :206-228 */
     double n_imag = 0.0001 * d->absorption / (4 * M_PI);
-    return CMPLX(disp_curve(d, wvl), n_imag * wvl);
+    return CMPLX(r32(disp_curve(d, wvl)), r32(n_imag * wvl));
 }
 
 static oray_t convert_to_sp(oray_t ray, vec_t normal) {  /* cmaterials.pyx:49-91 */
@@ -774,8 +785,25 @@ typedef struct {
 } sink_t;
 
 static void add_ray(sink_t *s, const oray_t *r) {
-    if (s->n < s->cap) s->rays[s->n] = *r;
-    else s->overflow = 1;
+    if (s->n < s->cap) {
+        s->rays[s->n] = *r;
+        if (g_f32) {
+            oray_t *o = &s->rays[s->n];
+            o->origin = r32v(o->origin);
+            o->direction = r32v(o->direction);
+            o->normal = r32v(o->normal);
+            o->E_vector = r32v(o->E_vector);
+            o->n_re = r32(o->n_re);
+            o->n_im = r32(o->n_im);
+            o->e1_re = r32(o->e1_re);
+            o->e1_im = r32(o->e1_im);
+            o->e2_re = r32(o->e2_re);
+            o->e2_im = r32(o->e2_im);
+            o->length = r32(o->length);
+        }
+    } else {
+        s->overflow = 1;
+    }
     s->n++;
 }
 
@@ -1377,11 +1405,10 @@ int oracle_lagrange_invariant(const ogausslet_t *gs, uint64_t n, const double *w
 /* trace_segment_c (ctracer.pyx:2062-2118) / trace_one_face_segment_c (:2121-2170).
  * one_list < 0 selects the non-sequential form.  Returns the number of children
  * (written only up to `cap`). */
-uint64_t oracle_trace_segment(const rpx_scene_desc_t *S, int one_list, int one_face, oray_t *rays, uint64_t n,
-                              float max_length, const double *wls, uint64_t n_wl, oray_t *out, uint64_t cap,
-                              uint32_t *face_hits) {
-    sink_t sink = {out, 0, cap, 0};
-    for (uint64_t i = 0; i < n; i++) {
+static void trace_segment_range(const rpx_scene_desc_t *S, int one_list, int one_face, oray_t *rays, uint64_t i0,
+                                uint64_t i1, float max_length, const double *wls, uint64_t n_wl, sink_t *sink,
+                                uint32_t *face_hits) {
+    for (uint64_t i = i0; i < i1; i++) {
         oray_t *ray = rays + i;
         int nearest = -1, nearest_list = -1;
         vec_t point;
@@ -1413,13 +1440,60 @@ uint64_t oracle_trace_segment(const rpx_scene_desc_t *S, int one_list, int one_f
             const rpx_face_t *face = &S->faces[nearest];
             orient_t orient;
             if (face_hits) face_hits[face->global_idx] += 1;
-            point = addvv_(ray->origin, multvs_(ray->direction, ray->length));
+            point = r32v(addvv_(ray->origin, multvs_(ray->direction, ray->length)));
             orient = compute_orientation(S, &S->face_lists[nearest_list], face, point);
+            orient.normal = r32v(orient.normal);
+            orient.tangent = r32v(orient.tangent);
             if (face->material >= 0)
-                eval_child_ray(S, &S->materials[face->material], ray, (uint32_t)i, point, orient, wls, n_wl, &sink);
+                eval_child_ray(S, &S->materials[face->material], ray, (uint32_t)i, point, orient, wls, n_wl, sink);
         }
     }
+}
+
+uint64_t oracle_trace_segment(const rpx_scene_desc_t *S, int one_list, int one_face, oray_t *rays, uint64_t n,
+                              float max_length, const double *wls, uint64_t n_wl, oray_t *out, uint64_t cap,
+                              uint32_t *face_hits) {
+    sink_t sink = {out, 0, cap, 0};
+    trace_segment_range(S, one_list, one_face, rays, 0, n, max_length, wls, n_wl, &sink, face_hits);
     return sink.n;
+}
+
+/* The same generation over `threads` OpenMP threads: each traces a
+ * contiguous block of rays into its own part of `out` (max_children slots
+ * per ray, so cap >= max_children * n), then the parts are moved down in
+ * block order.  Output and hit counts equal oracle_trace_segment's.
+ * Returns UINT64_MAX if cap is too small or a block overflowed. */
+uint64_t oracle_trace_segment_mt(const rpx_scene_desc_t *S, int one_list, int one_face, oray_t *rays, uint64_t n,
+                                 float max_length, const double *wls, uint64_t n_wl, oray_t *out, uint64_t cap,
+                                 uint32_t *face_hits, uint64_t n_hits, int max_children, int threads) {
+    if (threads < 1) threads = 1;
+    if ((uint64_t)max_children * n > cap) return UINT64_MAX;
+    uint64_t counts[256] = {0}, starts[257] = {0};
+    int bad = 0;
+    if (threads > 256) threads = 256;
+    uint32_t *hits = calloc((size_t)threads * (n_hits ? n_hits : 1), sizeof(uint32_t));
+    if (!hits) return UINT64_MAX;
+    for (int t = 0; t <= threads; t++) starts[t] = n * (uint64_t)t / (uint64_t)threads;
+#pragma omp parallel for num_threads(threads) schedule(static, 1) reduction(| : bad)
+    for (int t = 0; t < threads; t++) {
+        sink_t sink = {out + (uint64_t)max_children * starts[t], 0,
+                       (uint64_t)max_children * (starts[t + 1] - starts[t]), 0};
+        trace_segment_range(S, one_list, one_face, rays, starts[t], starts[t + 1], max_length, wls, n_wl, &sink,
+                            hits + (size_t)t * n_hits);
+        counts[t] = sink.n;
+        bad |= sink.n > sink.cap;
+    }
+    uint64_t total = 0;
+    if (!bad) {
+        for (int t = 0; t < threads; t++) {
+            memmove(out + total, out + (uint64_t)max_children * starts[t], counts[t] * sizeof(oray_t));
+            total += counts[t];
+        }
+        for (int t = 0; t < threads; t++)
+            for (uint64_t k = 0; k < n_hits; k++) face_hits[k] += hits[(size_t)t * n_hits + k];
+    }
+    free(hits);
+    return bad ? UINT64_MAX : total;
 }
 
 /* trace_gausslet_c (ctracer.pyx:2214-2281) with trace_parabasal_rays (:2350-2385);

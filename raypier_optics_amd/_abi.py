@@ -33,10 +33,11 @@ gausslet_dtype = np.dtype([('base_ray', ray_dtype),
 
 assert ray_dtype.itemsize == 188 and para_dtype.itemsize == 80 and gausslet_dtype.itemsize == 668
 
-ABI_VERSION = 8          # RPX_ABI_VERSION of include/rpx.h
+ABI_VERSION = 9          # RPX_ABI_VERSION of include/rpx.h
 
 # timing slots (RPX_TIMING_*)
-TIMING_KERNELS = ("k_intersect", "k_scan", "k_emit", "k_generation", "k_clear", "k_generation_split")
+TIMING_KERNELS = ("k_intersect", "k_scan", "k_emit", "k_generation", "k_clear", "k_generation_split",
+                  "k_field_sum", "k_field_sum_split")
 
 REFL_RAY = 1 << 0
 GAUSSLET = 1 << 1

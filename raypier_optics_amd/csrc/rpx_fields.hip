@@ -829,10 +829,12 @@ static uint64_t field_sum_chunk(uint64_t n_rays, uint64_t n_pts, int32_t precisi
 // pair kernel over gausslet chunks of `chunk` (partial sums into `part`,
 // 6 * n_pts doubles per chunk, when there is more than one chunk), then the
 // chunk-ordered reduction into out.  n_rays > 0 and n_pts > 0.
+// With `timed`, the pair kernel is bracketed by events whose elapsed time is
+// added to its timing slot once the stream has completed.
 static hipError_t field_sum_launch(const rpx_rays_t *rays, uint64_t n_rays, const double *modes, WlTable wl,
                                    const double *points, uint64_t n_pts, double time_ps, double *out,
                                    int32_t precision, uint64_t chunk, double *gp, double *part,
-                                   hipStream_t stream) {
+                                   hipStream_t stream, bool timed = false) {
     const double ct = 0.299792458 * time_ps;
     hipLaunchKernelGGL(k_field_prep, dim3((unsigned)((n_rays + 255) / 256)), dim3(256), 0, stream, *rays, n_rays,
                        modes, wl, ct, gp);
@@ -841,12 +843,24 @@ static hipError_t field_sum_launch(const rpx_rays_t *rays, uint64_t n_rays, cons
     const uint64_t n_chunks = (n_rays + chunk - 1) / chunk;
     double *dst = n_chunks > 1 ? part : out;
     const dim3 grid((unsigned)((n_pts + FS_BLOCK - 1) / FS_BLOCK), (unsigned)n_chunks);
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (timed && (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess)) timed = false;
+    if (timed) (void)hipEventRecord(ev[0], stream);
     if (precision == 0)
         hipLaunchKernelGGL(k_field_sum, grid, dim3(FS_BLOCK), 0, stream, gp, n_rays, chunk, points, n_pts, dst);
     else
         hipLaunchKernelGGL(k_field_sum_split, grid, dim3(FS_BLOCK), 0, stream, gp, n_rays, chunk, points, n_pts,
                            dst);
     e = hipGetLastError();
+    if (timed) {
+        float ms = 0;
+        if (e == hipSuccess && hipEventRecord(ev[1], stream) == hipSuccess &&
+            hipEventSynchronize(ev[1]) == hipSuccess && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess)
+            rpx_timing_add(precision == 0 ? RPX_TIMING_FIELD_SUM : RPX_TIMING_FIELD_SUM_SPLIT, ms,
+                           (double)n_rays * (double)n_pts);
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+    }
     if (e != hipSuccess || n_chunks == 1) return e;
     hipLaunchKernelGGL(k_field_reduce, dim3((unsigned)((6 * n_pts + 255) / 256)), dim3(256), 0, stream,
                        (const double *)part, (uint32_t)n_chunks, 6 * n_pts, out);
@@ -874,7 +888,8 @@ extern "C" rpx_status_t rpx_sum_gaussian_modes(const rpx_rays_t *rays, uint64_t 
     hipError_t e = hipSuccess;
     if (n_chunks > 1) e = hipMallocAsync((void **)&part, sizeof(double) * 6 * n_pts * n_chunks, stream);
     if (e == hipSuccess)
-        e = field_sum_launch(rays, n_rays, modes, wl, points, n_pts, time_ps, out, precision, chunk, gp, part, stream);
+        e = field_sum_launch(rays, n_rays, modes, wl, points, n_pts, time_ps, out, precision, chunk, gp, part, stream,
+                             rpx_timing_is_on());
     hipError_t e2 = hipFreeAsync(gp, stream);
     if (part) {
         hipError_t e3 = hipFreeAsync(part, stream);

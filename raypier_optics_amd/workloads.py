@@ -174,7 +174,11 @@ def singlet_c1(n_rays=10000, depth=4):
     return Workload("singlet_c1", rays, [wl], [lens, det], depth, 100.0, info)
 
 
-def asphere_doublet_c2(n_rays=4194304, seed=12345):
+def asphere_doublet_c2(n_rays=4194304, seed=12345, max_angle_deg=2.0, wl_range=(0.45, 0.65)):
+    """The aspheric + achromat system: 4,194,304 rays on a 6 mm disk with
+    angles U(+/-2 deg) and per-ray wavelengths U(0.45, 0.65) um by default;
+    max_angle_deg=0 and wl_range=(0.55, 0.55) give its on-axis
+    monochromatic bundle."""
     air = NondispersiveCurve(1.0)
     bk7, sf5 = glass(N_BK7), glass(N_SF5)
     mgf2 = NondispersiveCurve(1.38)
@@ -217,7 +221,7 @@ def asphere_doublet_c2(n_rays=4194304, seed=12345):
     focus = z_prev - y / u
     det = FaceList()
     det.faces = [CircularFace(diameter=80.0, z_plane=focus, material=OpaqueMaterial())]
-    rays, wl = disk_field(n_rays, 6.0, -5.0, 2.0, (0.45, 0.65), seed)
+    rays, wl = disk_field(n_rays, 6.0, -5.0, max_angle_deg, wl_range, seed)
     info = {"asphere_R_mm": 30.0, "asphere_CT_mm": 6.0, "doublet_z_mm": zA, "CT1_mm": ct1, "CT2_mm": ct2,
             "coating_um": t_coat, "focus_z_mm": focus, "seed": seed}
     return Workload("asphere_doublet_c2", rays, wl, [asphere, doublet, det], 10, 200.0, info)
