@@ -224,6 +224,7 @@ class SceneBuilder(object):
         rec["shape"] = -1
         rec["material"] = self.material_index(face.material)
         fill(rec, self)
+        rec["p"][14:16] = z_band(rec, face, self)
         return rec
 
     # ---------------------------------------------------------- lists
@@ -268,6 +269,79 @@ class SceneBuilder(object):
                            arr(self._zcoefs, _abi.zernike_coef_dtype),
                            arr(self._zinsns, _abi.zernike_insn_dtype),
                            n_glob, list(self._mat_objs), list(self._face_objs))
+
+
+def z_band(rec, face, builder=None):
+    """(lo, hi): a z range, in the face list's frame, holding every point at
+    which the face can accept a base-ray hit, widened by a margin for
+    rounding (the root of a near-tangent sphere intersection can sit about
+    1e-6 mm off the surface); (nan, nan) where this build does not bound it.
+    The kernels skip a face whose band lies outside the part of a ray's
+    segment closer than its nearest hit so far.
+
+    A DistortionFace over a planar base face gets the plane's z widened by a
+    bound on the distortion's sag over the face's aperture (the sum over its
+    Zernike terms of |N c| times the radial polynomial's absolute
+    coefficients at the aperture radius) and by 1e-3 mm plus twice that
+    bound for its fixed-point iteration, whose accepted point lies on or
+    next to the distorted surface (about 3.5e-4 mm from the plane on the
+    C3 mirror)."""
+    import math
+    from .cshapes import bound_radius
+    nan = (float("nan"), float("nan"))
+    t = int(rec["type"])
+    p = [float(x) for x in rec["p"]]
+    if t in (_abi.FACE_CIRCULAR, _abi.FACE_SHAPED_PLANAR, _abi.FACE_RECTANGULAR, _abi.FACE_POLYGON,
+             _abi.FACE_EXTRUDED_PLANAR):
+        if t == _abi.FACE_EXTRUDED_PLANAR:
+            lo, hi = p[4], p[5]
+        else:
+            z = {_abi.FACE_CIRCULAR: p[2], _abi.FACE_RECTANGULAR: p[3]}.get(t, p[0])
+            lo = hi = z
+        m = 1e-7 * (1.0 + abs(lo) + abs(hi))
+    elif t in (_abi.FACE_SPHERICAL, _abi.FACE_SHAPED_SPHERICAL):
+        if t == _abi.FACE_SPHERICAL:
+            R, z, r = p[1], p[2], abs(p[0]) / 2
+        else:
+            R, z, r = p[0], p[1], bound_radius(face.shape)
+        if R == 0.0 or not math.isfinite(R) or math.isnan(r):
+            return nan
+        aR = abs(R)
+        r = r * (1.0 + 1e-6)
+        depth = aR - math.sqrt(aR * aR - r * r) if r < aR else aR
+        lo, hi = (z - depth, z) if R > 0 else (z, z + depth)
+        m = 1e-6 * (1.0 + abs(z) + aR + min(r, aR))
+    elif t == _abi.FACE_ASPHERIC:
+        band = p[12]
+        if band < 0:
+            return nan
+        lo, hi = p[1] - band, p[1] + band
+        m = 1e-9 * (1.0 + abs(p[1]) + band)
+    elif t == _abi.FACE_DISTORTION and builder is not None:
+        base_rec = builder._extra[int(rec["aux0"])][0]
+        bt = int(base_rec["type"])
+        if bt not in (_abi.FACE_CIRCULAR, _abi.FACE_SHAPED_PLANAR, _abi.FACE_RECTANGULAR, _abi.FACE_POLYGON):
+            return nan
+        bp = [float(x) for x in base_rec["p"]]
+        z = {_abi.FACE_CIRCULAR: bp[2], _abi.FACE_RECTANGULAR: bp[3]}.get(bt, bp[0])
+        zrec = builder._zernikes[int(rec["aux1"])]
+        rho = bound_radius(face.shape) / float(zrec["unit_radius"])
+        if not math.isfinite(rho):
+            return nan
+        rho = max(rho, 1.0)
+        sag = 0.0
+        for c in builder._zcoefs[int(zrec["coef_begin"]): int(zrec["coef_begin"]) + int(zrec["n_coefs"])]:
+            deg = int(c["deg"][0])
+            sag += abs(float(c["nval"])) * sum(abs(float(a)) * rho ** i for i, a in enumerate(c["poly"][0][:deg + 1]))
+        if not math.isfinite(sag):
+            return nan
+        lo, hi = z - sag, z + sag
+        m = 1e-3 + 2.0 * sag + 1e-7 * (1.0 + abs(z))
+    else:
+        return nan
+    if not (math.isfinite(lo) and math.isfinite(hi)):
+        return nan
+    return lo - m, hi + m
 
 
 def compile_scene(face_sets):

@@ -298,6 +298,22 @@ RPX_DEV bool asph_outside_band(const rpx_face_t &f, V3 p1, V3 p2, int is_base) {
     return zhi < f.p[1] - m || zlo > f.p[1] + m;
 }
 
+// A base ray's nearest-face search only accepts a face's hit closer than
+// the nearest one so far.  p[14]..p[15] of a face record hold a z band (in
+// its face list's frame, widened for rounding on the host; NaN when not
+// bounded) containing every point at which the face can accept a base-ray
+// hit; an aspheric face's Newton tolerance widens it by a multiple of the
+// segment length, as in asph_outside_band.  If the part of the segment
+// closer than the current nearest hit -- z from z1 over zspan -- misses the
+// band, the face cannot give a nearer hit and its exact test is skipped.
+// Skipping removes only candidates the exact test would reject, so the
+// nearest face, its distance and the tie rule are unchanged.
+RPX_DEV bool face_out_of_reach(const rpx_face_t &f, double z1, double zspan, double slen) {
+    const double m = f.type == RPX_FACE_ASPHERIC ? 10.0 * f.p[10] * slen * (1.0 + f.p[13]) : 0.0;
+    const double z2 = z1 + zspan;
+    return fmax(z1, z2) < f.p[14] - m || fmin(z1, z2) > f.p[15] + m;
+}
+
 // ------------------------------------------------------------- intersect
 // Face record k of a table in global memory, for an index that is the same
 // in every lane: read through the constant address space, i.e. with scalar

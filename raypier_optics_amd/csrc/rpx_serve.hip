@@ -144,8 +144,10 @@ void k_batch(const BatchJob *jobs, BatchControl *ctl, uint64_t *status,
                 const rpx_face_list_t &fl = S.face_lists[l];
                 const V3 p1 = transform(fl.inv_trans, origin);
                 const V3 p2 = transform(fl.inv_trans, point);
+                const double slen = sep(p1, p2), zrate = (p2.z - p1.z) / slen;
                 for (int k = fl.first_face; k < fl.first_face + fl.n_faces; ++k) {
                     const rpx_face_t &fc = S.faces[k];
+                    if (face_out_of_reach(fc, p1.z, zrate * length, slen)) continue;
                     const double dist = face_intersect<F>(S, fc, p1, p2, 1);
                     if (fc.tolerance < dist && dist < length) {
                         length = dist;

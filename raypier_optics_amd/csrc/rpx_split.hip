@@ -103,10 +103,12 @@ void k_generation_split(SplitArgs A) {
             const rpx_face_list_t &fl = S.face_lists[l];
             const V3 p1 = transform(fl.inv_trans, origin);
             const V3 p2 = transform(fl.inv_trans, point);
+            const double slen = sep(p1, p2), zrate = (p2.z - p1.z) / slen;
             const int k0 = A.one_list < 0 ? fl.first_face : fl.first_face + A.one_face;
             const int k1 = A.one_list < 0 ? fl.first_face + fl.n_faces : k0 + 1;
             for (int k = k0; k < k1; ++k) {
                 auto test = [&](const rpx_face_t &fc) {
+                    if (face_out_of_reach(fc, p1.z, zrate * length, slen)) return;
                     const double dist = face_intersect<F>(S, fc, p1, p2, 1);
                     if (fc.tolerance < dist && dist < length) {
                         length = dist;

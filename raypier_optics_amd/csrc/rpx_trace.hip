@@ -342,8 +342,10 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_intersect(TraceArgs A) {
                 const rpx_face_list_t &fl = S.face_lists[l];
                 V3 p1 = transform(fl.inv_trans, origin);
                 V3 p2 = transform(fl.inv_trans, point);
+                const double slen = sep(p1, p2), zrate = (p2.z - p1.z) / slen;
                 for (int k = fl.first_face; k < fl.first_face + fl.n_faces; ++k) {
                     const rpx_face_t &fc = S.faces[k];
+                    if (face_out_of_reach(fc, p1.z, zrate * length, slen)) continue;
                     double dist = face_intersect<F>(S, fc, p1, p2, 1);
                     if (fc.tolerance < dist && dist < length) {
                         length = dist;
@@ -602,8 +604,10 @@ void k_generation(TraceArgs A) {
                 const rpx_face_list_t &fl = S.face_lists[l];
                 V3 p1 = transform(fl.inv_trans, origin);
                 V3 p2 = transform(fl.inv_trans, point);
+                const double slen = sep(p1, p2), zrate = (p2.z - p1.z) / slen;
                 for (int k = fl.first_face; k < fl.first_face + fl.n_faces; ++k) {
                     const rpx_face_t &fc = S.faces[k];
+                    if (face_out_of_reach(fc, p1.z, zrate * length, slen)) continue;
                     double dist = face_intersect<F>(S, fc, p1, p2, 1);
                     if (fc.tolerance < dist && dist < length) {
                         length = dist;
