@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RPX_ABI_VERSION 9
+#define RPX_ABI_VERSION 10
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t rpx_status_t;
@@ -682,8 +682,36 @@ rpx_status_t rpx_server_fail_allocations_after(rpx_server_t *server, int64_t n_a
  * to the plane z = detector_z.  A ray that leaves a surface's clear aperture,
  * misses it or is totally reflected is dropped.  The spot RMS is
  * sqrt(mean |p - mean p|^2) over the (x, y) of the kept rays' detector
- * points. */
+ * points.
+ *
+ * A surface with zernike_radius[s] = Ru > 0 carries a Zernike distortion
+ * (ZernikeDistortion, cdistortions.pyx:323-514, as DistortionFace adds it to
+ * its base face's sag, cfaces.pyx:2323-2438): its sag gains
+ *   D(x, y) = sum_k C[s][k] (x / Ru)^a_k (y / Ru)^b_k
+ * over the RPX_LENS_ZTERMS monomials a + b <= 6 in the order (0,0), (1,0),
+ * (0,1), (2,0), (1,1), (0,2), (3,0), ... (degree by degree, a descending).
+ * C (device, n_surfaces x RPX_LENS_ZTERMS) is the Zernike sum written in
+ * those monomials (the host expands sum_j c_j N_j R_n^m(r) cos/sin(m theta));
+ * the gradient with respect to C maps to the coefficients c_j by the same
+ * expansion.
+ *
+ * Objectives: RPX_LENS_SPOT_RMS as above; RPX_LENS_WAVEFRONT_RMS, the RMS
+ * over the kept rays of the wavefront error W - mean_lambda(W), with the
+ * piston removed per wavelength (at most RPX_LENS_MAX_WFE_WAVELENGTHS),
+ * where W is a ray's optical path from its origin (plus its
+ * accumulated_path) to the foot of the perpendicular from the on-axis
+ * point P = (0, 0, detector_z) on its last segment:
+ *   W = accumulated_path + sum_s n_s t_s + n_last (P - q_last) . d_last,
+ * the wavefront error against the sphere centred on P, to first order. */
 #define RPX_LENS_MAX_SURFACES 16
+#define RPX_LENS_ZTERMS 28
+#define RPX_LENS_MAX_WFE_WAVELENGTHS 16
+#define RPX_LENS_MOMENTS (2 + 2 * RPX_LENS_MAX_WFE_WAVELENGTHS)
+
+enum rpx_lens_objective {
+    RPX_LENS_SPOT_RMS = 0,
+    RPX_LENS_WAVEFRONT_RMS = 1
+};
 
 typedef struct {
     int32_t n_surfaces;
@@ -694,11 +722,32 @@ typedef struct {
     double aperture[RPX_LENS_MAX_SURFACES];     /* clear radius */
     int32_t medium_before[RPX_LENS_MAX_SURFACES];
     int32_t medium_after[RPX_LENS_MAX_SURFACES];
+    double zernike_radius[RPX_LENS_MAX_SURFACES];   /* Ru; 0 = no distortion */
 } rpx_lens_t;
 
 uint64_t rpx_lens_workspace_bytes(uint64_t n_rays, int32_t n_surfaces);
 
-/* Forward: moments[0..3] (device) = kept rays, sum x, sum y, sum x^2 + y^2. */
+/* Objective moments (device, RPX_LENS_MOMENTS doubles):
+ *   spot RMS:      kept rays, sum x, sum y, sum x^2 + y^2;
+ *   wavefront RMS: kept rays, sum (W - mean_lambda W)^2, then per wavelength
+ *                  l: moments[2 + 2l] = kept rays, moments[3 + 2l] = sum W.
+ * zcart may be null when no surface has a distortion. */
+rpx_status_t rpx_lens_eval(const rpx_lens_t *lens, const double *zcart, const rpx_rays_t *rays, uint64_t n_rays,
+                           const double *n_table, uint64_t n_wavelengths, int32_t objective, double *moments,
+                           void *workspace, uint64_t workspace_bytes, void *stream);
+
+/* grad (device, n_surfaces * (6 + RPX_LENS_ZTERMS) doubles):
+ * grad[s*6 + q] = d(objective)/d(param[s][q]);
+ * grad[6*n_surfaces + s*RPX_LENS_ZTERMS + k] = d(objective)/d(C[s][k]) (0
+ * where the surface has no distortion), given the moments of rpx_lens_eval
+ * for the same inputs.  Intersections are recomputed, not stored. */
+rpx_status_t rpx_lens_grad(const rpx_lens_t *lens, const double *zcart, const rpx_rays_t *rays, uint64_t n_rays,
+                           const double *n_table, uint64_t n_wavelengths, int32_t objective, const double *moments,
+                           double *grad, void *workspace, uint64_t workspace_bytes, void *stream);
+
+/* Spot RMS without distortions (rpx_lens_eval / rpx_lens_grad with
+ * RPX_LENS_SPOT_RMS and no zcart; zernike_radius is ignored).
+ * Forward: moments[0..3] (device) = kept rays, sum x, sum y, sum x^2 + y^2. */
 rpx_status_t rpx_lens_spot(const rpx_lens_t *lens, const rpx_rays_t *rays, uint64_t n_rays,
                            const double *n_table, uint64_t n_wavelengths,
                            double *moments, void *workspace, uint64_t workspace_bytes, void *stream);

@@ -33,7 +33,7 @@ gausslet_dtype = np.dtype([('base_ray', ray_dtype),
 
 assert ray_dtype.itemsize == 188 and para_dtype.itemsize == 80 and gausslet_dtype.itemsize == 668
 
-ABI_VERSION = 9          # RPX_ABI_VERSION of include/rpx.h
+ABI_VERSION = 10          # RPX_ABI_VERSION of include/rpx.h
 
 # timing slots (RPX_TIMING_*)
 TIMING_KERNELS = ("k_intersect", "k_scan", "k_emit", "k_generation", "k_clear", "k_generation_split",
@@ -163,6 +163,10 @@ class JobResult(ctypes.Structure):
 
 
 LENS_MAX_SURFACES = 16
+LENS_ZTERMS = 28                 # RPX_LENS_ZTERMS: monomials x^a y^b, a + b <= 6
+LENS_MAX_WFE_WAVELENGTHS = 16
+LENS_MOMENTS = 2 + 2 * LENS_MAX_WFE_WAVELENGTHS
+LENS_SPOT_RMS, LENS_WAVEFRONT_RMS = 0, 1
 
 
 class Lens(ctypes.Structure):
@@ -172,7 +176,8 @@ class Lens(ctypes.Structure):
                 ("param", (ctypes.c_double * 6) * LENS_MAX_SURFACES),
                 ("aperture", ctypes.c_double * LENS_MAX_SURFACES),
                 ("medium_before", ctypes.c_int32 * LENS_MAX_SURFACES),
-                ("medium_after", ctypes.c_int32 * LENS_MAX_SURFACES)]
+                ("medium_after", ctypes.c_int32 * LENS_MAX_SURFACES),
+                ("zernike_radius", ctypes.c_double * LENS_MAX_SURFACES)]
 
 
 SPLIT_F32 = 19   # RPX_SPLIT_F32_FIELDS (same order as the first 19 float64 fields)

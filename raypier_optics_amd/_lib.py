@@ -55,6 +55,10 @@ _SIGNATURES = {
     "rpx_server_clock": (c_f64, []),
     "rpx_server_fail_allocations_after": (c_i32, [c_void_p, ctypes.c_int64]),
     "rpx_lens_workspace_bytes": (c_u64, [c_u64, c_i32]),
+    "rpx_lens_eval": (c_i32, [P(_abi.Lens), c_void_p, P(_abi.Rays), c_u64, c_void_p, c_u64, c_i32, c_void_p,
+                              c_void_p, c_u64, c_void_p]),
+    "rpx_lens_grad": (c_i32, [P(_abi.Lens), c_void_p, P(_abi.Rays), c_u64, c_void_p, c_u64, c_i32, c_void_p,
+                              c_void_p, c_void_p, c_u64, c_void_p]),
     "rpx_lens_spot": (c_i32, [P(_abi.Lens), P(_abi.Rays), c_u64, c_void_p, c_u64, c_void_p, c_void_p, c_u64,
                               c_void_p]),
     "rpx_lens_spot_grad": (c_i32, [P(_abi.Lens), P(_abi.Rays), c_u64, c_void_p, c_u64, c_void_p, c_void_p,

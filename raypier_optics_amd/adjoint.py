@@ -1,20 +1,30 @@
-"""Differentiable spot RMS of a coaxial sequential lens (hand-written adjoint).
+"""Differentiable spot RMS and wavefront RMS of a coaxial sequential lens
+(hand-written adjoint).
 
 `SequentialLens` describes surfaces z = z_s + c r^2/(1+sqrt(1-(1+k)c^2 r^2))
 + A4 r^4 + A6 r^6 + A8 r^8 + A10 r^10 with per-surface parameters
-(c, k, A4, A6, A8, A10), the media on either side of each surface and a
-detector plane (include/rpx.h, rpx_lens_t).  `spot_rms(params, lens, rays)`
-is a torch.autograd.Function: its forward pass runs the forward trace kernel
-on the GPU, its backward pass the adjoint kernel, which recomputes every
-intersection instead of storing it, and returns d(spot RMS)/d(params).
+(c, k, A4, A6, A8, A10), optionally plus a Zernike distortion
+(ZernikeDistortion over ANSI indices j <= 27, n <= 6, as DistortionFace adds
+it to the sag), the media on either side of each surface and a detector
+plane (include/rpx.h, rpx_lens_t).
 
-`SequentialLens.face_lists(params)` builds the same system from drop-in
-faces (AsphericFace surfaces, FullDielectricDispersiveMaterial interfaces,
-an opaque CircularFace detector) so that the spot RMS can be evaluated by
-the float64 trace (tracer.trace_rays) as well -- which is how the gradient
-is checked by central finite differences.
+`spot_rms(params, lens, rays)` and `lens_objective(params, zcoefs, lens,
+rays, objective)` are torch.autograd.Functions: the forward pass runs the
+forward trace kernel on the GPU, the backward pass the adjoint kernel, which
+recomputes every intersection instead of storing it, and returns
+d(objective)/d(params) and d(objective)/d(Zernike coefficients).  The
+objectives are the detector spot RMS and the wavefront RMS (optical path to
+the on-axis detector point, piston removed per wavelength).
+
+`SequentialLens.face_lists(params, zcoefs)` builds the same system from
+drop-in faces (AsphericFace surfaces, DistortionFace over them for the
+distorted ones, FullDielectricDispersiveMaterial interfaces, an opaque
+CircularFace detector) so that the objectives can be evaluated by the
+float64 trace (tracer.trace_rays) as well -- which is how the gradients are
+checked by central finite differences.
 """
 import ctypes
+import math
 
 import numpy as np
 
@@ -23,10 +33,68 @@ from ._lib import load, check, require_gpu, stream_handle
 
 N_PARAM = 6
 PARAM_NAMES = ("c", "k", "A4", "A6", "A8", "A10")
+ZDEG = 6
+OBJECTIVES = {"spot": _abi.LENS_SPOT_RMS, "wavefront": _abi.LENS_WAVEFRONT_RMS}
+
+
+def monomials():
+    """(a, b) of the monomials x^a y^b in the device table's order: degree
+    by degree, a descending."""
+    return [(a, d - a) for d in range(ZDEG + 1) for a in range(d, -1, -1)]
+
+
+def _poly_mul(p, q):
+    out = {}
+    for (a1, b1), c1 in p.items():
+        for (a2, b2), c2 in q.items():
+            k = (a1 + a2, b1 + b2)
+            out[k] = out.get(k, 0) + c1 * c2
+    return out
+
+
+def zernike_monomials(j):
+    """Cartesian expansion of one ZernikeDistortion term in the unit-radius
+    coordinates: Z_j(x, y) = N R_n^|m|(r) cos(m theta) for m >= 0 and
+    N R_n^|m|(r) sin(|m| theta) for m < 0 (cdistortions.pyx:456-514),
+    N = sqrt(n+1) (m = 0) or sqrt(2(n+1)); returns the coefficients of
+    monomials() as a float64 array.  R_n^|m| r^-|m| is a polynomial in
+    r^2 = x^2 + y^2 and r^|m| (cos, sin)(|m| theta) = (Re, Im)(x + iy)^|m|."""
+    from .cdistortions import eval_nmk
+    n, m, _ = eval_nmk(int(j))
+    am = abs(m)
+    if n > ZDEG:
+        raise ValueError("Zernike term j=%d has n=%d > %d" % (j, n, ZDEG))
+    # (x + iy)^|m| = sum_k C(|m|, k) x^(|m|-k) (iy)^k
+    harm = {}
+    for k in range(am + 1):
+        ik = [1, 1j, -1, -1j][k % 4]
+        val = math.comb(am, k) * ik
+        part = val.real if m >= 0 else val.imag
+        if part:
+            harm[(am - k, k)] = harm.get((am - k, k), 0) + int(round(part))
+    radial = {}
+    for k in range((n - am) // 2 + 1):
+        a = (-1) ** k * math.factorial(n - k) // (math.factorial(k) * math.factorial((n + am) // 2 - k)
+                                                   * math.factorial((n - am) // 2 - k))
+        p = (n - 2 * k - am) // 2            # r^(2p) = (x^2 + y^2)^p
+        for i in range(p + 1):
+            key = (2 * i, 2 * (p - i))
+            radial[key] = radial.get(key, 0) + a * math.comb(p, i)
+    poly = _poly_mul(radial, harm)
+    N = math.sqrt(float(n + 1)) if m == 0 else math.sqrt(float(2 * (n + 1)))
+    idx = {ab: i for i, ab in enumerate(monomials())}
+    out = np.zeros(_abi.LENS_ZTERMS)
+    for ab, c in poly.items():
+        if c:
+            out[idx[ab]] += N * c
+    return out
 
 
 class SequentialLens(object):
-    def __init__(self, z, params, apertures, before, after, media, detector_z):
+    def __init__(self, z, params, apertures, before, after, media, detector_z, zernike=None):
+        """zernike: one entry per surface, None or (unit_radius, [(ANSI j,
+        coefficient), ...]) -- a ZernikeDistortion added to that surface's
+        sag."""
         self.z = np.asarray(z, np.float64)
         self.params = np.asarray(params, np.float64).reshape(-1, N_PARAM)
         self.apertures = np.asarray(apertures, np.float64)
@@ -36,17 +104,66 @@ class SequentialLens(object):
         self.detector_z = float(detector_z)
         if not 1 <= len(self.z) <= _abi.LENS_MAX_SURFACES:
             raise ValueError("1..%d surfaces" % _abi.LENS_MAX_SURFACES)
+        self.zernike = list(zernike) if zernike is not None else [None] * len(self.z)
+        if len(self.zernike) != len(self.z):
+            raise ValueError("one zernike entry per surface")
+        # per surface: unit radius, ANSI indices, (J, 28) expansion matrix
+        self._zj, self._zT, self.zernike_radius = [], [], np.zeros(len(self.z))
+        for s, zd in enumerate(self.zernike):
+            if zd is None:
+                self._zj.append([])
+                self._zT.append(np.zeros((0, _abi.LENS_ZTERMS)))
+                continue
+            ru, terms = zd
+            if not ru > 0:
+                raise ValueError("Zernike unit radius must be positive")
+            self.zernike_radius[s] = float(ru)
+            self._zj.append([int(j) for j, _ in terms])
+            self._zT.append(np.stack([zernike_monomials(j) for j, _ in terms]) if terms
+                            else np.zeros((0, _abi.LENS_ZTERMS)))
+
+    @property
+    def zcoefs(self):
+        """All Zernike coefficients, surface by surface, in term order."""
+        out = [c for zd in self.zernike if zd is not None for _, c in zd[1]]
+        return np.asarray(out, np.float64)
+
+    @property
+    def has_zernike(self):
+        return any(len(j) for j in self._zj)
+
+    def zernike_table(self, zcoefs=None):
+        """(n_surfaces, 28) monomial coefficients of every surface's
+        distortion for the flat coefficient vector zcoefs."""
+        zc = self.zcoefs if zcoefs is None else np.asarray(zcoefs, np.float64).reshape(-1)
+        out = np.zeros((self.n_surfaces, _abi.LENS_ZTERMS))
+        k = 0
+        for s, T in enumerate(self._zT):
+            J = T.shape[0]
+            if J:
+                out[s] = zc[k:k + J] @ T
+            k += J
+        return out
+
+    def zernike_grad(self, gmono):
+        """Gradient with respect to the flat coefficient vector from the
+        gradient with respect to the monomial table ((n_surfaces, 28))."""
+        out = []
+        for s, T in enumerate(self._zT):
+            if T.shape[0]:
+                out.append(T @ gmono[s])
+        return np.concatenate(out) if out else np.zeros(0)
 
     @property
     def n_surfaces(self):
         return len(self.z)
 
     @classmethod
-    def from_prescription(cls, presc, media, detector_z):
+    def from_prescription(cls, presc, media, detector_z, zernike=None):
         return cls([p["z"] for p in presc],
                    [[p["c"], p["k"]] + list(p["A"]) for p in presc],
                    [p["aperture"] for p in presc], [p["before"] for p in presc], [p["after"] for p in presc],
-                   media, detector_z)
+                   media, detector_z, zernike)
 
     @classmethod
     def from_workload(cls, w):
@@ -72,24 +189,37 @@ class SequentialLens(object):
             L.aperture[s] = self.apertures[s]
             L.medium_before[s] = self.before[s]
             L.medium_after[s] = self.after[s]
+            L.zernike_radius[s] = self.zernike_radius[s]
         return L
 
-    def face_lists(self, params=None):
+    def face_lists(self, params=None, zcoefs=None):
         """The lens as drop-in faces: [lens FaceList, detector FaceList]."""
         from .ctracer import FaceList
-        from .cfaces import AsphericFace, CircularFace
+        from .cfaces import AsphericFace, CircularFace, DistortionFace
         from .cshapes import CircleShape
         from .cmaterials import FullDielectricDispersiveMaterial, OpaqueMaterial
+        from .cdistortions import ZernikeDistortion
         P = self.params if params is None else np.asarray(params, np.float64).reshape(-1, N_PARAM)
+        zc = self.zcoefs if zcoefs is None else np.asarray(zcoefs, np.float64).reshape(-1)
         faces = []
+        k0 = 0
         for s in range(self.n_surfaces):
             c, k, A4, A6, A8, A10 = (float(x) for x in P[s])
             # inverted normals: the material's outside is the -z side (before)
             mat = FullDielectricDispersiveMaterial(dispersion_inside=self.media[self.after[s]],
                                                    dispersion_outside=self.media[self.before[s]])
-            faces.append(AsphericFace(shape=CircleShape(radius=float(self.apertures[s])), material=mat,
-                                      conic_const=k, z_height=float(self.z[s]), curvature=-1.0 / c,
-                                      A4=A4, A6=A6, A8=A8, A10=A10, invert_normals=True, atol=1e-16))
+            face = AsphericFace(shape=CircleShape(radius=float(self.apertures[s])), material=mat,
+                                conic_const=k, z_height=float(self.z[s]), curvature=-1.0 / c,
+                                A4=A4, A6=A6, A8=A8, A10=A10, invert_normals=True, atol=1e-16)
+            js = self._zj[s]
+            if js:
+                zd = ZernikeDistortion([(j, float(v)) for j, v in zip(js, zc[k0:k0 + len(js)])],
+                                       unit_radius=float(self.zernike_radius[s]))
+                k0 += len(js)
+                # the distorted normal is built with +z orientation (cfaces.pyx:2418-2431); the
+                # base face's inverted normals point -z, so the orientation is flipped back
+                face = DistortionFace(base_face=face, distortion=zd, material=mat, accuracy=1e-13, invert_normal=1)
+            faces.append(face)
         lens = FaceList()
         lens.faces = faces
         det = FaceList()
@@ -111,7 +241,79 @@ class _Device(object):
         self.ws = torch.empty(max(need, 8), dtype=torch.uint8, device="cuda")
 
 
+def _ztable(lens, zcoefs):
+    """Device monomial table of the lens's distortions, or None."""
+    import torch
+    if not lens.has_zernike:
+        return None
+    return torch.from_numpy(lens.zernike_table(zcoefs).reshape(-1)).to("cuda")
+
+
+def _eval(lens, d, params, zcoefs=None, objective=_abi.LENS_SPOT_RMS):
+    """Moments (device, LENS_MOMENTS doubles) of the objective."""
+    import torch
+    lib = load()
+    L = lens.struct(params)
+    s = d.dev.struct()
+    zt = _ztable(lens, zcoefs)
+    m = torch.zeros(_abi.LENS_MOMENTS, dtype=torch.float64, device="cuda")
+    check(lib.rpx_lens_eval(ctypes.byref(L), ctypes.c_void_p(zt.data_ptr() if zt is not None else 0), ctypes.byref(s),
+                            d.rays.n_rays, ctypes.c_void_p(d.table.data_ptr()), d.n_wl, int(objective),
+                            ctypes.c_void_p(m.data_ptr()), ctypes.c_void_p(d.ws.data_ptr()), d.ws.numel(),
+                            stream_handle()))
+    m._rpx_ztable = zt   # keeps the table alive while the stream uses it
+    return m
+
+
+def _value(m, objective):
+    """Objective value (0-d tensor) from its moments."""
+    import torch
+    if objective == _abi.LENS_SPOT_RMS:
+        return _sigma(m)
+    return torch.sqrt(torch.clamp(m[1] / m[0], min=0.0))
+
+
+def _lens_grad(lens, d, P, zcoefs, m, objective):
+    """(d objective / d params (n_surfaces, 6), d objective / d zcoefs) as
+    device tensors."""
+    import torch
+    lib = load()
+    L = lens.struct(P)
+    s = d.dev.struct()
+    zt = _ztable(lens, zcoefs)
+    S = lens.n_surfaces
+    g = torch.empty(S * (N_PARAM + _abi.LENS_ZTERMS), dtype=torch.float64, device="cuda")
+    check(lib.rpx_lens_grad(ctypes.byref(L), ctypes.c_void_p(zt.data_ptr() if zt is not None else 0), ctypes.byref(s),
+                            d.rays.n_rays, ctypes.c_void_p(d.table.data_ptr()), d.n_wl, int(objective),
+                            ctypes.c_void_p(m.data_ptr()), ctypes.c_void_p(g.data_ptr()),
+                            ctypes.c_void_p(d.ws.data_ptr()), d.ws.numel(), stream_handle()))
+    gp = g[:S * N_PARAM].view(S, N_PARAM)
+    gz = g[S * N_PARAM:].view(S, _abi.LENS_ZTERMS).cpu().numpy()
+    return gp, torch.from_numpy(lens.zernike_grad(gz))
+
+
+def objective_and_grad(lens, rays, objective="spot", params=None, zcoefs=None):
+    """Objective value and its gradients: (value, d/d params as an
+    (n_surfaces, 6) array, d/d zcoefs as a flat array in lens.zcoefs
+    order)."""
+    obj = OBJECTIVES[objective]
+    P = lens.params if params is None else np.asarray(params, np.float64).reshape(-1, N_PARAM)
+    zc = lens.zcoefs if zcoefs is None else np.asarray(zcoefs, np.float64).reshape(-1)
+    d = _Device(lens, rays)
+    m = _eval(lens, d, P, zc, obj)
+    gp, gz = _lens_grad(lens, d, P, zc, m, obj)
+    return float(_value(m, obj)), gp.cpu().numpy(), gz.numpy()
+
+
+def objective_moments(lens, rays, objective="spot", params=None, zcoefs=None):
+    """The objective's moments (see include/rpx.h, rpx_lens_eval)."""
+    obj = OBJECTIVES[objective]
+    d = _Device(lens, rays)
+    return _eval(lens, d, lens.params if params is None else params, zcoefs, obj).cpu().numpy()
+
+
 def _moments(lens, d, params):
+    """Spot moments through rpx_lens_spot, which ignores distortions."""
     import torch
     lib = load()
     L = lens.struct(params)
@@ -132,12 +334,15 @@ def _sigma(m):
 def spot_moments(lens, rays, params=None):
     """(kept rays, sum x, sum y, sum x^2+y^2) of the detector points, float64."""
     d = _Device(lens, rays)
-    return _moments(lens, d, lens.params if params is None else params).cpu().numpy()
+    return _eval(lens, d, lens.params if params is None else params)[:4].cpu().numpy()
 
 
 def spot_rms_and_grad(lens, rays, params=None):
-    """Spot RMS (mm) and d(spot RMS)/d(params) as a (n_surfaces, 6) array."""
-    import torch
+    """Spot RMS (mm) and d(spot RMS)/d(params) as a (n_surfaces, 6) array
+    (a lens with distortions keeps its Zernike coefficients fixed)."""
+    if lens.has_zernike:
+        v, gp, _ = objective_and_grad(lens, rays, "spot", params)
+        return v, gp
     P = lens.params if params is None else np.asarray(params, np.float64).reshape(-1, N_PARAM)
     d = _Device(lens, rays)
     m = _moments(lens, d, P)
@@ -146,6 +351,8 @@ def spot_rms_and_grad(lens, rays, params=None):
 
 
 def _grad(lens, d, P, m):
+    """d(spot RMS)/d(params) (device, flat) through rpx_lens_spot_grad,
+    which ignores distortions."""
     import torch
     lib = load()
     L = lens.struct(P)
@@ -181,15 +388,54 @@ def _function():
             g = g.to(ctx.param_device, ctx.param_dtype) * gout
             return g, None, None
 
-    return SpotRMS
+    class LensObjective(torch.autograd.Function):
+        """Spot RMS or wavefront RMS (0-d tensor) of `lens` for `rays` as a
+        function of the (n_surfaces, 6) parameter tensor and the flat
+        Zernike coefficient tensor (lens.zcoefs order)."""
+
+        @staticmethod
+        def forward(ctx, params, zcoefs, lens, rays, objective):
+            require_gpu()
+            obj = OBJECTIVES[objective]
+            P = params.detach().to("cpu", torch.float64).numpy()
+            Z = zcoefs.detach().to("cpu", torch.float64).numpy()
+            d = _Device(lens, rays)
+            m = _eval(lens, d, P, Z, obj)
+            ctx.lens, ctx.d, ctx.P, ctx.Z, ctx.m, ctx.obj = lens, d, P, Z, m, obj
+            ctx.pdev, ctx.pdtype, ctx.zdev, ctx.zdtype = params.device, params.dtype, zcoefs.device, zcoefs.dtype
+            return _value(m, obj).to(params.device, params.dtype)
+
+        @staticmethod
+        def backward(ctx, gout):
+            gp, gz = _lens_grad(ctx.lens, ctx.d, ctx.P, ctx.Z, ctx.m, ctx.obj)
+            gp = gp.to(ctx.pdev, ctx.pdtype) * gout.to(ctx.pdev)
+            gz = gz.to(ctx.zdev, ctx.zdtype) * gout.to(ctx.zdev)
+            return gp, gz, None, None, None
+
+    return SpotRMS, LensObjective
 
 
-_SPOT_RMS = None
+_FUNCS = None
 
 
 def spot_rms(params, lens, rays):
-    """Differentiable spot RMS: params is a (n_surfaces, 6) float64 tensor."""
-    global _SPOT_RMS
-    if _SPOT_RMS is None:
-        _SPOT_RMS = _function()
-    return _SPOT_RMS.apply(params, lens, rays)
+    """Differentiable spot RMS: params is a (n_surfaces, 6) float64 tensor
+    (a lens with distortions keeps its Zernike coefficients fixed)."""
+    global _FUNCS
+    if _FUNCS is None:
+        _FUNCS = _function()
+    if lens.has_zernike:
+        import torch
+        return _FUNCS[1].apply(params, torch.from_numpy(lens.zcoefs), lens, rays, "spot")
+    return _FUNCS[0].apply(params, lens, rays)
+
+
+def lens_objective(params, zcoefs, lens, rays, objective="spot"):
+    """Differentiable spot RMS ("spot") or wavefront RMS ("wavefront"):
+    params is an (n_surfaces, 6) float64 tensor, zcoefs the flat float64
+    tensor of the lens's Zernike coefficients (lens.zcoefs order; empty
+    without distortions)."""
+    global _FUNCS
+    if _FUNCS is None:
+        _FUNCS = _function()
+    return _FUNCS[1].apply(params, zcoefs, lens, rays, objective)

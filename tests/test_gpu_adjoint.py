@@ -101,3 +101,162 @@ def test_autograd_function(gpu):
     s2, g2 = spot_rms_and_grad(lens, rays)
     assert abs(float(sigma) - s2) == 0.0
     assert np.array_equal(P.grad.numpy(), 3.0 * g2)
+
+
+# ------------------------------------------- Zernike terms and the wavefront
+from raypier_optics_amd.adjoint import objective_and_grad, objective_moments, lens_objective  # noqa: E402
+
+# Terms whose gradient in the reference's ZernikeDistortion
+# (z_offset_and_gradient_c, cdistortions.pyx:456-514) is the derivative of its
+# sag, so that DistortionFace's normals are the surface's own: the drop-in
+# trace and the lens model describe the same surface.  The reference's
+# zernike_Rprime_c takes the memo slot of its third recursion term with the
+# half order of n-1 (cdistortions.pyx:210-213), which for even n reads or
+# fills another term's slot: for ANSI j = 11, 13, 16-19, 22-26 the normals
+# the trace reproduces are not the sag's derivative.  The lens model uses the
+# true derivative for every term (ZTERMS_ALL adds two such terms; checked
+# against the autograd reference and central differences of its own
+# forward).
+ZTERMS = [(3, 2e-4), (4, 1e-3), (7, -5e-4), (12, 3e-4), (14, 1e-4), (21, 5e-5)]
+ZTERMS_ALL = ZTERMS + [(13, 2e-4), (24, 1e-4)]
+
+
+def _zlens(seed, n=3000, distort=True, terms=ZTERMS):
+    """A C5 lens, with Zernike distortions on its first and third surfaces
+    (ANSI j up to 21 or 24, n = 6; coefficients 0.05-1 um)."""
+    w = workloads.lens_job_c5(seed, n_rays=n)
+    base = SequentialLens.from_workload(w)
+    zern = [None] * base.n_surfaces
+    if distort:
+        zern[0] = (float(base.apertures[0]), terms)
+        if base.n_surfaces > 2:
+            zern[2] = (1.2 * float(base.apertures[2]), [(5, -3e-4), (9, 2e-4)])
+    return w, SequentialLens.from_prescription(w.info["prescription"], w.info["media"], w.info["detector_z_mm"],
+                                               zern)
+
+
+def _trace_objective(lens, w, kind, params=None, zcoefs=None):
+    """Spot RMS or wavefront RMS of the float64 drop-in trace of the lens
+    built from AsphericFaces and DistortionFaces."""
+    faces = lens.face_lists(params, zcoefs)
+    traced, _ = trace_rays(w.collection(), faces, recursion_limit=lens.n_surfaces + 2, max_length=1000.0)
+    det = lens.n_surfaces
+    if kind == "spot":
+        pts, _ = metrics.detector_hits(traced, det)
+        p = pts.cpu().numpy()
+        c = p.mean(axis=0)
+        return float(np.sqrt(((p - c) ** 2).sum(axis=1).mean())), len(p)
+    Ws, wls = [], []
+    for g in traced:
+        if g.n_rays == 0:
+            continue
+        o, d, acc, _, n, end = metrics._path_fields(g)
+        wl = metrics._fields(g)[4]
+        m = end == det
+        P = torch.tensor([0.0, 0.0, lens.detector_z], dtype=torch.float64, device=o.device)
+        Ws.append((acc[m] + n[m] * ((P - o[m]) * d[m]).sum(dim=1)).cpu().numpy())
+        wls.append(wl[m].cpu().numpy())
+    W, wl = np.concatenate(Ws), np.concatenate(wls)
+    dev = np.zeros_like(W)
+    for l in np.unique(wl):
+        sel = wl == l
+        dev[sel] = W[sel] - W[sel].mean()
+    return float(np.sqrt((dev * dev).mean())), len(W)
+
+
+@pytest.mark.parametrize("kind", ["spot", "wavefront"])
+@pytest.mark.parametrize("distort", [True, False])
+def test_objective_matches_autograd_reference(gpu, kind, distort):
+    w, lens = _zlens(SEEDS[0], distort=distort, terms=ZTERMS_ALL)
+    r = w.rays
+    v_ref, gp_ref, gz_ref, keep = ar.objective_grad(lens, r["origin"], r["direction"], r["wavelength_idx"],
+                                                    w.wavelengths, kind)
+    v, gp, gz = objective_and_grad(lens, w.collection(), kind)
+    assert abs(v - v_ref) <= 1e-11 * v_ref, (v, v_ref)
+    floor = 1e-10 * np.abs(gp_ref).max(axis=0)
+    assert np.all(np.abs(gp - gp_ref) <= 1e-8 * np.abs(gp_ref) + floor), np.max(np.abs(gp - gp_ref) /
+                                                                              (np.abs(gp_ref) + floor))
+    assert gz.shape == gz_ref.shape == lens.zcoefs.shape
+    if distort:
+        zfloor = 1e-10 * np.abs(gz_ref).max()
+        assert np.all(np.abs(gz - gz_ref) <= 1e-8 * np.abs(gz_ref) + zfloor), (gz, gz_ref)
+
+
+@pytest.mark.parametrize("kind", ["spot", "wavefront"])
+def test_zernike_forward_matches_trace_rays(gpu, kind):
+    """The lens model's distorted surfaces are DistortionFaces over the
+    AsphericFaces: the drop-in trace gives the same objective."""
+    w, lens = _zlens(SEEDS[1])
+    m = objective_moments(lens, w.collection(), kind)
+    v = np.sqrt(m[3] / m[0] - (m[1:3] / m[0]) @ (m[1:3] / m[0])) if kind == "spot" else np.sqrt(m[1] / m[0])
+    v_trace, n_trace = _trace_objective(lens, w, kind)
+    assert int(m[0]) == n_trace
+    assert abs(v - v_trace) <= 1e-9 * v_trace, (v, v_trace)
+
+
+def test_zernike_gradient_matches_finite_differences(gpu):
+    """Wavefront RMS: d/d(Zernike coefficients) and d/d(c, k) against central
+    differences of the drop-in trace, |g_fd - g| h <= 1e-4 |g| h + 1e-13."""
+    w, lens = _zlens(SEEDS[0], n=2000)
+    v, gp, gz = objective_and_grad(lens, w.collection(), "wavefront")
+    Z0, P0 = lens.zcoefs, lens.params
+    worst = 0.0
+    for k in range(len(Z0)):
+        h = 1e-7
+        Zp, Zm = Z0.copy(), Z0.copy()
+        Zp[k] += h
+        Zm[k] -= h
+        fd = (_trace_objective(lens, w, "wavefront", zcoefs=Zp)[0] -
+              _trace_objective(lens, w, "wavefront", zcoefs=Zm)[0]) / (2 * h)
+        err, tol = abs(fd - gz[k]) * h, 1e-4 * abs(gz[k]) * h + 1e-13
+        worst = max(worst, err / tol)
+        assert err <= tol, (k, gz[k], fd)
+    for s, q in ((0, 0), (0, 1), (1, 0)):
+        h = 1e-6 * max(abs(P0[s, q]), 1e-2 if q == 0 else 1.0)
+        Pp, Pm = P0.copy(), P0.copy()
+        Pp[s, q] += h
+        Pm[s, q] -= h
+        fd = (_trace_objective(lens, w, "wavefront", params=Pp)[0] -
+              _trace_objective(lens, w, "wavefront", params=Pm)[0]) / (2 * h)
+        err, tol = abs(fd - gp[s, q]) * h, 1e-4 * abs(gp[s, q]) * h + 1e-13
+        worst = max(worst, err / tol)
+        assert err <= tol, (s, q, gp[s, q], fd)
+    print("worst FD mismatch %.3g of tolerance" % worst)
+
+
+def test_zernike_gradient_matches_own_finite_differences(gpu):
+    """Every term (ZTERMS_ALL), both objectives: d/d(Zernike coefficients)
+    against central differences of the kernels' own forward objective,
+    |g_fd - g| h <= 1e-5 |g| h + 1e-14."""
+    w, lens = _zlens(SEEDS[1], n=4000, terms=ZTERMS_ALL)
+    rays = w.collection()
+
+    def value(kind, Z):
+        m = objective_moments(lens, rays, kind, zcoefs=Z)
+        if kind == "spot":
+            c = m[1:3] / m[0]
+            return float(np.sqrt(m[3] / m[0] - c @ c))
+        return float(np.sqrt(m[1] / m[0]))
+
+    Z0 = lens.zcoefs
+    for kind in ("spot", "wavefront"):
+        _, _, gz = objective_and_grad(lens, rays, kind)
+        for k in range(len(Z0)):
+            h = 1e-7
+            Zp, Zm = Z0.copy(), Z0.copy()
+            Zp[k] += h
+            Zm[k] -= h
+            fd = (value(kind, Zp) - value(kind, Zm)) / (2 * h)
+            assert abs(fd - gz[k]) * h <= 1e-5 * abs(gz[k]) * h + 1e-14, (kind, k, gz[k], fd)
+
+
+def test_lens_objective_function(gpu):
+    w, lens = _zlens(SEEDS[0])
+    rays = w.collection()
+    P = torch.tensor(lens.params, dtype=torch.float64, requires_grad=True)
+    Z = torch.tensor(lens.zcoefs, dtype=torch.float64, requires_grad=True)
+    val = lens_objective(P, Z, lens, rays, "wavefront")
+    (val * 2.0).backward()
+    v2, gp, gz = objective_and_grad(lens, rays, "wavefront")
+    assert float(val) == v2
+    assert np.array_equal(P.grad.numpy(), 2.0 * gp) and np.array_equal(Z.grad.numpy(), 2.0 * gz)
