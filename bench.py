@@ -160,18 +160,23 @@ def roofline(r, measured):
                                   for i in range(len(names)) if launches[i]}}
 
 
-def serving_latency(n_jobs, qps, seed=2024, grad_fraction=0.3):
+def serving_latency(n_jobs, qps, seed=2024, grad_fraction=0.3, budget_bytes=3 << 29):
     """C5 stream: n_jobs random lens jobs (2-8 surfaces, rays log-uniform in
     [1e3, 2e6], built on the device) arriving as a Poisson process of rate
-    qps into the continuous-batching server; a grad_fraction of them also
-    request d(spot RMS)/d(surface parameters) from the adjoint kernel."""
+    qps into the continuous-batching server under a 1.5 GiB HBM budget (the
+    largest job's buffers take 1.1 GiB); a grad_fraction of them also request
+    d(spot RMS)/d(surface parameters) from the adjoint, which runs beside
+    the server's steps."""
     from raypier_optics_amd import workloads
     from raypier_optics_amd.serving import TraceServer, run_poisson
     jobs = [workloads.lens_job_c5(seed + k, device_rays=True) for k in range(n_jobs)]
     grad = np.random.default_rng(seed).random(n_jobs) < grad_fraction
-    server = TraceServer(hbm_budget_bytes=96 << 30, max_jobs=512, max_rays_in_flight=64 << 20)
+    server = TraceServer(hbm_budget_bytes=budget_bytes, max_jobs=512, max_rays_in_flight=64 << 20)
     run_poisson(server, jobs[:4], qps, seed=seed, gradient=grad[:4])     # warm: compile, graph build
+    held0, _ = server.stats()
+    deferred0 = server.submits_deferred
     results, wall = run_poisson(server, jobs, qps, seed=seed + 1, gradient=grad)
+    held, peak = server.stats()
     server.close()
     lat = np.array([r.latency for r in results]) * 1000.0
     n_int = sum(r.interactions for r in results)
@@ -187,7 +192,9 @@ def serving_latency(n_jobs, qps, seed=2024, grad_fraction=0.3):
            "workload": "lens_job_c5: 2-8 sphere/conic/asphere surfaces, N-BK7/N-SF11, rays logU(1e3, 2e6), "
                        "Poisson arrivals, %d%% gradient jobs; latency from arrival to result (host submit and "
                        "scene compile included)" % int(round(100 * grad_fraction)),
-           "hbm_budget_bytes": 96 << 30}
+           "hbm_budget_bytes": budget_bytes, "jobs_held_for_budget": held - held0, "peak_budget_bytes": peak,
+           "submissions_deferred_for_budget": server.submits_deferred - deferred0,
+           "admission_wait_ms_p99": float(np.percentile([1000.0 * (r.t_admit - r.t_submit) for r in results], 99))}
     return out
 
 

@@ -662,6 +662,11 @@ rpx_status_t rpx_server_step(rpx_server_t *server, rpx_job_result_t *finished, u
                              uint32_t *n_finished);
 /* Jobs waiting for admission and jobs on the device. */
 rpx_status_t rpx_server_counts(const rpx_server_t *server, uint32_t *waiting, uint32_t *active);
+/* Admission statistics since the server was created: jobs that were held
+ * back at least once because their buffers would have exceeded the HBM
+ * budget, and the most bytes the budget has counted at once. */
+rpx_status_t rpx_server_stats(const rpx_server_t *server, uint64_t *jobs_held_for_budget,
+                              uint64_t *peak_used_bytes);
 /* Seconds on the server's clock (the clock of t_submit / t_admit / t_finish). */
 double rpx_server_clock(void);
 /* Test hook (no counterpart in the reference): after n_allocations further

@@ -252,6 +252,7 @@ struct Job {
     uint64_t interactions = 0, children = 0;
     double t_submit = 0, t_admit = 0;
     int status = RPX_OK;
+    bool held_for_budget = false;   // once held back at admission by the HBM budget
 };
 
 }  // namespace
@@ -267,6 +268,7 @@ struct rpx_server {
     hipStream_t stream = nullptr;
     hipMemPool_t pool = nullptr;   // job buffers; keeps up to the HBM budget cached
     uint64_t budget = 0, used = 0;
+    uint64_t peak_used = 0, jobs_held_for_budget = 0;
     uint32_t max_jobs = 0;
     uint64_t max_tiles = 0;
     BatchJob *d_jobs = nullptr, *h_jobs = nullptr;
@@ -501,6 +503,7 @@ extern "C" rpx_status_t rpx_server_submit(rpx_server_t *s, const rpx_job_desc_t 
         }
     }
     s->used += j->cur.bytes();
+    s->peak_used = std::max(s->peak_used, s->used);
     s->waiting.push_back(j);
     *job_id = j->id;
     return RPX_OK;
@@ -538,7 +541,13 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
         // its input is already counted; the children buffer is what it adds
         const uint64_t need = std::max<uint64_t>((uint64_t)j->max_children * j->n, 1) * RAY_BYTES;
         const uint64_t tiles = (j->n + RPX_BLOCK - 1) / RPX_BLOCK;
-        if (!s->active.empty() && (s->used + need > s->budget || tiles_active + tiles > s->max_tiles)) break;
+        if (!s->active.empty() && (s->used + need > s->budget || tiles_active + tiles > s->max_tiles)) {
+            if (s->used + need > s->budget && !j->held_for_budget) {
+                j->held_for_budget = true;
+                ++s->jobs_held_for_budget;
+            }
+            break;
+        }
         s->waiting.pop_front();
         j->slot = s->free_slots.back();
         s->free_slots.pop_back();
@@ -554,6 +563,7 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
             continue;
         }
         s->used += j->next.bytes();
+        s->peak_used = std::max(s->peak_used, s->used);
         tiles_active += tiles;
         s->active.push_back(j);
     }
@@ -673,6 +683,8 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
                 continue;
             }
             s->used += j->next.bytes();
+            s->peak_used = std::max(s->peak_used, s->used);
+        s->peak_used = std::max(s->peak_used, s->used);
             still.push_back(j);
         }
         s->active.swap(still);
@@ -687,6 +699,14 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
 extern "C" rpx_status_t rpx_server_fail_allocations_after(rpx_server_t *s, int64_t n_allocations) {
     if (!s) return rpx_set_error(RPX_E_INVALID, "server is null");
     s->allocs_until_failure = n_allocations;
+    return RPX_OK;
+}
+
+extern "C" rpx_status_t rpx_server_stats(const rpx_server_t *s, uint64_t *jobs_held_for_budget,
+                                         uint64_t *peak_used_bytes) {
+    if (!s) return rpx_set_error(RPX_E_INVALID, "server is null");
+    if (jobs_held_for_budget) *jobs_held_for_budget = s->jobs_held_for_budget;
+    if (peak_used_bytes) *peak_used_bytes = s->peak_used;
     return RPX_OK;
 }
 

@@ -20,7 +20,7 @@ import time
 import numpy as np
 
 from . import _abi
-from ._lib import load, check, require_gpu
+from ._lib import load, check, require_gpu, RpxError
 from ._scene import compile_scene
 from .workloads import prepare_faces
 
@@ -64,6 +64,7 @@ class TraceServer(object):
         self._keep = {}         # job id -> objects the device job reads (scene, wavelengths)
         self._info = {}
         self._results = (_abi.JobResult * 64)()
+        self.submits_deferred = 0   # submissions run_poisson retried because waiting inputs filled the budget
 
     def close(self):
         if self.h is not None and self.h.value:
@@ -127,43 +128,144 @@ class TraceServer(object):
         check(self.lib.rpx_server_counts(self.h, ctypes.byref(w), ctypes.byref(a)))
         return w.value, a.value
 
+    def stats(self):
+        """(jobs held back at admission by the HBM budget, peak bytes the
+        budget counted)."""
+        held, peak = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        check(self.lib.rpx_server_stats(self.h, ctypes.byref(held), ctypes.byref(peak)))
+        return held.value, peak.value
+
     def busy(self):
         w, a = self.counts()
         return w + a > 0
+
+
+class _GradientWorker(object):
+    """Gradient jobs run on a thread and a HIP stream of their own, through
+    the adjoint's torch.autograd.Function (adjoint.spot_rms): a job's
+    forward and backward pass overlap the server's batched trace steps, and
+    each other's host work, instead of blocking the serving loop.  Both
+    streams run at the default priority."""
+
+    def __init__(self):
+        import queue
+        import threading
+        import torch
+        self.device = torch.cuda.current_device()
+        self.q_in, self.q_out = queue.Queue(), queue.Queue()
+        self.stream = torch.cuda.Stream()
+        self.thread = threading.Thread(target=self._run, name="rpx-gradient", daemon=True)
+        self.thread.start()
+
+    def _run(self):
+        import torch
+        from .adjoint import SequentialLens, spot_rms
+        torch.cuda.set_device(self.device)
+        with torch.cuda.stream(self.stream):
+            while True:
+                item = self.q_in.get()
+                if item is None:
+                    return
+                k, w = item
+                try:
+                    lens = SequentialLens.from_workload(w)
+                    P = torch.tensor(lens.params, dtype=torch.float64, requires_grad=True)
+                    sigma = spot_rms(P, lens, w.collection())
+                    sigma.backward()
+                    self.q_out.put((k, float(sigma.detach()), P.grad.numpy().copy(), None))
+                except Exception as e:     # reported with the job
+                    self.q_out.put((k, None, None, e))
+
+    def submit(self, k, workload):
+        self.q_in.put((k, workload))
+
+    def finished(self, timeout=None):
+        """Finished gradients (k, spot RMS, gradient, error); waits up to
+        `timeout` seconds for the first when given."""
+        import queue
+        out = []
+        try:
+            out.append(self.q_out.get(timeout=timeout) if timeout else self.q_out.get_nowait())
+            while True:
+                out.append(self.q_out.get_nowait())
+        except queue.Empty:
+            pass
+        return out
+
+    def close(self):
+        self.q_in.put(None)
+        self.thread.join()
 
 
 def run_poisson(server, jobs, qps, seed=0, gradient=None):
     """Submit `jobs` (a list of Workloads) at Poisson arrival times of rate
     `qps` while stepping the server; returns the JobResults in job order and
     the wall time.  A job is submitted at the first step boundary after its
-    arrival time; its latency runs from its arrival to its result.  Jobs k
+    arrival time (or, when the server refuses it because waiting inputs fill
+    its HBM budget, after the next step); its latency runs from its arrival
+    to its result.  Jobs k
     with gradient[k] true also get d(spot RMS)/d(surface parameters) from
-    the adjoint kernel once their trace is done (result.grad, result.grad_spot)
-    and their latency includes it."""
-    from .adjoint import SequentialLens, spot_rms_and_grad
+    the adjoint (result.grad, result.grad_spot): the gradient starts when the
+    job is submitted and runs beside the server's steps on a stream of its
+    own; the job's result is ready once its trace and its gradient both are,
+    and its latency includes both."""
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / qps, len(jobs)))
+    worker = _GradientWorker() if gradient is not None and any(gradient) else None
     t0 = server.clock()
     nxt = 0
-    results = {}
-    ids = {}
-    while len(results) < len(jobs):
-        now = server.clock() - t0
-        while nxt < len(jobs) and arrivals[nxt] <= now:
-            ids[server.submit(jobs[nxt], info=nxt)] = nxt
-            nxt += 1
-        if not server.busy():
-            if nxt < len(jobs):   # idle until the next arrival
-                time.sleep(max(0.0, min(arrivals[nxt] - now, 1e-3)))
+    results, ids = {}, {}
+    traced = {}      # k -> JobResult of a gradient job whose gradient is not back yet
+    grads = {}       # k -> (spot RMS, gradient, error, server time) of a gradient job not traced yet
+
+    def finish(k, r):
+        if worker is not None and gradient[k]:
+            val, g, err, t = grads.pop(k)
+            if err is not None:
+                raise err
+            r.grad_spot, r.grad = val, g
+            r.t_finish = max(r.t_finish, t)
+        r.latency = r.t_finish - (t0 + arrivals[k])
+        results[k] = r
+
+    def collect(timeout=None):
+        for k, val, g, err in worker.finished(timeout):
+            grads[k] = (val, g, err, server.clock())
+        for k in [k for k in traced if k in grads]:
+            finish(k, traced.pop(k))
+
+    try:
+        while len(results) < len(jobs):
+            now = server.clock() - t0
+            while nxt < len(jobs) and arrivals[nxt] <= now:
+                try:
+                    jid = server.submit(jobs[nxt], info=nxt)
+                except RpxError as e:
+                    if e.code != _abi.RPX_E_CAPACITY or not server.busy():
+                        raise
+                    server.submits_deferred += 1
+                    break             # the budget is full of waiting inputs: submit after the next step
+                ids[jid] = nxt
+                if worker is not None and gradient[nxt]:
+                    worker.submit(nxt, jobs[nxt])
+                nxt += 1
+            if worker is not None:
+                collect()
+            if not server.busy():
+                if nxt < len(jobs):   # idle until the next arrival
+                    time.sleep(max(0.0, min(arrivals[nxt] - now, 1e-3)))
+                elif traced:          # only gradients outstanding
+                    collect(timeout=1e-3)
                 continue
-        for r in server.step():
-            k = ids[r.job_id]
-            r.grad = None
-            if gradient is not None and gradient[k]:
-                w = jobs[k]
-                r.grad_spot, r.grad = spot_rms_and_grad(SequentialLens.from_workload(w), w.collection())
-                r.t_finish = server.clock()
-            r.latency = r.t_finish - (t0 + arrivals[k])
-            results[k] = r
+            for r in server.step():
+                k = ids[r.job_id]
+                r.grad = r.grad_spot = None
+                if worker is not None and gradient[k] and k not in grads:
+                    traced[k] = r
+                else:
+                    finish(k, r)
+    finally:
+        if worker is not None:
+            worker.close()
     wall = server.clock() - t0
     return [results[k] for k in range(len(jobs))], wall

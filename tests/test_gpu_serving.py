@@ -194,3 +194,46 @@ def test_submit_respects_budget(gpu):
         assert len(res) == 2 and all(r.status == 0 for r in res)
     finally:
         server.close()
+
+
+def test_run_poisson_gradients_beside_the_loop(gpu):
+    """Gradient jobs run through the adjoint's autograd.Function on a stream
+    and thread of their own while the server steps: every job finishes, and
+    each gradient is bit-identical to the same computation run alone."""
+    from raypier_optics_amd.serving import run_poisson
+    from raypier_optics_amd.adjoint import SequentialLens, spot_rms_and_grad
+    jobs = _jobs(range(6), [3000, 20000, 1000, 5000, 8000, 2000])
+    grad = [True, False, True, True, False, True]
+    server = TraceServer(hbm_budget_bytes=1 << 30, max_jobs=8, max_rays_in_flight=1 << 22)
+    try:
+        results, wall = run_poisson(server, jobs, 2000.0, seed=1, gradient=grad)
+    finally:
+        server.close()
+    assert len(results) == len(jobs) and wall > 0
+    for k, (w, r) in enumerate(zip(jobs, results)):
+        assert r.status == 0 and r.latency > 0
+        if grad[k]:
+            s, g = spot_rms_and_grad(SequentialLens.from_workload(w), w.collection())
+            assert r.grad_spot == s and np.array_equal(r.grad, g)
+        else:
+            assert r.grad is None
+
+
+def test_run_poisson_budget_backpressure(gpu):
+    """A budget that holds about two jobs' buffers: jobs are held back at
+    admission (and submissions refused while waiting inputs fill the
+    budget are retried after a step); every job still finishes, with the
+    results it has alone."""
+    from raypier_optics_amd.serving import run_poisson
+    jobs = _jobs(range(8), [70000] * 8)
+    server = TraceServer(hbm_budget_bytes=100 << 20, max_jobs=8, max_rays_in_flight=1 << 22)
+    try:
+        results, _ = run_poisson(server, jobs, 1e5, seed=2)
+        held, peak = server.stats()
+    finally:
+        server.close()
+    assert held > 0 and peak > 0
+    for w, r in zip(jobs[:2], results[:2]):
+        a = _alone(w)
+        assert r.status == 0 and (r.generations, r.interactions) == (a.generations, a.interactions)
+        assert r.moments == a.moments
