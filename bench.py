@@ -226,9 +226,10 @@ def adjoint_timing(n_rays=1 << 21, seed=4, reps=5):
             "spot_rms_mm": float(_sigma(m)), "ray_surface_gradients_per_s": n_rays * lens.n_surfaces / bwd}
 
 
-def michelson_c3_leg(n_rays=1 << 20, order=1):
-    """C3: the Michelson with a Zernike mirror and a grating (one order),
-    1,048,576 rays to depth 12, float64 and FP32-split; interactions/s.
+def michelson_c3_leg(n_rays=1 << 20, orders=(-1, 0, 1)):
+    """C3: the Michelson with a Zernike mirror and a grating with orders -1,
+    0, +1 (up to three children per hit), 1,048,576 rays to depth 12,
+    float64 and FP32-split; interactions/s.
 
     Each form is traced once untimed and then timed, like the C2 leg's
     warm-up steps: the generations of one trace reserve about 190 GB, and
@@ -241,9 +242,10 @@ def michelson_c3_leg(n_rays=1 << 20, order=1):
     lib = _lib.load()
     nk = len(_abi.TIMING_KERNELS)
     out = {"workload": "michelson_c3: N-BK7 beamsplitter cube, flat + Zernike (Noll 4-15) PEC mirrors, "
-                       "600 l/mm grating order %+d, relative thresholds 1e-4, depth 12" % order, "rays": n_rays}
+                       "600 l/mm grating orders %s in one multi-order grating, relative thresholds 1e-4, depth 12"
+                       % (tuple(orders),), "rays": n_rays}
     for name, tracer in (("fp64", trace_rays), ("fp32split", trace_rays_split)):
-        w = workloads.michelson_c3(n_rays=n_rays, order=order)
+        w = workloads.michelson_c3(n_rays=n_rays, orders=orders)
         for timed_run in (False, True):
             rc = w.collection()
             rc._ensure_device()

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RPX_ABI_VERSION 10
+#define RPX_ABI_VERSION 11
 
 /* ------------------------------------------------------------------ status */
 typedef int32_t rpx_status_t;
@@ -185,8 +185,14 @@ enum rpx_material_type {
 
 typedef struct rpx_material {
     int32_t type;                   /* rpx_material_type                         */
-    int32_t order;                  /* grating order                             */
-    int32_t pad[2];
+    int32_t order;                  /* grating order (the first, for several)    */
+    int32_t n_orders;               /* grating: 0 or 1 = the one `order`
+                                       (DiffractionGratingMaterial); 2 or 3 =
+                                       `order`, then the signed bytes of
+                                       more_orders, one child per propagating
+                                       order in that order (this build's
+                                       multi-order grating)                       */
+    int32_t more_orders;            /* second order in bits 0-7, third in 8-15   */
     double  reflection_threshold;
     double  transmission_threshold;
     double  n_inside[2];            /* complex (re, im), non-dispersive types    */

@@ -1126,39 +1126,44 @@ static void eval_child_ray(const rpx_scene_desc_t *S, const rpx_material_t *m, o
         }
         break;
     }
-    case RPX_MAT_GRATING: {  /* :1472-1542 */
-        vec_t normal = norm_(orient.normal);
-        vec_t tangent = norm_(orient.tangent);
-        vec_t tangent2 = cross_(normal, tangent);
-        double wavelen = wl_at(wls, n_wl, in_ray->wavelength_idx);
-        double line_spacing = 1000.0 / m->lines_per_mm;
-        vec_t reflected = norm_(in_ray->direction);
-        double k_z = dotprod_(normal, reflected);
-        double k_y = dotprod_(tangent2, reflected);
-        double k_x = dotprod_(tangent, reflected);
-        int sign = (k_z < 0.0) ? 1 : -1;
-        oray_t sp;
-        vec_t og = V(m->origin[0], m->origin[1], m->origin[2]);
-        k_x = k_x - m->order * wavelen / (line_spacing * in_ray->n_re);
-        k_z = 1 - (k_x * k_x) - (k_y * k_y);
-        if (k_z < 0) return;
-        k_z = sign * sqrt(k_z);
-        reflected = multvs_(tangent, k_x);
-        reflected = addvv_(reflected, multvs_(tangent2, k_y));
-        reflected = addvv_(reflected, multvs_(normal, k_z));
-        sp = convert_to_sp(*in_ray, normal);
-        sp.accumulated_path += sp.length * sp.n_re;
-        sp.origin = point;
-        sp.normal = normal;
-        sp.direction = reflected;
-        sp.e1_re = -sp.e1_re * m->efficiency;
-        sp.e1_im = -sp.e1_im * m->efficiency;
-        sp.e2_re = sp.e2_re * m->efficiency;
-        sp.e2_im = sp.e2_im * m->efficiency;
-        sp.parent_idx = idx;
-        sp.ray_type_id |= RPX_REFL_RAY;
-        sp.phase += 1000.0 * dotprod_(subvv_(og, point), tangent) * m->order * 2 * M_PI / line_spacing;
-        add_ray(new_rays, &sp);
+    case RPX_MAT_GRATING: {  /* :1472-1542; several orders: each order's child in list order (this
+                                build's multi-order grating, n_orders 2 or 3) */
+        int n_orders = m->n_orders > 1 ? m->n_orders : 1;
+        for (int o = 0; o < n_orders; o++) {
+            int order = o == 0 ? m->order : (int)(int8_t)((uint32_t)m->more_orders >> (8 * (o - 1)));
+            vec_t normal = norm_(orient.normal);
+            vec_t tangent = norm_(orient.tangent);
+            vec_t tangent2 = cross_(normal, tangent);
+            double wavelen = wl_at(wls, n_wl, in_ray->wavelength_idx);
+            double line_spacing = 1000.0 / m->lines_per_mm;
+            vec_t reflected = norm_(in_ray->direction);
+            double k_z = dotprod_(normal, reflected);
+            double k_y = dotprod_(tangent2, reflected);
+            double k_x = dotprod_(tangent, reflected);
+            int sign = (k_z < 0.0) ? 1 : -1;
+            oray_t sp;
+            vec_t og = V(m->origin[0], m->origin[1], m->origin[2]);
+            k_x = k_x - order * wavelen / (line_spacing * in_ray->n_re);
+            k_z = 1 - (k_x * k_x) - (k_y * k_y);
+            if (k_z < 0) continue;
+            k_z = sign * sqrt(k_z);
+            reflected = multvs_(tangent, k_x);
+            reflected = addvv_(reflected, multvs_(tangent2, k_y));
+            reflected = addvv_(reflected, multvs_(normal, k_z));
+            sp = convert_to_sp(*in_ray, normal);
+            sp.accumulated_path += sp.length * sp.n_re;
+            sp.origin = point;
+            sp.normal = normal;
+            sp.direction = reflected;
+            sp.e1_re = -sp.e1_re * m->efficiency;
+            sp.e1_im = -sp.e1_im * m->efficiency;
+            sp.e2_re = sp.e2_re * m->efficiency;
+            sp.e2_im = sp.e2_im * m->efficiency;
+            sp.parent_idx = idx;
+            sp.ray_type_id |= RPX_REFL_RAY;
+            sp.phase += 1000.0 * dotprod_(subvv_(og, point), tangent) * order * 2 * M_PI / line_spacing;
+            add_ray(new_rays, &sp);
+        }
         break;
     }
     default:

@@ -33,7 +33,7 @@ gausslet_dtype = np.dtype([('base_ray', ray_dtype),
 
 assert ray_dtype.itemsize == 188 and para_dtype.itemsize == 80 and gausslet_dtype.itemsize == 668
 
-ABI_VERSION = 10          # RPX_ABI_VERSION of include/rpx.h
+ABI_VERSION = 11          # RPX_ABI_VERSION of include/rpx.h
 
 # timing slots (RPX_TIMING_*)
 TIMING_KERNELS = ("k_intersect", "k_scan", "k_emit", "k_generation", "k_clear", "k_generation_split",
@@ -88,7 +88,7 @@ shape_op_dtype = np.dtype([('op', np.int32), ('a', np.int32), ('b', np.int32), (
 dispersion_dtype = np.dtype([('formula_id', np.int32), ('n_coefs', np.int32), ('absorption', np.float64),
                              ('coefs', np.float64, (MAX_DISP_COEFS,))])
 
-material_dtype = np.dtype([('type', np.int32), ('order', np.int32), ('pad', np.int32, (2,)),
+material_dtype = np.dtype([('type', np.int32), ('order', np.int32), ('n_orders', np.int32), ('more_orders', np.int32),
                            ('reflection_threshold', np.float64), ('transmission_threshold', np.float64),
                            ('n_inside', np.float64, (2,)), ('n_outside', np.float64, (2,)),
                            ('n_coating', np.float64, (2,)), ('thickness', np.float64),

@@ -58,7 +58,11 @@ class SceneTables(object):
         best = 0
         for f in self.faces[:n_list]:
             if f["material"] >= 0:
-                best = max(best, mc.get(int(self.materials[f["material"]]["type"]), 0))
+                m = self.materials[f["material"]]
+                k = mc.get(int(m["type"]), 0)
+                if int(m["type"]) == _abi.MAT_GRATING:
+                    k = max(1, int(m["n_orders"]))
+                best = max(best, k)
         return best
 
     def key(self):

@@ -293,7 +293,13 @@ class CoatedDispersiveMaterial(_DispersiveBase):
 
 
 class DiffractionGratingMaterial(InterfaceMaterial):
-    """Reflective grating, one diffraction order (cmaterials.pyx:1437-1599)."""
+    """Reflective grating, one diffraction order (cmaterials.pyx:1437-1599).
+
+    orders=(o1, o2[, o3]) -- not in the reference, whose material has one
+    `order` -- makes one grating emit up to three orders: one child per
+    propagating order, in the listed order, each exactly the child a
+    one-order grating of that order emits (same efficiency for all).  An
+    evanescent order emits nothing, as in the reference."""
 
     def __init__(self, **kwds):
         super().__init__()
@@ -301,6 +307,16 @@ class DiffractionGratingMaterial(InterfaceMaterial):
         self.order = kwds.get("order", 1)
         self.efficiency = kwds.get("efficiency", 1.0)
         self.origin = kwds.get("origin", (0.0, 0.0, 0.0))
+        orders = kwds.get("orders")
+        if orders is not None:
+            orders = tuple(int(o) for o in orders)
+            if not 1 <= len(orders) <= 3 or any(not -128 <= o <= 127 for o in orders):
+                raise ValueError("orders: one to three orders in -128..127")
+            self.order = orders[0]
+        self.orders = orders
+
+    def max_children(self):
+        return len(self.orders) if self.orders else 1
 
     @property
     def origin(self):
@@ -314,5 +330,9 @@ class DiffractionGratingMaterial(InterfaceMaterial):
         rec["type"] = _abi.MAT_GRATING
         rec["lines_per_mm"] = self.lines_per_mm
         rec["order"] = int(self.order)
+        if self.orders and len(self.orders) > 1:
+            rec["n_orders"] = len(self.orders)
+            more = [o & 0xFF for o in self.orders[1:]]
+            rec["more_orders"] = more[0] | ((more[1] if len(more) > 1 else 0) << 8)
         rec["efficiency"] = self.efficiency
         rec["origin"] = self._origin

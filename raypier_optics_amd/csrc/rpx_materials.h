@@ -15,11 +15,17 @@ struct Ray {
     uint32_t wl_idx, parent_idx, end_face_idx, ray_ident, ray_type_id;
 };
 
-// Collects the (at most two) children of one ray as a two-slot shift
-// register: r0 is the latest child, r1 the one before.  Conditional or
-// indexed slot writes make the compiler keep the pair in scratch memory;
+// Emitters take children in order through operator(); kCap is how many a
+// material should build and hand over, skip() counts one more child the
+// emitter does not take (a multi-order grating's third order, built again
+// after the children's positions are known).
+//
+// PairEmit collects the (at most two) children of one ray as a two-slot
+// shift register: r0 is the latest child, r1 the one before.  Conditional
+// or indexed slot writes make the compiler keep the pair in scratch memory;
 // the unconditional shift keeps both in registers.
 struct PairEmit {
+    static constexpr int kCap = 2;
     Ray r0, r1;
     int n = 0;
     RPX_DEV void operator()(const Ray &x) {
@@ -27,8 +33,16 @@ struct PairEmit {
         r0 = x;
         ++n;
     }
+    RPX_DEV void skip() { ++n; }
     RPX_DEV const Ray &first() const { return n > 1 ? r1 : r0; }
 };
+
+// Order k (0-based) of a grating: `order`, then the signed bytes of
+// more_orders (multi-order gratings, n_orders 2 or 3).
+RPX_DEV int grating_order(const rpx_material_t &m, int k) {
+    return k == 0 ? m.order : (int)(int8_t)((uint32_t)m.more_orders >> (8 * (k - 1)));
+}
+RPX_DEV int grating_orders(const rpx_material_t &m) { return m.n_orders > 1 ? m.n_orders : 1; }
 
 RPX_DEV Ray load_ray(const rpx_rays_t &r, uint64_t i) {
     const double *f = r.f64;
@@ -177,6 +191,56 @@ RPX_DEV double power_frac(double nre, Cx a, Cx b, double P_in) {
 
 // ------------------------------------------------------------- children
 // InterfaceMaterial.eval_child_ray_c of every supported material.
+// DiffractionGratingMaterial.eval_child_ray_c (cmaterials.pyx:1472-1542):
+// one child per propagating order, orders in list order (one order in the
+// reference); children past the emitter's kCap are counted, not built.
+template <class Emit>
+RPX_DEV void grating_children(const rpx_material_t &m, const Ray &in, uint32_t idx, V3 point,
+                              const Orientation &orient, const WlTable &wavelengths, Emit &emit) {
+    V3 normal = norm(orient.normal);
+    V3 tangent = norm(orient.tangent);
+    V3 tangent2 = cross(normal, tangent);
+    double wavelen = wavelengths[in.wl_idx];
+    double line_spacing = 1000.0 / m.lines_per_mm;
+    const V3 dir = norm(in.direction);
+    const double k_z0 = dot(normal, dir);
+    const double k_y = dot(tangent2, dir);
+    const double k_x0 = dot(tangent, dir);
+    int sign = (k_z0 < 0.0) ? 1 : -1;
+    double n_re = in.n.re;
+    const int n_orders = grating_orders(m);
+    int built = 0;
+    for (int o = 0; o < n_orders; ++o) {
+        const int order = grating_order(m, o);
+        double k_x = k_x0 - order * wavelen / (line_spacing * n_re);
+        double k_z = 1 - (k_x * k_x) - (k_y * k_y);
+        if (k_z < 0) continue;
+        if (built >= Emit::kCap) {
+            emit.skip();
+            continue;
+        }
+        ++built;
+        k_z = sign * sqrt(k_z);
+        V3 reflected = mul(tangent, k_x);
+        reflected = add(reflected, mul(tangent2, k_y));
+        reflected = add(reflected, mul(normal, k_z));
+        Ray sp = convert_to_sp(in, normal);
+        sp.accum += sp.length * sp.n.re;
+        sp.origin = point;
+        sp.normal = normal;
+        sp.direction = reflected;
+        sp.E1.re = -sp.E1.re * m.efficiency;
+        sp.E1.im = -sp.E1.im * m.efficiency;
+        sp.E2.re = sp.E2.re * m.efficiency;
+        sp.E2.im = sp.E2.im * m.efficiency;
+        sp.parent_idx = idx;
+        sp.ray_type_id |= RPX_REFL_RAY;
+        V3 og = v3(m.origin[0], m.origin[1], m.origin[2]);
+        sp.phase += 1000.0 * dot(sub(og, point), tangent) * order * 2 * M_PI / line_spacing;
+        emit(sp);
+    }
+}
+
 template <uint32_t F, class Emit>
 RPX_DEV void eval_children(const rpx_material_t &m, const Ray &in, uint32_t idx, V3 point,
                            const Orientation &orient, const WlTable &wavelengths, Emit &emit) {
@@ -455,39 +519,8 @@ RPX_DEV void eval_children(const rpx_material_t &m, const Ray &in, uint32_t idx,
         }
     } break;
     case RPX_MAT_GRATING: {
-        if constexpr ((F & M_GRATING) == 0) return;  // cmaterials.pyx:1472-1542
-        V3 normal = norm(orient.normal);
-        V3 tangent = norm(orient.tangent);
-        V3 tangent2 = cross(normal, tangent);
-        double wavelen = wavelengths[in.wl_idx];
-        double line_spacing = 1000.0 / m.lines_per_mm;
-        V3 reflected = norm(in.direction);
-        double k_z = dot(normal, reflected);
-        double k_y = dot(tangent2, reflected);
-        double k_x = dot(tangent, reflected);
-        int sign = (k_z < 0.0) ? 1 : -1;
-        double n_re = in.n.re;
-        k_x = k_x - m.order * wavelen / (line_spacing * n_re);
-        k_z = 1 - (k_x * k_x) - (k_y * k_y);
-        if (k_z < 0) return;
-        k_z = sign * sqrt(k_z);
-        reflected = mul(tangent, k_x);
-        reflected = add(reflected, mul(tangent2, k_y));
-        reflected = add(reflected, mul(normal, k_z));
-        Ray sp = convert_to_sp(in, normal);
-        sp.accum += sp.length * sp.n.re;
-        sp.origin = point;
-        sp.normal = normal;
-        sp.direction = reflected;
-        sp.E1.re = -sp.E1.re * m.efficiency;
-        sp.E1.im = -sp.E1.im * m.efficiency;
-        sp.E2.re = sp.E2.re * m.efficiency;
-        sp.E2.im = sp.E2.im * m.efficiency;
-        sp.parent_idx = idx;
-        sp.ray_type_id |= RPX_REFL_RAY;
-        V3 og = v3(m.origin[0], m.origin[1], m.origin[2]);
-        sp.phase += 1000.0 * dot(sub(og, point), tangent) * m.order * 2 * M_PI / line_spacing;
-        emit(sp);
+        if constexpr ((F & M_GRATING) == 0) return;
+        grating_children(m, in, idx, point, orient, wavelengths, emit);
     } break;
     default:  // RPX_MAT_NONE, RPX_MAT_OPAQUE: no children
         break;

@@ -44,6 +44,7 @@ __global__ void k_query_zernike(SceneView S, int zi, const double *xy, uint64_t 
 }
 
 struct SlotEmit {
+    static constexpr int kCap = 3;
     const rpx_rays_t *out;
     uint64_t base;
     int n = 0;
@@ -51,6 +52,7 @@ struct SlotEmit {
         if (n < 3) store_ray(*out, base + n, r);
         ++n;
     }
+    RPX_DEV void skip() { ++n; }
 };
 
 __global__ void k_eval_child(SceneView S, int mat, rpx_rays_t in, uint64_t n, uint32_t idx_base, const double *pts,

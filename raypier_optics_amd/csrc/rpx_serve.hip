@@ -470,6 +470,9 @@ extern "C" rpx_status_t rpx_server_submit(rpx_server_t *s, const rpx_job_desc_t 
     if ((d->n_rays + RPX_BLOCK - 1) / RPX_BLOCK + 1 > s->max_tiles)
         return rpx_set_error(RPX_E_CAPACITY, "job larger than the server's rays in flight");
     if (d->n_wavelengths && !d->wavelengths) return rpx_set_error(RPX_E_INVALID, "wavelength table is null");
+    if (d->scene->view.max_children > 2)
+        return rpx_set_error(RPX_E_UNSUPPORTED, "the server traces at most two children per ray (a multi-order "
+                             "grating is traced by rpx_trace_rays)");
     // a waiting job's input copy counts against the budget from submission
     const uint64_t in_bytes = std::max<uint64_t>(d->n_rays, 1) * RAY_BYTES;
     if (s->used + in_bytes > s->budget)

@@ -175,6 +175,50 @@ RPX_DEV RayS begin_child(const RayS &in, double length, F3 normal_f, V3 point, u
     return sp;
 }
 
+// The grating's children, FP32-split (the float64 grating_children's
+// emitter protocol: one child per propagating order, those past kCap
+// counted, not built).
+template <class Emit>
+RPX_DEV void grating_children_split(const rpx_material_t &m, const RayS &in, double length, uint32_t idx, V3 point,
+                                    const Orientation &orient, const WlTable &wavelengths, Emit &emit) {
+    const V3 normal = norm(orient.normal);
+    const F3 normal_f = f3_of(normal);
+    const V3 tangent = norm(orient.tangent);
+    const V3 tangent2 = cross(normal, tangent);
+    const double wavelen = wavelengths[in.wl_idx];
+    const double line_spacing = 1000.0 / m.lines_per_mm;
+    const V3 dir = norm(v3_of(in.direction));
+    const double k_z0 = dot(normal, dir);
+    const double k_y = dot(tangent2, dir);
+    const double k_x0 = dot(tangent, dir);
+    const int sign = (k_z0 < 0.0) ? 1 : -1;
+    const int n_orders = grating_orders(m);
+    int built = 0;
+    for (int o = 0; o < n_orders; ++o) {
+        const int order = grating_order(m, o);
+        const double k_x = k_x0 - order * wavelen / (line_spacing * (double)in.n.re);
+        double k_z = 1 - (k_x * k_x) - (k_y * k_y);
+        if (k_z < 0) continue;
+        if (built >= Emit::kCap) {
+            emit.skip();
+            continue;
+        }
+        ++built;
+        k_z = sign * sqrt(k_z);
+        const V3 reflected = add(add(mul(tangent, k_x), mul(tangent2, k_y)), mul(normal, k_z));
+        RayS sp = begin_child(in, length, normal_f, point, idx);
+        sp.length = (float)length;   // the grating child keeps the hit length (cmaterials.pyx:1472-1542)
+        const float eff = (float)m.efficiency;
+        sp.direction = f3_of(reflected);
+        sp.E1 = cfscale(sp.E1, -eff);
+        sp.E2 = cfscale(sp.E2, eff);
+        sp.ray_type_id |= RPX_REFL_RAY;
+        const V3 og = v3(m.origin[0], m.origin[1], m.origin[2]);
+        sp.phase += 1000.0 * dot(sub(og, point), tangent) * order * 2 * M_PI / line_spacing;
+        emit(sp);
+    }
+}
+
 // InterfaceMaterial.eval_child_ray_c of every material, FP32-split.
 template <uint32_t F, class Emit>
 RPX_DEV void eval_children_split(const rpx_material_t &m, const RayS &in, double length, uint32_t idx,
@@ -380,30 +424,7 @@ RPX_DEV void eval_children_split(const rpx_material_t &m, const RayS &in, double
     } break;
     case RPX_MAT_GRATING: {
         if constexpr ((F & M_GRATING) == 0) return;
-        const V3 tangent = norm(orient.tangent);
-        const V3 tangent2 = cross(normal, tangent);
-        const double wavelen = wavelengths[in.wl_idx];
-        const double line_spacing = 1000.0 / m.lines_per_mm;
-        V3 reflected = norm(v3_of(in.direction));
-        double k_z = dot(normal, reflected);
-        const double k_y = dot(tangent2, reflected);
-        double k_x = dot(tangent, reflected);
-        const int sign = (k_z < 0.0) ? 1 : -1;
-        k_x = k_x - m.order * wavelen / (line_spacing * (double)in.n.re);
-        k_z = 1 - (k_x * k_x) - (k_y * k_y);
-        if (k_z < 0) return;
-        k_z = sign * sqrt(k_z);
-        reflected = add(add(mul(tangent, k_x), mul(tangent2, k_y)), mul(normal, k_z));
-        RayS sp = begin_child(in, length, normal_f, point, idx);
-        sp.length = (float)length;   // the grating child keeps the hit length (cmaterials.pyx:1472-1542)
-        const float eff = (float)m.efficiency;
-        sp.direction = f3_of(reflected);
-        sp.E1 = cfscale(sp.E1, -eff);
-        sp.E2 = cfscale(sp.E2, eff);
-        sp.ray_type_id |= RPX_REFL_RAY;
-        const V3 og = v3(m.origin[0], m.origin[1], m.origin[2]);
-        sp.phase += 1000.0 * dot(sub(og, point), tangent) * m.order * 2 * M_PI / line_spacing;
-        emit(sp);
+        grating_children_split(m, in, length, idx, point, orient, wavelengths, emit);
     } break;
     default:  // RPX_MAT_NONE, RPX_MAT_OPAQUE: no children
         break;

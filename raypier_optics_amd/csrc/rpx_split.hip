@@ -43,6 +43,7 @@ RPX_DEV rpx_rays_split_t advance_split(rpx_rays_split_t r, uint64_t off) {
 // Collects the (at most two) children of one ray as a two-slot shift
 // register (r0 latest), which the compiler keeps in registers.
 struct PairEmitS {
+    static constexpr int kCap = 2;
     RayS r0, r1;
     int n = 0;
     RPX_DEV void operator()(const RayS &x) {
@@ -50,7 +51,22 @@ struct PairEmitS {
         r0 = x;
         ++n;
     }
+    RPX_DEV void skip() { ++n; }
     RPX_DEV const RayS &first() const { return n > 1 ? r1 : r0; }
+};
+
+// After a generation's children are placed: stores children from index 2 on
+// (a multi-order grating's third order) at pos + index, within `limit`.
+struct TailEmitS {
+    static constexpr int kCap = 3;
+    const rpx_rays_split_t *out;
+    uint64_t pos, limit;
+    int n = 0;
+    RPX_DEV void operator()(const RayS &r) {
+        if (n >= 2 && pos + n < limit) store_ray_s(*out, pos + n, r);
+        ++n;
+    }
+    RPX_DEV void skip() { ++n; }
 };
 
 template <uint32_t F>
@@ -88,6 +104,8 @@ void k_generation_split(SplitArgs A) {
     const uint64_t i = (uint64_t)tile * RPX_BLOCK + threadIdx.x;
     PairEmitS ce;
     bool hit = false;
+    int hit_rec = -1, hit_list = -1;
+    double hit_length = 0;
     if (i < n_in) {
         const float *f = in.f32;
         const uint64_t c = in.capacity;
@@ -95,7 +113,6 @@ void k_generation_split(SplitArgs A) {
         const V3 dir = v3(f[RPX_S_DX * c + i], f[RPX_S_DY * c + i], f[RPX_S_DZ * c + i]);
         double length = A.max_length;
         uint32_t end_face = RPX_NO_FACE;
-        int hit_rec = -1, hit_list = -1;
         const V3 point = add(origin, mul(dir, A.max_length));
         const int l0 = A.one_list < 0 ? 0 : A.one_list;
         const int l1 = A.one_list < 0 ? S.n_face_lists : A.one_list + 1;
@@ -128,6 +145,7 @@ void k_generation_split(SplitArgs A) {
         }
         in.f32[RPX_S_LENGTH * c + i] = (float)length;
         in.u32[RPX_END_FACE_IDX * c + i] = end_face;
+        hit_length = length;
         if (hit_rec >= 0) {
             hit = true;
             atomicAdd(&lds_hist[hit_rec], 1u);
@@ -143,7 +161,8 @@ void k_generation_split(SplitArgs A) {
             }
         }
     }
-    const uint32_t count = (uint32_t)min(ce.n, 2);
+    // a multi-order grating (all-features variant only) can have a third child
+    const uint32_t count = (uint32_t)min(ce.n, (F & M_GRATING) ? 3 : 2);
     uint32_t block_total;
     const uint32_t local = block_scan_small(count, lds_wave, &block_total);
     {
@@ -158,6 +177,18 @@ void k_generation_split(SplitArgs A) {
     const uint64_t pos = lds_excl + local;
     if (count > 0 && pos < out_limit) store_ray_s(out, pos, ce.first());
     if (count > 1 && pos + 1 < out_limit) store_ray_s(out, pos + 1, ce.r0);
+    if constexpr ((F & M_GRATING) != 0) {
+        if (count > 2 && pos + 2 < out_limit) {
+            // the third child is built again now that its position is known
+            const rpx_face_t &fc = S.faces[hit_rec];
+            RayS r = load_ray_s(in, i);
+            r.length = (float)hit_length;
+            const V3 hp = add(v3_of(r.origin), mul(v3_of(r.direction), hit_length));
+            const Orientation o = compute_orientation<F>(S, S.face_lists[hit_list], fc, hp);
+            TailEmitS te{&out, pos, out_limit};
+            grating_children_split(S.materials[fc.material], r, hit_length, (uint32_t)i, hp, o, A.wavelengths, te);
+        }
+    }
     if (threadIdx.x == 0) {
         add_hits(A.ws.ctl, tile, lds_hits);
         if ((uint64_t)tile == (n_in - 1) / RPX_BLOCK) {

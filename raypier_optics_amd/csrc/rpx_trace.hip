@@ -61,12 +61,13 @@ extern "C" int32_t rpx_struct_size(const char *name) {
 
 // ----------------------------------------------------------------- scene
 
-static int material_max_children(int type) {
-    switch (type) {
+static int material_max_children(const rpx_material_t &m) {
+    switch (m.type) {
+    case RPX_MAT_GRATING:
+        return m.n_orders > 1 ? m.n_orders : 1;
     case RPX_MAT_TRANSPARENT:
     case RPX_MAT_PEC:
     case RPX_MAT_DIELECTRIC:
-    case RPX_MAT_GRATING:
         return 1;
     case RPX_MAT_PARTIAL:
     case RPX_MAT_FULL_DIELECTRIC:
@@ -120,6 +121,9 @@ static rpx_status_t validate_scene(const rpx_scene_desc_t *d) {
         if (f.type < RPX_FACE_CIRCULAR || f.type > RPX_FACE_DISTORTION)
             return set_error(RPX_E_UNSUPPORTED, "unknown face type " + std::to_string(f.type));
         if (f.material >= d->n_materials) return set_error(RPX_E_INVALID, "face material index");
+        if (f.material >= 0 && d->materials[f.material].type == RPX_MAT_GRATING &&
+            (d->materials[f.material].n_orders < 0 || d->materials[f.material].n_orders > 3))
+            return set_error(RPX_E_INVALID, "a grating has more than three orders");
         if (f.shape >= d->n_shape_ops) return set_error(RPX_E_INVALID, "face shape index");
         if (f.type == RPX_FACE_POLYGON &&
             (f.aux1 < 1 || f.aux0 < 0 || f.aux0 + f.aux1 > d->n_vertices))
@@ -238,7 +242,7 @@ extern "C" rpx_status_t rpx_scene_create(const rpx_scene_desc_t *d, rpx_scene_t 
     for (int i = 0; i < nlist; ++i) {
         const rpx_face_t &f = d->faces[i];
         if (f.global_idx + 1 > nglob) nglob = f.global_idx + 1;
-        int mc = f.material >= 0 ? material_max_children(d->materials[f.material].type) : 0;
+        int mc = f.material >= 0 ? material_max_children(d->materials[f.material]) : 0;
         if (mc > maxc) maxc = mc;
     }
     v.max_children = maxc;
@@ -312,8 +316,10 @@ struct TraceArgs {
 };
 
 struct CountEmit {
+    static constexpr int kCap = 3;
     int n = 0;
     RPX_DEV void operator()(const Ray &) { ++n; }
+    RPX_DEV void skip() { ++n; }
 };
 
 template <bool GAUSS, uint32_t F>
@@ -468,6 +474,7 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t *totals, const uin
 // Writes at most `room` children (the count reserved by the counting pass),
 // so a lane can never write outside its slot range.
 struct StoreEmit {
+    static constexpr int kCap = 3;
     const rpx_rays_t *out;
     uint64_t pos;
     uint32_t room;
@@ -476,16 +483,33 @@ struct StoreEmit {
         store_ray(*out, pos++, r);
         --room;
     }
+    RPX_DEV void skip() {}
+};
+
+// After a generation's children are placed: stores children from index 2 on
+// (a multi-order grating's third order) at pos + index, within `limit`.
+struct TailEmit {
+    static constexpr int kCap = 3;
+    const rpx_rays_t *out;
+    uint64_t pos, limit;
+    int n = 0;
+    RPX_DEV void operator()(const Ray &r) {
+        if (n >= 2 && pos + n < limit) store_ray(*out, pos + n, r);
+        ++n;
+    }
+    RPX_DEV void skip() { ++n; }
 };
 
 // Collects up to 3 children of one gausslet base ray.
 struct CollectEmit {
+    static constexpr int kCap = 3;
     Ray r[3];
     int n = 0;
     RPX_DEV void operator()(const Ray &x) {
         if (n < 3) r[n] = x;
         ++n;
     }
+    RPX_DEV void skip() { ++n; }
 };
 
 template <bool GAUSS, uint32_t F>
@@ -589,6 +613,7 @@ void k_generation(TraceArgs A) {
     const uint64_t i = (uint64_t)tile * RPX_BLOCK + threadIdx.x;
     PairEmit ce;
     bool hit = false;
+    int hit_rec = -1, hit_list = -1;
     if (i < n_in) {
         const double *f = in.f64;
         const uint64_t c = in.capacity;
@@ -596,7 +621,6 @@ void k_generation(TraceArgs A) {
         V3 dir = v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]);
         double length = A.max_length;
         uint32_t end_face = RPX_NO_FACE;
-        int hit_rec = -1, hit_list = -1;
         V3 point = add(origin, mul(dir, A.max_length));
         if (A.one_list < 0) {
             // trace_segment_c / FaceList.intersect_c (ctracer.pyx:2093-2104, 1882-1904)
@@ -649,7 +673,8 @@ void k_generation(TraceArgs A) {
             }
         }
     }
-    const uint32_t count = (uint32_t)min(ce.n, 2);
+    // a multi-order grating (all-features variant only) can have a third child
+    const uint32_t count = (uint32_t)min(ce.n, (F & M_GRATING) ? 3 : 2);
     uint32_t block_total;
     const uint32_t local = block_scan_small(count, lds_wave, &block_total);
     {
@@ -664,6 +689,17 @@ void k_generation(TraceArgs A) {
     const uint64_t pos = lds_excl + local;
     if (count > 0 && pos < out_limit) store_ray(out, pos, ce.first());
     if (count > 1 && pos + 1 < out_limit) store_ray(out, pos + 1, ce.r0);
+    if constexpr ((F & M_GRATING) != 0) {
+        if (count > 2 && pos + 2 < out_limit) {
+            // the third child is built again now that its position is known
+            const rpx_face_t &fc = S.faces[hit_rec];
+            Ray r = load_ray(in, i);
+            V3 hp = add(r.origin, mul(r.direction, r.length));
+            Orientation o = compute_orientation<F>(S, S.face_lists[hit_list], fc, hp);
+            TailEmit te{&out, pos, out_limit};
+            grating_children(S.materials[fc.material], r, (uint32_t)i, hp, o, A.wavelengths, te);
+        }
+    }
     if (threadIdx.x == 0) {
         add_hits(A.ws.ctl, tile, lds_hits);
         if ((uint64_t)tile == (n_in - 1) / RPX_BLOCK) {
@@ -950,6 +986,9 @@ extern "C" rpx_status_t rpx_trace_gausslet(const rpx_scene_t *scene, int32_t fac
     if (!scene) return set_error(RPX_E_INVALID, "scene is null");
     if (face_list >= scene->view.n_face_lists || (face_list >= 0 && face_idx < 0))
         return set_error(RPX_E_INVALID, "face list / face index out of range");
+    if (scene->view.max_children > 2)
+        return set_error(RPX_E_UNSUPPORTED, "gausslets through a multi-order grating (its parabasal rays follow "
+                         "one order)");
     return run_generation(scene, face_list, face_list >= 0 ? face_idx : -1, true, in, n_in, max_length,
                           wavelengths, n_wavelengths, out, n_out, face_hits, workspace, workspace_bytes, stream);
 }

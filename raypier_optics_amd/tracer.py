@@ -33,14 +33,19 @@ def _pool_rays(n, max_children, remaining):
     room for the generations still allowed (at most _CHAIN_GENERATIONS) at
     about n rays each, plus the worst case the chain has to be sure of before
     it launches a generation without waiting (its own size bound and its
-    children's), rounded up to a power of two and capped at 40 % of the
-    device memory free now.  (A field stride of 2^k rays puts every SoA plane
-    on the same boundary; on C2 the generation kernel ran 3-4 % faster with
-    such strides than with 15 x 2^22 or odd ones.)"""
+    children's), rounded up to a power of two below 2^24 rays and to a
+    multiple of 2^20 above, and capped at 40 % of the device memory free now.
+    (A field stride of 2^k rays puts every SoA plane on the same boundary; on
+    C2 the generation kernel ran 3-4 % faster with such strides than with
+    15 x 2^22 or odd ones.  Large pools round finer, so that the hundreds of
+    GB a deep branching trace keeps are not doubled by rounding.)"""
     torch = require_gpu()
     mc = max(int(max_children), 1)
     want = n * (min(remaining, _CHAIN_GENERATIONS) + 1 + 2 * mc)
-    want = 1 << max(int(want - 1).bit_length(), 0)
+    if want <= 1 << 24:
+        want = 1 << max(int(want - 1).bit_length(), 0)
+    else:
+        want = ((want + (1 << 20) - 1) >> 20) << 20
     free, _ = torch.cuda.mem_get_info()
     cap = int(0.4 * free) // (_abi.N_F64 * 8 + _abi.N_U32 * 4)
     return max(min(want, cap), mc * n, 1)

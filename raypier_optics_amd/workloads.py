@@ -481,7 +481,7 @@ def _placed(faces, columns, translation):
     return fl
 
 
-def michelson_c3(n_rays=1048576, order=1, seed=2025, depth=12, threshold=1e-4, wavelength=0.6328):
+def michelson_c3(n_rays=1048576, order=1, seed=2025, depth=12, threshold=1e-4, wavelength=0.6328, orders=None):
     """Michelson interferometer in the x-y plane, light along +x:
       * 25 mm N-BK7 beamsplitter cube at the origin -- four extruded side
         faces and two polygon end faces with FullDielectricMaterial (relative
@@ -491,9 +491,10 @@ def michelson_c3(n_rays=1048576, order=1, seed=2025, depth=12, threshold=1e-4, w
       * arm +y: PEC mirror at y = 50 with a Zernike distortion, Noll terms
         4-15 (ANSI-indexed), coefficients N(0, 0.05 um), unit radius 15 mm;
       * output port -y: 600 lines/mm reflective grating at y = -50 tilted by
-        10 degrees about z, one diffraction order `order` per trace (the
-        reference's grating material has a single order; orders -1, 0, +1
-        are three traces);
+        10 degrees about z, one diffraction order `order` per trace as the
+        reference's grating material has, or with orders=(-1, 0, 1) this
+        build's multi-order grating: up to three children per hit, each
+        order's child as a one-order grating emits it;
       * an opaque 200 mm box around everything.
     Rays: n_rays collimated along +x from x = -40 on a 4 mm disk, random
     elliptical polarisation (E1 = cos(chi) e^{i phi1}, E2 = sin(chi) e^{i phi2}).
@@ -531,7 +532,7 @@ def michelson_c3(n_rays=1048576, order=1, seed=2025, depth=12, threshold=1e-4, w
     ex = (math.cos(th), math.sin(th), 0.0)
     grating = _placed([ShapedPlanarFace(shape=CircleShape(radius=20.0), z_height=0.0,
                                         material=DiffractionGratingMaterial(lines_per_mm=600, order=order,
-                                                                           efficiency=0.33,
+                                                                           efficiency=0.33, orders=orders,
                                                                            origin=(0.0, -50.0, 0.0)))],
                       [ex, (0, 0, 1), (math.sin(th), -math.cos(th), 0.0)], (0.0, -50.0, 0.0))
     b = 100.0
@@ -556,6 +557,6 @@ def michelson_c3(n_rays=1048576, order=1, seed=2025, depth=12, threshold=1e-4, w
     phi1, phi2 = 2 * np.pi * rng.random(n_rays), 2 * np.pi * rng.random(n_rays)
     r["E1_amp"] = np.cos(chi) * np.exp(1j * phi1)
     r["E2_amp"] = np.sin(chi) * np.exp(1j * phi2)
-    info = {"order": order, "zernike_noll_4_15_mm": [float(c) for c in coefs], "n_glass": n_glass,
+    info = {"order": order, "orders": orders, "zernike_noll_4_15_mm": [float(c) for c in coefs], "n_glass": n_glass,
             "threshold": threshold, "seed": seed}
     return Workload("michelson_c3", r, [wavelength], [cube, mirror1, mirror2, grating, box], depth, 500.0, info)

@@ -6,6 +6,9 @@
   match their descendants in the full device trace (topology bit-exact,
   values within the 1e-9 parity bound).  Every generation is also
   parent-major with at most two children per parent.
+* C3 with the three-order grating, 1,048,576 rays to depth 12: the same
+  sampled-descendant check for 300 input rays, the ancestry followed on the
+  device (the generations hold about 7e8 rays).
 * C4, 262,144 gausslets x 512^2 points, float64 field sum.  The field at 24
   sampled grid points equals the oracle's sum over all gausslets (1e-9 of the
   grid's max|E|), with the same mode coefficients.
@@ -59,6 +62,57 @@ def test_c2_full_size_sampled_parity(gpu):
         compare_generation(got, ref[g], what="C2 full size, sampled gen %d" % g)
         sel_prev = sel
     assert len(gens) >= len(ref)
+
+
+def _gather(gen, sel):
+    """Host ray_dtype records of rows `sel` (device index tensor) of a
+    device-resident generation."""
+    from raypier_optics_amd._device import DeviceRays
+    dev = gen._ensure_device()
+    out = DeviceRays(max(len(sel), 1))
+    out.f64()[:, :len(sel)] = dev.f64()[:, :gen.n_rays][:, sel]
+    out.u32()[:, :len(sel)] = dev.u32()[:, :gen.n_rays][:, sel]
+    return out.download(len(sel))
+
+
+def test_c3_full_size_sampled_parity(gpu):
+    """C3 with the three-order grating, 1,048,576 rays to depth 12 (about
+    7e8 rays over all generations): the descendants of 300 sampled input
+    rays, found on the device by following parent_idx, against the oracle's
+    trace of those rays alone (topology bit-exact, values within 1e-9);
+    every generation parent-major with at most three children per parent."""
+    import torch
+    w = workloads.michelson_c3(orders=(-1, 0, 1))
+    traced, _ = trace_rays(w.collection(), w.face_lists, recursion_limit=w.recursion_limit,
+                           max_length=w.max_length)
+    n0 = traced[0].n_rays
+    assert n0 == 1048576 and len(traced) == 12
+    rng = np.random.default_rng(12)
+    S = np.sort(rng.choice(n0, 300, replace=False))
+    sub_rays = w.rays[S].copy()
+    workloads.prepare_faces(w.face_lists, w.wavelengths, w.max_length)
+    ref, _ = oracle.trace_rays(compile_scene(w.face_lists), sub_rays, w.max_length, w.wavelengths,
+                               w.recursion_limit)
+    S_dev = torch.as_tensor(S, device="cuda")
+    anc = torch.arange(n0, device="cuda")
+    sel_prev = None
+    for g, gen in enumerate(traced):
+        n = gen.n_rays
+        if g > 0:
+            parent = gen._ensure_device().u32()[1, :n].long() & 0xFFFFFFFF
+            assert bool((parent[1:] >= parent[:-1]).all()), "generation %d not parent-major" % g
+            assert int(torch.bincount(parent).max()) <= 3
+            anc = anc[parent]
+        sel = torch.nonzero(torch.isin(anc, S_dev)).flatten()
+        got = _gather(gen, sel)
+        assert len(got) == len(ref[g]), (g, len(got), len(ref[g]))
+        assert not got["wavelength_idx"].any()                  # one wavelength: index 0 everywhere
+        if g > 0:
+            got["parent_idx"] = np.searchsorted(sel_prev, got["parent_idx"]).astype(np.uint32)
+        compare_generation(got, ref[g], what="C3 full size, sampled gen %d" % g)
+        sel_prev = sel.cpu().numpy()
+    del traced
+    torch.cuda.empty_cache()
 
 
 def test_c4_full_size_field_samples(gpu):

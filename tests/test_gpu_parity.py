@@ -280,11 +280,13 @@ def test_trace_chain_pool_resumes(gpu, monkeypatch, limit):
         assert got[g].parent is got[g - 1]
 
 
-def test_trace_chain_matches_generation_path(gpu):
+@pytest.mark.parametrize("orders", [None, (-1, 0, 1)])
+def test_trace_chain_matches_generation_path(gpu, orders):
     """The device-resident loop and the one-call-per-generation path
     (trace_segment on each generation) give bit-identical generations and
-    face counts on the Michelson (C3) scene at 4,096 rays, depth 12."""
-    w = workloads.michelson_c3(n_rays=4096, order=1)
+    face counts on the Michelson (C3) scene at 4,096 rays, depth 12, with a
+    one-order and a three-order grating."""
+    w = workloads.michelson_c3(n_rays=4096, order=1, orders=orders)
     got, faces = trace_rays(w.collection(), w.face_lists, recursion_limit=w.recursion_limit,
                             max_length=w.max_length)
     counts_chain = [f.count for f in faces]
@@ -527,12 +529,21 @@ def test_field_sum_gausslet_chunks(gpu, n_g, n_x):
         assert np.abs(got - ref).max() <= tol * scale, prec
 
 
-@pytest.mark.parametrize("order", (-1, 0, 1))
+@pytest.mark.parametrize("order", (-1, 0, 1, "multi"))
 def test_trace_c3_michelson(gpu, order):
     """Michelson (beamsplitter cube, flat and Zernike-distorted PEC mirrors,
-    600 l/mm grating in one order), 2,048 rays to depth 12: every generation
-    bit-exact in topology and within 1e-9 in values."""
-    ref, hits = _oracle_trace(workloads.michelson_c3(n_rays=2048, order=order))
-    got, faces = _gpu_trace(workloads.michelson_c3(n_rays=2048, order=order))
-    _compare_traces(got, ref, hits, faces, "C3 order %d" % order)
+    600 l/mm grating in one order, or orders -1, 0, +1 in one multi-order
+    grating), 2,048 rays to depth 12: every generation bit-exact in topology
+    and within 1e-9 in values."""
+    kw = {"orders": (-1, 0, 1)} if order == "multi" else {"order": order}
+    ref, hits = _oracle_trace(workloads.michelson_c3(n_rays=2048, **kw))
+    got, faces = _gpu_trace(workloads.michelson_c3(n_rays=2048, **kw))
+    _compare_traces(got, ref, hits, faces, "C3 order %s" % order)
     assert len(got) == 12
+    if order == "multi":
+        triples = 0
+        for g in got[1:]:
+            counts = np.bincount(g.copy_as_array()["parent_idx"].astype(np.int64))
+            assert counts.max() <= 3
+            triples += int((counts == 3).sum())
+        assert triples > 0

@@ -156,7 +156,7 @@ def test_split_materials(gpu, k):
     assert np.abs(a["phase"] - b["phase"]).max() <= 2e-3
 
 
-@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("order", [0, 1, "multi"])
 def test_split_c3_michelson(gpu, order):
     """Flat optics (beamsplitter cube, plane and Zernike-distorted PEC
     mirrors, 600 l/mm grating) to depth 12: every generation of the split
@@ -167,8 +167,9 @@ def test_split_c3_michelson(gpu, order):
     +1 the grating adds 1000*((origin - p).t)*2*pi/spacing, 3770 rad per mm
     at 600 lines/mm, and at y = -50 mm a float32 origin coordinate is
     rounded by up to 1.9e-6 mm, so three coordinates bound the phase error
-    at 1.2e-2 rad."""
-    w = lambda: workloads.michelson_c3(n_rays=4096, order=order)
+    at 1.2e-2 rad (the multi-order grating, orders -1, 0, +1, likewise)."""
+    kw = {"orders": (-1, 0, 1)} if order == "multi" else {"order": order}
+    w = lambda: workloads.michelson_c3(n_rays=4096, **kw)
     _, t64, f64, t32, f32 = _both(w)
     _compare_topology(t64, t32)
     assert len(t64) >= 6
@@ -187,14 +188,14 @@ def test_split_c3_michelson(gpu, order):
         scale = np.maximum(np.abs(ra["E1_amp"]), np.abs(ra["E2_amp"]))[:, None]
         worst["field"] = max(worst["field"], float((np.abs(fa - fb) / np.maximum(scale, 1e-30)).max()))
         worst["phase"] = max(worst["phase"], float(np.abs(ra["phase"] - rb["phase"]).max()))
-    print("order %d worst split - float64:" % order, worst)
+    print("order %s worst split - float64:" % order, worst)
     assert worst["origin"] <= 1e-4
     assert worst["direction"] <= 1e-6
     assert worst["field"] <= 1e-5
     assert worst["phase"] <= (1.2e-2 if order else 2e-3)
 
 
-@pytest.mark.parametrize("small_pools", [False, True])
+@pytest.mark.parametrize("small_pools", [False, True, "multi"])
 def test_split_chain_matches_generation_path(gpu, monkeypatch, small_pools):
     """The device-resident FP32-split loop (rpx_trace_rays_split), with
     ordinary pools and with the smallest pools it accepts (so that it stops
@@ -204,9 +205,9 @@ def test_split_chain_matches_generation_path(gpu, monkeypatch, small_pools):
     import torch
     from raypier_optics_amd import tracer, split
     from raypier_optics_amd.ctracer import _TraceContext
-    if small_pools:
+    if small_pools is True:
         monkeypatch.setattr(tracer, "_pool_rays", lambda n, mc, remaining: max(mc * n, 1))
-    w = workloads.michelson_c3(n_rays=4096, order=1)
+    w = workloads.michelson_c3(n_rays=4096, **({"orders": (-1, 0, 1)} if small_pools == "multi" else {"order": 1}))
     got, faces = trace_rays_split(w.collection(), w.face_lists, recursion_limit=w.recursion_limit,
                                   max_length=w.max_length)
     counts_chain = [f.count for f in faces]
