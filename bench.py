@@ -241,6 +241,7 @@ def michelson_c3_leg(n_rays=1 << 20, orders=(-1, 0, 1)):
     from raypier_optics_amd.split import trace_rays_split
     lib = _lib.load()
     nk = len(_abi.TIMING_KERNELS)
+    torch.cuda.empty_cache()       # earlier legs' cached blocks
     out = {"workload": "michelson_c3: N-BK7 beamsplitter cube, flat + Zernike (Noll 4-15) PEC mirrors, "
                        "600 l/mm grating orders %s in one multi-order grating, relative thresholds 1e-4, depth 12"
                        % (tuple(orders),), "rays": n_rays}

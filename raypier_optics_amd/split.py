@@ -176,7 +176,7 @@ def _split_generation(ctx, rays, max_length, one_list=-1, one_face=-1):
 def _split_chain(ctx, rays, max_length, limit):
     """The FP32-split generations on the device (rpx_trace_rays_split), with
     the pool and resume bookkeeping of tracer._trace_chain."""
-    from .tracer import _pool_rays
+    from .tracer import _pool_rays, _alloc_pool
     lib = load()
     traced = [rays]
     current = rays
@@ -184,7 +184,8 @@ def _split_chain(ctx, rays, max_length, limit):
     hits = (ctypes.c_uint32 * max(ctx.tables.n_global_faces, 1))()
     while remaining > 0 and current.n_rays > 0:
         n = current.n_rays
-        pool = SplitRays(_pool_rays(n, ctx.max_children, remaining))
+        pool = _alloc_pool(SplitRays, _pool_rays(n, ctx.max_children, remaining),
+                           max(ctx.max_children, 1) * n if remaining > 1 else 1)
         ws_bytes = int(lib.rpx_chain_workspace_bytes(max(n, pool.capacity), remaining))
         ws = ctx.chain_workspace(ws_bytes)
         counts = (ctypes.c_uint64 * (remaining + 1))()

@@ -41,14 +41,37 @@ def _pool_rays(n, max_children, remaining):
     GB a deep branching trace keeps are not doubled by rounding.)"""
     torch = require_gpu()
     mc = max(int(max_children), 1)
+    if remaining <= 1:
+        return 1            # the last generation's children are not kept
     want = n * (min(remaining, _CHAIN_GENERATIONS) + 1 + 2 * mc)
     if want <= 1 << 24:
         want = 1 << max(int(want - 1).bit_length(), 0)
     else:
         want = ((want + (1 << 20) - 1) >> 20) << 20
-    free, _ = torch.cuda.mem_get_info()
+    # memory the pool can come from: the device's memory less what tensors
+    # hold now (free or cached by the allocator), so that a repeated trace
+    # asks for the same pools as before and gets them from the cache
+    _, total = torch.cuda.mem_get_info()
+    free = total - torch.cuda.memory_allocated()
     cap = int(0.4 * free) // (_abi.N_F64 * 8 + _abi.N_U32 * 4)
     return max(min(want, cap), mc * n, 1)
+
+
+def _alloc_pool(make, want, minimum):
+    """make(capacity) for the largest capacity from `want` down to `minimum`
+    (halving) that the device can hold: after an out-of-memory error, the
+    allocator's cached blocks are released and a smaller pool is tried (a
+    smaller pool only means an earlier restart of the device-resident loop)."""
+    torch = require_gpu()
+    cap = max(int(want), int(minimum), 1)
+    while True:
+        try:
+            return make(cap)
+        except torch.OutOfMemoryError:
+            torch.cuda.empty_cache()
+            if cap <= minimum:
+                raise
+            cap = max(cap // 2, int(minimum))
 
 
 def _trace_chain(ctx, rays, max_length, limit):
@@ -69,7 +92,8 @@ def _trace_chain(ctx, rays, max_length, limit):
         n = current.n_rays
         dev_in = current._ensure_device()
         wl = _wavelength_table(current, ctx)
-        pool = DeviceRays(_pool_rays(n, ctx.max_children, remaining))
+        pool = _alloc_pool(DeviceRays, _pool_rays(n, ctx.max_children, remaining),
+                           max(ctx.max_children, 1) * n if remaining > 1 else 1)
         ws_bytes = int(lib.rpx_chain_workspace_bytes(max(n, pool.capacity), remaining))
         ws = ctx.chain_workspace(ws_bytes)
         counts = (ctypes.c_uint64 * (remaining + 1))()
