@@ -311,13 +311,25 @@ def field_c4(n_gausslets=262144, grid=512):
     from raypier_optics_amd import workloads, cfields
     from raypier_optics_amd.find_focus import find_ray_focus
     from raypier_optics_amd.tracer import trace_rays
+    from raypier_optics_amd import _abi, _lib
+    lib = _lib.load()
+    nk = len(_abi.TIMING_KERNELS)
     gc, faces, info = workloads.gausslet_c4(n_gausslets=n_gausslets, grid=grid)
     gc._ensure_device()
+    trace_rays(gc, faces, recursion_limit=3, max_length=200.0)      # warm (lengths are reset per call)
     torch.cuda.synchronize()
+    lib.rpx_timing_enable(1)
+    lib.rpx_timing_read(None, None, None)
     t0 = time.perf_counter()
     traced, _ = trace_rays(gc, faces, recursion_limit=3, max_length=200.0)
     torch.cuda.synchronize()
     t_trace = time.perf_counter() - t0
+    ms = (ctypes.c_double * nk)()
+    launches = (ctypes.c_uint64 * nk)()
+    nbytes = (ctypes.c_double * nk)()
+    lib.rpx_timing_read(ms, launches, nbytes)
+    lib.rpx_timing_enable(0)
+    gausslet_roofline = roofline({"ms": list(ms), "launches": list(launches), "bytes": list(nbytes)}, None)
     out = traced[2]
     focus = find_ray_focus(out)
     pts = workloads.field_grid(focus, grid, info["span_mm"])
@@ -325,9 +337,6 @@ def field_c4(n_gausslets=262144, grid=512):
     wl = out.wavelengths
     cfields.sum_gaussian_modes(out, modes, wl, pts[:256], precision=0)      # warm
     cfields.sum_gaussian_modes(out, modes, wl, pts[:256], precision=1)
-    from raypier_optics_amd import _abi, _lib
-    lib = _lib.load()
-    nk = len(_abi.TIMING_KERNELS)
     res, kern = {}, {}
     for prec in (0, 1):
         torch.cuda.synchronize()
@@ -357,6 +366,7 @@ def field_c4(n_gausslets=262144, grid=512):
             "max_abs_dE_over_max_E": float(np.abs(e32 - e64).max() / amp),
             "phase_err_rad_where_E_ge_0p1max": float(np.abs(np.angle(e32[mask] / e64[mask])).max()),
             "focus_mm": [float(x) for x in focus],
+            "gausslet_trace_roofline": gausslet_roofline,
             "roofline": field_roofline("k_field_sum", kern[0], 78.6),
             "roofline_split": field_roofline("k_field_sum_split", kern[1], 157.3)}
 

@@ -777,8 +777,9 @@ rpx_status_t rpx_lens_spot_grad(const rpx_lens_t *lens, const rpx_rays_t *rays, 
  * of: ms = total duration, launches = number of launches, bytes = bytes the
  * kernel's algorithm reads + writes (per the ray record sizes); then resets
  * the totals.  Ray generations run k_clear + k_generation (FP32-split:
- * k_clear + k_generation_split); gausslet generations run k_intersect +
- * k_scan + k_emit.  The two field-sum slots time the pair kernel of
+ * k_clear + k_generation_split); gausslet generations run k_clear +
+ * k_gausslet_generation (slots 0-2 belonged to an earlier three-launch
+ * gausslet generation and stay empty).  The two field-sum slots time the pair kernel of
  * rpx_sum_gaussian_modes (k_field_sum, k_field_sum_split) and count
  * gausslet x point pairs in `bytes`; while timing is on, that call returns
  * after its pair kernel has completed. */
@@ -791,7 +792,8 @@ enum {
     RPX_TIMING_SPLIT = 5,
     RPX_TIMING_FIELD_SUM = 6,
     RPX_TIMING_FIELD_SUM_SPLIT = 7,
-    RPX_TIMING_SLOTS = 8
+    RPX_TIMING_GAUSSLET = 8,
+    RPX_TIMING_SLOTS = 9
 };
 rpx_status_t rpx_timing_enable(int32_t on);
 rpx_status_t rpx_timing_read(double *ms, uint64_t *launches, double *bytes);

@@ -37,7 +37,7 @@ ABI_VERSION = 11          # RPX_ABI_VERSION of include/rpx.h
 
 # timing slots (RPX_TIMING_*)
 TIMING_KERNELS = ("k_intersect", "k_scan", "k_emit", "k_generation", "k_clear", "k_generation_split",
-                  "k_field_sum", "k_field_sum_split")
+                  "k_field_sum", "k_field_sum_split", "k_gausslet_generation")
 
 REFL_RAY = 1 << 0
 GAUSSLET = 1 << 1

@@ -33,6 +33,7 @@ struct GenControl {
     uint32_t wg_done;        // device-resident chain: workgroups that have finished their tile
     uint32_t pad[6];
     uint32_t hit_shards[RPX_SHARDS];                  // single pass: rays that hit, per shard
+    uint32_t parent_shards[RPX_SHARDS];               // single-pass gausslets: rays with children, per shard
     uint32_t face_totals[RPX_LDS_FACES];              // device-resident chain: hits per face record
     uint32_t face_shards[RPX_SHARDS][RPX_LDS_FACES];  // single pass: hits per face record, per shard
 };
@@ -168,7 +169,10 @@ static __global__ __launch_bounds__(256) void k_clear(uint64_t *tile_status, uin
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nb) tile_status[k] = 0;
     if (k < RPX_SHARDS * RPX_LDS_FACES) (&ctl->face_shards[0][0])[k] = 0;
-    if (k < RPX_SHARDS) ctl->hit_shards[k] = 0;
+    if (k < RPX_SHARDS) {
+        ctl->hit_shards[k] = 0;
+        ctl->parent_shards[k] = 0;
+    }
     if (k < RPX_LDS_FACES) ctl->face_totals[k] = 0;
     if (k == 0) {
         ctl->children = 0;

@@ -1,15 +1,13 @@
 // Generation tracer for gfx950: one generation of rays (or gausslets) against
 // a device-resident scene, with stable parent-major compaction of children.
 //
-// Pipeline per generation (three launches, no inter-block waiting):
-//   k_intersect  one ray per lane: nearest-face search over every face list
-//                (faces staged in LDS), in-place parent update (length,
-//                end_face_idx), Face.count histogram, child count via the
-//                material; per-block child totals by wavefront ballots.
-//   k_scan       one workgroup: exclusive scan of the block totals.
-//   k_emit       per-block exclusive scan of child counts (ballot bit-planes +
-//                LDS), then each lane re-evaluates its material and writes its
-//                children at their parent-major position.
+// One launch per generation (after k_clear): each workgroup takes a tile of
+// 256 rays, runs the nearest-face search (scene staged in LDS), updates the
+// parents in place (length, end_face_idx), counts face hits, evaluates the
+// material, and places the children by a decoupled look-back over the tiles
+// in parent-major order (k_generation for rays; k_gausslet_generation for
+// gausslets, which builds children and parabasal rays after the look-back).
+// rpx_trace_rays chains the ray generations on the device (rpx_gen.h).
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <string.h>
@@ -322,155 +320,6 @@ struct CountEmit {
     RPX_DEV void skip() { ++n; }
 };
 
-template <bool GAUSS, uint32_t F>
-__global__ __launch_bounds__(RPX_BLOCK) void k_intersect(TraceArgs A) {
-    __shared__ SceneLds lds_scene;
-    __shared__ uint32_t lds_hist[RPX_LDS_FACES];
-    __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
-    const SceneView S = stage_scene(A.S, lds_scene);
-    for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
-    __syncthreads();
-
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t count = 0, tag = 0;
-    if (i < A.n_in) {
-        const double *f = A.in.f64;
-        const uint64_t c = A.in.capacity;
-        V3 origin = v3(f[RPX_OX * c + i], f[RPX_OY * c + i], f[RPX_OZ * c + i]);
-        V3 dir = v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]);
-        double length = GAUSS ? f[RPX_LENGTH * c + i] : A.max_length;
-        uint32_t end_face = RPX_NO_FACE;
-        int hit_rec = -1, hit_list = -1;
-        V3 point = add(origin, mul(dir, A.max_length));
-        if (A.one_list < 0) {
-            // trace_segment_c / FaceList.intersect_c (ctracer.pyx:2093-2104, 1882-1904)
-            for (int l = 0; l < S.n_face_lists; ++l) {
-                const rpx_face_list_t &fl = S.face_lists[l];
-                V3 p1 = transform(fl.inv_trans, origin);
-                V3 p2 = transform(fl.inv_trans, point);
-                const double slen = sep(p1, p2), zrate = (p2.z - p1.z) / slen;
-                for (int k = fl.first_face; k < fl.first_face + fl.n_faces; ++k) {
-                    const rpx_face_t &fc = S.faces[k];
-                    if (face_out_of_reach(fc, p1.z, zrate * length, slen)) continue;
-                    double dist = face_intersect<F>(S, fc, p1, p2, 1);
-                    if (fc.tolerance < dist && dist < length) {
-                        length = dist;
-                        end_face = (uint32_t)fc.global_idx;
-                        hit_rec = k;
-                        hit_list = l;
-                    }
-                }
-            }
-        } else {
-            // FaceList.intersect_one_face_c (ctracer.pyx:1861-1879)
-            const rpx_face_list_t &fl = S.face_lists[A.one_list];
-            const int k = fl.first_face + A.one_face;
-            const rpx_face_t &fc = S.faces[k];
-            V3 p1 = transform(fl.inv_trans, origin);
-            V3 p2 = transform(fl.inv_trans, point);
-            double dist = face_intersect<F>(S, fc, p1, p2, 1);
-            if (fc.tolerance < dist && dist < length) {
-                length = dist;
-                end_face = (uint32_t)fc.global_idx;
-                hit_rec = k;
-                hit_list = A.one_list;
-            }
-        }
-        A.in.f64[RPX_LENGTH * c + i] = length;
-        A.in.u32[RPX_END_FACE_IDX * c + i] = end_face;
-        if (hit_rec >= 0) {
-            atomicAdd(&lds_hist[hit_rec], 1u);
-            const rpx_face_t &fc = S.faces[hit_rec];
-            const rpx_face_list_t &fl = S.face_lists[hit_list];
-            bool para_ok = true;
-            if (GAUSS) {
-                // trace_parabasal_rays (ctracer.pyx:2363-2373): every parabasal
-                // ray must hit the same face; lengths update in place.
-                const double *P = A.in.para;
-                for (int j = 0; j < RPX_N_PARA && para_ok; ++j) {
-                    const uint64_t b = (uint64_t)j * RPX_PARA_FIELDS;
-                    V3 po = v3(P[(b + 0) * c + i], P[(b + 1) * c + i], P[(b + 2) * c + i]);
-                    V3 pd = v3(P[(b + 3) * c + i], P[(b + 4) * c + i], P[(b + 5) * c + i]);
-                    double plen = P[(b + 9) * c + i];
-                    V3 pend = add(po, mul(pd, A.max_length));
-                    V3 q1 = transform(fl.inv_trans, po);
-                    V3 q2 = transform(fl.inv_trans, pend);
-                    double dist = face_intersect<F>(S, fc, q1, q2, 0);
-                    if (fc.tolerance < dist && dist < plen) {
-                        A.in.para[(b + 9) * c + i] = dist;
-                    } else {
-                        para_ok = false;
-                    }
-                }
-            }
-            if (para_ok && fc.material >= 0) {
-                Ray r = load_ray(A.in, i);
-                r.length = length;
-                r.end_face_idx = end_face;
-                V3 hp = add(r.origin, mul(r.direction, length));
-                Orientation o = compute_orientation<F>(S, fl, fc, hp);
-                CountEmit ce;
-                eval_children<F>(S.materials[fc.material], r, (uint32_t)i, hp, o, A.wavelengths, ce);
-                count = (uint32_t)ce.n;
-            }
-            tag = count | ((uint32_t)(hit_rec + 1) << 8);
-        }
-        A.ws.tags[i] = tag;
-    }
-    uint32_t block_total;
-    block_scan_small(count, lds_wave, &block_total);
-    __syncthreads();
-    uint32_t block_par;
-    block_scan_small(count ? 1u : 0u, lds_wave, &block_par);
-    if (threadIdx.x == 0) {
-        A.ws.block_totals[blockIdx.x] = block_total;
-        A.ws.block_parents[blockIdx.x] = block_par;
-    }
-    __syncthreads();
-    if (A.face_hits) {
-        for (int k = threadIdx.x; k < S.n_list_faces; k += blockDim.x) {
-            uint32_t h = lds_hist[k];
-            if (h) atomicAdd(&A.face_hits[S.faces[k].global_idx], h);
-        }
-    }
-}
-
-// Exclusive scan of block totals in a single workgroup of 1024 lanes.
-__global__ __launch_bounds__(1024) void k_scan(const uint32_t *totals, const uint32_t *parents, uint64_t nb,
-                                               uint64_t *offsets, GenControl *ctl) {
-    __shared__ uint64_t lds[1024];
-    uint64_t carry = 0;
-    uint64_t mine = 0;
-    for (uint64_t k = threadIdx.x; k < nb; k += 1024) mine += parents[k];
-    for (uint64_t base = 0; base < nb; base += 1024) {
-        uint64_t k = base + threadIdx.x;
-        uint64_t v = (k < nb) ? totals[k] : 0;
-        lds[threadIdx.x] = v;
-        __syncthreads();
-        for (int s = 1; s < 1024; s <<= 1) {  // Hillis-Steele inclusive scan
-            uint64_t add = (threadIdx.x >= (unsigned)s) ? lds[threadIdx.x - s] : 0;
-            __syncthreads();
-            lds[threadIdx.x] += add;
-            __syncthreads();
-        }
-        if (k < nb) offsets[k] = carry + lds[threadIdx.x] - v;
-        uint64_t chunk = lds[1023];
-        __syncthreads();
-        carry += chunk;
-    }
-    __syncthreads();
-    lds[threadIdx.x] = mine;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (threadIdx.x < (unsigned)s) lds[threadIdx.x] += lds[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        ctl->children = carry;
-        ctl->parents = lds[0];
-    }
-}
-
 // Writes at most `room` children (the count reserved by the counting pass),
 // so a lane can never write outside its slot range.
 struct StoreEmit {
@@ -512,64 +361,178 @@ struct CollectEmit {
     RPX_DEV void skip() { ++n; }
 };
 
-template <bool GAUSS, uint32_t F>
-__global__ __launch_bounds__(RPX_BLOCK) void k_emit(TraceArgs A) {
+// One gausslet generation in one launch (trace_gausslet_c, ctracer.pyx:
+// 2214-2281, with trace_parabasal_rays, :2350-2385): each lane takes one
+// gausslet; the base ray's nearest-face search and its parabasal rays'
+// intersections with that face (every one must hit it, lengths updated in
+// place) come first, with the child count from the material; the decoupled
+// look-back then places the children, and the lane builds them and their
+// parabasal rays straight into their slots (children's lengths reset to
+// max_length, :2280).  Gausslet children are built after the look-back, not
+// held in registers through it: a child carries six parabasal rays.
+template <uint32_t F>
+__global__ __launch_bounds__(RPX_BLOCK) void k_gausslet_generation(TraceArgs A) {
     __shared__ SceneLds lds_scene;
+    __shared__ uint32_t lds_hist[RPX_LDS_FACES];
     __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
+    __shared__ uint32_t lds_tile, lds_hits, lds_parents;
+    __shared__ uint64_t lds_excl;
+    uint32_t my_tile = 0;
+    if (threadIdx.x == 0) my_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
     const SceneView S = stage_scene(A.S, lds_scene);
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t tag = (i < A.n_in) ? A.ws.tags[i] : 0u;
-    const uint32_t count = tag & 0xffu;
+    for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
+    if (threadIdx.x == 0) {
+        lds_tile = my_tile;
+        lds_hits = 0;
+        lds_parents = 0;
+    }
+    __syncthreads();
+    const uint32_t tile = lds_tile;
+    const uint64_t i = (uint64_t)tile * RPX_BLOCK + threadIdx.x;
+    const uint64_t c = A.in.capacity;
+    uint32_t count = 0;
+    int hit_rec = -1, hit_list = -1;
+    if (i < A.n_in) {
+        const double *f = A.in.f64;
+        V3 origin = v3(f[RPX_OX * c + i], f[RPX_OY * c + i], f[RPX_OZ * c + i]);
+        V3 dir = v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]);
+        double length = f[RPX_LENGTH * c + i];
+        uint32_t end_face = RPX_NO_FACE;
+        V3 point = add(origin, mul(dir, A.max_length));
+        if (A.one_list < 0) {
+            // trace_segment_c / FaceList.intersect_c (ctracer.pyx:2093-2104, 1882-1904)
+            for (int l = 0; l < S.n_face_lists; ++l) {
+                const rpx_face_list_t &fl = S.face_lists[l];
+                V3 p1 = transform(fl.inv_trans, origin);
+                V3 p2 = transform(fl.inv_trans, point);
+                const double slen = sep(p1, p2), zrate = (p2.z - p1.z) / slen;
+                for (int k = fl.first_face; k < fl.first_face + fl.n_faces; ++k) {
+                    const rpx_face_t &fc = S.faces[k];
+                    if (face_out_of_reach(fc, p1.z, zrate * length, slen)) continue;
+                    double dist = face_intersect<F>(S, fc, p1, p2, 1);
+                    if (fc.tolerance < dist && dist < length) {
+                        length = dist;
+                        end_face = (uint32_t)fc.global_idx;
+                        hit_rec = k;
+                        hit_list = l;
+                    }
+                }
+            }
+        } else {
+            // FaceList.intersect_one_face_c (ctracer.pyx:1861-1879)
+            const rpx_face_list_t &fl = S.face_lists[A.one_list];
+            const int k = fl.first_face + A.one_face;
+            const rpx_face_t &fc = S.faces[k];
+            V3 p1 = transform(fl.inv_trans, origin);
+            V3 p2 = transform(fl.inv_trans, point);
+            double dist = face_intersect<F>(S, fc, p1, p2, 1);
+            if (fc.tolerance < dist && dist < length) {
+                length = dist;
+                end_face = (uint32_t)fc.global_idx;
+                hit_rec = k;
+                hit_list = A.one_list;
+            }
+        }
+        A.in.f64[RPX_LENGTH * c + i] = length;
+        A.in.u32[RPX_END_FACE_IDX * c + i] = end_face;
+        if (hit_rec >= 0) {
+            atomicAdd(&lds_hist[hit_rec], 1u);
+            atomicAdd(&lds_hits, 1u);
+            const rpx_face_t &fc = S.faces[hit_rec];
+            const rpx_face_list_t &fl = S.face_lists[hit_list];
+            // trace_parabasal_rays (ctracer.pyx:2363-2373): every parabasal
+            // ray must hit the same face; lengths update in place
+            bool para_ok = true;
+            const double *P = A.in.para;
+            for (int j = 0; j < RPX_N_PARA && para_ok; ++j) {
+                const uint64_t b = (uint64_t)j * RPX_PARA_FIELDS;
+                V3 po = v3(P[(b + 0) * c + i], P[(b + 1) * c + i], P[(b + 2) * c + i]);
+                V3 pd = v3(P[(b + 3) * c + i], P[(b + 4) * c + i], P[(b + 5) * c + i]);
+                double plen = P[(b + 9) * c + i];
+                V3 pend = add(po, mul(pd, A.max_length));
+                V3 q1 = transform(fl.inv_trans, po);
+                V3 q2 = transform(fl.inv_trans, pend);
+                double dist = face_intersect<F>(S, fc, q1, q2, 0);
+                if (fc.tolerance < dist && dist < plen) {
+                    A.in.para[(b + 9) * c + i] = dist;
+                } else {
+                    para_ok = false;
+                }
+            }
+            if (para_ok && fc.material >= 0) {
+                Ray r = load_ray(A.in, i);
+                r.length = length;
+                r.end_face_idx = end_face;
+                V3 hp = add(r.origin, mul(r.direction, length));
+                Orientation o = compute_orientation<F>(S, fl, fc, hp);
+                CountEmit ce;
+                eval_children<F>(S.materials[fc.material], r, (uint32_t)i, hp, o, A.wavelengths, ce);
+                count = (uint32_t)min(ce.n, 3);
+            }
+        }
+    }
     uint32_t block_total;
     const uint32_t local = block_scan_small(count, lds_wave, &block_total);
-    if (count == 0) return;
-    const uint64_t pos = A.ws.block_offsets[blockIdx.x] + local;
-    const int rec = (int)(tag >> 8) - 1;
-    const rpx_face_t &fc = S.faces[rec];
-    const rpx_face_list_t &fl = S.face_lists[fc.face_list];
-    const rpx_material_t &m = S.materials[fc.material];
-    Ray r = load_ray(A.in, i);
-    V3 hp = add(r.origin, mul(r.direction, r.length));
-    Orientation o = compute_orientation<F>(S, fl, fc, hp);
-    if (!GAUSS) {
-        StoreEmit se{&A.out, pos, count};
-        eval_children<F>(m, r, (uint32_t)i, hp, o, A.wavelengths, se);
-        return;
+    {
+        const uint64_t mp = __ballot(count > 0);
+        if ((threadIdx.x & 63) == 0 && mp) atomicAdd(&lds_parents, (uint32_t)__popcll(mp));
     }
-    // Gausslets: children of the base ray, then 6 parabasal rays each
-    // (ctracer.pyx:2375-2385), lengths reset to max_length (:2280).
-    CollectEmit ce;
-    eval_children<F>(m, r, (uint32_t)i, hp, o, A.wavelengths, ce);
-    const uint64_t c = A.in.capacity, co = A.out.capacity;
-    const int nk = min(min(ce.n, 3), (int)count);
-    for (int k = 0; k < nk; ++k) {
-        Ray child = ce.r[k];
-        child.length = A.max_length;
-        store_ray(A.out, pos + k, child);
+    if (threadIdx.x < 64) {
+        const uint64_t excl = lookback_tagged(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error, A.status_tag);
+        if (threadIdx.x == 0) lds_excl = excl;
     }
-    for (int j = 0; j < RPX_N_PARA; ++j) {
-        const uint64_t b = (uint64_t)j * RPX_PARA_FIELDS;
-        const double *P = A.in.para;
-        V3 po = v3(P[(b + 0) * c + i], P[(b + 1) * c + i], P[(b + 2) * c + i]);
-        V3 pd = v3(P[(b + 3) * c + i], P[(b + 4) * c + i], P[(b + 5) * c + i]);
-        double plen = P[(b + 9) * c + i];
-        V3 pp = add(po, mul(pd, plen));
-        Orientation po_ = compute_orientation<F>(S, fl, fc, pp);
+    __syncthreads();
+    const uint64_t pos = lds_excl + local;
+    if (count > 0 && pos + count <= A.out.capacity) {
+        const rpx_face_t &fc = S.faces[hit_rec];
+        const rpx_face_list_t &fl = S.face_lists[hit_list];
+        const rpx_material_t &m = S.materials[fc.material];
+        Ray r = load_ray(A.in, i);
+        V3 hp = add(r.origin, mul(r.direction, r.length));
+        Orientation o = compute_orientation<F>(S, fl, fc, hp);
+        CollectEmit ce;
+        eval_children<F>(m, r, (uint32_t)i, hp, o, A.wavelengths, ce);
+        const uint64_t co = A.out.capacity;
+        const int nk = min(min(ce.n, 3), (int)count);
         for (int k = 0; k < nk; ++k) {
-            Para q = eval_parabasal<F>(m, ce.r[k], pd, pp, po_, ce.r[k].ray_type_id, A.wavelengths);
-            double *Q = A.out.para;
-            const uint64_t oi = pos + k;
-            Q[(b + 0) * co + oi] = q.origin.x;
-            Q[(b + 1) * co + oi] = q.origin.y;
-            Q[(b + 2) * co + oi] = q.origin.z;
-            Q[(b + 3) * co + oi] = q.direction.x;
-            Q[(b + 4) * co + oi] = q.direction.y;
-            Q[(b + 5) * co + oi] = q.direction.z;
-            Q[(b + 6) * co + oi] = q.normal.x;
-            Q[(b + 7) * co + oi] = q.normal.y;
-            Q[(b + 8) * co + oi] = q.normal.z;
-            Q[(b + 9) * co + oi] = A.max_length;
+            Ray child = ce.r[k];
+            child.length = A.max_length;
+            store_ray(A.out, pos + k, child);
         }
+        for (int j = 0; j < RPX_N_PARA; ++j) {
+            const uint64_t b = (uint64_t)j * RPX_PARA_FIELDS;
+            const double *P = A.in.para;
+            V3 po = v3(P[(b + 0) * c + i], P[(b + 1) * c + i], P[(b + 2) * c + i]);
+            V3 pd = v3(P[(b + 3) * c + i], P[(b + 4) * c + i], P[(b + 5) * c + i]);
+            double plen = P[(b + 9) * c + i];
+            V3 pp = add(po, mul(pd, plen));
+            Orientation po_ = compute_orientation<F>(S, fl, fc, pp);
+            for (int k = 0; k < nk; ++k) {
+                Para q = eval_parabasal<F>(m, ce.r[k], pd, pp, po_, ce.r[k].ray_type_id, A.wavelengths);
+                double *Q = A.out.para;
+                const uint64_t oi = pos + k;
+                Q[(b + 0) * co + oi] = q.origin.x;
+                Q[(b + 1) * co + oi] = q.origin.y;
+                Q[(b + 2) * co + oi] = q.origin.z;
+                Q[(b + 3) * co + oi] = q.direction.x;
+                Q[(b + 4) * co + oi] = q.direction.y;
+                Q[(b + 5) * co + oi] = q.direction.z;
+                Q[(b + 6) * co + oi] = q.normal.x;
+                Q[(b + 7) * co + oi] = q.normal.y;
+                Q[(b + 8) * co + oi] = q.normal.z;
+                Q[(b + 9) * co + oi] = A.max_length;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        add_hits(A.ws.ctl, tile, lds_hits);
+        if (lds_parents) atomicAdd(&A.ws.ctl->parent_shards[tile % RPX_SHARDS], lds_parents);
+        if ((uint64_t)tile == (A.n_in - 1) / RPX_BLOCK) A.ws.ctl->children = lds_excl + block_total;
+    }
+    uint32_t *shard = A.ws.ctl->face_shards[tile % RPX_SHARDS];
+    for (int k = threadIdx.x; k < S.n_list_faces; k += blockDim.x) {
+        uint32_t h = lds_hist[k];
+        if (h) atomicAdd(&shard[k], h);
     }
 }
 
@@ -724,38 +687,19 @@ void k_generation(TraceArgs A) {
 }
 
 // ------------------------------------------------------------- variants
-template <uint32_t F>
-static void launch_intersect_f(bool gauss, dim3 grid, hipStream_t stream, const TraceArgs &A) {
-    if (gauss)
-        hipLaunchKernelGGL((k_intersect<true, F>), grid, dim3(RPX_BLOCK), 0, stream, A);
-    else
-        hipLaunchKernelGGL((k_intersect<false, F>), grid, dim3(RPX_BLOCK), 0, stream, A);
-}
-
-template <uint32_t F>
-static void launch_emit_f(bool gauss, dim3 grid, hipStream_t stream, const TraceArgs &A) {
-    if (gauss)
-        hipLaunchKernelGGL((k_emit<true, F>), grid, dim3(RPX_BLOCK), 0, stream, A);
-    else
-        hipLaunchKernelGGL((k_emit<false, F>), grid, dim3(RPX_BLOCK), 0, stream, A);
-}
-
-static void launch_intersect(bool gauss, uint32_t variant, dim3 grid, hipStream_t stream, const TraceArgs &A) {
-    if (variant == V_SPHERES) launch_intersect_f<V_SPHERES>(gauss, grid, stream, A);
-    else if (variant == V_LENS) launch_intersect_f<V_LENS>(gauss, grid, stream, A);
-    else launch_intersect_f<V_ALL>(gauss, grid, stream, A);
-}
-
 static void launch_generation(uint32_t variant, dim3 grid, hipStream_t stream, const TraceArgs &A) {
     if (variant == V_SPHERES) hipLaunchKernelGGL((k_generation<V_SPHERES>), grid, dim3(RPX_BLOCK), 0, stream, A);
     else if (variant == V_LENS) hipLaunchKernelGGL((k_generation<V_LENS>), grid, dim3(RPX_BLOCK), 0, stream, A);
     else hipLaunchKernelGGL((k_generation<V_ALL>), grid, dim3(RPX_BLOCK), 0, stream, A);
 }
 
-static void launch_emit(bool gauss, uint32_t variant, dim3 grid, hipStream_t stream, const TraceArgs &A) {
-    if (variant == V_SPHERES) launch_emit_f<V_SPHERES>(gauss, grid, stream, A);
-    else if (variant == V_LENS) launch_emit_f<V_LENS>(gauss, grid, stream, A);
-    else launch_emit_f<V_ALL>(gauss, grid, stream, A);
+static void launch_gausslet_generation(uint32_t variant, dim3 grid, hipStream_t stream, const TraceArgs &A) {
+    if (variant == V_SPHERES)
+        hipLaunchKernelGGL((k_gausslet_generation<V_SPHERES>), grid, dim3(RPX_BLOCK), 0, stream, A);
+    else if (variant == V_LENS)
+        hipLaunchKernelGGL((k_gausslet_generation<V_LENS>), grid, dim3(RPX_BLOCK), 0, stream, A);
+    else
+        hipLaunchKernelGGL((k_gausslet_generation<V_ALL>), grid, dim3(RPX_BLOCK), 0, stream, A);
 }
 
 // ------------------------------------------------------------- timing
@@ -903,57 +847,48 @@ static rpx_status_t run_generation(const rpx_scene_t *scene, int one_list, int o
                                                  std::to_string(ctl.children) + " children");
         return RPX_OK;
     }
+    // gausslets: one single-pass launch as well
+    if (out->capacity > 0 && (!out->f64 || !out->u32 || !out->para))
+        return set_error(RPX_E_INVALID, "output gausslets have null fields");
     if (timing) HIP_TRY(hipEventRecord(ev.e[0], stream));
-    launch_intersect(gauss, variant, dim3((unsigned)nb), stream, A);
+    hipLaunchKernelGGL(k_clear, dim3((unsigned)((std::max<uint64_t>(nb, RPX_SHARDS * RPX_LDS_FACES) + 255) / 256)),
+                       dim3(256), 0, stream, A.ws.tile_status, nb, A.ws.ctl);
     HIP_TRY(hipGetLastError());
     if (timing) HIP_TRY(hipEventRecord(ev.e[1], stream));
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, A.ws.block_totals, A.ws.block_parents, nb,
-                       A.ws.block_offsets, A.ws.ctl);
+    launch_gausslet_generation(variant, dim3((unsigned)nb), stream, A);
     HIP_TRY(hipGetLastError());
     if (timing) HIP_TRY(hipEventRecord(ev.e[2], stream));
-    uint64_t totals[2] = {0, 0};
-    HIP_TRY(hipMemcpyAsync(totals, A.ws.ctl, sizeof(totals), hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
-    const uint64_t total = totals[0], parents_with_children = totals[1];
-    *n_out = total;
-    float t_int = 0, t_scan = 0;
-    if (timing) {
-        HIP_TRY(hipEventElapsedTime(&t_int, ev.e[0], ev.e[1]));
-        HIP_TRY(hipEventElapsedTime(&t_scan, ev.e[1], ev.e[2]));
-        // algorithmic bytes: parent geometry read, length/end_face written
-        const double geom = gauss ? (48.0 + 6 * 48.0) : 48.0;
-        std::lock_guard<std::mutex> g(g_timing_mutex);
-        g_timing_ms[RPX_TIMING_INTERSECT] += t_int;
-        g_timing_ms[RPX_TIMING_SCAN] += t_scan;
-        g_timing_launches[RPX_TIMING_INTERSECT] += 1;
-        g_timing_launches[RPX_TIMING_SCAN] += 1;
-        g_timing_bytes[RPX_TIMING_INTERSECT] += (double)n_in * (geom + 12.0);
-        g_timing_bytes[RPX_TIMING_SCAN] += (double)nb * 16.0;
+    GenControl ctl;
+    HIP_TRY(finish_generation(ctl, A.ws.ctl, scene->view.faces, scene->view.n_list_faces, face_hits, stream));
+    if (ctl.error) return set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
+    uint64_t hits = 0, parents = 0;
+    for (int s2 = 0; s2 < RPX_SHARDS; ++s2) {
+        hits += ctl.hit_shards[s2];
+        parents += ctl.parent_shards[s2];
     }
-    if (total == 0) return RPX_OK;
-    if (total > out->capacity)
+    *n_out = ctl.children;
+    if (timing) {
+        float t_clear = 0, t_gen = 0;
+        HIP_TRY(hipEventElapsedTime(&t_clear, ev.e[0], ev.e[1]));
+        HIP_TRY(hipEventElapsedTime(&t_gen, ev.e[1], ev.e[2]));
+        std::lock_guard<std::mutex> g(g_timing_mutex);
+        g_timing_ms[RPX_TIMING_CLEAR] += t_clear;
+        g_timing_launches[RPX_TIMING_CLEAR] += 1;
+        g_timing_bytes[RPX_TIMING_CLEAR] += (double)nb * 8.0;
+        // algorithmic bytes: base and parabasal geometry of every gausslet
+        // plus its length and end face, the rest of the base ray (100) and
+        // the parabasal normals and lengths (6 x 32) of every gausslet with
+        // children, and every child record (188 + 6 x 80)
+        g_timing_ms[RPX_TIMING_GAUSSLET] += t_gen;
+        g_timing_launches[RPX_TIMING_GAUSSLET] += 1;
+        g_timing_bytes[RPX_TIMING_GAUSSLET] += (double)n_in * (48.0 + 6 * 48.0 + 12.0) +
+                                               (double)parents * (100.0 + 6 * 32.0) +
+                                               (double)ctl.children * (188.0 + 6 * 80.0);
+        (void)hits;
+    }
+    if (ctl.children > out->capacity)
         return set_error(RPX_E_CAPACITY, "output capacity " + std::to_string(out->capacity) + " below " +
-                                             std::to_string(total) + " children");
-    st = check_rays(out, total, gauss, "output rays");
-    if (st) return st;
-    if (timing) HIP_TRY(hipEventRecord(ev.e[3], stream));
-    launch_emit(gauss, variant, dim3((unsigned)nb), stream, A);
-    HIP_TRY(hipGetLastError());
-    if (timing) HIP_TRY(hipEventRecord(ev.e[4], stream));
-    HIP_TRY(hipStreamSynchronize(stream));
-    if (timing) {
-        float t_emit = 0;
-        HIP_TRY(hipEventElapsedTime(&t_emit, ev.e[3], ev.e[4]));
-        // algorithmic bytes: the 100 non-geometry bytes of each parent with
-        // children (its geometry was counted in the intersect pass) plus every
-        // child record written
-        const double rest = gauss ? 100.0 + 6 * 80.0 : 100.0;
-        const double rec = gauss ? 188.0 + 6 * 80.0 : 188.0;
-        std::lock_guard<std::mutex> g(g_timing_mutex);
-        g_timing_ms[RPX_TIMING_EMIT] += t_emit;
-        g_timing_launches[RPX_TIMING_EMIT] += 1;
-        g_timing_bytes[RPX_TIMING_EMIT] += (double)total * rec + (double)parents_with_children * rest;
-    }
+                                             std::to_string(ctl.children) + " children");
     return RPX_OK;
 }
 
