@@ -687,7 +687,6 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
             }
             s->used += j->next.bytes();
             s->peak_used = std::max(s->peak_used, s->used);
-        s->peak_used = std::max(s->peak_used, s->used);
             still.push_back(j);
         }
         s->active.swap(still);
