@@ -28,6 +28,18 @@ def _prepare(input_rays, face_lists, max_length, reset=True):
 _CHAIN_GENERATIONS = 16
 
 
+_DEVICE_BYTES = {}
+
+
+def _device_bytes(torch):
+    """Total memory of the current device (looked up once: the query is a
+    driver call of tens of microseconds)."""
+    dev = torch.cuda.current_device()
+    if dev not in _DEVICE_BYTES:
+        _DEVICE_BYTES[dev] = int(torch.cuda.get_device_properties(dev).total_memory)
+    return _DEVICE_BYTES[dev]
+
+
 def _pool_rays(n, max_children, remaining):
     """Pool size for a device-resident chain from a generation of n rays:
     room for the generations still allowed (at most _CHAIN_GENERATIONS) at
@@ -51,8 +63,7 @@ def _pool_rays(n, max_children, remaining):
     # memory the pool can come from: the device's memory less what tensors
     # hold now (free or cached by the allocator), so that a repeated trace
     # asks for the same pools as before and gets them from the cache
-    _, total = torch.cuda.mem_get_info()
-    free = total - torch.cuda.memory_allocated()
+    free = _device_bytes(torch) - torch.cuda.memory_allocated()
     cap = int(0.4 * free) // (_abi.N_F64 * 8 + _abi.N_U32 * 4)
     return max(min(want, cap), mc * n, 1)
 
