@@ -798,6 +798,20 @@ enum {
 rpx_status_t rpx_timing_enable(int32_t on);
 rpx_status_t rpx_timing_read(double *ms, uint64_t *launches, double *bytes);
 
+/* Tile order of the device-resident generation chains (rpx_trace_rays,
+ * rpx_trace_rays_split; no counterpart in the reference).  A chain first
+ * runs with each workgroup's tile given by its launch order, every look-back
+ * wait bounded in time; if a wait runs out (the hardware started a later
+ * workgroup before an earlier one could run), the chain is traced again from
+ * its input with tiles handed out by an atomic counter, which needs no
+ * dispatch order.  Results are the same either way.
+ * rpx_tile_order_fallbacks: chains traced again so far in this process.
+ * rpx_debug_lookback_ticks: test hook -- the wait bound in ticks of the
+ * device's real-time counter (0 restores the default of 50 ms; 1 makes the
+ * first wait of every launch-order tile run out, forcing the fallback). */
+uint64_t rpx_tile_order_fallbacks(void);
+rpx_status_t rpx_debug_lookback_ticks(uint64_t ticks);
+
 /* Per-object queries, batched over n device-resident inputs.
  *   rpx_query_face: what = 0 -> out[i] = Face.intersect_c(p1[i], p2[i],
  *   is_base_ray).dist (cfaces.pyx, -1 for no intersection); what = 1 ->

@@ -47,6 +47,8 @@ _SIGNATURES = {
     "rpx_trace_rays_split": (c_i32, [c_void_p, P(_abi.RaysSplit), c_u64, c_f32, c_void_p, c_u64, c_i32,
                                      P(_abi.RaysSplit), c_void_p, c_void_p, P(c_i32), c_void_p, c_void_p, c_u64,
                                      c_void_p]),
+    "rpx_tile_order_fallbacks": (c_u64, []),
+    "rpx_debug_lookback_ticks": (c_i32, [c_u64]),
     "rpx_server_create": (c_i32, [c_u64, c_u32, c_u64, c_void_p, P(c_void_p)]),
     "rpx_server_destroy": (c_i32, [c_void_p]),
     "rpx_server_submit": (c_i32, [c_void_p, P(_abi.JobDesc), P(c_u64)]),

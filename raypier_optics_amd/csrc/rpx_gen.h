@@ -219,10 +219,15 @@ static __device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
 // predecessors' aggregates 64 tiles at a time until an inclusive prefix is
 // found, publishes this tile's inclusive prefix and returns the exclusive one.
 // The scan starts at tile `first` (tiles before it belong to another scan)
-// and reads only words carrying its `tag`; a wait that runs out of polls adds
-// 1 to *error.
+// and reads only words carrying its `tag`; a wait that runs out of polls, or
+// (max_ticks > 0) lasts longer than max_ticks ticks of the 100 MHz-class
+// real-time counter, adds 1 to *error and ends the wait with the part of the
+// prefix seen so far (never more than the true prefix, so every position
+// derived from it stays within the true bounds).  max_ticks ==
+// RPX_LOOKBACK_EXPIRED ends the first wait at once (a test hook).
+#define RPX_LOOKBACK_EXPIRED 1ull
 static __device__ uint64_t lookback_tagged(uint64_t *status, uint64_t tile, uint64_t first, uint64_t agg,
-                                           uint64_t *error, uint64_t tag) {
+                                           uint64_t *error, uint64_t tag, uint64_t max_ticks = 0) {
     const int lane = threadIdx.x & 63;
     const uint64_t tagbits = (tag & ST_TAG_MASK) << ST_TAG_SHIFT;
     if (tile == first) {
@@ -233,6 +238,7 @@ static __device__ uint64_t lookback_tagged(uint64_t *status, uint64_t tile, uint
     uint64_t excl = 0;
     int64_t end = (int64_t)tile - 1;
     uint32_t polls = 0;
+    const uint64_t t0 = max_ticks ? __builtin_amdgcn_s_memrealtime() : 0;
     for (;;) {
         const int64_t idx = end - lane;
         uint64_t st = (idx >= (int64_t)first) ? status_load(&status[idx]) : (ST_PREFIX | tagbits);
@@ -242,7 +248,11 @@ static __device__ uint64_t lookback_tagged(uint64_t *status, uint64_t tile, uint
         const int first_prefix = pm ? (__ffsll((unsigned long long)pm) - 1) : 64;
         const uint64_t window = (first_prefix >= 63) ? ~0ull : ((2ull << first_prefix) - 1);
         if (__ballot(flag == 0) & window) {
-            if (++polls > RPX_LOOKBACK_POLLS) {
+            bool expired = ++polls > RPX_LOOKBACK_POLLS;
+            if (max_ticks)
+                expired = expired || max_ticks == RPX_LOOKBACK_EXPIRED ||
+                          __builtin_amdgcn_s_memrealtime() - t0 > max_ticks;
+            if (expired) {
                 if (lane == 0) atomicAdd((unsigned long long *)error, 1ull);
                 break;
             }
@@ -423,7 +433,10 @@ static inline GenControl *pinned_heads(int n, GenControl **dev) {
 }
 
 // What a generation launch of a chain needs besides the kernel's own
-// arguments.
+// arguments.  Tile order: with by_launch, a workgroup's tile is tile_base +
+// blockIdx.x (a generation's second launch starts at the first launch's
+// grid) and its look-back waits are bounded by lookback_ticks; otherwise
+// tiles are handed out by the generation's atomic counter.
 struct ChainGen {
     GenLink *link;
     GenControl *ctl;
@@ -431,7 +444,27 @@ struct ChainGen {
     uint64_t tag;
     GenControl *head_out;
     int32_t in_pool, store_children;
+    uint32_t tile_base;
+    int32_t by_launch;
+    uint64_t lookback_ticks;
 };
+
+// Ticks of the device's real-time counter in `ms` milliseconds (the counter
+// s_memrealtime reads; its rate is the device's wall-clock rate).
+static inline uint64_t realtime_ticks(double ms) {
+    thread_local int dev_cached = -1;
+    thread_local uint64_t khz = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (dev != dev_cached) {
+        int rate = 0;
+        if (hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, dev) != hipSuccess || rate <= 0)
+            rate = 100000;   // 100 MHz
+        khz = (uint64_t)rate;
+        dev_cached = dev;
+    }
+    return std::max<uint64_t>((uint64_t)(ms * (double)khz), 2);
+}
 
 struct ChainSpec {
     uint64_t n_in, pool_rays, max_children;
@@ -451,14 +484,54 @@ static inline rpx_status_t check_chain(const ChainSpec &c, const rpx_scene_t *sc
     return RPX_OK;
 }
 
+// Status of one chain attempt whose launch-order look-back outlasted its
+// bound (never returned to callers).
+static constexpr rpx_status_t RPX_CHAIN_RETRY = -1;
+
+// Default bound on one look-back wait with launch-order tiles, in ms.  A
+// tile waits only for tiles of workgroups dispatched before it, whose work
+// takes microseconds to a few milliseconds; the bound is met only if the
+// hardware did not dispatch in workgroup order.
+#define RPX_LOOKBACK_BOUND_MS 50.0
+
+template <class Launch>
+static rpx_status_t run_chain_once(const ChainSpec &c, const rpx_scene_t *scene, void *workspace, Launch &launch,
+                                   uint64_t *counts, uint64_t *offsets, int32_t *n_traced, uint32_t *face_hits,
+                                   hipStream_t stream, bool by_launch);
+
 // Runs the chain; `launch(gen, grid, stream)` launches one generation's
 // kernel (grid workgroups) with the chain fields of `gen` and returns the
 // launch error.  counts / offsets / n_traced / face_hits (host) as
 // rpx_trace_rays documents them.
+//
+// Tiles are taken in launch order (tile = workgroup id): taking them from an
+// atomic counter costs one same-address device atomic per tile, and those
+// serialise at about 12 ns each, a floor of 0.19 ms per 16,384-tile
+// generation.  Launch order needs the hardware to start workgroups in id
+// order for the look-back's waits to end; each wait is therefore bounded,
+// and a chain in which one ran out is traced again from its input with
+// ticket order, which guarantees progress whatever the dispatch order.  (A
+// run-out wait leaves positions below the true ones, inside every buffer,
+// and the rerun rewrites every generation and the input's lengths and end
+// faces.)
 template <class Launch>
 static rpx_status_t run_chain(const ChainSpec &c, const rpx_scene_t *scene, void *workspace, Launch &&launch,
                               uint64_t *counts, uint64_t *offsets, int32_t *n_traced, uint32_t *face_hits,
                               hipStream_t stream) {
+    // an attempt adds its hit totals only once it has ended without a run-out wait
+    rpx_status_t st = run_chain_once(c, scene, workspace, launch, counts, offsets, n_traced, face_hits, stream, true);
+    if (st != RPX_CHAIN_RETRY) return st;
+    rpx_note_tile_order_fallback();
+    return run_chain_once(c, scene, workspace, launch, counts, offsets, n_traced, face_hits, stream, false);
+}
+
+template <class Launch>
+static rpx_status_t run_chain_once(const ChainSpec &c, const rpx_scene_t *scene, void *workspace, Launch &launch,
+                                   uint64_t *counts, uint64_t *offsets, int32_t *n_traced, uint32_t *face_hits,
+                                   hipStream_t stream, bool by_launch) {
+    const uint64_t ticks = rpx_lookback_ticks_override() ? rpx_lookback_ticks_override()
+                                                         : realtime_ticks(RPX_LOOKBACK_BOUND_MS);
+    *n_traced = 0;
     const int32_t L = c.max_generations;
     const uint64_t mc = c.max_children, max_gen = std::max<uint64_t>(c.n_in, c.pool_rays);
     GenControl *heads_dev = nullptr;
@@ -499,12 +572,19 @@ static rpx_status_t run_chain(const ChainSpec &c, const rpx_scene_t *scene, void
         cg.head_out = heads_dev + g;
         cg.in_pool = g > 0;
         cg.store_children = g + 1 < L ? 1 : 0;
+        cg.tile_base = 0;
+        cg.by_launch = by_launch ? 1 : 0;
+        cg.lookback_ticks = by_launch ? ticks : 0;
         return cg;
     };
     auto close_gen = [&](int g) -> rpx_status_t {
         if (closed[g]) return RPX_OK;
         const uint64_t need = tiles_of(counts[g]);
-        if (need > grid[g]) RPX_HIP_TRY(launch(gen(g), need - grid[g], stream));
+        if (need > grid[g]) {
+            ChainGen cg = gen(g);
+            cg.tile_base = (uint32_t)grid[g];
+            RPX_HIP_TRY(launch(cg, need - grid[g], stream));
+        }
         RPX_HIP_TRY(hipEventRecord(evs[2 * g + 1], stream));
         closed[g] = 1;
         return RPX_OK;
@@ -586,8 +666,10 @@ static rpx_status_t run_chain(const ChainSpec &c, const rpx_scene_t *scene, void
             if (rpx_status_t st = read_heads_through(launched - 1)) return st;
         RPX_HIP_TRY(hipEventSynchronize(evs[2 * L]));
         for (int g = 0; g < launched; ++g)
-            if (heads[g].error)
+            if (heads[g].error) {
+                if (by_launch) return RPX_CHAIN_RETRY;
                 return rpx_set_error(RPX_E_HIP, "generation scan: a look-back wait exceeded its poll limit");
+            }
     }
     int traced = 0;
     for (int g = 0; g < launched; ++g) {

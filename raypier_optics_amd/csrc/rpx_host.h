@@ -27,6 +27,13 @@ void rpx_timing_add(int slot, float ms, double bytes);
 // Records the message for rpx_last_error() and returns `code`.
 rpx_status_t rpx_set_error(rpx_status_t code, const std::string &msg);
 
+// Launch-order tiles of the device-resident generation chains: the bound on
+// one look-back wait in real-time-counter ticks (0: the default bound; set by
+// rpx_debug_lookback_ticks), and the count of chains that ran again with
+// ticket order after a wait outlasted it.
+uint64_t rpx_lookback_ticks_override();
+void rpx_note_tile_order_fallback();
+
 // RPX_OK if the scene can be traced (every face has a global index >= 0,
 // which indexes the per-face hit counters), else RPX_E_INVALID.
 rpx_status_t rpx_check_trace_scene(const rpx_scene_t *scene);

@@ -30,6 +30,10 @@ struct SplitArgs {
     int32_t in_pool, store_children;
     uint64_t status_tag;
     GenControl *head_out;
+    // tile order, as TraceArgs
+    uint32_t tile_base = 0;
+    int32_t by_launch = 0;
+    uint64_t lookback_ticks = 0;
 };
 
 // The rays from element `off` of an FP32-split SoA buffer on (same stride).
@@ -84,15 +88,33 @@ void k_generation_split(SplitArgs A) {
     if (A.link) {
         const GenLink L = A.link[0];
         n_in = L.count;
-        if ((uint64_t)blockIdx.x * RPX_BLOCK >= n_in) return;
         if (A.in_pool) in = advance_split(A.in, L.offset);
         out = advance_split(A.out, L.out_offset);
         out_limit = A.store_children ? (A.pool_rays > L.out_offset ? A.pool_rays - L.out_offset : 0) : 0;
     }
-    // the tile number is fetched first, so that its round trip overlaps the
-    // scene staging
+    const uint64_t n_tiles = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
     uint32_t my_tile = 0;
-    if (threadIdx.x == 0) my_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
+    if (A.by_launch) {
+        my_tile = A.tile_base + blockIdx.x;
+        if (my_tile >= n_tiles) return;
+    } else {
+        if (blockIdx.x >= n_tiles) return;
+        // the tile number is fetched first, so that its round trip overlaps
+        // the scene staging
+        if (threadIdx.x == 0) my_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
+    }
+    // with launch-order tiles the ray's geometry is loaded before the scene
+    // is staged, so that the two latencies overlap
+    const float *f = in.f32;
+    const uint64_t c = in.capacity;
+    F3 o32 = f3(0, 0, 0), d32 = f3(0, 0, 0);
+    if (A.by_launch) {
+        const uint64_t i0 = (uint64_t)my_tile * RPX_BLOCK + threadIdx.x;
+        if (i0 < n_in) {
+            o32 = f3(f[RPX_S_OX * c + i0], f[RPX_S_OY * c + i0], f[RPX_S_OZ * c + i0]);
+            d32 = f3(f[RPX_S_DX * c + i0], f[RPX_S_DY * c + i0], f[RPX_S_DZ * c + i0]);
+        }
+    }
     const SceneView S = stage_scene(A.S, lds_scene);
     for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
     if (threadIdx.x == 0) {
@@ -107,10 +129,12 @@ void k_generation_split(SplitArgs A) {
     int hit_rec = -1, hit_list = -1;
     double hit_length = 0;
     if (i < n_in) {
-        const float *f = in.f32;
-        const uint64_t c = in.capacity;
-        const V3 origin = v3(f[RPX_S_OX * c + i], f[RPX_S_OY * c + i], f[RPX_S_OZ * c + i]);
-        const V3 dir = v3(f[RPX_S_DX * c + i], f[RPX_S_DY * c + i], f[RPX_S_DZ * c + i]);
+        if (!A.by_launch) {
+            o32 = f3(f[RPX_S_OX * c + i], f[RPX_S_OY * c + i], f[RPX_S_OZ * c + i]);
+            d32 = f3(f[RPX_S_DX * c + i], f[RPX_S_DY * c + i], f[RPX_S_DZ * c + i]);
+        }
+        const V3 origin = v3_of(o32);
+        const V3 dir = v3_of(d32);
         double length = A.max_length;
         uint32_t end_face = RPX_NO_FACE;
         const V3 point = add(origin, mul(dir, A.max_length));
@@ -170,7 +194,8 @@ void k_generation_split(SplitArgs A) {
         if ((threadIdx.x & 63) == 0) atomicAdd(&lds_hits, (uint32_t)__popcll(mh));
     }
     if (threadIdx.x < 64) {
-        const uint64_t excl = lookback_tagged(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error, A.status_tag);
+        const uint64_t excl = lookback_tagged(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error, A.status_tag,
+                                              A.lookback_ticks);
         if (threadIdx.x == 0) lds_excl = excl;
     }
     __syncthreads();
@@ -421,6 +446,9 @@ extern "C" rpx_status_t rpx_trace_rays_split(const rpx_scene_t *scene, const rpx
         A.in_pool = cg.in_pool;
         A.in = cg.in_pool ? *pool : *in;
         A.store_children = cg.store_children;
+        A.tile_base = cg.tile_base;
+        A.by_launch = cg.by_launch;
+        A.lookback_ticks = cg.lookback_ticks;
         launch_split(variant, dim3((unsigned)grid), stream, A);
         return hipGetLastError();
     };

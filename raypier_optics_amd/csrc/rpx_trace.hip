@@ -9,6 +9,7 @@
 // gausslets, which builds children and parabasal rays after the look-back).
 // rpx_trace_rays chains the ray generations on the device (rpx_gen.h).
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <algorithm>
 #include <string.h>
 #include <stdio.h>
@@ -311,6 +312,12 @@ struct TraceArgs {
     int32_t in_pool, store_children;
     uint64_t status_tag;     // look-back tag of this generation (0 outside a chain)
     GenControl *head_out;    // chain: where the last workgroup to finish writes the control head
+    // tile order (k_generation): with by_launch, tile = tile_base +
+    // blockIdx.x and look-back waits are bounded by lookback_ticks (ChainGen);
+    // otherwise tiles come from the generation's atomic counter
+    uint32_t tile_base = 0;
+    int32_t by_launch = 0;
+    uint64_t lookback_ticks = 0;
 };
 
 struct CountEmit {
@@ -560,11 +567,28 @@ void k_generation(TraceArgs A) {
         out_limit = A.store_children ? (A.pool_rays > L.out_offset ? A.pool_rays - L.out_offset : 0) : 0;
     }
     const uint32_t n_tiles = (uint32_t)((n_in + RPX_BLOCK - 1) / RPX_BLOCK);
-    if (blockIdx.x >= n_tiles) return;
-    // the tile number is fetched first, so that its round trip overlaps the
-    // scene staging
     uint32_t my_tile = 0;
-    if (threadIdx.x == 0) my_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
+    if (A.by_launch) {
+        my_tile = A.tile_base + blockIdx.x;
+        if (my_tile >= n_tiles) return;
+    } else {
+        if (blockIdx.x >= n_tiles) return;
+        // the tile number is fetched first, so that its round trip overlaps
+        // the scene staging
+        if (threadIdx.x == 0) my_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
+    }
+    // with launch-order tiles the ray's geometry is loaded before the scene
+    // is staged, so that the two latencies overlap
+    const double *f = in.f64;
+    const uint64_t c = in.capacity;
+    V3 origin = v3(0, 0, 0), dir = v3(0, 0, 0);
+    if (A.by_launch) {
+        const uint64_t i0 = (uint64_t)my_tile * RPX_BLOCK + threadIdx.x;
+        if (i0 < n_in) {
+            origin = v3(f[RPX_OX * c + i0], f[RPX_OY * c + i0], f[RPX_OZ * c + i0]);
+            dir = v3(f[RPX_DX * c + i0], f[RPX_DY * c + i0], f[RPX_DZ * c + i0]);
+        }
+    }
     const SceneView S = stage_scene(A.S, lds_scene);
     for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
     if (threadIdx.x == 0) {
@@ -578,10 +602,10 @@ void k_generation(TraceArgs A) {
     bool hit = false;
     int hit_rec = -1, hit_list = -1;
     if (i < n_in) {
-        const double *f = in.f64;
-        const uint64_t c = in.capacity;
-        V3 origin = v3(f[RPX_OX * c + i], f[RPX_OY * c + i], f[RPX_OZ * c + i]);
-        V3 dir = v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]);
+        if (!A.by_launch) {
+            origin = v3(f[RPX_OX * c + i], f[RPX_OY * c + i], f[RPX_OZ * c + i]);
+            dir = v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]);
+        }
         double length = A.max_length;
         uint32_t end_face = RPX_NO_FACE;
         V3 point = add(origin, mul(dir, A.max_length));
@@ -645,7 +669,8 @@ void k_generation(TraceArgs A) {
         if ((threadIdx.x & 63) == 0) atomicAdd(&lds_hits, (uint32_t)__popcll(mh));
     }
     if (threadIdx.x < 64) {
-        const uint64_t excl = lookback_tagged(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error, A.status_tag);
+        const uint64_t excl = lookback_tagged(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error, A.status_tag,
+                                              A.lookback_ticks);
         if (threadIdx.x == 0) lds_excl = excl;
     }
     __syncthreads();
@@ -740,6 +765,20 @@ extern "C" rpx_status_t rpx_timing_read(double *ms, uint64_t *launches, double *
     }
     return RPX_OK;
 }
+
+// ------------------------------------------------ tile order of the chains
+static std::atomic<uint64_t> g_lookback_ticks{0};
+static std::atomic<uint64_t> g_tile_order_fallbacks{0};
+
+uint64_t rpx_lookback_ticks_override() { return g_lookback_ticks.load(); }
+void rpx_note_tile_order_fallback() { g_tile_order_fallbacks.fetch_add(1); }
+
+extern "C" rpx_status_t rpx_debug_lookback_ticks(uint64_t ticks) {
+    g_lookback_ticks.store(ticks);
+    return RPX_OK;
+}
+
+extern "C" uint64_t rpx_tile_order_fallbacks(void) { return g_tile_order_fallbacks.load(); }
 
 // ------------------------------------------------------------- host side
 rpx_status_t rpx_check_trace_scene(const rpx_scene_t *scene) {
@@ -986,6 +1025,9 @@ extern "C" rpx_status_t rpx_trace_rays(const rpx_scene_t *scene, const rpx_rays_
         A.in_pool = cg.in_pool;
         A.in = cg.in_pool ? *pool : *in;
         A.store_children = cg.store_children;
+        A.tile_base = cg.tile_base;
+        A.by_launch = cg.by_launch;
+        A.lookback_ticks = cg.lookback_ticks;
         launch_generation(variant, dim3((unsigned)grid), stream, A);
         return hipGetLastError();
     };

@@ -307,6 +307,47 @@ def test_trace_chain_matches_generation_path(gpu, orders):
     assert counts_chain == [f.count for f in all_faces]
 
 
+@pytest.mark.parametrize("form", ["f64", "split"])
+def test_trace_chain_tile_order_fallback(gpu, form):
+    """The device-resident loop takes tiles in launch order and bounds each
+    look-back wait; a wait that runs out makes it trace the chain again with
+    tiles from the atomic counter.  With every wait made to run out at once
+    (rpx_debug_lookback_ticks(1)), the fallback runs, and its generations and
+    face counts equal an ordinary trace's bit for bit (C3 Michelson, 4,096
+    rays, depth 12, three grating orders; float64 and FP32-split)."""
+    import torch
+    from raypier_optics_amd import _lib
+    from raypier_optics_amd.split import trace_rays_split
+    lib = _lib.load()
+    run = trace_rays if form == "f64" else trace_rays_split
+
+    def trace():
+        w = workloads.michelson_c3(n_rays=4096, orders=(-1, 0, 1))
+        got, faces = run(w.collection(), w.face_lists, recursion_limit=w.recursion_limit, max_length=w.max_length)
+        if form == "f64":
+            gens = [g.copy_as_array().tobytes() for g in got]
+        else:
+            gens = [[getattr(g.dev, part)()[:, : g.n_rays].cpu() for part in ("f64", "f32", "u32")] for g in got]
+        return gens, [f.count for f in faces]
+
+    ref, ref_counts = trace()
+    before = int(lib.rpx_tile_order_fallbacks())
+    assert lib.rpx_debug_lookback_ticks(1) == 0
+    try:
+        got, counts = trace()
+    finally:
+        lib.rpx_debug_lookback_ticks(0)
+    assert int(lib.rpx_tile_order_fallbacks()) > before
+    assert counts == ref_counts
+    assert len(got) == len(ref)
+    for g, (a, b) in enumerate(zip(got, ref)):
+        if form == "f64":
+            assert a == b, g
+        else:
+            for x, y in zip(a, b):
+                assert torch.equal(x, y), g
+
+
 def test_trace_one_face_sequence(gpu):
     w = workloads.singlet_c1(n_rays=3000)
     lens, det = w.face_lists
