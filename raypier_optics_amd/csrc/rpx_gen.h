@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include "rpx_internal.h"
+#include "rpx_materials.h"
 #include "rpx_host.h"
 
 namespace rpx {
@@ -387,16 +388,32 @@ static __global__ void k_chain_init(ChainWs w, uint64_t n_tiles, int32_t n_gen, 
 // generation of a chain into its host head (the generations themselves write
 // only their child count there).  One workgroup per generation.
 static __global__ __launch_bounds__(64) void k_chain_fold(const GenControl *ctl, GenControl *heads,
-                                                           int n_list_faces) {
+                                                           int n_list_faces, const rpx_face_t *faces,
+                                                           const rpx_material_t *materials) {
     const GenControl &c = ctl[blockIdx.x];
     GenControl &h = heads[blockIdx.x];
+    __shared__ uint32_t lds_total[64];
+    uint32_t mine = 0;
     for (int k = threadIdx.x; k < n_list_faces; k += 64) {
         uint32_t t = 0;
         for (int q = 0; q < RPX_SHARDS; ++q) t += c.face_shards[q][k];
         h.face_totals[k] = t;
+        const int m = faces[k].material;
+        if (m >= 0 && material_emits(materials[m])) mine += t;
     }
-    if (threadIdx.x < RPX_SHARDS) h.hit_shards[threadIdx.x] = c.hit_shards[threadIdx.x];
-    if (threadIdx.x == 0) h.error = c.error;
+    // every ray that hit a face is counted once, on that face: the rays whose
+    // material was evaluated are the face totals of faces whose material can
+    // have children (the generation kernels of a chain leave the hit shards
+    // alone)
+    lds_total[threadIdx.x] = mine;
+    __syncthreads();
+    if (threadIdx.x < RPX_SHARDS) h.hit_shards[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        uint32_t all = 0;
+        for (int k = 0; k < 64; ++k) all += lds_total[k];
+        h.hit_shards[0] = all;
+        h.error = c.error;
+    }
 }
 
 // Pinned, device-mapped control heads, one per generation of a chain, per
@@ -659,7 +676,8 @@ static rpx_status_t run_chain_once(const ChainSpec &c, const rpx_scene_t *scene,
             if (rpx_status_t st = read_heads_through(launched - 2)) return st;
         if (rpx_status_t st = close_gen(launched - 1)) return st;
         hipLaunchKernelGGL(k_chain_fold, dim3((unsigned)launched), dim3(64), 0, stream, (const GenControl *)cw.ctl,
-                           heads_dev, scene->view.n_list_faces);
+                           heads_dev, scene->view.n_list_faces, (const rpx_face_t *)scene->view.faces,
+                           (const rpx_material_t *)scene->view.materials);
         RPX_HIP_TRY(hipGetLastError());
         RPX_HIP_TRY(hipEventRecord(evs[2 * L], stream));
         if (launched > synced)

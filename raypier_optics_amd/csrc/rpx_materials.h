@@ -184,6 +184,13 @@ struct WlTable {
     RPX_DEV double operator[](uint32_t i) const { return (i < n) ? wl[i] : __builtin_nan(""); }
 };
 
+// Whether a material can have children: OpaqueMaterial and the base
+// InterfaceMaterial never do, so a ray ending on them needs no material
+// evaluation and none of its record beyond the geometry.
+RPX_DEV bool material_emits(const rpx_material_t &m) {
+    return m.type != RPX_MAT_NONE && m.type != RPX_MAT_OPAQUE;
+}
+
 RPX_DEV double power_frac(double nre, Cx a, Cx b, double P_in) {
     double ca = cabs_(a), cb = cabs_(b);
     return nre * (ca * ca + cb * cb) / P_in;

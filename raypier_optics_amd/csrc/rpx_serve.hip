@@ -168,7 +168,7 @@ void k_batch(const BatchJob *jobs, BatchControl *ctl, uint64_t *status,
                     mx = hp.x; my = hp.y; mz = hp.z;
                     mr = hp.x * hp.x + hp.y * hp.y + hp.z * hp.z;
                 }
-                if (fc.material >= 0) {
+                if (fc.material >= 0 && material_emits(S.materials[fc.material])) {
                     const rpx_face_list_t &fl = S.face_lists[hit_list];
                     Ray r = load_ray(J.in, i);
                     r.length = length;

@@ -171,10 +171,10 @@ void k_generation_split(SplitArgs A) {
         in.u32[RPX_END_FACE_IDX * c + i] = end_face;
         hit_length = length;
         if (hit_rec >= 0) {
-            hit = true;
             atomicAdd(&lds_hist[hit_rec], 1u);
             const rpx_face_t &fc = S.faces[hit_rec];
-            if (fc.material >= 0) {
+            if (fc.material >= 0 && material_emits(S.materials[fc.material])) {
+                hit = true;   // counted: the rest of its record is read
                 const rpx_face_list_t &fl = S.face_lists[hit_list];
                 RayS r = load_ray_s(in, i);
                 r.length = (float)length;
@@ -215,7 +215,9 @@ void k_generation_split(SplitArgs A) {
         }
     }
     if (threadIdx.x == 0) {
-        add_hits(A.ws.ctl, tile, lds_hits);
+        // a chain's hit total is the sum of its face totals (k_chain_fold);
+        // one word per generation would take every tile's atomic in one line
+        if (!A.link) add_hits(A.ws.ctl, tile, lds_hits);
         if ((uint64_t)tile == (n_in - 1) / RPX_BLOCK) {
             const uint64_t kids = lds_excl + block_total;
             A.ws.ctl->children = kids;
