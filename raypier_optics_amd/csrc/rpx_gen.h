@@ -485,6 +485,7 @@ static inline uint64_t realtime_ticks(double ms) {
 
 struct ChainSpec {
     uint64_t n_in, pool_rays, max_children;
+    uint32_t block = RPX_BLOCK;               // rays per tile of the generation kernel (<= the workspace's)
     int32_t max_generations;
     int timing_slot;                          // RPX_TIMING_* of the generation kernel
     double bytes_in, bytes_hit, bytes_child;  // algorithmic bytes per input ray / hit / child
@@ -579,7 +580,7 @@ static rpx_status_t run_chain_once(const ChainSpec &c, const rpx_scene_t *scene,
     known[0] = 1;
     int launched = 0, synced = 0;
     rpx_status_t result = RPX_OK;
-    auto tiles_of = [](uint64_t n) { return (n + RPX_BLOCK - 1) / RPX_BLOCK; };
+    auto tiles_of = [&](uint64_t n) { return (n + c.block - 1) / c.block; };
     auto gen = [&](int g) {
         ChainGen cg;
         cg.link = cw.link + g;
