@@ -62,9 +62,15 @@ def _pool_rays(n, max_children, remaining):
         want = ((want + (1 << 20) - 1) >> 20) << 20
     # memory the pool can come from: the device's memory less what tensors
     # hold now (free or cached by the allocator), so that a repeated trace
-    # asks for the same pools as before and gets them from the cache
-    free = _device_bytes(torch) - torch.cuda.memory_allocated()
-    cap = int(0.4 * free) // (_abi.N_F64 * 8 + _abi.N_U32 * 4)
+    # asks for the same pools as before and gets them from the cache.  The
+    # allocator's statistics cost tens of microseconds per query, so they are
+    # read only for a pool above a tenth of the device, which the cap could
+    # bind (a smaller pool that does not fit is halved by _alloc_pool)
+    ray_bytes = _abi.N_F64 * 8 + _abi.N_U32 * 4
+    total = _device_bytes(torch)
+    if want * ray_bytes <= total // 10:
+        return max(want, mc * n, 1)
+    cap = int(0.4 * (total - torch.cuda.memory_allocated())) // ray_bytes
     return max(min(want, cap), mc * n, 1)
 
 
