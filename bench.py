@@ -160,7 +160,7 @@ def roofline(r, measured):
                                   for i in range(len(names)) if launches[i]}}
 
 
-def serving_latency(n_jobs, qps, seed=2024, grad_fraction=0.3, budget_bytes=3 << 29):
+def serving_latency(n_jobs, qps, seed=2024, grad_fraction=0.3, budget_bytes=3 << 29, generations_per_step=2):
     """C5 stream: n_jobs random lens jobs (2-8 surfaces, rays log-uniform in
     [1e3, 2e6], built on the device) arriving as a Poisson process of rate
     qps into the continuous-batching server under a 1.5 GiB HBM budget (the
@@ -171,7 +171,8 @@ def serving_latency(n_jobs, qps, seed=2024, grad_fraction=0.3, budget_bytes=3 <<
     from raypier_optics_amd.serving import TraceServer, run_poisson
     jobs = [workloads.lens_job_c5(seed + k, device_rays=True) for k in range(n_jobs)]
     grad = np.random.default_rng(seed).random(n_jobs) < grad_fraction
-    server = TraceServer(hbm_budget_bytes=budget_bytes, max_jobs=512, max_rays_in_flight=64 << 20)
+    server = TraceServer(hbm_budget_bytes=budget_bytes, max_jobs=512, max_rays_in_flight=64 << 20,
+                         generations_per_step=generations_per_step)
     run_poisson(server, jobs[:4], qps, seed=seed, gradient=grad[:4])     # warm: compile, graph build
     held0, _ = server.stats()
     deferred0 = server.submits_deferred
@@ -192,6 +193,7 @@ def serving_latency(n_jobs, qps, seed=2024, grad_fraction=0.3, budget_bytes=3 <<
            "workload": "lens_job_c5: 2-8 sphere/conic/asphere surfaces, N-BK7/N-SF11, rays logU(1e3, 2e6), "
                        "Poisson arrivals, %d%% gradient jobs; latency from arrival to result (host submit and "
                        "scene compile included)" % int(round(100 * grad_fraction)),
+           "generations_per_step": generations_per_step,
            "hbm_budget_bytes": budget_bytes, "jobs_held_for_budget": held - held0, "peak_budget_bytes": peak,
            "submissions_deferred_for_budget": server.submits_deferred - deferred0,
            "admission_wait_ms_p99": float(np.percentile([1000.0 * (r.t_admit - r.t_submit) for r in results], 99))}
@@ -479,6 +481,10 @@ def main():
         "roofline": roofline(r64, measured=args.rays),
         "fp32split": fp32split,
         "latency": serving_latency(args.serve_jobs, args.serve_qps) if args.serve_jobs > 0 else None,
+        # the same stream with one generation per server step, for comparison
+        "latency_one_generation_per_step": {k: v for k, v in serving_latency(
+            args.serve_jobs, args.serve_qps, generations_per_step=1).items()
+            if k in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "jobs_per_s")} if args.serve_jobs > 0 else None,
         "adjoint": adjoint_timing(),
         "field": field_c4() if args.field else None,
         "michelson": michelson_c3_leg() if args.michelson else None,

@@ -43,9 +43,11 @@ struct BatchJob {
     double max_length;
     WlTable wl;
     int32_t detector;
+    int32_t gens;                    // generations this step may trace (1 or 2)
+    int32_t max_children;
     int32_t pad;
     double *moments;                 // device [6], accumulated over generations
-    uint64_t *counts;                // device [4]: children, hits, parents, unused
+    uint64_t *counts;                // device [4]: children, hits, parents, second generation traced (0/1)
 };
 
 struct BatchControl {
@@ -62,7 +64,7 @@ __global__ __launch_bounds__(256) void k_batch_clear(uint64_t *status, uint64_t 
     if (k < max_tiles && k < ctl->total_tiles) status[k] = 0;
     if (k < ctl->n_jobs) {
         uint64_t *c = jobs[k].counts;
-        c[0] = 0; c[1] = 0; c[2] = 0; c[3] = 0;
+        c[0] = 0; c[1] = 0; c[2] = 0;
     }
 }
 
@@ -87,6 +89,51 @@ __global__ __launch_bounds__(256) void k_batch_reduce(const BatchJob *jobs, cons
     }
     if (threadIdx.x == 0)
         for (int q = 0; q < 5; ++q) J.moments[q] += red[0][q];
+}
+
+// Lays out a step's second generation on the device, from the first one's
+// child counts: a job that may trace two generations (gens == 2) and whose
+// children fit its first generation's buffer at max_children each traces
+// them from its output buffer into its input buffer, which the first
+// generation no longer needs; the other jobs get no tiles.  Jobs keep their
+// order, so the tiles stay laid end to end.  Each job's counts move to its
+// second block of four words, whose last word records that the second
+// generation was traced.  One workgroup: each thread takes a run of jobs.
+__global__ __launch_bounds__(256) void k_batch_relayout(BatchJob *jobs, BatchControl *ctl) {
+    __shared__ uint64_t run_tiles[256];
+    const uint32_t n = (uint32_t)ctl->n_jobs;
+    const uint32_t per = (n + 255) / 256;
+    const uint32_t b = threadIdx.x * per, e = min(n, b + per);
+    auto traces = [](const BatchJob &J, uint64_t kids) {
+        return J.gens >= 2 && kids > 0 && kids * (uint64_t)J.max_children <= J.in.capacity;
+    };
+    uint64_t mine = 0;
+    for (uint32_t j = b; j < e; ++j) {
+        const uint64_t kids = jobs[j].counts[0];
+        if (traces(jobs[j], kids)) mine += (kids + RPX_BLOCK - 1) / RPX_BLOCK;
+    }
+    run_tiles[threadIdx.x] = mine;
+    __syncthreads();
+    uint64_t base = 0;
+    for (uint32_t t = 0; t < threadIdx.x; ++t) base += run_tiles[t];
+    for (uint32_t j = b; j < e; ++j) {
+        BatchJob &J = jobs[j];
+        const uint64_t kids = J.counts[0];
+        const bool go = traces(J, kids);
+        const rpx_rays_t first_in = J.in;
+        J.in = J.out;
+        J.out = first_in;
+        J.n_in = go ? kids : 0;
+        J.tile_begin = base;
+        base += go ? (kids + RPX_BLOCK - 1) / RPX_BLOCK : 0;
+        J.tile_end = base;
+        J.counts += 4;
+        J.counts[3] = go ? 1 : 0;
+    }
+    if (threadIdx.x == 255) {
+        ctl->total_tiles = base;
+        ctl->tile_counter = 0;
+    }
 }
 
 template <uint32_t F>
@@ -257,10 +304,12 @@ struct Job {
 
 }  // namespace
 
+// A step's graph: one generation (clear, batch) or two (then also the
+// second generation's clear2, batch2 after the relayout).
 struct StepGraph {
     hipGraph_t g = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipGraphNode_t clear = nullptr, batch = nullptr;
+    hipGraphNode_t clear = nullptr, batch = nullptr, clear2 = nullptr, batch2 = nullptr;
     void *batch_func = nullptr;
 };
 
@@ -277,10 +326,12 @@ struct rpx_server {
     double *d_partials = nullptr;
     uint64_t *d_counts = nullptr, *h_counts = nullptr;
     double *d_moments = nullptr, *h_moments = nullptr;
-    StepGraph graphs[3];
+    StepGraph graphs[3], graphs2[3];   // one / two generations per step, per kernel variant
+    uint32_t gens_per_step = 2;
     void *clear_args[4] = {&d_status, &max_tiles, &d_jobs, &d_ctl};
     void *batch_args[4] = {&d_jobs, &d_ctl, &d_status, &d_partials};
     void *reduce_args[3] = {&d_jobs, &d_ctl, &d_partials};
+    void *relayout_args[2] = {&d_jobs, &d_ctl};
     std::deque<Job *> waiting;
     std::vector<Job *> active;
     std::vector<int> free_slots;
@@ -319,54 +370,72 @@ static void free_buffer(rpx_server_t *s, Buffer &b) {
     } while (0)
 
 template <uint32_t F>
-static rpx_status_t build_graph(rpx_server_t *s, StepGraph &G) {
+static rpx_status_t build_graph(rpx_server_t *s, StepGraph &G, bool two) {
     RPX_GRAPH_TRY(hipGraphCreate(&G.g, 0));
-    hipGraphNode_t n[8];
-    RPX_GRAPH_TRY(hipGraphAddMemcpyNode1D(&n[0], G.g, nullptr, 0, s->d_jobs, s->h_jobs,
+    hipGraphNode_t last, node;
+    RPX_GRAPH_TRY(hipGraphAddMemcpyNode1D(&node, G.g, nullptr, 0, s->d_jobs, s->h_jobs,
                                           sizeof(BatchJob) * s->max_jobs, hipMemcpyHostToDevice));
-    RPX_GRAPH_TRY(hipGraphAddMemcpyNode1D(&n[1], G.g, &n[0], 1, s->d_ctl, s->h_ctl, sizeof(BatchControl),
+    last = node;
+    RPX_GRAPH_TRY(hipGraphAddMemcpyNode1D(&node, G.g, &last, 1, s->d_ctl, s->h_ctl, sizeof(BatchControl),
                                           hipMemcpyHostToDevice));
-    hipKernelNodeParams p{};
-    p.func = (void *)k_batch_clear;
-    p.gridDim = dim3(1);
-    p.blockDim = dim3(256);
-    p.kernelParams = s->clear_args;
-    RPX_GRAPH_TRY(hipGraphAddKernelNode(&n[2], G.g, &n[1], 1, &p));
-    p.func = (void *)k_batch<F>;
-    p.blockDim = dim3(RPX_BLOCK);
-    p.kernelParams = s->batch_args;
-    RPX_GRAPH_TRY(hipGraphAddKernelNode(&n[3], G.g, &n[2], 1, &p));
-    p.func = (void *)k_batch_reduce;
-    p.gridDim = dim3(s->max_jobs);
-    p.blockDim = dim3(256);
-    p.kernelParams = s->reduce_args;
-    RPX_GRAPH_TRY(hipGraphAddKernelNode(&n[4], G.g, &n[3], 1, &p));
-    RPX_GRAPH_TRY(hipGraphAddMemcpyNode1D(&n[5], G.g, &n[4], 1, s->h_counts, s->d_counts,
-                                          sizeof(uint64_t) * 4 * s->max_jobs, hipMemcpyDeviceToHost));
-    RPX_GRAPH_TRY(hipGraphAddMemcpyNode1D(&n[6], G.g, &n[5], 1, s->h_moments, s->d_moments,
+    last = node;
+    auto add_kernel = [&](void *func, dim3 grid, dim3 block, void **args, hipGraphNode_t *out) -> hipError_t {
+        hipKernelNodeParams p{};
+        p.func = func;
+        p.gridDim = grid;
+        p.blockDim = block;
+        p.kernelParams = args;
+        hipError_t e = hipGraphAddKernelNode(&node, G.g, &last, 1, &p);
+        if (e == hipSuccess) {
+            last = node;
+            if (out) *out = node;
+        }
+        return e;
+    };
+    // per generation: status and counts clear, the tile kernel, the moment sums
+    RPX_GRAPH_TRY(add_kernel((void *)k_batch_clear, dim3(1), dim3(256), s->clear_args, &G.clear));
+    RPX_GRAPH_TRY(add_kernel((void *)k_batch<F>, dim3(1), dim3(RPX_BLOCK), s->batch_args, &G.batch));
+    RPX_GRAPH_TRY(add_kernel((void *)k_batch_reduce, dim3(s->max_jobs), dim3(256), s->reduce_args, nullptr));
+    if (two) {
+        RPX_GRAPH_TRY(add_kernel((void *)k_batch_relayout, dim3(1), dim3(256), s->relayout_args, nullptr));
+        RPX_GRAPH_TRY(add_kernel((void *)k_batch_clear, dim3(1), dim3(256), s->clear_args, &G.clear2));
+        RPX_GRAPH_TRY(add_kernel((void *)k_batch<F>, dim3(1), dim3(RPX_BLOCK), s->batch_args, &G.batch2));
+        RPX_GRAPH_TRY(add_kernel((void *)k_batch_reduce, dim3(s->max_jobs), dim3(256), s->reduce_args, nullptr));
+    }
+    RPX_GRAPH_TRY(hipGraphAddMemcpyNode1D(&node, G.g, &last, 1, s->h_counts, s->d_counts,
+                                          sizeof(uint64_t) * 8 * s->max_jobs, hipMemcpyDeviceToHost));
+    last = node;
+    RPX_GRAPH_TRY(hipGraphAddMemcpyNode1D(&node, G.g, &last, 1, s->h_moments, s->d_moments,
                                           sizeof(double) * 6 * s->max_jobs, hipMemcpyDeviceToHost));
-    RPX_GRAPH_TRY(hipGraphAddMemcpyNode1D(&n[7], G.g, &n[6], 1, s->h_ctl_back, s->d_ctl, sizeof(BatchControl),
+    last = node;
+    RPX_GRAPH_TRY(hipGraphAddMemcpyNode1D(&node, G.g, &last, 1, s->h_ctl_back, s->d_ctl, sizeof(BatchControl),
                                           hipMemcpyDeviceToHost));
     RPX_GRAPH_TRY(hipGraphInstantiate(&G.exec, G.g, nullptr, nullptr, 0));
-    G.clear = n[2];
-    G.batch = n[3];
     G.batch_func = (void *)k_batch<F>;
     return RPX_OK;
 }
 
-// Point the two tile-sized kernels of the graph at this step's tile count.
-static rpx_status_t size_graph(rpx_server_t *s, StepGraph &G, uint64_t tiles, uint64_t n_jobs) {
+// Point the tile-sized kernels of the graph at this step's tile counts: the
+// first generation's exact count, and for a two-generation graph a bound on
+// the second's (its workgroups past the relayout's count leave at once).
+static rpx_status_t size_graph(rpx_server_t *s, StepGraph &G, uint64_t tiles, uint64_t n_jobs, uint64_t tiles2) {
     hipKernelNodeParams p{};
-    p.func = (void *)k_batch_clear;
-    p.gridDim = dim3((unsigned)((std::max(tiles, n_jobs) + 255) / 256));
-    p.blockDim = dim3(256);
-    p.kernelParams = s->clear_args;
-    RPX_GRAPH_TRY(hipGraphExecKernelNodeSetParams(G.exec, G.clear, &p));
-    p.func = G.batch_func;
-    p.gridDim = dim3((unsigned)tiles);
-    p.blockDim = dim3(RPX_BLOCK);
-    p.kernelParams = s->batch_args;
-    RPX_GRAPH_TRY(hipGraphExecKernelNodeSetParams(G.exec, G.batch, &p));
+    auto set = [&](hipGraphNode_t clear, hipGraphNode_t batch, uint64_t t) -> rpx_status_t {
+        p.func = (void *)k_batch_clear;
+        p.gridDim = dim3((unsigned)((std::max(t, n_jobs) + 255) / 256));
+        p.blockDim = dim3(256);
+        p.kernelParams = s->clear_args;
+        RPX_GRAPH_TRY(hipGraphExecKernelNodeSetParams(G.exec, clear, &p));
+        p.func = G.batch_func;
+        p.gridDim = dim3((unsigned)std::max<uint64_t>(t, 1));
+        p.blockDim = dim3(RPX_BLOCK);
+        p.kernelParams = s->batch_args;
+        RPX_GRAPH_TRY(hipGraphExecKernelNodeSetParams(G.exec, batch, &p));
+        return RPX_OK;
+    };
+    if (rpx_status_t st = set(G.clear, G.batch, tiles)) return st;
+    if (G.clear2)
+        if (rpx_status_t st = set(G.clear2, G.batch2, tiles2)) return st;
     return RPX_OK;
 }
 
@@ -411,12 +480,12 @@ extern "C" rpx_status_t rpx_server_create(uint64_t hbm_budget_bytes, uint32_t ma
         hipMalloc(&s->d_ctl, sizeof(BatchControl)) != hipSuccess ||
         hipMalloc(&s->d_status, sizeof(uint64_t) * s->max_tiles) != hipSuccess ||
         hipMalloc(&s->d_partials, sizeof(double) * 5 * s->max_tiles) != hipSuccess ||
-        hipMalloc(&s->d_counts, sizeof(uint64_t) * 4 * max_jobs) != hipSuccess ||
+        hipMalloc(&s->d_counts, sizeof(uint64_t) * 8 * max_jobs) != hipSuccess ||
         hipMalloc(&s->d_moments, sizeof(double) * 6 * max_jobs) != hipSuccess ||
         hipHostMalloc(&s->h_jobs, sizeof(BatchJob) * max_jobs, 0) != hipSuccess ||
         hipHostMalloc(&s->h_ctl, sizeof(BatchControl), 0) != hipSuccess ||
         hipHostMalloc(&s->h_ctl_back, sizeof(BatchControl), 0) != hipSuccess ||
-        hipHostMalloc(&s->h_counts, sizeof(uint64_t) * 4 * max_jobs, 0) != hipSuccess ||
+        hipHostMalloc(&s->h_counts, sizeof(uint64_t) * 8 * max_jobs, 0) != hipSuccess ||
         hipHostMalloc(&s->h_moments, sizeof(double) * 6 * max_jobs, 0) != hipSuccess)
         return fail(rpx_set_error(RPX_E_HIP, "server allocation failed"));
     memset(s->h_jobs, 0, sizeof(BatchJob) * max_jobs);
@@ -439,10 +508,11 @@ extern "C" rpx_status_t rpx_server_destroy(rpx_server_t *s) {
         delete j;
     }
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    for (auto &G : s->graphs) {
-        if (G.exec) (void)hipGraphExecDestroy(G.exec);
-        if (G.g) (void)hipGraphDestroy(G.g);
-    }
+    for (StepGraph *set : {s->graphs, s->graphs2})
+        for (int k = 0; k < 3; ++k) {
+            if (set[k].exec) (void)hipGraphExecDestroy(set[k].exec);
+            if (set[k].g) (void)hipGraphDestroy(set[k].g);
+        }
     (void)hipFree(s->d_jobs);
     (void)hipFree(s->d_ctl);
     (void)hipFree(s->d_status);
@@ -614,23 +684,36 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
             b.wl.wl = j->desc.wavelengths;
             b.wl.n = j->desc.wavelengths ? j->desc.n_wavelengths : 0;
             b.detector = j->desc.detector_face;
+            b.max_children = j->max_children;
+            // a second generation in this step needs one left before the
+            // recursion limit; the device decides whether its children fit
+            b.gens = (s->gens_per_step >= 2 && j->gen + 2 <= j->desc.recursion_limit) ? 2 : 1;
             b.moments = s->d_moments + j->slot * 6;
-            b.counts = s->d_counts + j->slot * 4;
+            b.counts = s->d_counts + j->slot * 8;
             if (j->variant == V_ALL || (j->variant == V_LENS && variant == V_SPHERES)) variant = j->variant;
         }
+        // bound on the second generation's tiles: a job traces it only if its
+        // first generation's children fit its input buffer max_children each
+        uint64_t tiles2 = 0;
+        for (size_t k = 0; k < batch.size(); ++k)
+            if (s->h_jobs[k].gens >= 2)
+                tiles2 += (batch[k]->cur.cap / std::max(1, batch[k]->max_children) + RPX_BLOCK - 1) / RPX_BLOCK;
+        const bool two = tiles2 > 0 && tiles2 <= s->max_tiles;
+        if (!two)
+            for (size_t k = 0; k < batch.size(); ++k) s->h_jobs[k].gens = 1;
         s->h_ctl->total_tiles = tile;
         s->h_ctl->n_jobs = batch.size();
         s->h_ctl->error = 0;
         s->h_ctl->tile_counter = 0;
         const int vi = variant == V_SPHERES ? 0 : variant == V_LENS ? 1 : 2;
-        StepGraph &G = s->graphs[vi];
+        StepGraph &G = two ? s->graphs2[vi] : s->graphs[vi];
         if (!G.exec) {
-            rpx_status_t st = vi == 0 ? build_graph<V_SPHERES>(s, G)
-                            : vi == 1 ? build_graph<V_LENS>(s, G)
-                                      : build_graph<V_ALL>(s, G);
+            rpx_status_t st = vi == 0 ? build_graph<V_SPHERES>(s, G, two)
+                            : vi == 1 ? build_graph<V_LENS>(s, G, two)
+                                      : build_graph<V_ALL>(s, G, two);
             if (st) return st;
         }
-        rpx_status_t st = size_graph(s, G, tile, batch.size());
+        rpx_status_t st = size_graph(s, G, tile, batch.size(), tiles2);
         if (st) return st;
         RPX_HIP_TRY(hipGraphLaunch(G.exec, s->stream));
         RPX_HIP_TRY(hipStreamSynchronize(s->stream));
@@ -661,32 +744,53 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
                 continue;
             }
             ++b;
-            const uint64_t *c = s->h_counts + j->slot * 4;
+            const uint64_t *c = s->h_counts + j->slot * 8;
             const uint64_t kids = c[0];
             j->interactions += c[1];
             j->children += kids;
             j->gen += 1;
-            s->used -= std::min(s->used, j->cur.bytes());
-            free_buffer(s, j->cur);
-            j->cur = j->next;
-            j->next = Buffer();
-            j->n = kids;
-            if (kids > j->cur.cap) {
-                j->status = RPX_E_CAPACITY;
-                j->n = 0;
+            if (two && c[7] == 1) {
+                // the second generation ran from `next` (the first one's
+                // children) back into `cur`, which now holds the job's
+                // current generation; `next` is free to take the next step's
+                j->interactions += c[5];
+                j->children += c[4];
+                j->gen += 1;
+                j->n = c[4];
+                if (j->n > j->cur.cap) {
+                    j->status = RPX_E_CAPACITY;
+                    j->n = 0;
+                }
+            } else {
+                s->used -= std::min(s->used, j->cur.bytes());
+                free_buffer(s, j->cur);
+                j->cur = j->next;
+                j->next = Buffer();
+                j->n = kids;
+                if (kids > j->cur.cap) {
+                    j->status = RPX_E_CAPACITY;
+                    j->n = 0;
+                }
             }
             if (j->n == 0 || j->gen >= j->desc.recursion_limit) {
                 finish(s, j, t);
                 continue;
             }
-            rpx_status_t st = alloc_buffer(s, j->next, (uint64_t)j->max_children * j->n);
-            if (st != RPX_OK) {            // the job ends with the error; the others carry on
-                j->status = st;
-                finish(s, j, t);
-                continue;
+            const uint64_t need = (uint64_t)j->max_children * j->n;
+            if (j->next.p && j->next.cap < need) {
+                s->used -= std::min(s->used, j->next.bytes());
+                free_buffer(s, j->next);
             }
-            s->used += j->next.bytes();
-            s->peak_used = std::max(s->peak_used, s->used);
+            if (!j->next.p) {
+                rpx_status_t st = alloc_buffer(s, j->next, need);
+                if (st != RPX_OK) {            // the job ends with the error; the others carry on
+                    j->status = st;
+                    finish(s, j, t);
+                    continue;
+                }
+                s->used += j->next.bytes();
+                s->peak_used = std::max(s->peak_used, s->used);
+            }
             still.push_back(j);
         }
         s->active.swap(still);
@@ -695,6 +799,13 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
         finished[(*n_finished)++] = s->done.front();
         s->done.pop_front();
     }
+    return RPX_OK;
+}
+
+extern "C" rpx_status_t rpx_server_set_generations_per_step(rpx_server_t *s, uint32_t generations) {
+    if (!s) return rpx_set_error(RPX_E_INVALID, "server is null");
+    if (generations < 1 || generations > 2) return rpx_set_error(RPX_E_INVALID, "generations per step must be 1 or 2");
+    s->gens_per_step = generations;
     return RPX_OK;
 }
 

@@ -52,15 +52,23 @@ class JobResult(object):
 
 class TraceServer(object):
     """Owns an rpx_server_t: HBM budget in bytes, at most `max_jobs` jobs on
-    the device, at most `max_rays_in_flight` rays per step."""
+    the device, at most `max_rays_in_flight` rays per step, and up to
+    `generations_per_step` (1 or 2) generations of each job per step."""
 
-    def __init__(self, hbm_budget_bytes=64 << 30, max_jobs=256, max_rays_in_flight=32 << 20):
+    def __init__(self, hbm_budget_bytes=64 << 30, max_jobs=256, max_rays_in_flight=32 << 20,
+                 generations_per_step=2):
         require_gpu()
         self.lib = load()
         h = ctypes.c_void_p()
         check(self.lib.rpx_server_create(int(hbm_budget_bytes), int(max_jobs), int(max_rays_in_flight), None,
                                          ctypes.byref(h)))
         self.h = h
+        try:
+            check(self.lib.rpx_server_set_generations_per_step(h, int(generations_per_step)))
+        except Exception:
+            self.lib.rpx_server_destroy(h)
+            self.h = None
+            raise
         self._keep = {}         # job id -> objects the device job reads (scene, wavelengths)
         self._info = {}
         self._results = (_abi.JobResult * 64)()
