@@ -675,13 +675,14 @@ rpx_status_t rpx_server_stats(const rpx_server_t *server, uint64_t *jobs_held_fo
                               uint64_t *peak_used_bytes);
 /* Seconds on the server's clock (the clock of t_submit / t_admit / t_finish). */
 double rpx_server_clock(void);
-/* Generations a step may trace per job: 1, or 2 (the default; no
- * counterpart in the reference).  With 2, a step's graph traces a second
- * generation after the first without a host round trip: the device lays out
- * the second generation's tiles from the first one's child counts, and a job
- * traces it when one is left before its recursion limit and its children fit
- * the buffer its first generation came from (max_children each), which the
- * second generation writes into.  A job's results are the same either way. */
+/* Generations a step may trace per job: 1 to 4 (the default is 4; no
+ * counterpart in the reference).  After the first, each generation of a step
+ * is traced without a host round trip: the device lays out its tiles from
+ * the previous generation's child counts, and a job traces it when it has a
+ * generation left before its recursion limit and its children fit, at
+ * max_children each, in the buffer the previous generation came from, which
+ * it writes into (a job's two buffers alternate).  A job's results are the
+ * same for every setting. */
 rpx_status_t rpx_server_set_generations_per_step(rpx_server_t *server, uint32_t generations);
 /* Test hook (no counterpart in the reference): after n_allocations further
  * job-buffer allocations succeed, the next one fails once, as if the device

@@ -1,8 +1,10 @@
 // Continuous-batching trace server.
 //
 // Every active job owns a float64 ray generation in HBM.  One server step
-// traces the current generation of all active jobs together: the jobs' rays
-// are laid end to end as tiles of RPX_BLOCK rays, and one launch of one
+// traces the current generation of all active jobs together, and then up to
+// three more without a host round trip (each laid out on the device from
+// the previous one's child counts, k_batch_relayout): the jobs' rays are
+// laid end to end as tiles of RPX_BLOCK rays, and one launch of one
 // workgroup per tile takes tile numbers from an atomic counter.  A workgroup
 // stages the scene of the job its tile belongs to in LDS, traces the tile as
 // k_generation does (same face, orientation and material code), places the
@@ -43,11 +45,11 @@ struct BatchJob {
     double max_length;
     WlTable wl;
     int32_t detector;
-    int32_t gens;                    // generations this step may trace (1 or 2)
+    int32_t gens;                    // generations this step may trace (1..RPX_SERVE_MAX_GENS)
     int32_t max_children;
     int32_t pad;
     double *moments;                 // device [6], accumulated over generations
-    uint64_t *counts;                // device [4]: children, hits, parents, second generation traced (0/1)
+    uint64_t *counts;                // device [4] per generation of the step: children, hits, parents, traced (0/1)
 };
 
 struct BatchControl {
@@ -55,8 +57,13 @@ struct BatchControl {
     uint64_t n_jobs;
     uint64_t error;
     uint32_t tile_counter;
-    uint32_t pad;
+    uint32_t next_block;             // the generation of the step the next relayout lays out (1, 2, ...)
 };
+
+// Generations one server step can trace per job, and the count words each
+// generation of a step has per job.
+#define RPX_SERVE_MAX_GENS 4
+#define RPX_SERVE_COUNT_WORDS (4 * RPX_SERVE_MAX_GENS)
 
 __global__ __launch_bounds__(256) void k_batch_clear(uint64_t *status, uint64_t max_tiles, BatchJob *jobs,
                                                       BatchControl *ctl) {
@@ -91,21 +98,23 @@ __global__ __launch_bounds__(256) void k_batch_reduce(const BatchJob *jobs, cons
         for (int q = 0; q < 5; ++q) J.moments[q] += red[0][q];
 }
 
-// Lays out a step's second generation on the device, from the first one's
-// child counts: a job that may trace two generations (gens == 2) and whose
-// children fit its first generation's buffer at max_children each traces
-// them from its output buffer into its input buffer, which the first
-// generation no longer needs; the other jobs get no tiles.  Jobs keep their
-// order, so the tiles stay laid end to end.  Each job's counts move to its
-// second block of four words, whose last word records that the second
-// generation was traced.  One workgroup: each thread takes a run of jobs.
+// Lays out a step's next generation on the device, from the previous one's
+// child counts: a job that may trace it (gens > the generation's index in
+// the step) and whose children fit, max_children each, in the buffer the
+// previous generation came from traces them from the previous generation's
+// output into that buffer, which is no longer needed (the two buffers of a
+// job alternate); the other jobs get no tiles.  Jobs keep their order, so
+// the tiles stay laid end to end.  Each job's counts move to the next block
+// of four words, whose last word records whether the generation was traced.
+// One workgroup: each thread takes a run of jobs.
 __global__ __launch_bounds__(256) void k_batch_relayout(BatchJob *jobs, BatchControl *ctl) {
     __shared__ uint64_t run_tiles[256];
     const uint32_t n = (uint32_t)ctl->n_jobs;
+    const uint32_t blk = ctl->next_block;
     const uint32_t per = (n + 255) / 256;
     const uint32_t b = threadIdx.x * per, e = min(n, b + per);
-    auto traces = [](const BatchJob &J, uint64_t kids) {
-        return J.gens >= 2 && kids > 0 && kids * (uint64_t)J.max_children <= J.in.capacity;
+    auto traces = [blk](const BatchJob &J, uint64_t kids) {
+        return J.gens > (int32_t)blk && kids > 0 && kids * (uint64_t)J.max_children <= J.in.capacity;
     };
     uint64_t mine = 0;
     for (uint32_t j = b; j < e; ++j) {
@@ -133,6 +142,7 @@ __global__ __launch_bounds__(256) void k_batch_relayout(BatchJob *jobs, BatchCon
     if (threadIdx.x == 255) {
         ctl->total_tiles = base;
         ctl->tile_counter = 0;
+        ctl->next_block = blk + 1;
     }
 }
 
@@ -304,12 +314,14 @@ struct Job {
 
 }  // namespace
 
-// A step's graph: one generation (clear, batch) or two (then also the
-// second generation's clear2, batch2 after the relayout).
+// A step's graph: per generation a clear and a tile kernel whose grids are
+// set before each replay (and a relayout before every generation but the
+// first).
 struct StepGraph {
     hipGraph_t g = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipGraphNode_t clear = nullptr, batch = nullptr, clear2 = nullptr, batch2 = nullptr;
+    int gens = 0;                                        // generations the graph traces
+    hipGraphNode_t clear[RPX_SERVE_MAX_GENS] = {}, batch[RPX_SERVE_MAX_GENS] = {};
     void *batch_func = nullptr;
 };
 
@@ -326,8 +338,8 @@ struct rpx_server {
     double *d_partials = nullptr;
     uint64_t *d_counts = nullptr, *h_counts = nullptr;
     double *d_moments = nullptr, *h_moments = nullptr;
-    StepGraph graphs[3], graphs2[3];   // one / two generations per step, per kernel variant
-    uint32_t gens_per_step = 2;
+    StepGraph graphs[RPX_SERVE_MAX_GENS][3];   // by generations per step - 1, then kernel variant
+    uint32_t gens_per_step = RPX_SERVE_MAX_GENS;
     void *clear_args[4] = {&d_status, &max_tiles, &d_jobs, &d_ctl};
     void *batch_args[4] = {&d_jobs, &d_ctl, &d_status, &d_partials};
     void *reduce_args[3] = {&d_jobs, &d_ctl, &d_partials};
@@ -370,7 +382,7 @@ static void free_buffer(rpx_server_t *s, Buffer &b) {
     } while (0)
 
 template <uint32_t F>
-static rpx_status_t build_graph(rpx_server_t *s, StepGraph &G, bool two) {
+static rpx_status_t build_graph(rpx_server_t *s, StepGraph &G, int gens) {
     RPX_GRAPH_TRY(hipGraphCreate(&G.g, 0));
     hipGraphNode_t last, node;
     RPX_GRAPH_TRY(hipGraphAddMemcpyNode1D(&node, G.g, nullptr, 0, s->d_jobs, s->h_jobs,
@@ -392,18 +404,19 @@ static rpx_status_t build_graph(rpx_server_t *s, StepGraph &G, bool two) {
         }
         return e;
     };
-    // per generation: status and counts clear, the tile kernel, the moment sums
-    RPX_GRAPH_TRY(add_kernel((void *)k_batch_clear, dim3(1), dim3(256), s->clear_args, &G.clear));
-    RPX_GRAPH_TRY(add_kernel((void *)k_batch<F>, dim3(1), dim3(RPX_BLOCK), s->batch_args, &G.batch));
-    RPX_GRAPH_TRY(add_kernel((void *)k_batch_reduce, dim3(s->max_jobs), dim3(256), s->reduce_args, nullptr));
-    if (two) {
-        RPX_GRAPH_TRY(add_kernel((void *)k_batch_relayout, dim3(1), dim3(256), s->relayout_args, nullptr));
-        RPX_GRAPH_TRY(add_kernel((void *)k_batch_clear, dim3(1), dim3(256), s->clear_args, &G.clear2));
-        RPX_GRAPH_TRY(add_kernel((void *)k_batch<F>, dim3(1), dim3(RPX_BLOCK), s->batch_args, &G.batch2));
+    // per generation: (after the first, the device relayout) status and
+    // counts clear, the tile kernel, the moment sums
+    for (int k = 0; k < gens; ++k) {
+        if (k > 0)
+            RPX_GRAPH_TRY(add_kernel((void *)k_batch_relayout, dim3(1), dim3(256), s->relayout_args, nullptr));
+        RPX_GRAPH_TRY(add_kernel((void *)k_batch_clear, dim3(1), dim3(256), s->clear_args, &G.clear[k]));
+        RPX_GRAPH_TRY(add_kernel((void *)k_batch<F>, dim3(1), dim3(RPX_BLOCK), s->batch_args, &G.batch[k]));
         RPX_GRAPH_TRY(add_kernel((void *)k_batch_reduce, dim3(s->max_jobs), dim3(256), s->reduce_args, nullptr));
     }
+    G.gens = gens;
     RPX_GRAPH_TRY(hipGraphAddMemcpyNode1D(&node, G.g, &last, 1, s->h_counts, s->d_counts,
-                                          sizeof(uint64_t) * 8 * s->max_jobs, hipMemcpyDeviceToHost));
+                                          sizeof(uint64_t) * RPX_SERVE_COUNT_WORDS * s->max_jobs,
+                                          hipMemcpyDeviceToHost));
     last = node;
     RPX_GRAPH_TRY(hipGraphAddMemcpyNode1D(&node, G.g, &last, 1, s->h_moments, s->d_moments,
                                           sizeof(double) * 6 * s->max_jobs, hipMemcpyDeviceToHost));
@@ -416,9 +429,9 @@ static rpx_status_t build_graph(rpx_server_t *s, StepGraph &G, bool two) {
 }
 
 // Point the tile-sized kernels of the graph at this step's tile counts: the
-// first generation's exact count, and for a two-generation graph a bound on
-// the second's (its workgroups past the relayout's count leave at once).
-static rpx_status_t size_graph(rpx_server_t *s, StepGraph &G, uint64_t tiles, uint64_t n_jobs, uint64_t tiles2) {
+// first generation's exact count, and bounds on the later ones' (their
+// workgroups past the relayout's count leave at once).
+static rpx_status_t size_graph(rpx_server_t *s, StepGraph &G, const uint64_t *tiles, uint64_t n_jobs) {
     hipKernelNodeParams p{};
     auto set = [&](hipGraphNode_t clear, hipGraphNode_t batch, uint64_t t) -> rpx_status_t {
         p.func = (void *)k_batch_clear;
@@ -433,9 +446,8 @@ static rpx_status_t size_graph(rpx_server_t *s, StepGraph &G, uint64_t tiles, ui
         RPX_GRAPH_TRY(hipGraphExecKernelNodeSetParams(G.exec, batch, &p));
         return RPX_OK;
     };
-    if (rpx_status_t st = set(G.clear, G.batch, tiles)) return st;
-    if (G.clear2)
-        if (rpx_status_t st = set(G.clear2, G.batch2, tiles2)) return st;
+    for (int k = 0; k < G.gens; ++k)
+        if (rpx_status_t st = set(G.clear[k], G.batch[k], tiles[k])) return st;
     return RPX_OK;
 }
 
@@ -480,12 +492,12 @@ extern "C" rpx_status_t rpx_server_create(uint64_t hbm_budget_bytes, uint32_t ma
         hipMalloc(&s->d_ctl, sizeof(BatchControl)) != hipSuccess ||
         hipMalloc(&s->d_status, sizeof(uint64_t) * s->max_tiles) != hipSuccess ||
         hipMalloc(&s->d_partials, sizeof(double) * 5 * s->max_tiles) != hipSuccess ||
-        hipMalloc(&s->d_counts, sizeof(uint64_t) * 8 * max_jobs) != hipSuccess ||
+        hipMalloc(&s->d_counts, sizeof(uint64_t) * RPX_SERVE_COUNT_WORDS * max_jobs) != hipSuccess ||
         hipMalloc(&s->d_moments, sizeof(double) * 6 * max_jobs) != hipSuccess ||
         hipHostMalloc(&s->h_jobs, sizeof(BatchJob) * max_jobs, 0) != hipSuccess ||
         hipHostMalloc(&s->h_ctl, sizeof(BatchControl), 0) != hipSuccess ||
         hipHostMalloc(&s->h_ctl_back, sizeof(BatchControl), 0) != hipSuccess ||
-        hipHostMalloc(&s->h_counts, sizeof(uint64_t) * 8 * max_jobs, 0) != hipSuccess ||
+        hipHostMalloc(&s->h_counts, sizeof(uint64_t) * RPX_SERVE_COUNT_WORDS * max_jobs, 0) != hipSuccess ||
         hipHostMalloc(&s->h_moments, sizeof(double) * 6 * max_jobs, 0) != hipSuccess)
         return fail(rpx_set_error(RPX_E_HIP, "server allocation failed"));
     memset(s->h_jobs, 0, sizeof(BatchJob) * max_jobs);
@@ -508,10 +520,10 @@ extern "C" rpx_status_t rpx_server_destroy(rpx_server_t *s) {
         delete j;
     }
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    for (StepGraph *set : {s->graphs, s->graphs2})
-        for (int k = 0; k < 3; ++k) {
-            if (set[k].exec) (void)hipGraphExecDestroy(set[k].exec);
-            if (set[k].g) (void)hipGraphDestroy(set[k].g);
+    for (auto &row : s->graphs)
+        for (StepGraph &G : row) {
+            if (G.exec) (void)hipGraphExecDestroy(G.exec);
+            if (G.g) (void)hipGraphDestroy(G.g);
         }
     (void)hipFree(s->d_jobs);
     (void)hipFree(s->d_ctl);
@@ -685,35 +697,47 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
             b.wl.n = j->desc.wavelengths ? j->desc.n_wavelengths : 0;
             b.detector = j->desc.detector_face;
             b.max_children = j->max_children;
-            // a second generation in this step needs one left before the
-            // recursion limit; the device decides whether its children fit
-            b.gens = (s->gens_per_step >= 2 && j->gen + 2 <= j->desc.recursion_limit) ? 2 : 1;
+            // generations this step may trace: the job's remaining ones, up
+            // to the server's setting; the device decides whether each later
+            // generation's children fit
+            b.gens = (int32_t)std::min<int64_t>(s->gens_per_step, (int64_t)j->desc.recursion_limit - j->gen);
             b.moments = s->d_moments + j->slot * 6;
-            b.counts = s->d_counts + j->slot * 8;
+            b.counts = s->d_counts + j->slot * RPX_SERVE_COUNT_WORDS;
             if (j->variant == V_ALL || (j->variant == V_LENS && variant == V_SPHERES)) variant = j->variant;
         }
-        // bound on the second generation's tiles: a job traces it only if its
-        // first generation's children fit its input buffer max_children each
-        uint64_t tiles2 = 0;
-        for (size_t k = 0; k < batch.size(); ++k)
-            if (s->h_jobs[k].gens >= 2)
-                tiles2 += (batch[k]->cur.cap / std::max(1, batch[k]->max_children) + RPX_BLOCK - 1) / RPX_BLOCK;
-        const bool two = tiles2 > 0 && tiles2 <= s->max_tiles;
-        if (!two)
-            for (size_t k = 0; k < batch.size(); ++k) s->h_jobs[k].gens = 1;
+        // tiles of each generation of the step: the first's exact, the later
+        // ones' bounded by the buffer they are written into (the job's two
+        // buffers alternate, the second generation writing into `cur`), over
+        // max_children; the step traces as many generations as any job may
+        // and the status table holds
+        uint64_t tiles[RPX_SERVE_MAX_GENS] = {tile};
+        int gens = 1;
+        for (int g = 1; g < (int)s->gens_per_step; ++g) {
+            uint64_t t = 0;
+            for (size_t k = 0; k < batch.size(); ++k) {
+                if (s->h_jobs[k].gens <= g) continue;
+                const Buffer &into = (g % 2 == 1) ? batch[k]->cur : batch[k]->next;
+                t += (into.cap / std::max(1, batch[k]->max_children) + RPX_BLOCK - 1) / RPX_BLOCK;
+            }
+            if (t == 0 || t > s->max_tiles) break;
+            tiles[g] = t;
+            gens = g + 1;
+        }
+        for (size_t k = 0; k < batch.size(); ++k) s->h_jobs[k].gens = std::min(s->h_jobs[k].gens, (int32_t)gens);
         s->h_ctl->total_tiles = tile;
         s->h_ctl->n_jobs = batch.size();
         s->h_ctl->error = 0;
         s->h_ctl->tile_counter = 0;
+        s->h_ctl->next_block = 1;
         const int vi = variant == V_SPHERES ? 0 : variant == V_LENS ? 1 : 2;
-        StepGraph &G = two ? s->graphs2[vi] : s->graphs[vi];
+        StepGraph &G = s->graphs[gens - 1][vi];
         if (!G.exec) {
-            rpx_status_t st = vi == 0 ? build_graph<V_SPHERES>(s, G, two)
-                            : vi == 1 ? build_graph<V_LENS>(s, G, two)
-                                      : build_graph<V_ALL>(s, G, two);
+            rpx_status_t st = vi == 0 ? build_graph<V_SPHERES>(s, G, gens)
+                            : vi == 1 ? build_graph<V_LENS>(s, G, gens)
+                                      : build_graph<V_ALL>(s, G, gens);
             if (st) return st;
         }
-        rpx_status_t st = size_graph(s, G, tile, batch.size(), tiles2);
+        rpx_status_t st = size_graph(s, G, tiles, batch.size());
         if (st) return st;
         RPX_HIP_TRY(hipGraphLaunch(G.exec, s->stream));
         RPX_HIP_TRY(hipStreamSynchronize(s->stream));
@@ -744,33 +768,26 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
                 continue;
             }
             ++b;
-            const uint64_t *c = s->h_counts + j->slot * 8;
-            const uint64_t kids = c[0];
+            // the first generation went from `cur` into `next`; each later one
+            // traced went back the other way
+            const uint64_t *c = s->h_counts + j->slot * RPX_SERVE_COUNT_WORDS;
             j->interactions += c[1];
-            j->children += kids;
+            j->children += c[0];
             j->gen += 1;
-            if (two && c[7] == 1) {
-                // the second generation ran from `next` (the first one's
-                // children) back into `cur`, which now holds the job's
-                // current generation; `next` is free to take the next step's
-                j->interactions += c[5];
-                j->children += c[4];
+            j->n = c[0];
+            bool in_cur = false;
+            for (int g = 1; g < gens && c[4 * g + 3] == 1; ++g) {
+                j->interactions += c[4 * g + 1];
+                j->children += c[4 * g];
                 j->gen += 1;
-                j->n = c[4];
-                if (j->n > j->cur.cap) {
-                    j->status = RPX_E_CAPACITY;
-                    j->n = 0;
-                }
-            } else {
-                s->used -= std::min(s->used, j->cur.bytes());
-                free_buffer(s, j->cur);
-                j->cur = j->next;
-                j->next = Buffer();
-                j->n = kids;
-                if (kids > j->cur.cap) {
-                    j->status = RPX_E_CAPACITY;
-                    j->n = 0;
-                }
+                j->n = c[4 * g];
+                in_cur = !in_cur;
+            }
+            // `cur` holds the job's current generation, `next` a free buffer
+            if (!in_cur) std::swap(j->cur, j->next);
+            if (j->n > j->cur.cap) {
+                j->status = RPX_E_CAPACITY;
+                j->n = 0;
             }
             if (j->n == 0 || j->gen >= j->desc.recursion_limit) {
                 finish(s, j, t);
@@ -804,7 +821,8 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
 
 extern "C" rpx_status_t rpx_server_set_generations_per_step(rpx_server_t *s, uint32_t generations) {
     if (!s) return rpx_set_error(RPX_E_INVALID, "server is null");
-    if (generations < 1 || generations > 2) return rpx_set_error(RPX_E_INVALID, "generations per step must be 1 or 2");
+    if (generations < 1 || generations > RPX_SERVE_MAX_GENS)
+        return rpx_set_error(RPX_E_INVALID, "generations per step must be 1.." + std::to_string(RPX_SERVE_MAX_GENS));
     s->gens_per_step = generations;
     return RPX_OK;
 }

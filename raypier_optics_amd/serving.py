@@ -56,7 +56,7 @@ class TraceServer(object):
     `generations_per_step` (1 or 2) generations of each job per step."""
 
     def __init__(self, hbm_budget_bytes=64 << 30, max_jobs=256, max_rays_in_flight=32 << 20,
-                 generations_per_step=2):
+                 generations_per_step=4):
         require_gpu()
         self.lib = load()
         h = ctypes.c_void_p()

@@ -60,15 +60,15 @@ def test_server_alone_vs_mix(gpu):
         assert a.detector_hits > 0
 
 
-def test_server_one_and_two_generations_per_step(gpu):
-    """A step traces up to two generations of each job without a host round
-    trip (the second laid out on the device from the first one's counts).
-    Every job's generations, interactions, children and detector moments are
-    bit-identical to one generation per step, alone and in a mix with a
-    budget that lets only some jobs run at once."""
+def test_server_generations_per_step(gpu):
+    """A step traces up to four generations of each job without a host round
+    trip (each after the first laid out on the device from the previous
+    one's counts).  Every job's generations, interactions, children and
+    detector moments are bit-identical to one generation per step, in a mix
+    with a budget that lets only some jobs run at once."""
     jobs = _jobs()
     res, steps = {}, {}
-    for gens in (1, 2):
+    for gens in (1, 2, 4):
         server = TraceServer(hbm_budget_bytes=96 << 20, max_jobs=16, max_rays_in_flight=1 << 22,
                              generations_per_step=gens)
         ids = [server.submit(w) for w in jobs]
@@ -78,13 +78,14 @@ def test_server_one_and_two_generations_per_step(gpu):
             steps[gens] += 1
         server.close()
         res[gens] = [out[i] for i in ids]
-    for k, (a, b) in enumerate(zip(res[1], res[2])):
-        _same(a, b, "job %d" % k)
+    for gens in (2, 4):
+        for k, (a, b) in enumerate(zip(res[1], res[gens])):
+            _same(a, b, "job %d, %d generations per step" % (k, gens))
     # the first step of a job traces one generation (its input buffer holds
-    # only its rays); later steps trace two
-    assert steps[2] < steps[1] * 0.75, steps
+    # only its rays); later steps trace up to the setting
+    assert steps[4] < steps[2] < steps[1] * 0.75, steps
     with pytest.raises(Exception):
-        TraceServer(generations_per_step=3)
+        TraceServer(generations_per_step=5)
 
 
 def test_server_matches_trace_rays(gpu):
