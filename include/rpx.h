@@ -347,6 +347,21 @@ rpx_status_t rpx_trace_rays(const rpx_scene_t *scene,
                             void *workspace, uint64_t workspace_bytes,
                             void *stream);
 
+/* The same loop for a gausslet collection (core/tracer.py:9-47 with
+ * ctracer.pyx:2214-2282 per generation: trace_gausslet_c and the parabasal
+ * rays), as rpx_trace_rays with gausslet buffers (`para` set on `in` and the
+ * pool) and max_length in double precision (as rpx_trace_gausslet).  For scenes without a decomposition (ResampleGaussletMaterial)
+ * face, whose captures need the host between generations. */
+rpx_status_t rpx_trace_gausslets(const rpx_scene_t *scene,
+                            const rpx_rays_t *in, uint64_t n_in,
+                            double max_length,
+                            const double *wavelengths, uint64_t n_wavelengths,
+                            int32_t max_generations, rpx_rays_t *pool,
+                            uint64_t *counts, uint64_t *offsets, int32_t *n_traced,
+                            uint32_t *face_hits,
+                            void *workspace, uint64_t workspace_bytes,
+                            void *stream);
+
 /* Sequential variant, one prescribed face: ctracer.pyx:2121-2170
  * trace_one_face_segment_c.  face_list / face_idx select FaceList and the
  * index of the face inside it. */

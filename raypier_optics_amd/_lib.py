@@ -34,6 +34,8 @@ _SIGNATURES = {
     "rpx_chain_workspace_bytes": (c_u64, [c_u64, c_i32]),
     "rpx_trace_rays": (c_i32, [c_void_p, P(_abi.Rays), c_u64, c_f32, c_void_p, c_u64, c_i32, P(_abi.Rays),
                                c_void_p, c_void_p, P(c_i32), c_void_p, c_void_p, c_u64, c_void_p]),
+    "rpx_trace_gausslets": (c_i32, [c_void_p, P(_abi.Rays), c_u64, c_f64, c_void_p, c_u64, c_i32, P(_abi.Rays),
+                               c_void_p, c_void_p, P(c_i32), c_void_p, c_void_p, c_u64, c_void_p]),
     "rpx_trace_one_face_segment": (c_i32, [c_void_p, c_i32, c_i32, P(_abi.Rays), c_u64, c_f32, c_void_p, c_u64,
                                            P(_abi.Rays), P(c_u64), c_void_p, c_void_p, c_u64, c_void_p]),
     "rpx_trace_gausslet": (c_i32, [c_void_p, c_i32, c_i32, P(_abi.Rays), c_u64, c_f64, c_void_p, c_u64,

@@ -412,6 +412,9 @@ static __global__ __launch_bounds__(64) void k_chain_fold(const GenControl *ctl,
         uint32_t all = 0;
         for (int k = 0; k < 64; ++k) all += lds_total[k];
         h.hit_shards[0] = all;
+        uint64_t parents = 0;   // gausslet generations: gausslets with children
+        for (int q = 0; q < RPX_SHARDS; ++q) parents += c.parent_shards[q];
+        h.parents = parents;
         h.error = c.error;
     }
 }
@@ -486,6 +489,7 @@ static inline uint64_t realtime_ticks(double ms) {
 struct ChainSpec {
     uint64_t n_in, pool_rays, max_children;
     uint32_t block = RPX_BLOCK;               // rays per tile of the generation kernel (<= the workspace's)
+    bool bytes_per_parent = false;            // bytes_hit per gausslet with children (gausslet chains)
     int32_t max_generations;
     int timing_slot;                          // RPX_TIMING_* of the generation kernel
     double bytes_in, bytes_hit, bytes_child;  // algorithmic bytes per input ray / hit / child
@@ -705,6 +709,7 @@ static rpx_status_t run_chain_once(const ChainSpec &c, const rpx_scene_t *scene,
             RPX_HIP_TRY(hipEventElapsedTime(&ms, evs[2 * g], evs[2 * g + 1]));
             uint64_t hits = 0;
             for (int s2 = 0; s2 < RPX_SHARDS; ++s2) hits += heads[g].hit_shards[s2];
+            if (c.bytes_per_parent) hits = heads[g].parents;
             const bool stored = g + 1 < L;
             rpx_timing_add(c.timing_slot, ms, (double)counts[g] * c.bytes_in + (double)hits * c.bytes_hit +
                                                   (stored ? (double)counts[g + 1] * c.bytes_child : 0.0));

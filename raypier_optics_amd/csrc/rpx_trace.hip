@@ -384,8 +384,26 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_gausslet_generation(TraceArgs A) 
     __shared__ uint32_t lds_wave[RPX_BLOCK / 64];
     __shared__ uint32_t lds_tile, lds_hits, lds_parents;
     __shared__ uint64_t lds_excl;
+    // in a device-resident chain, size and pool place come from the link,
+    // and tiles follow the chain's tile order (as k_generation)
+    uint64_t n_in = A.n_in, out_limit = A.out.capacity;
+    rpx_rays_t in = A.in, out = A.out;
+    if (A.link) {
+        const GenLink L = A.link[0];
+        n_in = L.count;
+        if (A.in_pool) in = advance_rays(A.in, L.offset);
+        out = advance_rays(A.out, L.out_offset);
+        out_limit = A.store_children ? (A.pool_rays > L.out_offset ? A.pool_rays - L.out_offset : 0) : 0;
+    }
+    const uint64_t n_tiles = (n_in + RPX_BLOCK - 1) / RPX_BLOCK;
     uint32_t my_tile = 0;
-    if (threadIdx.x == 0) my_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
+    if (A.by_launch) {
+        my_tile = A.tile_base + blockIdx.x;
+        if (my_tile >= n_tiles) return;
+    } else {
+        if (blockIdx.x >= n_tiles) return;
+        if (threadIdx.x == 0) my_tile = atomicAdd(&A.ws.ctl->tile_counter, 1u);
+    }
     const SceneView S = stage_scene(A.S, lds_scene);
     for (int k = threadIdx.x; k < RPX_LDS_FACES; k += blockDim.x) lds_hist[k] = 0;
     if (threadIdx.x == 0) {
@@ -396,14 +414,18 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_gausslet_generation(TraceArgs A) 
     __syncthreads();
     const uint32_t tile = lds_tile;
     const uint64_t i = (uint64_t)tile * RPX_BLOCK + threadIdx.x;
-    const uint64_t c = A.in.capacity;
+    const uint64_t c = in.capacity;
     uint32_t count = 0;
     int hit_rec = -1, hit_list = -1;
-    if (i < A.n_in) {
-        const double *f = A.in.f64;
+    if (i < n_in) {
+        const double *f = in.f64;
         V3 origin = v3(f[RPX_OX * c + i], f[RPX_OY * c + i], f[RPX_OZ * c + i]);
         V3 dir = v3(f[RPX_DX * c + i], f[RPX_DY * c + i], f[RPX_DZ * c + i]);
-        double length = f[RPX_LENGTH * c + i];
+        // in a chain every gausslet starts at max_length (trace_rays resets
+        // the input, children are written so), which keeps a generation
+        // repeatable after a run-out look-back (its in-place lengths are then
+        // not read back)
+        double length = A.link ? A.max_length : f[RPX_LENGTH * c + i];
         uint32_t end_face = RPX_NO_FACE;
         V3 point = add(origin, mul(dir, A.max_length));
         if (A.one_list < 0) {
@@ -440,8 +462,8 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_gausslet_generation(TraceArgs A) 
                 hit_list = A.one_list;
             }
         }
-        A.in.f64[RPX_LENGTH * c + i] = length;
-        A.in.u32[RPX_END_FACE_IDX * c + i] = end_face;
+        in.f64[RPX_LENGTH * c + i] = length;
+        in.u32[RPX_END_FACE_IDX * c + i] = end_face;
         if (hit_rec >= 0) {
             atomicAdd(&lds_hist[hit_rec], 1u);
             atomicAdd(&lds_hits, 1u);
@@ -450,24 +472,24 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_gausslet_generation(TraceArgs A) 
             // trace_parabasal_rays (ctracer.pyx:2363-2373): every parabasal
             // ray must hit the same face; lengths update in place
             bool para_ok = true;
-            const double *P = A.in.para;
+            const double *P = in.para;
             for (int j = 0; j < RPX_N_PARA && para_ok; ++j) {
                 const uint64_t b = (uint64_t)j * RPX_PARA_FIELDS;
                 V3 po = v3(P[(b + 0) * c + i], P[(b + 1) * c + i], P[(b + 2) * c + i]);
                 V3 pd = v3(P[(b + 3) * c + i], P[(b + 4) * c + i], P[(b + 5) * c + i]);
-                double plen = P[(b + 9) * c + i];
+                double plen = A.link ? A.max_length : P[(b + 9) * c + i];
                 V3 pend = add(po, mul(pd, A.max_length));
                 V3 q1 = transform(fl.inv_trans, po);
                 V3 q2 = transform(fl.inv_trans, pend);
                 double dist = face_intersect<F>(S, fc, q1, q2, 0);
                 if (fc.tolerance < dist && dist < plen) {
-                    A.in.para[(b + 9) * c + i] = dist;
+                    in.para[(b + 9) * c + i] = dist;
                 } else {
                     para_ok = false;
                 }
             }
             if (para_ok && fc.material >= 0) {
-                Ray r = load_ray(A.in, i);
+                Ray r = load_ray(in, i);
                 r.length = length;
                 r.end_face_idx = end_face;
                 V3 hp = add(r.origin, mul(r.direction, length));
@@ -485,30 +507,31 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_gausslet_generation(TraceArgs A) 
         if ((threadIdx.x & 63) == 0 && mp) atomicAdd(&lds_parents, (uint32_t)__popcll(mp));
     }
     if (threadIdx.x < 64) {
-        const uint64_t excl = lookback_tagged(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error, A.status_tag);
+        const uint64_t excl = lookback_tagged(A.ws.tile_status, tile, 0, block_total, &A.ws.ctl->error, A.status_tag,
+                                              A.lookback_ticks);
         if (threadIdx.x == 0) lds_excl = excl;
     }
     __syncthreads();
     const uint64_t pos = lds_excl + local;
-    if (count > 0 && pos + count <= A.out.capacity) {
+    if (count > 0 && pos + count <= out_limit) {
         const rpx_face_t &fc = S.faces[hit_rec];
         const rpx_face_list_t &fl = S.face_lists[hit_list];
         const rpx_material_t &m = S.materials[fc.material];
-        Ray r = load_ray(A.in, i);
+        Ray r = load_ray(in, i);
         V3 hp = add(r.origin, mul(r.direction, r.length));
         Orientation o = compute_orientation<F>(S, fl, fc, hp);
         CollectEmit ce;
         eval_children<F>(m, r, (uint32_t)i, hp, o, A.wavelengths, ce);
-        const uint64_t co = A.out.capacity;
+        const uint64_t co = out.capacity;
         const int nk = min(min(ce.n, 3), (int)count);
         for (int k = 0; k < nk; ++k) {
             Ray child = ce.r[k];
             child.length = A.max_length;
-            store_ray(A.out, pos + k, child);
+            store_ray(out, pos + k, child);
         }
         for (int j = 0; j < RPX_N_PARA; ++j) {
             const uint64_t b = (uint64_t)j * RPX_PARA_FIELDS;
-            const double *P = A.in.para;
+            const double *P = in.para;
             V3 po = v3(P[(b + 0) * c + i], P[(b + 1) * c + i], P[(b + 2) * c + i]);
             V3 pd = v3(P[(b + 3) * c + i], P[(b + 4) * c + i], P[(b + 5) * c + i]);
             double plen = P[(b + 9) * c + i];
@@ -516,7 +539,7 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_gausslet_generation(TraceArgs A) 
             Orientation po_ = compute_orientation<F>(S, fl, fc, pp);
             for (int k = 0; k < nk; ++k) {
                 Para q = eval_parabasal<F>(m, ce.r[k], pd, pp, po_, ce.r[k].ray_type_id, A.wavelengths);
-                double *Q = A.out.para;
+                double *Q = out.para;
                 const uint64_t oi = pos + k;
                 Q[(b + 0) * co + oi] = q.origin.x;
                 Q[(b + 1) * co + oi] = q.origin.y;
@@ -534,7 +557,17 @@ __global__ __launch_bounds__(RPX_BLOCK) void k_gausslet_generation(TraceArgs A) 
     if (threadIdx.x == 0) {
         add_hits(A.ws.ctl, tile, lds_hits);
         if (lds_parents) atomicAdd(&A.ws.ctl->parent_shards[tile % RPX_SHARDS], lds_parents);
-        if ((uint64_t)tile == (A.n_in - 1) / RPX_BLOCK) A.ws.ctl->children = lds_excl + block_total;
+        if ((uint64_t)tile == (n_in - 1) / RPX_BLOCK) {
+            const uint64_t kids = lds_excl + block_total;
+            A.ws.ctl->children = kids;
+            if (A.link) {
+                const uint64_t at = A.link[0].out_offset;
+                A.link[1].count = kids;
+                A.link[1].offset = at;
+                A.link[1].out_offset = at + kids;
+                A.head_out->children = kids;   // host-mapped: read once the generation has ended
+            }
+        }
     }
     uint32_t *shard = A.ws.ctl->face_shards[tile % RPX_SHARDS];
     for (int k = threadIdx.x; k < S.n_list_faces; k += blockDim.x) {
@@ -1031,6 +1064,74 @@ extern "C" rpx_status_t rpx_trace_rays(const rpx_scene_t *scene, const rpx_rays_
         A.by_launch = cg.by_launch;
         A.lookback_ticks = cg.lookback_ticks;
         launch_generation(variant, dim3((unsigned)grid), stream, A);
+        return hipGetLastError();
+    };
+    return run_chain(spec, scene, workspace, launch, counts, offsets, n_traced, face_hits, (hipStream_t)stream_);
+}
+
+// rpx_trace_gausslets: the gausslet generations of trace_rays as one
+// device-resident chain (run_chain in rpx_gen.h).
+extern "C" rpx_status_t rpx_trace_gausslets(const rpx_scene_t *scene, const rpx_rays_t *in, uint64_t n_in,
+                                       double max_length, const double *wavelengths, uint64_t n_wavelengths,
+                                       int32_t max_generations, rpx_rays_t *pool, uint64_t *counts,
+                                       uint64_t *offsets, int32_t *n_traced, uint32_t *face_hits,
+                                       void *workspace, uint64_t workspace_bytes, void *stream_) {
+    if (!n_traced || !counts || !offsets) return set_error(RPX_E_INVALID, "counts / offsets / n_traced are null");
+    *n_traced = 0;
+    counts[0] = n_in;
+    offsets[0] = 0;
+    if (rpx_status_t st = rpx_check_trace_scene(scene)) return st;
+    if (rpx_status_t st = check_rays(in, n_in, true, "input gausslets")) return st;
+    if (!pool || (pool->capacity > 0 && (!pool->f64 || !pool->u32 || !pool->para)))
+        return set_error(RPX_E_INVALID, "pool");
+    if (max_generations < 0) return set_error(RPX_E_INVALID, "max_generations is negative");
+    if (n_in == 0 || max_generations == 0) return RPX_OK;
+    if (!wavelengths && n_wavelengths) return set_error(RPX_E_INVALID, "wavelength table is null");
+    if (!face_hits && scene->view.n_list_faces > 0) return set_error(RPX_E_INVALID, "face_hits is null");
+    ChainSpec spec;
+    spec.n_in = n_in;
+    spec.pool_rays = pool->capacity;
+    spec.max_generations = max_generations;
+    spec.max_children = (uint64_t)std::max(scene->view.max_children, 0);
+    spec.timing_slot = RPX_TIMING_GAUSSLET;
+    // as the one-call-per-generation gausslet path counts them: base and
+    // parabasal geometry plus length and end face of every gausslet, the
+    // rest of the base ray and the parabasal normals and lengths of every
+    // gausslet with children, and every child record
+    spec.bytes_in = 48.0 + 6 * 48.0 + 12.0;
+    spec.bytes_hit = 100.0 + 6 * 32.0;
+    spec.bytes_child = 188.0 + 6 * 80.0;
+    spec.bytes_per_parent = true;
+    if (rpx_status_t st = check_chain(spec, scene, workspace, workspace_bytes)) return st;
+    const uint32_t variant = scene_variant(scene);
+    TraceArgs A;
+    A.S = scene->view;
+    A.out = *pool;
+    A.n_in = 0;
+    A.max_length = (double)max_length;
+    A.wavelengths.wl = wavelengths;
+    A.wavelengths.n = wavelengths ? n_wavelengths : 0;
+    A.face_hits = nullptr;           // per-face totals come back in the heads
+    A.ws.tags = nullptr;
+    A.ws.block_totals = nullptr;
+    A.ws.block_parents = nullptr;
+    A.ws.block_offsets = nullptr;
+    A.one_list = -1;
+    A.one_face = -1;
+    A.pool_rays = pool->capacity;
+    auto launch = [&](const ChainGen &cg, uint64_t grid, hipStream_t stream) {
+        A.link = cg.link;
+        A.ws.ctl = cg.ctl;
+        A.ws.tile_status = cg.tile_status;
+        A.status_tag = cg.tag;
+        A.head_out = cg.head_out;
+        A.in_pool = cg.in_pool;
+        A.in = cg.in_pool ? *pool : *in;
+        A.store_children = cg.store_children;
+        A.tile_base = cg.tile_base;
+        A.by_launch = cg.by_launch;
+        A.lookback_ticks = cg.lookback_ticks;
+        launch_gausslet_generation(variant, dim3((unsigned)grid), stream, A);
         return hipGetLastError();
     };
     return run_chain(spec, scene, workspace, launch, counts, offsets, n_traced, face_hits, (hipStream_t)stream_);

@@ -40,7 +40,7 @@ def _device_bytes(torch):
     return _DEVICE_BYTES[dev]
 
 
-def _pool_rays(n, max_children, remaining):
+def _pool_rays(n, max_children, remaining, gausslet=False):
     """Pool size for a device-resident chain from a generation of n rays:
     room for the generations still allowed (at most _CHAIN_GENERATIONS) at
     about n rays each, plus the worst case the chain has to be sure of before
@@ -66,7 +66,7 @@ def _pool_rays(n, max_children, remaining):
     # allocator's statistics cost tens of microseconds per query, so they are
     # read only for a pool above a tenth of the device, which the cap could
     # bind (a smaller pool that does not fit is halved by _alloc_pool)
-    ray_bytes = _abi.N_F64 * 8 + _abi.N_U32 * 4
+    ray_bytes = _abi.N_F64 * 8 + _abi.N_U32 * 4 + (_abi.N_PARA * _abi.PARA_FIELDS * 8 if gausslet else 0)
     total = _device_bytes(torch)
     if want * ray_bytes <= total // 10:
         return max(want, mc * n, 1)
@@ -92,15 +92,18 @@ def _alloc_pool(make, want, minimum):
 
 
 def _trace_chain(ctx, rays, max_length, limit):
-    """trace_rays' generation loop for a RayCollection on the device
-    (rpx_trace_rays): generations are written into pools and traced with no
-    host round trip between them; a pool that cannot be sure to hold the
-    next generation ends the call, and the loop continues from that
-    generation with a fresh pool.  Returns the traced collections, the input
-    first."""
+    """trace_rays' generation loop for a RayCollection (rpx_trace_rays) or a
+    GaussletCollection (rpx_trace_gausslets) on the device: generations are
+    written into pools and traced with no host round trip between them; a
+    pool that cannot be sure to hold the next generation ends the call, and
+    the loop continues from that generation with a fresh pool.  Returns the
+    traced collections, the input first."""
     from ._device import DeviceRays, DeviceRaysView
     torch = require_gpu()
     lib = load()
+    gauss = isinstance(rays, GaussletCollection)
+    cls = GaussletCollection if gauss else RayCollection
+    entry = lib.rpx_trace_gausslets if gauss else lib.rpx_trace_rays
     traced = [rays]
     current = rays
     remaining = limit
@@ -109,7 +112,8 @@ def _trace_chain(ctx, rays, max_length, limit):
         n = current.n_rays
         dev_in = current._ensure_device()
         wl = _wavelength_table(current, ctx)
-        pool = _alloc_pool(DeviceRays, _pool_rays(n, ctx.max_children, remaining),
+        pool = _alloc_pool(lambda cap: DeviceRays(cap, gausslet=gauss),
+                           _pool_rays(n, ctx.max_children, remaining, gauss),
                            max(ctx.max_children, 1) * n if remaining > 1 else 1)
         ws_bytes = int(lib.rpx_chain_workspace_bytes(max(n, pool.capacity), remaining))
         ws = ctx.chain_workspace(ws_bytes)
@@ -117,10 +121,9 @@ def _trace_chain(ctx, rays, max_length, limit):
         offsets = (ctypes.c_uint64 * (remaining + 1))()
         n_traced = ctypes.c_int32(0)
         s_in, s_pool = dev_in.struct(), pool.struct()
-        code = lib.rpx_trace_rays(ctx.handle, ctypes.byref(s_in), n, float(max_length), ctypes.c_void_p(wl.data_ptr()),
-                                  len(current.wavelengths), remaining, ctypes.byref(s_pool), counts, offsets,
-                                  ctypes.byref(n_traced), hits, ctypes.c_void_p(ws.data_ptr()), ws_bytes,
-                                  stream_handle())
+        code = entry(ctx.handle, ctypes.byref(s_in), n, float(max_length), ctypes.c_void_p(wl.data_ptr()),
+                     len(current.wavelengths), remaining, ctypes.byref(s_pool), counts, offsets,
+                     ctypes.byref(n_traced), hits, ctypes.c_void_p(ws.data_ptr()), ws_bytes, stream_handle())
         if code != _abi.RPX_E_CAPACITY:
             check(code)
         current._device_changed()
@@ -130,7 +133,7 @@ def _trace_chain(ctx, rays, max_length, limit):
         last = k_traced + 1 if code == _abi.RPX_E_CAPACITY else k_traced
         parent = current
         for k in range(1, last):
-            gen = RayCollection.from_device(DeviceRaysView(pool, offsets[k], counts[k]), counts[k])
+            gen = cls.from_device(DeviceRaysView(pool, offsets[k], counts[k]), counts[k])
             gen.parent = parent
             traced.append(gen)
             parent = gen
@@ -154,15 +157,20 @@ def trace_rays(input_rays, face_lists, recursion_limit=100, max_length=100.0):
     with recursion_limit=L the L-th traced generation is not returned (its
     parents still receive their length and end_face_idx)."""
     # the device-resident loop writes every input ray's length itself (the
-    # distance to its face, or max_length), so the reset is the kernel's
-    chain = isinstance(input_rays, RayCollection)
-    all_faces = _prepare(input_rays, face_lists, max_length, reset=not chain or input_rays.n_rays == 0
+    # distance to its face, or max_length), so the reset is the kernel's; the
+    # gausslet generation reads the input lengths (the reference resets them)
+    rays_chain = isinstance(input_rays, RayCollection)
+    all_faces = _prepare(input_rays, face_lists, max_length, reset=not rays_chain or input_rays.n_rays == 0
                          or recursion_limit <= 0)
     face_sets = list(face_lists)
     for fs in face_sets:
         fs.sync_transforms()
     ctx = _TraceContext(face_sets, all_faces)
     decomp_faces = [f for f in all_faces if f.material.is_decomp_material()]
+    # gausslets that end on a decomposition face are captured and decomposed
+    # on the host between generations, so such scenes keep the per-generation
+    # loop below
+    chain = rays_chain or (isinstance(input_rays, GaussletCollection) and not decomp_faces)
     if chain:
         traced_rays = _trace_chain(ctx, input_rays, max_length, recursion_limit) \
             if input_rays.n_rays > 0 and recursion_limit > 0 else []

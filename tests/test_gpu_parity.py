@@ -271,7 +271,7 @@ def test_trace_chain_pool_resumes(gpu, monkeypatch, limit):
     against the oracle, for several recursion limits (the last generation's
     children are counted and discarded)."""
     from raypier_optics_amd import tracer
-    monkeypatch.setattr(tracer, "_pool_rays", lambda n, mc, remaining: max(mc * n, 1))
+    monkeypatch.setattr(tracer, "_pool_rays", lambda n, mc, remaining, *_: max(mc * n, 1))
     w = workloads.singlet_c1(n_rays=3000, depth=limit)
     ref, hits = _oracle_trace(w)
     got, faces = _gpu_trace(workloads.singlet_c1(n_rays=3000, depth=limit))
@@ -307,14 +307,15 @@ def test_trace_chain_matches_generation_path(gpu, orders):
     assert counts_chain == [f.count for f in all_faces]
 
 
-@pytest.mark.parametrize("form", ["f64", "split"])
+@pytest.mark.parametrize("form", ["f64", "split", "gausslet"])
 def test_trace_chain_tile_order_fallback(gpu, form):
     """The device-resident loop takes tiles in launch order and bounds each
     look-back wait; a wait that runs out makes it trace the chain again with
     tiles from the atomic counter.  With every wait made to run out at once
     (rpx_debug_lookback_ticks(1)), the fallback runs, and its generations and
     face counts equal an ordinary trace's bit for bit (C3 Michelson, 4,096
-    rays, depth 12, three grating orders; float64 and FP32-split)."""
+    rays, depth 12, three grating orders, float64 and FP32-split; and the
+    gausslet scene of test_trace_gausslets)."""
     import torch
     from raypier_optics_amd import _lib
     from raypier_optics_amd.split import trace_rays_split
@@ -322,6 +323,10 @@ def test_trace_chain_tile_order_fallback(gpu, form):
     run = trace_rays if form == "f64" else trace_rays_split
 
     def trace():
+        if form == "gausslet":
+            w, g = gausslet_workload()
+            got, faces = trace_rays(g, w.face_lists, recursion_limit=3, max_length=w.max_length)
+            return [x.copy_as_array().tobytes() for x in got], [f.count for f in faces]
         w = workloads.michelson_c3(n_rays=4096, orders=(-1, 0, 1))
         got, faces = run(w.collection(), w.face_lists, recursion_limit=w.recursion_limit, max_length=w.max_length)
         if form == "f64":
@@ -341,7 +346,7 @@ def test_trace_chain_tile_order_fallback(gpu, form):
     assert counts == ref_counts
     assert len(got) == len(ref)
     for g, (a, b) in enumerate(zip(got, ref)):
-        if form == "f64":
+        if form in ("f64", "gausslet"):
             assert a == b, g
         else:
             for x, y in zip(a, b):
@@ -421,6 +426,36 @@ def test_trace_gausslets(gpu):
             scale = np.linalg.norm(b["para_rays"][f], axis=-1)[..., None]
             assert np.all(np.abs(a["para_rays"][f] - b["para_rays"][f]) <= 1e-9 * np.maximum(scale, 1.0))
         assert np.array_equal(np.isinf(a["para_rays"]["length"]), np.isinf(b["para_rays"]["length"]))
+
+
+@pytest.mark.parametrize("small_pools", [False, True])
+def test_gausslet_chain_matches_generation_path(gpu, monkeypatch, small_pools):
+    """trace_rays on a gausslet collection runs the device-resident loop
+    (rpx_trace_gausslets), also with the smallest pools it accepts (so that
+    it stops and resumes); the one-call-per-generation path (trace_gausslet
+    on each generation) gives bit-identical generations, parabasal rays
+    included, and the same face counts."""
+    from raypier_optics_amd import tracer
+    from raypier_optics_amd.ctracer import trace_gausslet
+    if small_pools:
+        monkeypatch.setattr(tracer, "_pool_rays", lambda n, mc, remaining, *_: max(mc * n, 1))
+    w, g = gausslet_workload()
+    got, faces = trace_rays(g, w.face_lists, recursion_limit=3, max_length=w.max_length)
+    counts_chain = [f.count for f in faces]
+    w2, g2 = gausslet_workload()
+    all_faces = tracer._prepare(g2, w2.face_lists, w2.max_length)
+    seq = [g2]
+    while len(seq) < 3 and seq[-1].n_rays > 0:
+        kids = trace_gausslet(seq[-1], w2.face_lists, all_faces, max_length=w2.max_length)
+        if kids.n_rays == 0:
+            break
+        seq.append(kids)
+    if len(seq) == 3:   # the last generation is traced too (its children are discarded)
+        trace_gausslet(seq[-1], w2.face_lists, all_faces, max_length=w2.max_length)
+    assert len(got) == len(seq) == 3
+    for k, (a, b) in enumerate(zip(got, seq)):
+        assert a.copy_as_array().tobytes() == b.copy_as_array().tobytes(), k
+    assert counts_chain == [f.count for f in all_faces]
 
 
 def _compare_gausslets(got, ref, what):

@@ -206,7 +206,7 @@ def test_split_chain_matches_generation_path(gpu, monkeypatch, small_pools):
     from raypier_optics_amd import tracer, split
     from raypier_optics_amd.ctracer import _TraceContext
     if small_pools is True:
-        monkeypatch.setattr(tracer, "_pool_rays", lambda n, mc, remaining: max(mc * n, 1))
+        monkeypatch.setattr(tracer, "_pool_rays", lambda n, mc, remaining, *_: max(mc * n, 1))
     w = workloads.michelson_c3(n_rays=4096, **({"orders": (-1, 0, 1)} if small_pools == "multi" else {"order": 1}))
     got, faces = trace_rays_split(w.collection(), w.face_lists, recursion_limit=w.recursion_limit,
                                   max_length=w.max_length)
