@@ -160,7 +160,7 @@ def roofline(r, measured):
                                   for i in range(len(names)) if launches[i]}}
 
 
-def serving_latency(n_jobs, qps, seed=2024, grad_fraction=0.3, budget_bytes=3 << 29, generations_per_step=4):
+def serving_latency(n_jobs, qps, seed=2024, grad_fraction=0.3, budget_bytes=3 << 29, generations_per_step=2):
     """C5 stream: n_jobs random lens jobs (2-8 surfaces, rays log-uniform in
     [1e3, 2e6], built on the device) arriving as a Poisson process of rate
     qps into the continuous-batching server under a 1.5 GiB HBM budget (the

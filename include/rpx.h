@@ -690,7 +690,7 @@ rpx_status_t rpx_server_stats(const rpx_server_t *server, uint64_t *jobs_held_fo
                               uint64_t *peak_used_bytes);
 /* Seconds on the server's clock (the clock of t_submit / t_admit / t_finish). */
 double rpx_server_clock(void);
-/* Generations a step may trace per job: 1 to 4 (the default is 4; no
+/* Generations a step may trace per job: 1 to 4 (the default is 2; no
  * counterpart in the reference).  After the first, each generation of a step
  * is traced without a host round trip: the device lays out its tiles from
  * the previous generation's child counts, and a job traces it when it has a

@@ -20,6 +20,15 @@
 // Admission is first come, first served while the job's buffers (its input
 // generation plus max_children times as many children) fit the HBM budget;
 // the first waiting job is always admitted when nothing is active.
+//
+// Job buffers come from a heap of the server's own (rpx_heap.h): chunks of
+// device memory carved best-fit, freed blocks merged with their neighbours,
+// a chunk returned to the driver only when the chunks exceed the budget, so
+// steady serving makes no allocator call.  (Allocating from a HIP memory
+// pool instead was measured to take tens of milliseconds per call, and up to
+// 1.8 s, once a process had created and destroyed a few pools; plain
+// hipMalloc / hipFree per buffer wait for the device and cost up to
+// milliseconds each: one such call stalls every job in the step.)
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <chrono>
@@ -30,6 +39,7 @@
 #include "rpx_materials.h"
 #include "rpx_host.h"
 #include "rpx_gen.h"
+#include "rpx_heap.h"
 
 using namespace rpx;
 
@@ -325,9 +335,23 @@ struct StepGraph {
     void *batch_func = nullptr;
 };
 
+// Device memory for the job-buffer heap (hipFree waits for the device; the
+// heap calls it only when its chunks exceed the budget, between steps).
+struct DeviceChunks {
+    void *get(uint64_t bytes) {
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        return p;
+    }
+    void put(void *p) { (void)hipFree(p); }
+};
+
 struct rpx_server {
     hipStream_t stream = nullptr;
-    hipMemPool_t pool = nullptr;   // job buffers; keeps up to the HBM budget cached
+    rpx::BlockHeap<DeviceChunks> heap;   // job buffers
     uint64_t budget = 0, used = 0;
     uint64_t peak_used = 0, jobs_held_for_budget = 0;
     uint32_t max_jobs = 0;
@@ -339,7 +363,7 @@ struct rpx_server {
     uint64_t *d_counts = nullptr, *h_counts = nullptr;
     double *d_moments = nullptr, *h_moments = nullptr;
     StepGraph graphs[RPX_SERVE_MAX_GENS][3];   // by generations per step - 1, then kernel variant
-    uint32_t gens_per_step = RPX_SERVE_MAX_GENS;
+    uint32_t gens_per_step = 2;
     void *clear_args[4] = {&d_status, &max_tiles, &d_jobs, &d_ctl};
     void *batch_args[4] = {&d_jobs, &d_ctl, &d_status, &d_partials};
     void *reduce_args[3] = {&d_jobs, &d_ctl, &d_partials};
@@ -360,16 +384,18 @@ static rpx_status_t alloc_buffer(rpx_server_t *s, Buffer &b, uint64_t cap) {
     }
     if (s->allocs_until_failure > 0) --s->allocs_until_failure;
     b.cap = std::max<uint64_t>(cap, 1);
-    hipError_t e = hipMallocFromPoolAsync(&b.p, b.bytes(), s->pool, s->stream);
-    if (e != hipSuccess) {
+    b.p = s->heap.alloc(b.bytes());
+    if (!b.p) {
         b = Buffer();
-        return rpx_set_error(RPX_E_HIP, std::string("job buffer allocation: ") + hipGetErrorString(e));
+        return rpx_set_error(RPX_E_HIP, "job buffer allocation: out of device memory");
     }
     return RPX_OK;
 }
 
+// Every kernel that used the buffer has completed: the server synchronises
+// its stream after each step and after an input copy.
 static void free_buffer(rpx_server_t *s, Buffer &b) {
-    if (b.p) (void)hipFreeAsync(b.p, s->stream);
+    if (b.p) (void)s->heap.free(b.p);
     b.p = nullptr;
     b.cap = 0;
 }
@@ -462,6 +488,7 @@ extern "C" rpx_status_t rpx_server_create(uint64_t hbm_budget_bytes, uint32_t ma
     if (max_rays_in_flight == 0) return rpx_set_error(RPX_E_INVALID, "max_rays_in_flight is 0");
     rpx_server_t *s = new rpx_server_t();
     s->budget = hbm_budget_bytes;
+    s->heap.set_limits(hbm_budget_bytes, std::min<uint64_t>(hbm_budget_bytes, 256ull << 20));
     s->max_jobs = max_jobs;
     s->max_tiles = (max_rays_in_flight + RPX_BLOCK - 1) / RPX_BLOCK + max_jobs;
     auto fail = [&](rpx_status_t st) {
@@ -469,24 +496,6 @@ extern "C" rpx_status_t rpx_server_create(uint64_t hbm_budget_bytes, uint32_t ma
         return st;
     };
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return fail(rpx_set_error(RPX_E_HIP, hipGetErrorString(e)));
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return fail(rpx_set_error(RPX_E_NO_DEVICE, "no HIP device"));
-    // A pool of the server's own whose release threshold is the budget: the
-    // buffers freed each step stay mapped for the next step's allocations
-    // instead of going back to the driver at every synchronisation.
-    hipMemPoolProps props = {};
-    props.allocType = hipMemAllocationTypePinned;
-    props.handleTypes = hipMemHandleTypeNone;
-    props.location.type = hipMemLocationTypeDevice;
-    props.location.id = dev;
-    e = hipMemPoolCreate(&s->pool, &props);
-    if (e != hipSuccess) {
-        s->pool = nullptr;
-        return fail(rpx_set_error(RPX_E_HIP, hipGetErrorString(e)));
-    }
-    uint64_t keep = hbm_budget_bytes;
-    e = hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold, &keep);
     if (e != hipSuccess) return fail(rpx_set_error(RPX_E_HIP, hipGetErrorString(e)));
     if (hipMalloc(&s->d_jobs, sizeof(BatchJob) * max_jobs) != hipSuccess ||
         hipMalloc(&s->d_ctl, sizeof(BatchControl)) != hipSuccess ||
@@ -536,7 +545,7 @@ extern "C" rpx_status_t rpx_server_destroy(rpx_server_t *s) {
     (void)hipHostFree(s->h_ctl_back);
     (void)hipHostFree(s->h_counts);
     (void)hipHostFree(s->h_moments);
-    if (s->pool) (void)hipMemPoolDestroy(s->pool);   // every job buffer was freed and the stream synchronised
+    s->heap.release_all();   // every job buffer is back and the stream synchronised
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return RPX_OK;

@@ -14,6 +14,7 @@ and the spot of its detector hits.
             ...
 """
 import ctypes
+import gc
 import math
 import time
 
@@ -53,10 +54,10 @@ class JobResult(object):
 class TraceServer(object):
     """Owns an rpx_server_t: HBM budget in bytes, at most `max_jobs` jobs on
     the device, at most `max_rays_in_flight` rays per step, and up to
-    `generations_per_step` (1 or 2) generations of each job per step."""
+    `generations_per_step` (1 to 4) generations of each job per step."""
 
     def __init__(self, hbm_budget_bytes=64 << 30, max_jobs=256, max_rays_in_flight=32 << 20,
-                 generations_per_step=4):
+                 generations_per_step=2):
         require_gpu()
         self.lib = load()
         h = ctypes.c_void_p()
@@ -101,7 +102,10 @@ class TraceServer(object):
         rc.reset_length(workload.max_length)   # as trace_rays does (core/tracer.py:20)
         dev = rc._ensure_device()
         wl = rc._device_wavelengths()
-        torch.cuda.synchronize()
+        # the rays and wavelengths were written on this thread's stream; the
+        # server reads them from a stream of its own (a device-wide
+        # synchronize would also wait for gradient work on other streams)
+        torch.cuda.current_stream().synchronize()
         if detector_face is None:
             detector_face = len(all_faces) - 1
         s = dev.struct()
@@ -211,16 +215,20 @@ def run_poisson(server, jobs, qps, seed=0, gradient=None):
     the wall time.  A job is submitted at the first step boundary after its
     arrival time (or, when the server refuses it because waiting inputs fill
     its HBM budget, after the next step); its latency runs from its arrival
-    to its result.  Jobs k
-    with gradient[k] true also get d(spot RMS)/d(surface parameters) from
-    the adjoint (result.grad, result.grad_spot): the gradient starts when the
-    job is submitted and runs beside the server's steps on a stream of its
-    own; the job's result is ready once its trace and its gradient both are,
-    and its latency includes both."""
+    to its result.  Jobs k with gradient[k] true also get d(spot RMS)/d(surface
+    parameters) from the adjoint (result.grad, result.grad_spot): the gradient
+    starts when the job is submitted and runs beside the server's steps on a
+    stream of its own; the job's result is ready once its trace and its gradient both are,
+    and its latency includes both.
+
+    The objects alive when the stream starts are frozen out of Python's
+    cyclic garbage collector for its duration: a full collection of a heap
+    with torch imported scans about 2e5 objects (75-90 ms), which would stall
+    the step loop and every job in flight; collections during the stream
+    scan only the objects made since."""
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / qps, len(jobs)))
     worker = _GradientWorker() if gradient is not None and any(gradient) else None
-    t0 = server.clock()
     nxt = 0
     results, ids = {}, {}
     traced = {}      # k -> JobResult of a gradient job whose gradient is not back yet
@@ -242,6 +250,9 @@ def run_poisson(server, jobs, qps, seed=0, gradient=None):
         for k in [k for k in traced if k in grads]:
             finish(k, traced.pop(k))
 
+    gc.collect()
+    gc.freeze()
+    t0 = server.clock()
     try:
         while len(results) < len(jobs):
             now = server.clock() - t0
@@ -275,5 +286,6 @@ def run_poisson(server, jobs, qps, seed=0, gradient=None):
     finally:
         if worker is not None:
             worker.close()
+        gc.unfreeze()
     wall = server.clock() - t0
     return [results[k] for k in range(len(jobs))], wall

@@ -153,7 +153,7 @@ def test_server_admits_while_running(gpu):
 
 
 def test_two_servers_at_once(gpu):
-    """Two servers live together, each with its own stream and memory pool,
+    """Two servers live together, each with its own stream and buffer cache,
     stepped in turn: every job's result is bit-identical between them."""
     jobs = _jobs(seeds=range(20, 26))
     servers = [TraceServer(hbm_budget_bytes=64 << 20, max_jobs=4, max_rays_in_flight=1 << 22) for _ in range(2)]
