@@ -287,6 +287,9 @@ int32_t     rpx_struct_size(const char *name);
  * Face / InterfaceMaterial objects in ctracer.pyx:2062-2118 and the
  * FaceList.sync_transforms calls of ctracer.pyx:2173-2211. */
 rpx_status_t rpx_scene_create(const rpx_scene_desc_t *desc, rpx_scene_t **out);
+/* No work that reads the scene may still be in flight: its device tables are
+ * reused by the next scene created on that device, with no device-wide
+ * synchronisation. */
 rpx_status_t rpx_scene_destroy(rpx_scene_t *scene);
 /* Largest number of children one ray can produce in this scene. */
 int32_t      rpx_scene_max_children(const rpx_scene_t *scene);

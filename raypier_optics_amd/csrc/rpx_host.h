@@ -6,13 +6,29 @@
 
 struct rpx_scene {
     rpx::SceneView view;
-    void *device_block;
+    void *device_block;       // the device tables, a block of the device's scene heap
+    int device;               // the device they live on
     int32_t n_global_faces;
     int32_t list_sizes[16];
     uint32_t features;        // face / material families present (F_*, M_*)
     bool negative_face_idx;   // a face has global_idx < 0: usable for selection, not for tracing
     std::vector<rpx_shape_op_t> host_shape_ops;   // for validating query roots
     std::vector<int32_t> face_global_idx;         // Face.idx of each face record (host copy)
+};
+
+// Device memory for an rpx::BlockHeap (rpx_heap.h): chunks from hipMalloc,
+// returned with hipFree (which waits for the device, so heaps return chunks
+// rarely).
+struct DeviceChunks {
+    void *get(uint64_t bytes) {
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        return p;
+    }
+    void put(void *p) { (void)hipFree(p); }
 };
 
 // Checks that ops[begin..root] is a well-formed postfix program.

@@ -335,23 +335,9 @@ struct StepGraph {
     void *batch_func = nullptr;
 };
 
-// Device memory for the job-buffer heap (hipFree waits for the device; the
-// heap calls it only when its chunks exceed the budget, between steps).
-struct DeviceChunks {
-    void *get(uint64_t bytes) {
-        void *p = nullptr;
-        if (hipMalloc(&p, bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            return nullptr;
-        }
-        return p;
-    }
-    void put(void *p) { (void)hipFree(p); }
-};
-
 struct rpx_server {
     hipStream_t stream = nullptr;
-    rpx::BlockHeap<DeviceChunks> heap;   // job buffers
+    rpx::BlockHeap<DeviceChunks> heap;   // job buffers (chunks returned only above the budget, between steps)
     uint64_t budget = 0, used = 0;
     uint64_t peak_used = 0, jobs_held_for_budget = 0;
     uint32_t max_jobs = 0;

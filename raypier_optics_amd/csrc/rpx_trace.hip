@@ -18,6 +18,8 @@
 #include <mutex>
 #include "rpx_materials.h"
 #include "rpx_host.h"
+#include "rpx_heap.h"
+#include <map>
 #include "rpx_gen.h"
 
 using namespace rpx;
@@ -191,6 +193,42 @@ static size_t push_table(std::vector<char> &blob, const T *src, int n) {
     return off;
 }
 
+// The device tables of every scene on one device come from one heap: 2 MiB
+// chunks carved into blocks, so creating and destroying a scene makes no
+// hipMalloc / hipFree (hipFree waits for the whole device: a serving job's
+// submission paid about 0.5 ms for it, up to 55 ms).  A destroyed scene's
+// block is reused by the next scene at once; no work that reads a scene may
+// be in flight when it is destroyed.  The heaps live until the process ends.
+namespace {
+struct SceneHeaps {
+    std::mutex m;
+    std::map<int, rpx::BlockHeap<DeviceChunks> *> by_device;
+};
+
+SceneHeaps &scene_heaps() {
+    static SceneHeaps *h = new SceneHeaps();
+    return *h;
+}
+
+void *scene_block_alloc(int dev, uint64_t bytes) {
+    SceneHeaps &H = scene_heaps();
+    std::lock_guard<std::mutex> lock(H.m);
+    rpx::BlockHeap<DeviceChunks> *&heap = H.by_device[dev];
+    if (!heap) {
+        heap = new rpx::BlockHeap<DeviceChunks>();
+        heap->set_limits(64ull << 20, 2ull << 20);
+    }
+    return heap->alloc(bytes);
+}
+
+void scene_block_free(int dev, void *p) {
+    SceneHeaps &H = scene_heaps();
+    std::lock_guard<std::mutex> lock(H.m);
+    auto it = H.by_device.find(dev);
+    if (it != H.by_device.end() && it->second) (void)it->second->free(p);
+}
+}  // namespace
+
 extern "C" rpx_status_t rpx_scene_create(const rpx_scene_desc_t *d, rpx_scene_t **out) {
     if (!out) return set_error(RPX_E_INVALID, "out is null");
     *out = nullptr;
@@ -209,15 +247,18 @@ extern "C" rpx_status_t rpx_scene_create(const rpx_scene_desc_t *d, rpx_scene_t 
     size_t o_zc = push_table(blob, d->zernike_coefs, d->n_zernike_coefs);
     size_t o_zi = push_table(blob, d->zernike_insns, d->n_zernike_insns);
     blob.resize(((blob.size() + 255) & ~size_t(255)) + 256);
-    char *dev = nullptr;
-    HIP_TRY(hipMalloc((void **)&dev, blob.size()));
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device));
+    char *dev = (char *)scene_block_alloc(device, blob.size());
+    if (!dev) return set_error(RPX_E_HIP, "scene upload: out of device memory");
     hipError_t e = hipMemcpy(dev, blob.data(), blob.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        (void)hipFree(dev);
+        scene_block_free(device, dev);
         return set_error(RPX_E_HIP, std::string("scene upload: ") + hipGetErrorString(e));
     }
     rpx_scene *s = new rpx_scene();
     s->device_block = dev;
+    s->device = device;
     SceneView &v = s->view;
     v.face_lists = (const rpx_face_list_t *)(dev + o_fl);
     v.faces = (const rpx_face_t *)(dev + o_f);
@@ -278,9 +319,8 @@ extern "C" rpx_status_t rpx_scene_create(const rpx_scene_desc_t *d, rpx_scene_t 
 
 extern "C" rpx_status_t rpx_scene_destroy(rpx_scene_t *s) {
     if (!s) return RPX_OK;
-    hipError_t e = hipFree(s->device_block);
+    scene_block_free(s->device, s->device_block);
     delete s;
-    if (e != hipSuccess) return set_error(RPX_E_HIP, hipGetErrorString(e));
     return RPX_OK;
 }
 
