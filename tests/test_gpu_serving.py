@@ -265,3 +265,32 @@ def test_run_poisson_budget_backpressure(gpu):
         a = _alone(w)
         assert r.status == 0 and (r.generations, r.interactions) == (a.generations, a.interactions)
         assert r.moments == a.moments
+
+
+def test_servers_one_after_another(gpu):
+    """Eight servers created, run and closed in turn in one process, each under
+    a budget small enough that its buffer heap releases chunks: every server
+    returns the same results, and no step stalls (with job buffers from a
+    HIP memory pool per server, allocation calls from the seventh server on
+    took up to seconds)."""
+    import time
+    jobs = _jobs(seeds=range(40, 46), sizes=[30000, 2000, 60000, 9000, 45000, 1000])
+    first, worst = None, 0.0
+    for k in range(8):
+        # the inputs (28 MB) fit; the largest job's children (23 MB) wait for room
+        server = TraceServer(hbm_budget_bytes=48 << 20, max_jobs=4, max_rays_in_flight=1 << 22)
+        try:
+            ids = [server.submit(w) for w in jobs]
+            res = {}
+            while server.busy():
+                t0 = time.perf_counter()
+                res.update({r.job_id: r for r in server.step()})
+                worst = max(worst, time.perf_counter() - t0)
+        finally:
+            server.close()
+        got = [res[j] for j in ids]
+        if first is None:
+            first = got
+        for a, b in zip(first, got):
+            _same(a, b, "server %d" % k)
+    assert worst < 0.25, "slowest step %.3f s" % worst
