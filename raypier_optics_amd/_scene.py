@@ -294,7 +294,7 @@ def z_band(rec, face, builder=None):
     from .cshapes import bound_radius
     nan = (float("nan"), float("nan"))
     t = int(rec["type"])
-    p = [float(x) for x in rec["p"]]
+    p = rec["p"].tolist()
     if t in (_abi.FACE_CIRCULAR, _abi.FACE_SHAPED_PLANAR, _abi.FACE_RECTANGULAR, _abi.FACE_POLYGON,
              _abi.FACE_EXTRUDED_PLANAR):
         if t == _abi.FACE_EXTRUDED_PLANAR:
@@ -326,7 +326,7 @@ def z_band(rec, face, builder=None):
         bt = int(base_rec["type"])
         if bt not in (_abi.FACE_CIRCULAR, _abi.FACE_SHAPED_PLANAR, _abi.FACE_RECTANGULAR, _abi.FACE_POLYGON):
             return nan
-        bp = [float(x) for x in base_rec["p"]]
+        bp = base_rec["p"].tolist()
         z = {_abi.FACE_CIRCULAR: bp[2], _abi.FACE_RECTANGULAR: bp[3]}.get(bt, bp[0])
         zrec = builder._zernikes[int(rec["aux1"])]
         rho = bound_radius(face.shape) / float(zrec["unit_radius"])

@@ -1012,7 +1012,9 @@ class _TraceContext(object):
         self.all_faces = all_faces
         self.tables = compile_scene(self.face_sets)
         self.handle = self.tables.handle()
-        self.max_children = self.tables.max_children
+        # the same count as SceneTables.max_children, read from the uploaded
+        # scene (a numpy walk over the face records costs tens of us a trace)
+        self.max_children = int(load().rpx_scene_max_children(self.handle))
         self._hits = None
         self._ws = None
 

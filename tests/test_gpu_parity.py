@@ -623,3 +623,22 @@ def test_trace_c3_michelson(gpu, order):
             assert counts.max() <= 3
             triples += int((counts == 3).sum())
         assert triples > 0
+
+
+def test_scene_max_children_matches_tables(gpu):
+    """The trace reads max_children from the uploaded scene
+    (rpx_scene_max_children); it equals the host tables' own count on scenes
+    of one, two and three children per ray."""
+    from raypier_optics_amd import workloads as wl
+    from raypier_optics_amd._lib import load
+    from raypier_optics_amd._scene import compile_scene
+    cases = [wl.asphere_doublet_c2(n_rays=64), wl.lens_job_c5(7, n_rays=64),
+             wl.michelson_c3(n_rays=64, orders=(-1, 0, 1)), wl.michelson_c3(n_rays=64, orders=(1,))]
+    seen = set()
+    for w in cases:
+        wl.prepare_faces(w.face_lists, w.wavelengths, w.max_length)
+        t = compile_scene(w.face_lists)
+        got = int(load().rpx_scene_max_children(t.handle()))
+        assert got == t.max_children, (w.name, got, t.max_children)
+        seen.add(got)
+    assert seen >= {2, 3}
