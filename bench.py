@@ -392,6 +392,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=128)
+    ap.add_argument("--serve-repeat", type=int, default=0,
+                    help="also run the serving stream this many more times, a fresh server each time (p50/p99 of each)")
     ap.add_argument("--field", type=int, default=1, help="run the C4 gausslet field leg (0 to skip)")
     ap.add_argument("--michelson", type=int, default=1, help="run the C3 Michelson leg (0 to skip)")
     ap.add_argument("--serve-qps", type=float, default=200.0)
@@ -485,6 +487,11 @@ def main():
         "latency_one_generation_per_step": {k: v for k, v in serving_latency(
             args.serve_jobs, args.serve_qps, generations_per_step=1).items()
             if k in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "jobs_per_s")} if args.serve_jobs > 0 else None,
+        # servers created and closed in turn in one process (buffer heaps, not
+        # a memory pool per server, keep the later ones as fast as the first)
+        "latency_repeats": [{k: v for k, v in serving_latency(args.serve_jobs, args.serve_qps, seed=2024 + k).items()
+                             if k in ("p50_ms", "p99_ms", "mean_ms", "max_ms")} for k in range(args.serve_repeat)]
+        if args.serve_jobs > 0 and args.serve_repeat > 0 else None,
         "adjoint": adjoint_timing(),
         "field": field_c4() if args.field else None,
         "michelson": michelson_c3_leg() if args.michelson else None,
