@@ -702,6 +702,11 @@ double rpx_server_clock(void);
  * it writes into (a job's two buffers alternate).  A job's results are the
  * same for every setting. */
 rpx_status_t rpx_server_set_generations_per_step(rpx_server_t *server, uint32_t generations);
+/* How a step's work is enqueued: on != 0 (the default) replays a HIP graph
+ * built once per kernel variant and generation count, 0 launches the same
+ * copies and kernels one by one on the server's stream.  Results are the
+ * same either way (no counterpart in the reference). */
+rpx_status_t rpx_server_set_graph_launch(rpx_server_t *server, int32_t on);
 /* Test hook (no counterpart in the reference): after n_allocations further
  * job-buffer allocations succeed, the next one fails once, as if the device
  * were out of memory; n_allocations < 0 cancels a pending failure. */

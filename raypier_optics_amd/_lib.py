@@ -59,6 +59,7 @@ _SIGNATURES = {
     "rpx_server_stats": (c_i32, [c_void_p, c_void_p, c_void_p]),
     "rpx_server_clock": (c_f64, []),
     "rpx_server_set_generations_per_step": (c_i32, [c_void_p, c_u32]),
+    "rpx_server_set_graph_launch": (c_i32, [c_void_p, c_i32]),
     "rpx_server_fail_allocations_after": (c_i32, [c_void_p, ctypes.c_int64]),
     "rpx_lens_workspace_bytes": (c_u64, [c_u64, c_i32]),
     "rpx_lens_eval": (c_i32, [P(_abi.Lens), c_void_p, P(_abi.Rays), c_u64, c_void_p, c_u64, c_i32, c_void_p,

@@ -15,7 +15,8 @@
 // table upload, status clear, the tile kernel, the moment sums, counts
 // download -- is one HIP graph, built once per kernel variant and replayed
 // every step with the tables rewritten in pinned host memory and the grids of
-// its two tile-sized kernels reset.
+// its two tile-sized kernels reset (launch_step enqueues the same work node by
+// node, when graph launch is turned off; the graph measured faster).
 //
 // Admission is first come, first served while the job's buffers (its input
 // generation plus max_children times as many children) fit the HBM budget;
@@ -350,6 +351,7 @@ struct rpx_server {
     double *d_moments = nullptr, *h_moments = nullptr;
     StepGraph graphs[RPX_SERVE_MAX_GENS][3];   // by generations per step - 1, then kernel variant
     uint32_t gens_per_step = 2;
+    bool graph_launch = true;    // replay the step graph (else launch its nodes one by one)
     void *clear_args[4] = {&d_status, &max_tiles, &d_jobs, &d_ctl};
     void *batch_args[4] = {&d_jobs, &d_ctl, &d_status, &d_partials};
     void *reduce_args[3] = {&d_jobs, &d_ctl, &d_partials};
@@ -460,6 +462,34 @@ static rpx_status_t size_graph(rpx_server_t *s, StepGraph &G, const uint64_t *ti
     };
     for (int k = 0; k < G.gens; ++k)
         if (rpx_status_t st = set(G.clear[k], G.batch[k], tiles[k])) return st;
+    return RPX_OK;
+}
+
+// The step graph's work enqueued node by node on the server's stream: the
+// job table (only the batched jobs' entries) and control block up, per
+// generation the relayout (after the first), clear, tile kernel and moment
+// sums, then the counts, moments and control block back.
+template <uint32_t F>
+static rpx_status_t launch_step(rpx_server_t *s, int gens, const uint64_t *tiles, uint64_t n_jobs) {
+    hipStream_t st = s->stream;
+    RPX_HIP_TRY(hipMemcpyAsync(s->d_jobs, s->h_jobs, sizeof(BatchJob) * n_jobs, hipMemcpyHostToDevice, st));
+    RPX_HIP_TRY(hipMemcpyAsync(s->d_ctl, s->h_ctl, sizeof(BatchControl), hipMemcpyHostToDevice, st));
+    for (int k = 0; k < gens; ++k) {
+        if (k > 0) hipLaunchKernelGGL(k_batch_relayout, dim3(1), dim3(256), 0, st, s->d_jobs, s->d_ctl);
+        const uint64_t t = tiles[k];
+        hipLaunchKernelGGL(k_batch_clear, dim3((unsigned)((std::max(t, n_jobs) + 255) / 256)), dim3(256), 0, st,
+                           s->d_status, s->max_tiles, s->d_jobs, s->d_ctl);
+        hipLaunchKernelGGL((k_batch<F>), dim3((unsigned)std::max<uint64_t>(t, 1)), dim3(RPX_BLOCK), 0, st,
+                           s->d_jobs, s->d_ctl, s->d_status, s->d_partials);
+        hipLaunchKernelGGL(k_batch_reduce, dim3((unsigned)n_jobs), dim3(256), 0, st, s->d_jobs, s->d_ctl,
+                           s->d_partials);
+    }
+    RPX_HIP_TRY(hipGetLastError());
+    RPX_HIP_TRY(hipMemcpyAsync(s->h_counts, s->d_counts, sizeof(uint64_t) * RPX_SERVE_COUNT_WORDS * s->max_jobs,
+                               hipMemcpyDeviceToHost, st));
+    RPX_HIP_TRY(hipMemcpyAsync(s->h_moments, s->d_moments, sizeof(double) * 6 * s->max_jobs, hipMemcpyDeviceToHost,
+                               st));
+    RPX_HIP_TRY(hipMemcpyAsync(s->h_ctl_back, s->d_ctl, sizeof(BatchControl), hipMemcpyDeviceToHost, st));
     return RPX_OK;
 }
 
@@ -725,16 +755,23 @@ extern "C" rpx_status_t rpx_server_step(rpx_server_t *s, rpx_job_result_t *finis
         s->h_ctl->tile_counter = 0;
         s->h_ctl->next_block = 1;
         const int vi = variant == V_SPHERES ? 0 : variant == V_LENS ? 1 : 2;
-        StepGraph &G = s->graphs[gens - 1][vi];
-        if (!G.exec) {
-            rpx_status_t st = vi == 0 ? build_graph<V_SPHERES>(s, G, gens)
-                            : vi == 1 ? build_graph<V_LENS>(s, G, gens)
-                                      : build_graph<V_ALL>(s, G, gens);
+        if (s->graph_launch) {
+            StepGraph &G = s->graphs[gens - 1][vi];
+            if (!G.exec) {
+                rpx_status_t st = vi == 0 ? build_graph<V_SPHERES>(s, G, gens)
+                                : vi == 1 ? build_graph<V_LENS>(s, G, gens)
+                                          : build_graph<V_ALL>(s, G, gens);
+                if (st) return st;
+            }
+            rpx_status_t st = size_graph(s, G, tiles, batch.size());
+            if (st) return st;
+            RPX_HIP_TRY(hipGraphLaunch(G.exec, s->stream));
+        } else {
+            rpx_status_t st = vi == 0 ? launch_step<V_SPHERES>(s, gens, tiles, batch.size())
+                            : vi == 1 ? launch_step<V_LENS>(s, gens, tiles, batch.size())
+                                      : launch_step<V_ALL>(s, gens, tiles, batch.size());
             if (st) return st;
         }
-        rpx_status_t st = size_graph(s, G, tiles, batch.size());
-        if (st) return st;
-        RPX_HIP_TRY(hipGraphLaunch(G.exec, s->stream));
         RPX_HIP_TRY(hipStreamSynchronize(s->stream));
         if (s->h_ctl_back->error) {
             // the launch's counts and moments cannot be trusted: every job in
@@ -840,5 +877,11 @@ extern "C" rpx_status_t rpx_server_counts(const rpx_server_t *s, uint32_t *waiti
     if (!s) return rpx_set_error(RPX_E_INVALID, "server is null");
     if (waiting) *waiting = (uint32_t)s->waiting.size();
     if (active) *active = (uint32_t)s->active.size();
+    return RPX_OK;
+}
+
+extern "C" rpx_status_t rpx_server_set_graph_launch(rpx_server_t *s, int32_t on) {
+    if (!s) return rpx_set_error(RPX_E_INVALID, "null server");
+    s->graph_launch = on != 0;
     return RPX_OK;
 }

@@ -54,10 +54,12 @@ class JobResult(object):
 class TraceServer(object):
     """Owns an rpx_server_t: HBM budget in bytes, at most `max_jobs` jobs on
     the device, at most `max_rays_in_flight` rays per step, and up to
-    `generations_per_step` (1 to 4) generations of each job per step."""
+    `generations_per_step` (1 to 4) generations of each job per step;
+    `graph_launch` (the default) replays each step as a HIP graph; False
+    launches its copies and kernels one by one."""
 
     def __init__(self, hbm_budget_bytes=64 << 30, max_jobs=256, max_rays_in_flight=32 << 20,
-                 generations_per_step=2):
+                 generations_per_step=2, graph_launch=True):
         require_gpu()
         self.lib = load()
         h = ctypes.c_void_p()
@@ -66,6 +68,7 @@ class TraceServer(object):
         self.h = h
         try:
             check(self.lib.rpx_server_set_generations_per_step(h, int(generations_per_step)))
+            check(self.lib.rpx_server_set_graph_launch(h, int(bool(graph_launch))))
         except Exception:
             self.lib.rpx_server_destroy(h)
             self.h = None

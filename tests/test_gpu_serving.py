@@ -294,3 +294,23 @@ def test_servers_one_after_another(gpu):
         for a, b in zip(first, got):
             _same(a, b, "server %d" % k)
     assert worst < 0.25, "slowest step %.3f s" % worst
+
+
+def test_graph_launch_same_results(gpu):
+    """A step replayed as a HIP graph (the default) and the same step
+    enqueued node by node give bit-identical job results, at one and two
+    generations per step."""
+    jobs = _jobs(seeds=range(50, 56))
+    for gens in (1, 2):
+        res = []
+        for graph in (False, True):
+            server = TraceServer(hbm_budget_bytes=96 << 20, max_jobs=8, max_rays_in_flight=1 << 22,
+                                 generations_per_step=gens, graph_launch=graph)
+            try:
+                ids = [server.submit(w) for w in jobs]
+                done = {r.job_id: r for r in _drain(server)}
+            finally:
+                server.close()
+            res.append([done[j] for j in ids])
+        for k, (a, b) in enumerate(zip(*res)):
+            _same(a, b, "job %d, %d per step" % (k, gens))
